@@ -1,0 +1,249 @@
+/*
+ * spike_hip.h -- C ABI of the MI355X (gfx950) simulation backend for Spike.
+ *
+ * This is the boundary between Spike's C++ front-end (SpikingModel, neuron groups,
+ * synapse groups, plasticity rules, SpikingActivityMonitor) and the hand-written HIP
+ * kernels.  The C++ "HIP" backend classes registered in Spike's backend factory
+ * (Spike/Backend/Macros.hpp:54-70 pattern) translate every backend virtual call into
+ * one of the entry points below.  Everything here is plain C: opaque handles,
+ * pointers, sizes and int status codes (0 = success).  Host pointers are host memory;
+ * the library owns all device memory and one HIP stream per network.
+ *
+ * Conventions
+ *   - Presynaptic ids follow Spike's encoding: id >= 0 is a LIF neuron, id < 0 is input
+ *     neuron (-id - 1) (Spike/Neurons/Neurons.hpp:34-35).
+ *   - Synapse arrays are passed in the caller's order; the library keeps its own
+ *     delay-sorted CSR internally and returns weights in the caller's order.
+ *   - Times are float seconds exactly as the reference passes them to its backends
+ *     (t = start + (s*G)*dt, Spike/Models/SpikingModel.cpp:294-296); the timestep index
+ *     is roundf(t/dt) as in Spike/Backend/CUDA/Neurons/LIFSpikingNeurons.cu:84.
+ *   - Synaptic input is accumulated into the per-delay input ring as exact 64-bit
+ *     fixed-point sums (2^-frac_bits units), so results do not depend on the order in
+ *     which spikes are delivered.
+ */
+#ifndef SPIKE_HIP_H
+#define SPIKE_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct spk_network spk_network;
+
+/* status codes */
+enum {
+  SPK_OK = 0,
+  SPK_ERR_ARG = 1,       /* invalid argument or size */
+  SPK_ERR_STATE = 2,     /* call out of order (e.g. before spk_prepare) */
+  SPK_ERR_HIP = 3,       /* a HIP runtime call failed */
+  SPK_ERR_NOMEM = 4,     /* device or host allocation failed */
+  SPK_ERR_CAPACITY = 5,  /* a bounded device store overflowed */
+  SPK_ERR_NODEVICE = 6   /* no usable gfx950 device */
+};
+
+/* synapse models (Spike/Synapses/{Voltage,Conductance}SpikingSynapses.hpp) */
+enum { SPK_SYN_VOLTAGE = 0, SPK_SYN_CONDUCTANCE = 1 };
+
+/* populations */
+enum { SPK_POP_NEURONS = 0, SPK_POP_INPUTS = 1 };
+
+/* run-time switches for spk_run (bit flags) */
+enum {
+  SPK_RUN_PLASTICITY = 1,   /* apply plasticity rules (SpikingModel::run plasticity_on) */
+  SPK_RUN_FORCE_LAUNCHES = 2 /* use the per-phase kernels instead of the fused kernel */
+};
+
+/* Weight-dependent STDP parameters.
+ * Replaces weightdependent_stdp_plasticity_parameters_struct
+ * (Spike/Plasticity/WeightDependentSTDPPlasticity.hpp:28-40). */
+typedef struct {
+  float a_minus, a_plus, tau_minus, tau_plus, lambda, alpha, w_max;
+  int nearest_spike_only;
+} spk_wd_stdp_params;
+
+/* Vogels inhibitory STDP parameters.
+ * Replaces inhibitory_stdp_plasticity_parameters_struct
+ * (Spike/Plasticity/InhibitorySTDPPlasticity.hpp:38-46). */
+typedef struct {
+  float tau_istdp, learningrate, targetrate, momentumrate;
+} spk_vogels_stdp_params;
+
+/* Evans STDP parameters.
+ * Replaces evans_stdp_plasticity_parameters_struct
+ * (Spike/Plasticity/EvansSTDPPlasticity.hpp:33-40). */
+typedef struct {
+  float decay_term_tau_D, model_parameter_alpha_D,
+        synaptic_neurotransmitter_concentration_alpha_C, decay_term_tau_C,
+        learning_rate_rho;
+} spk_evans_stdp_params;
+
+/* Sizes fixed by spk_prepare. */
+typedef struct {
+  int n_neurons, n_inputs, n_labels, synapse_kind;
+  int timestep_grouping;     /* G: sub-steps per group (Spike's min axonal delay) */
+  int ring_slots;            /* B = max_delay + 2G + 1 (SpikingSynapses.cu:48) */
+  int history_words;         /* 32-bit words of spike history per neuron */
+  int frac_bits;             /* fixed-point fraction bits of the input ring */
+  int min_delay, max_delay;  /* in timesteps */
+  int64_t n_synapses;
+  int n_rules, n_monitors;
+} spk_info;
+
+/* Per-run timing of the last spk_run call, from HIP events on the network's stream. */
+typedef struct {
+  double total_ms;           /* whole run, first launch to last completion */
+  double kernel_ms;          /* summed duration of the dominant (fused) kernel */
+  int64_t kernel_launches;   /* launches of the dominant kernel */
+  int64_t groups;            /* timestep groups simulated */
+  int64_t spikes_neurons;    /* LIF spikes emitted */
+  int64_t spikes_inputs;     /* Poisson input spikes emitted */
+  int64_t synaptic_events;   /* deliveries: sum over spikes of out-degree */
+  int64_t external_events;   /* per-neuron external Poisson events */
+} spk_timing;
+
+/* ---------------------------------------------------------------------------- */
+/* library */
+
+/* Message of the last failed call on this thread. */
+const char* spk_last_error(void);
+/* ABI version (major*100+minor). */
+int spk_abi_version(void);
+/* Number of visible HIP devices; 0 when none. Does not fail. */
+int spk_device_count(void);
+
+/* ---------------------------------------------------------------------------- */
+/* network lifetime
+ * Replaces the allocation half of Backend::<Type>::prepare() of every backend type
+ * (Spike/Backend/Backend.hpp:15-25) with one network object per SpikingModel. */
+int spk_create(int n_neurons, int n_inputs, int synapse_kind, int n_labels,
+               float timestep, spk_network** out);
+int spk_destroy(spk_network* net);
+/* Use an existing HIP stream (hipStream_t passed as void*); NULL keeps the own one. */
+int spk_set_stream(spk_network* net, void* hip_stream);
+
+/* LIF neurons.  Replaces Backend::LIFSpikingNeurons::prepare
+ * (CUDA LIFSpikingNeurons.cu:26-66, SpikingNeurons.cu:38-42).
+ * decay_c[i] = dt/tau_m, resist[i] = decay_c[i]*R (the value the reference stores in
+ * membrane_resistances_R on the device); refractory period and background current are
+ * scalars as in Spike/Neurons/LIFSpikingNeurons.cpp:24,37. */
+int spk_set_lif(spk_network* net, const float* v0, const float* vreset,
+                const float* threshold, const float* decay_c, const float* resist,
+                float refractory_s, float background_current);
+
+/* Poisson inputs.  Replaces Backend::PoissonInputSpikingNeurons::prepare
+ * (CUDA PoissonInputSpikingNeurons.cu:36-47): rates[s*n_inputs + i] in Hz. */
+int spk_set_poisson_rates(spk_network* net, const float* rates, int n_stimuli);
+/* InputSpikingNeurons::select_stimulus (Spike/Neurons/InputSpikingNeurons.cpp:12-19). */
+int spk_select_stimulus(spk_network* net, int stimulus);
+/* Philox key of all random streams (replaces RandomStateManager, CUDA
+ * RandomStateManager.cu:14-50). */
+int spk_set_seed(spk_network* net, uint64_t seed);
+/* Per-neuron external Poisson drive (no reference counterpart: the Brunel network's
+ * external population nu_ext, generated in registers instead of through synapses).
+ * Each event adds `weight` to the neuron's input for that timestep. rate_hz=0 disables. */
+int spk_set_external_drive(spk_network* net, float rate_hz, float weight);
+
+/* Conductance synapse labels.  Replaces ConductanceSpikingSynapses::prepare
+ * (CUDA ConductanceSpikingSynapses.cu:19-87): decay_g[l] = expf(-dt/tau_g[l]). */
+int spk_set_labels(spk_network* net, const float* decay_g, const float* reversal);
+
+/* Synapses.  Replaces Backend::SpikingSynapses::prepare / Synapses::prepare
+ * (CUDA SpikingSynapses.cu:44-117, Synapses.cu:87-102).  delay in timesteps (>=1),
+ * label in [0, n_labels).  scale may be NULL (all 1). */
+int spk_set_synapses(spk_network* net, int64_t n, const int32_t* pre, const int32_t* post,
+                     const float* weight, const float* scale, const int32_t* delay,
+                     const int32_t* label);
+
+/* Plasticity rules over synapse indices (caller's order).  Replace
+ * Backend::STDPPlasticity::prepare (CUDA STDPPlasticity.cu:13-41) plus the rule's
+ * prepare (WeightDependentSTDPPlasticity.cu:28-43, InhibitorySTDPPlasticity.cu:196-214,
+ * EvansSTDPPlasticity.cu:30-37).  rule_id receives the index used by
+ * spk_update_plasticity. */
+int spk_add_wd_stdp(spk_network* net, const int64_t* synapses, int64_t n,
+                    const spk_wd_stdp_params* p, int* rule_id);
+int spk_add_vogels_stdp(spk_network* net, const int64_t* synapses, int64_t n,
+                        const spk_vogels_stdp_params* p, int* rule_id);
+int spk_add_evans_stdp(spk_network* net, const int64_t* synapses, int64_t n,
+                       const spk_evans_stdp_params* p, int* rule_id);
+
+/* Spike monitors.  Replaces Backend::SpikingActivityMonitor::prepare
+ * (CUDA SpikingActivityMonitor.cu:22-33): a bounded device store of `capacity`
+ * (id, time) records for one population. */
+int spk_add_monitor(spk_network* net, int population, int64_t capacity, int* monitor_id);
+
+/* Build device structures.  G = SpikingModel::timestep_grouping (SpikingModel.cpp:98).
+ * After this call neurons/synapses/rules/monitors are fixed. */
+int spk_prepare(spk_network* net, int timestep_grouping);
+/* Change G after prepare (VogelsAbbottNet.cpp:229-231 sets it to 1); must not exceed
+ * the value given to spk_prepare. */
+int spk_set_timestep_grouping(spk_network* net, int timestep_grouping);
+int spk_get_info(const spk_network* net, spk_info* info);
+
+/* reset_state of SpikingModel (SpikingModel.cpp:181-190): V=v0, last spike=-1000,
+ * spike history, input ring, conductances and plasticity traces cleared, Poisson
+ * inputs disarmed.  Weights are kept. */
+int spk_reset_state(spk_network* net);
+/* reset of the input population only (PoissonInputSpikingNeurons::select_stimulus). */
+int spk_reset_inputs(spk_network* net);
+
+/* ---------------------------------------------------------------------------- */
+/* per-phase updates: the backend state_update entry points called once per timestep
+ * group by SpikingModel::perform_per_step_model_instructions (SpikingModel.cpp:197-212) */
+
+/* Backend::SpikingNeurons::state_update for LIF (LIFSpikingNeurons.cu:72-195). */
+int spk_update_neurons(spk_network* net, float t);
+/* Backend::SpikingNeurons::state_update for Poisson (PoissonInputSpikingNeurons.cu:49-130). */
+int spk_update_inputs(spk_network* net, float t);
+/* apply_stdp_to_synapse_weights / update_synaptic_efficacies_or_weights of one rule
+ * (WeightDependentSTDPPlasticity.cu:45-163, InhibitorySTDPPlasticity.cu:50-155,
+ * EvansSTDPPlasticity.cu:40-143). */
+int spk_update_plasticity(spk_network* net, int rule_id, float t);
+/* Backend::SpikingSynapses::state_update (SpikingSynapses.cu:119-202). */
+int spk_update_synapses(spk_network* net, float t);
+/* Backend::SpikingActivityMonitor::collect_spikes_for_timestep
+ * (SpikingActivityMonitor.cu:52-98). */
+int spk_collect_spikes(spk_network* net, int monitor_id, float t);
+
+/* Fused run: the whole SpikingModel::run group loop (SpikingModel.cpp:214-236) on the
+ * device for n_groups groups starting at start_time.  Monitor records produced during
+ * the run are drained into the library's host-side store (spk_monitor_take). */
+int spk_run(spk_network* net, float start_time, int64_t n_groups, int flags);
+int spk_last_timing(const spk_network* net, spk_timing* timing);
+int spk_synchronize(spk_network* net);
+
+/* ---------------------------------------------------------------------------- */
+/* read-out */
+
+/* Records currently in the device store of a monitor
+ * (SpikingActivityMonitor.cu:35-39).  Overflow is reported as a count > capacity. */
+int spk_monitor_device_count(spk_network* net, int monitor_id, int64_t* count);
+/* Copy device records (SpikingActivityMonitor.cu:41-50) and clear the device store. */
+int spk_monitor_drain(spk_network* net, int monitor_id, int32_t* ids, float* times,
+                      int64_t cap, int64_t* n);
+/* Host-side records accumulated by spk_run since the last take; ids are local to the
+ * population, sorted by (time, id).  Pass ids=NULL to query the count only. */
+int spk_monitor_take(spk_network* net, int monitor_id, int32_t* ids, float* times,
+                     int64_t cap, int64_t* n);
+
+/* Synapses::copy_to_frontend / copy_to_backend (CUDA Synapses.cu:104-121). */
+int spk_get_weights(spk_network* net, float* weights);
+int spk_set_weights(spk_network* net, const float* weights);
+/* Neuron state for checks: membrane potentials (LIF) and last spike times. */
+int spk_get_membrane(spk_network* net, float* v);
+int spk_get_last_spike(spk_network* net, int population, float* last_spike);
+/* Per-neuron per-label conductances (conductance synapses only), [n_neurons*n_labels]. */
+int spk_get_conductances(spk_network* net, float* g);
+/* The input ring: ring[slot*(n_neurons*n_labels) + post*n_labels + label]. */
+int spk_get_ring(spk_network* net, int64_t* ring);
+/* Spike history bits, words[w*n_pop + i] for the population. */
+int spk_get_history(spk_network* net, int population, uint32_t* words);
+/* Plasticity traces of a rule in its synapse order (two arrays of n). */
+int spk_get_traces(spk_network* net, int rule_id, float* trace_a, float* trace_b);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPIKE_HIP_H */

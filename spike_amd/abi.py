@@ -1,0 +1,323 @@
+"""ctypes binding of the C ABI in include/spike_hip.h (libspike_hip.so).
+
+This module is plumbing for tests and the benchmark: the product's host interface is
+Spike's C++ front-end, whose HIP backend calls the same entry points.  Loading fails
+loudly when the shared library has not been built; there is no fallback path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+
+from .netdesc import NetDesc, RULE_WD, RULE_VOGELS, RULE_EVANS
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_lib", "libspike_hip.so")
+
+SPK_RUN_PLASTICITY = 1
+SPK_RUN_FORCE_LAUNCHES = 2
+
+
+class SpikeError(RuntimeError):
+    pass
+
+
+class WDParams(C.Structure):
+    _fields_ = [("a_minus", C.c_float), ("a_plus", C.c_float), ("tau_minus", C.c_float),
+                ("tau_plus", C.c_float), ("lambda_", C.c_float), ("alpha", C.c_float),
+                ("w_max", C.c_float), ("nearest_spike_only", C.c_int)]
+
+
+class VogelsParams(C.Structure):
+    _fields_ = [("tau_istdp", C.c_float), ("learningrate", C.c_float),
+                ("targetrate", C.c_float), ("momentumrate", C.c_float)]
+
+
+class EvansParams(C.Structure):
+    _fields_ = [("decay_term_tau_D", C.c_float), ("model_parameter_alpha_D", C.c_float),
+                ("synaptic_neurotransmitter_concentration_alpha_C", C.c_float),
+                ("decay_term_tau_C", C.c_float), ("learning_rate_rho", C.c_float)]
+
+
+class Info(C.Structure):
+    _fields_ = [("n_neurons", C.c_int), ("n_inputs", C.c_int), ("n_labels", C.c_int),
+                ("synapse_kind", C.c_int), ("timestep_grouping", C.c_int),
+                ("ring_slots", C.c_int), ("history_words", C.c_int), ("frac_bits", C.c_int),
+                ("min_delay", C.c_int), ("max_delay", C.c_int), ("n_synapses", C.c_int64),
+                ("n_rules", C.c_int), ("n_monitors", C.c_int)]
+
+
+class Timing(C.Structure):
+    _fields_ = [("total_ms", C.c_double), ("kernel_ms", C.c_double),
+                ("kernel_launches", C.c_int64), ("groups", C.c_int64),
+                ("spikes_neurons", C.c_int64), ("spikes_inputs", C.c_int64),
+                ("synaptic_events", C.c_int64), ("external_events", C.c_int64)]
+
+
+P = C.c_void_p
+fp = C.POINTER(C.c_float)
+ip = C.POINTER(C.c_int32)
+lp = C.POINTER(C.c_int64)
+up = C.POINTER(C.c_uint32)
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "spk_last_error": (C.c_char_p, []),
+    "spk_abi_version": (C.c_int, []),
+    "spk_device_count": (C.c_int, []),
+    "spk_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.POINTER(P)]),
+    "spk_destroy": (C.c_int, [P]),
+    "spk_set_stream": (C.c_int, [P, P]),
+    "spk_set_lif": (C.c_int, [P, fp, fp, fp, fp, fp, C.c_float, C.c_float]),
+    "spk_set_poisson_rates": (C.c_int, [P, fp, C.c_int]),
+    "spk_select_stimulus": (C.c_int, [P, C.c_int]),
+    "spk_set_seed": (C.c_int, [P, C.c_uint64]),
+    "spk_set_external_drive": (C.c_int, [P, C.c_float, C.c_float]),
+    "spk_set_labels": (C.c_int, [P, fp, fp]),
+    "spk_set_synapses": (C.c_int, [P, C.c_int64, ip, ip, fp, fp, ip, ip]),
+    "spk_add_wd_stdp": (C.c_int, [P, lp, C.c_int64, C.POINTER(WDParams), C.POINTER(C.c_int)]),
+    "spk_add_vogels_stdp": (C.c_int, [P, lp, C.c_int64, C.POINTER(VogelsParams), C.POINTER(C.c_int)]),
+    "spk_add_evans_stdp": (C.c_int, [P, lp, C.c_int64, C.POINTER(EvansParams), C.POINTER(C.c_int)]),
+    "spk_add_monitor": (C.c_int, [P, C.c_int, C.c_int64, C.POINTER(C.c_int)]),
+    "spk_prepare": (C.c_int, [P, C.c_int]),
+    "spk_set_timestep_grouping": (C.c_int, [P, C.c_int]),
+    "spk_get_info": (C.c_int, [P, C.POINTER(Info)]),
+    "spk_reset_state": (C.c_int, [P]),
+    "spk_reset_inputs": (C.c_int, [P]),
+    "spk_update_neurons": (C.c_int, [P, C.c_float]),
+    "spk_update_inputs": (C.c_int, [P, C.c_float]),
+    "spk_update_plasticity": (C.c_int, [P, C.c_int, C.c_float]),
+    "spk_update_synapses": (C.c_int, [P, C.c_float]),
+    "spk_collect_spikes": (C.c_int, [P, C.c_int, C.c_float]),
+    "spk_run": (C.c_int, [P, C.c_float, C.c_int64, C.c_int]),
+    "spk_last_timing": (C.c_int, [P, C.POINTER(Timing)]),
+    "spk_synchronize": (C.c_int, [P]),
+    "spk_monitor_device_count": (C.c_int, [P, C.c_int, lp]),
+    "spk_monitor_drain": (C.c_int, [P, C.c_int, ip, fp, C.c_int64, lp]),
+    "spk_monitor_take": (C.c_int, [P, C.c_int, ip, fp, C.c_int64, lp]),
+    "spk_get_weights": (C.c_int, [P, fp]),
+    "spk_set_weights": (C.c_int, [P, fp]),
+    "spk_get_membrane": (C.c_int, [P, fp]),
+    "spk_get_last_spike": (C.c_int, [P, C.c_int, fp]),
+    "spk_get_conductances": (C.c_int, [P, fp]),
+    "spk_get_ring": (C.c_int, [P, lp]),
+    "spk_get_history": (C.c_int, [P, C.c_int, up]),
+    "spk_get_traces": (C.c_int, [P, C.c_int, fp, fp]),
+}
+
+_lib: Optional[C.CDLL] = None
+
+
+def load(path: str = LIB_PATH) -> C.CDLL:
+    """Load libspike_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise SpikeError(f"{path} is missing: build it with __graft_entry__.build()")
+    lib = C.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _ptr(a: Optional[np.ndarray], ctype):
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+def check(rc: int):
+    if rc != 0:
+        msg = load().spk_last_error().decode(errors="replace")
+        raise SpikeError(f"spike_hip error {rc}: {msg}")
+
+
+class Network:
+    """A network on the GPU built from a NetDesc through the C ABI."""
+
+    def __init__(self, d: NetDesc, monitor_capacity: Optional[int] = None):
+        lib = load()
+        self.lib = lib
+        self.desc = d
+        h = P()
+        check(lib.spk_create(d.n_neurons, d.n_inputs, d.kind, d.n_labels, float(d.dt), C.byref(h)))
+        self.h = h
+        try:
+            self._build(d, monitor_capacity)
+        except Exception:
+            lib.spk_destroy(self.h)
+            self.h = None
+            raise
+
+    def _build(self, d: NetDesc, monitor_capacity):
+        lib, h = self.lib, self.h
+        f32 = lambda a: np.ascontiguousarray(a, dtype=np.float32)
+        i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32)
+        self._keep = []
+        if d.n_neurons:
+            arrs = [f32(x) for x in (d.v0, d.vreset, d.threshold, d.decay_c, d.resist)]
+            check(lib.spk_set_lif(h, *[_ptr(a, C.c_float) for a in arrs],
+                                  float(d.refractory), float(d.background)))
+        if d.n_inputs:
+            rates = f32(d.rates if d.rates is not None else np.zeros((1, d.n_inputs)))
+            rates = rates.reshape(-1, d.n_inputs)
+            check(lib.spk_set_poisson_rates(h, _ptr(rates, C.c_float), rates.shape[0]))
+        check(lib.spk_set_seed(h, int(d.seed)))
+        if d.ext_rate:
+            check(lib.spk_set_external_drive(h, float(d.ext_rate), float(d.ext_weight)))
+        if d.decay_g is not None:
+            dg, rv = f32(d.decay_g), f32(d.reversal)
+            check(lib.spk_set_labels(h, _ptr(dg, C.c_float), _ptr(rv, C.c_float)))
+        if d.n_synapses:
+            pre, post, dl = i32(d.pre), i32(d.post), i32(d.delay)
+            w = f32(d.weight)
+            sc = None if d.scale is None else f32(d.scale)
+            lb = None if d.label is None else i32(d.label)
+            check(lib.spk_set_synapses(h, d.n_synapses, _ptr(pre, C.c_int32), _ptr(post, C.c_int32),
+                                       _ptr(w, C.c_float), _ptr(sc, C.c_float), _ptr(dl, C.c_int32),
+                                       _ptr(lb, C.c_int32)))
+        self.rule_ids = []
+        for r in d.rules:
+            syn = np.ascontiguousarray(r.synapses, dtype=np.int64)
+            rid = C.c_int()
+            if r.kind == RULE_WD:
+                p = WDParams(**{("lambda_" if k == "lambda" else k): v for k, v in r.params.items()})
+                check(lib.spk_add_wd_stdp(h, _ptr(syn, C.c_int64), len(syn), C.byref(p), C.byref(rid)))
+            elif r.kind == RULE_VOGELS:
+                p = VogelsParams(**r.params)
+                check(lib.spk_add_vogels_stdp(h, _ptr(syn, C.c_int64), len(syn), C.byref(p), C.byref(rid)))
+            elif r.kind == RULE_EVANS:
+                p = EvansParams(**r.params)
+                check(lib.spk_add_evans_stdp(h, _ptr(syn, C.c_int64), len(syn), C.byref(p), C.byref(rid)))
+            self.rule_ids.append(rid.value)
+        self.monitor_ids = []
+        for pop in d.monitors:
+            npop = d.n_neurons if pop == 0 else d.n_inputs
+            cap = monitor_capacity or max(1, 60 * npop)
+            mid = C.c_int()
+            check(lib.spk_add_monitor(h, int(pop), int(cap), C.byref(mid)))
+            self.monitor_ids.append(mid.value)
+        check(lib.spk_prepare(h, d.timestep_grouping()))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.spk_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- control -------------------------------------------------------------------
+    def info(self) -> Info:
+        i = Info()
+        check(self.lib.spk_get_info(self.h, C.byref(i)))
+        return i
+
+    def reset_state(self):
+        check(self.lib.spk_reset_state(self.h))
+
+    def set_grouping(self, g: int):
+        check(self.lib.spk_set_timestep_grouping(self.h, int(g)))
+
+    def run(self, start: float, ngroups: int, plasticity: bool = True, force_launches=False) -> Timing:
+        flags = (SPK_RUN_PLASTICITY if plasticity else 0) | (SPK_RUN_FORCE_LAUNCHES if force_launches else 0)
+        check(self.lib.spk_run(self.h, float(start), int(ngroups), flags))
+        t = Timing()
+        check(self.lib.spk_last_timing(self.h, C.byref(t)))
+        return t
+
+    def step_phases(self, t: float, plasticity: bool = True):
+        """One group through the per-phase entry points, in SpikingModel order."""
+        lib, h = self.lib, self.h
+        check(lib.spk_update_neurons(h, float(t)))
+        check(lib.spk_update_inputs(h, float(t)))
+        if plasticity:
+            for r in self.rule_ids:
+                check(lib.spk_update_plasticity(h, r, float(t)))
+        check(lib.spk_update_synapses(h, float(t)))
+        for m in self.monitor_ids:
+            check(lib.spk_collect_spikes(h, m, float(t)))
+
+    # -- read-out ------------------------------------------------------------------
+    def membrane(self) -> np.ndarray:
+        v = np.empty(self.desc.n_neurons, np.float32)
+        check(self.lib.spk_get_membrane(self.h, _ptr(v, C.c_float)))
+        return v
+
+    def last_spike(self, pop=0) -> np.ndarray:
+        n = self.desc.n_neurons if pop == 0 else self.desc.n_inputs
+        v = np.empty(n, np.float32)
+        check(self.lib.spk_get_last_spike(self.h, pop, _ptr(v, C.c_float)))
+        return v
+
+    def weights(self) -> np.ndarray:
+        w = np.empty(self.desc.n_synapses, np.float32)
+        check(self.lib.spk_get_weights(self.h, _ptr(w, C.c_float)))
+        return w
+
+    def set_weights(self, w: np.ndarray):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        check(self.lib.spk_set_weights(self.h, _ptr(w, C.c_float)))
+
+    def conductances(self) -> np.ndarray:
+        g = np.empty(self.desc.n_neurons * self.desc.n_labels, np.float32)
+        check(self.lib.spk_get_conductances(self.h, _ptr(g, C.c_float)))
+        return g
+
+    def ring(self) -> np.ndarray:
+        i = self.info()
+        r = np.empty(i.ring_slots * self.desc.n_neurons * self.desc.n_labels, np.int64)
+        check(self.lib.spk_get_ring(self.h, _ptr(r, C.c_int64)))
+        return r
+
+    def history(self, pop=0) -> np.ndarray:
+        i = self.info()
+        n = self.desc.n_neurons if pop == 0 else self.desc.n_inputs
+        r = np.empty(i.history_words * n, np.uint32)
+        check(self.lib.spk_get_history(self.h, pop, _ptr(r, C.c_uint32)))
+        return r
+
+    def traces(self, rule: int):
+        n = len(self.desc.rules[rule].synapses)
+        a = np.empty(n, np.float32)
+        b = np.empty(n, np.float32)
+        check(self.lib.spk_get_traces(self.h, rule, _ptr(a, C.c_float), _ptr(b, C.c_float)))
+        return a, b
+
+    def take_spikes(self, monitor: int = 0):
+        """Records of a monitor: those spk_run drained to the host, then those still in
+        the device store (per-phase collection), each sorted by (time, id)."""
+        hi, ht = self._take_host(monitor)
+        c = C.c_int64()
+        check(self.lib.spk_monitor_device_count(self.h, monitor, C.byref(c)))
+        if c.value == 0:
+            return hi, ht
+        di, dt = self.drain_spikes(monitor, cap=int(c.value))
+        return np.concatenate([hi, di]), np.concatenate([ht, dt])
+
+    def _take_host(self, monitor: int = 0):
+        n = C.c_int64()
+        check(self.lib.spk_monitor_take(self.h, monitor, None, None, 0, C.byref(n)))
+        ids = np.empty(n.value, np.int32)
+        times = np.empty(n.value, np.float32)
+        check(self.lib.spk_monitor_take(self.h, monitor, _ptr(ids, C.c_int32), _ptr(times, C.c_float),
+                                        n.value, C.byref(n)))
+        return ids, times
+
+    def drain_spikes(self, monitor: int = 0, cap: int = 1 << 24):
+        ids = np.empty(cap, np.int32)
+        times = np.empty(cap, np.float32)
+        n = C.c_int64()
+        check(self.lib.spk_monitor_drain(self.h, monitor, _ptr(ids, C.c_int32), _ptr(times, C.c_float),
+                                         cap, C.byref(n)))
+        return ids[: n.value].copy(), times[: n.value].copy()

@@ -1,0 +1,478 @@
+// kernels.hpp -- per-phase gfx950 kernels of the Spike backend.
+//
+// One lane per neuron for the neuron phases, one 64-lane wavefront per spiking
+// presynaptic neuron for delivery.  State is structure-of-arrays FP32; the input ring is
+// int64 fixed point (exact, order-independent sums); spike lists are compacted with a
+// wavefront ballot + popcount prefix and one atomic per wavefront.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "philox.hpp"
+
+#pragma clang fp contract(off)
+
+namespace spk {
+
+constexpr int kWave = 64;
+constexpr int kMaxLabels = 8;
+constexpr int kMaxExtTable = 64;
+
+// ---------------------------------------------------------------------------------
+// spike history: bit (step mod 32W) of neuron i lives in word hist[(b/32)*npop + i]
+
+__device__ __forceinline__ int hist_bitpos(int64_t step, int W) {
+  const int64_t bits = (int64_t)W * 32;
+  int64_t b = step % bits;
+  if (b < 0) b += bits;
+  return (int)b;
+}
+
+__device__ __forceinline__ bool hist_get(const uint32_t* __restrict__ hist, int npop, int W,
+                                         int i, int64_t step) {
+  const int b = hist_bitpos(step, W);
+  return (hist[(size_t)(b >> 5) * npop + i] >> (b & 31)) & 1u;
+}
+
+// Append `id` to list (when `pred`) with one atomic per wavefront.  Every lane of the
+// wavefront must call this (the ballot spans the whole wavefront).
+__device__ __forceinline__ void wave_append(bool pred, uint32_t id, uint32_t* __restrict__ list,
+                                            int* __restrict__ count, int cap) {
+  const uint64_t m = __ballot(pred);
+  if (m == 0) return;
+  const int lane = (int)__lane_id();
+  const int leader = __ffsll((unsigned long long)m) - 1;
+  int base = 0;
+  if (lane == leader) base = atomicAdd(count, __popcll(m));
+  base = __shfl(base, leader);
+  if (pred) {
+    const int pos = base + __popcll(m & ((1ull << lane) - 1ull));
+    if (pos < cap) list[pos] = id;
+  }
+}
+
+// ---------------------------------------------------------------------------------
+// group bookkeeping: clear the G per-sub-step counts of a list slot, record its time
+
+__global__ void k_begin_group(int* __restrict__ counts, int G, float* __restrict__ slot_time,
+                              int* __restrict__ slot_tidx, int slot, float t, int tidx) {
+  const int i = threadIdx.x;
+  for (int g = i; g < G; g += blockDim.x) counts[g] = 0;
+  if (i == 0) {
+    slot_time[slot] = t;
+    slot_tidx[slot] = tidx;
+  }
+}
+
+// ---------------------------------------------------------------------------------
+// LIF neurons (LIFSpikingNeurons.cu:91-195; injection InlineDeviceFunctions.hpp:22-111)
+
+struct LifArgs {
+  float* V;
+  float* last;
+  uint32_t* hist;
+  int64_t* ring;
+  float* gcond;               // [N*L] conductance synapses only
+  const float* v0;
+  const float* vreset;
+  const float* thr;
+  const float* c;
+  const float* rp;
+  const float* decay_g;       // [L]
+  const float* erev;          // [L]
+  const uint32_t* ext_cdf;    // [ext_k]
+  uint32_t* lists;            // [G][cap]
+  int* counts;                // [G]
+  unsigned long long* ext_events;
+  int N, L, G, B, W, cap, ext_k, tidx;
+  float t, dt, tref, ibg, inv_scale;
+  int64_t q_ext;              // fixed-point value of one external event
+  uint64_t seed;
+};
+
+template <int KIND>
+__global__ __launch_bounds__(256) void k_lif(LifArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool active = i < a.N;
+  const int ii = active ? i : 0;
+  float V = a.V[ii];
+  float last = a.last[ii];
+  const float cc = a.c[ii];
+  const float cv0 = cc * a.v0[ii];
+  const float omc = 1.0f - cc;
+  const float cib = cc * a.ibg;
+  const float thr = a.thr[ii];
+  const float vr = a.vreset[ii];
+  const float rp = a.rp[ii];
+  const size_t NL = (size_t)a.N * a.L;
+  float gl[kMaxLabels];
+  if (KIND == 1) {
+#pragma unroll
+    for (int l = 0; l < kMaxLabels; ++l) gl[l] = (l < a.L) ? a.gcond[(size_t)ii * a.L + l] : 0.0f;
+  }
+  int cur_w = -1;
+  uint32_t word = 0;
+  unsigned long long ext_sum = 0;
+  const float half_dt = 0.5f * a.dt;
+  for (int g = 0; g < a.G; ++g) {
+    const int64_t step = (int64_t)a.tidx + g;
+    const int b = hist_bitpos(step, a.W);
+    const int w = b >> 5;
+    if (w != cur_w) {
+      if (cur_w >= 0 && active) a.hist[(size_t)cur_w * a.N + i] = word;
+      word = active ? a.hist[(size_t)w * a.N + i] : 0u;
+      cur_w = w;
+    }
+    word &= ~(1u << (b & 31));
+    // external per-neuron Poisson drive, joined to label 0 of this step's input
+    int64_t ext = 0;
+    if (a.ext_k > 0) {
+      const uint32_t u = draw_u32(a.seed, step, (uint32_t)ii, kStreamExternal);
+      int k = 0;
+      for (int j = 0; j < a.ext_k; ++j) k += (u >= a.ext_cdf[j]) ? 1 : 0;
+      ext = (int64_t)k * a.q_ext;
+      ext_sum += (unsigned long long)k;
+    }
+    const size_t slot = (size_t)((a.tidx + g) % a.B);
+    int64_t* rs = a.ring + slot * NL + (size_t)ii * a.L;
+    float dV;
+    if (KIND == 0) {
+      float total = 0.0f;
+      for (int l = 0; l < a.L; ++l) {
+        int64_t q = active ? rs[l] : 0;
+        if (l == 0) q += ext;
+        if (active) rs[l] = 0;
+        total += (float)q * a.inv_scale;
+      }
+      dV = total;
+    } else {
+      float total = 0.0f;
+#pragma unroll
+      for (int l = 0; l < kMaxLabels; ++l) {
+        if (l < a.L) {
+          int64_t q = active ? rs[l] : 0;
+          if (l == 0) q += ext;
+          if (active) rs[l] = 0;
+          const float in = (float)q * a.inv_scale;
+          gl[l] = a.decay_g[l] * gl[l] + in;
+          total += gl[l] * (a.erev[l] - V);
+        }
+      }
+      dV = total * rp;
+    }
+    const float tg = a.t + (float)g * a.dt;
+    bool spk = false;
+    if (((tg - last) - a.tref) > half_dt) {
+      V = ((cv0 + omc * V) + cib) + dV;
+      if (V >= thr) {
+        spk = true;
+        last = tg;
+        V = vr;
+      }
+    }
+    spk = spk && active;
+    if (spk) word |= (1u << (b & 31));
+    wave_append(spk, (uint32_t)i, a.lists + (size_t)g * a.cap, a.counts + g, a.cap);
+  }
+  if (active) {
+    if (cur_w >= 0) a.hist[(size_t)cur_w * a.N + i] = word;
+    a.V[i] = V;
+    a.last[i] = last;
+    if (KIND == 1) {
+      for (int l = 0; l < a.L; ++l) a.gcond[(size_t)i * a.L + l] = gl[l];
+    }
+  }
+  if (a.ext_k > 0) {
+    // one atomic per wavefront for the event counter
+    unsigned long long s = active ? ext_sum : 0ull;
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    if (__lane_id() == 0 && s) atomicAdd(a.ext_events, s);
+  }
+}
+
+// ---------------------------------------------------------------------------------
+// Poisson inputs (PoissonInputSpikingNeurons.cu:73-130), Bernoulli form
+
+struct PoissonArgs {
+  float* last;
+  uint32_t* hist;
+  const uint32_t* thr;   // [Nin] of the selected stimulus
+  uint32_t* lists;
+  int* counts;
+  int N, Nin, G, W, cap, tidx;
+  int64_t arm_step;
+  float t, dt;
+  uint64_t seed;
+};
+
+__global__ __launch_bounds__(256) void k_poisson(PoissonArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool active = i < a.Nin;
+  const int ii = active ? i : 0;
+  const uint32_t T = a.thr[ii];
+  float last = a.last[ii];
+  int cur_w = -1;
+  uint32_t word = 0;
+  for (int g = 0; g < a.G; ++g) {
+    const int64_t step = (int64_t)a.tidx + g;
+    const int b = hist_bitpos(step, a.W);
+    const int w = b >> 5;
+    if (w != cur_w) {
+      if (cur_w >= 0 && active) a.hist[(size_t)cur_w * a.Nin + i] = word;
+      word = active ? a.hist[(size_t)w * a.Nin + i] : 0u;
+      cur_w = w;
+    }
+    word &= ~(1u << (b & 31));
+    const uint32_t u = draw_u32(a.seed, step, (uint32_t)ii, kStreamInput);
+    const bool spk = active && (u < T) && (step != a.arm_step);
+    if (spk) {
+      word |= (1u << (b & 31));
+      last = a.t + (float)g * a.dt;
+    }
+    wave_append(spk, (uint32_t)(a.N + i), a.lists + (size_t)g * a.cap, a.counts + g, a.cap);
+  }
+  if (active) {
+    if (cur_w >= 0) a.hist[(size_t)cur_w * a.Nin + i] = word;
+    a.last[i] = last;
+  }
+}
+
+// ---------------------------------------------------------------------------------
+// Delivery (SpikingSynapses.cu:168-202): one wavefront per spike, CSR rows sorted by
+// delay, int64 fixed-point atomics into the ring.
+
+struct PropArgs {
+  const uint32_t* lists;
+  const int* counts;
+  const int64_t* rowptr;
+  const int32_t* tgt;
+  const float* w;
+  const float* scale;       // nullptr: uniform scale0
+  const uint16_t* delay;
+  const uint8_t* label;     // nullptr: single label
+  int64_t* ring;
+  unsigned long long* events;
+  float scale0, two_k;
+  int N, L, G, B, cap, tidx;
+};
+
+__global__ __launch_bounds__(256) void k_propagate(PropArgs a) {
+  const int lane = (int)__lane_id();
+  const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) / kWave);
+  const int nwaves = (int)((gridDim.x * blockDim.x) / kWave);
+  const size_t NL = (size_t)a.N * a.L;
+  const int base_slot = a.tidx % a.B;
+  unsigned long long ev = 0;
+  for (int g = 0; g < a.G; ++g) {
+    int n = a.counts[g];
+    if (n > a.cap) n = a.cap;
+    const uint32_t* list = a.lists + (size_t)g * a.cap;
+    for (int j = wave; j < n; j += nwaves) {
+      const uint32_t u = list[j];
+      const int64_t s0 = a.rowptr[u], s1 = a.rowptr[u + 1];
+      if (lane == 0) ev += (unsigned long long)(s1 - s0);
+      for (int64_t s = s0 + lane; s < s1; s += kWave) {
+        const int post = a.tgt[s];
+        const int d = a.delay[s];
+        const float sc = a.scale ? a.scale[s] : a.scale0;
+        const float wv = a.w[s] * sc;
+        const long long q = __float2ll_rn(wv * a.two_k);
+        const int slot = (base_slot + d + g) % a.B;
+        const int l = a.label ? (int)a.label[s] : 0;
+        atomicAdd((unsigned long long*)(a.ring + (size_t)slot * NL + (size_t)post * a.L + l),
+                  (unsigned long long)q);
+      }
+    }
+  }
+  if (lane == 0 && ev) atomicAdd(a.events, ev);
+}
+
+// ---------------------------------------------------------------------------------
+// Plasticity rules, one lane per plastic synapse (rule's own order)
+
+struct StdpArgs {
+  const int64_t* pos;      // CSR position of each plastic synapse
+  const int32_t* pre;      // unified presynaptic id (inputs at N + i)
+  const int32_t* tgt;
+  const uint16_t* delay;
+  float* w;
+  float* ta;
+  float* tb;
+  const float* last_n;
+  const float* last_in;
+  const uint32_t* hist_n;
+  const uint32_t* hist_in;
+  const float* tab;        // rule constants / tables
+  double dterm;
+  int64_t P;
+  int N, Nin, G, W, tidx, nearest;
+  float t, dt;
+};
+
+// WeightDependentSTDPPlasticity.cu:67-163. tab = {am, ap, lambda, alpha, wmax,
+// Em[0..G], Ep[0..G], iEm[0..G], iEp[0..G]}
+__global__ __launch_bounds__(256) void k_stdp_wd(StdpArgs a) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= a.P) return;
+  const int64_t s = a.pos[j];
+  const int u = a.pre[j];
+  const int post = a.tgt[s];
+  const float am = a.tab[0], ap = a.tab[1], lam = a.tab[2], alp = a.tab[3], wmax = a.tab[4];
+  const float* Em = a.tab + 5;
+  const float* Ep = Em + (a.G + 1);
+  const float* iEm = Ep + (a.G + 1);
+  const float* iEp = iEm + (a.G + 1);
+  const int G = a.G;
+  float xpre = a.ta[j], xpost = a.tb[j];
+  const float lpre = (u < a.N) ? a.last_n[u] : a.last_in[u - a.N];
+  int pg = (int)roundf((lpre - a.t) / a.dt);
+  const int qg = (int)roundf((a.last_n[post] - a.t) / a.dt);
+  if (pg >= G) pg *= -1;
+  const float w = a.w[s];
+  xpost *= Em[G];
+  xpre *= Ep[G];
+  if (a.nearest) {
+    if (qg >= 0) xpost = am * Em[G - qg];
+    if (pg >= 0) xpre = ap * Ep[G - pg];
+  } else {
+    xpost += (qg >= 0) ? am * Em[G - qg] : 0.0f;
+    xpre += (pg >= 0) ? ap * Ep[G - pg] : 0.0f;
+  }
+  float upd = 0.0f;
+  if (pg >= 0) {
+    float tmp = xpost;
+    tmp += (qg > pg) ? -am * Em[G - qg] : 0.0f;
+    tmp *= iEm[G - pg];
+    upd -= ((lam * alp) * w) * tmp;
+  }
+  if (qg >= 0) {
+    float tmp = xpre;
+    tmp += (pg > qg) ? -ap * Ep[G - pg] : 0.0f;
+    tmp *= iEp[G - qg];
+    upd += (lam * (wmax - w)) * tmp;
+  }
+  float nw = w + upd;
+  if (nw < 0.0f) nw = 0.0f;
+  a.w[s] = nw;
+  a.ta[j] = xpre;
+  a.tb[j] = xpost;
+}
+
+__device__ __forceinline__ bool pre_bit(const StdpArgs& a, int u, int64_t step) {
+  return (u < a.N) ? hist_get(a.hist_n, a.N, a.W, u, step)
+                   : hist_get(a.hist_in, a.Nin, a.W, u - a.N, step);
+}
+
+// InhibitorySTDPPlasticity.cu:238-321. tab = {decay, lr}; dterm = lr*(2*rate*tau) (double)
+__global__ __launch_bounds__(256) void k_stdp_vogels(StdpArgs a) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= a.P) return;
+  const int64_t s = a.pos[j];
+  const int u = a.pre[j];
+  const int post = a.tgt[s];
+  const int d = a.delay[s];
+  const float decay = a.tab[0], lr = a.tab[1];
+  float xpre = a.ta[j], xpost = a.tb[j];
+  float w = a.w[s];
+  for (int g = 0; g < a.G; ++g) {
+    xpre *= decay;
+    xpost *= decay;
+    const int64_t step = (int64_t)a.tidx + g;
+    const bool pr = pre_bit(a, u, step - d);
+    const bool po = hist_get(a.hist_n, a.N, a.W, post, step);
+    if (pr) xpre += 1.0f;
+    if (po) xpost += 1.0f;
+    float upd = 0.0f;
+    if (pr) {
+      upd += lr * xpost;
+      upd = (float)((double)upd + (-a.dterm));
+    }
+    if (po) upd += lr * xpre;
+    w += upd;
+    if (w < 0.0f) w = 0.0f;
+  }
+  a.w[s] = w;
+  a.ta[j] = xpre;
+  a.tb[j] = xpost;
+}
+
+// EvansSTDPPlasticity.cu:59-143. tab = {dC, dD, aC, aD}
+__global__ __launch_bounds__(256) void k_stdp_evans(StdpArgs a) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= a.P) return;
+  const int64_t s = a.pos[j];
+  const int u = a.pre[j];
+  const int post = a.tgt[s];
+  const int d = a.delay[s];
+  const float dC = a.tab[0], dD = a.tab[1], aC = a.tab[2], aD = a.tab[3];
+  float C = a.ta[j], D = a.tb[j];
+  float w = a.w[s];
+  for (int g = 0; g < a.G; ++g) {
+    C = dC * C;
+    D = dD * D;
+    const int64_t step = (int64_t)a.tidx + g;
+    const bool pr = pre_bit(a, u, step - d);
+    const bool po = hist_get(a.hist_n, a.N, a.W, post, step);
+    if (pr) C += aC * (1.0f - C);
+    if (po) D += aD * (1.0f - D);
+    float upd = 0.0f;
+    const float wo = w;
+    if (pr) upd -= wo * D;
+    if (po) upd += (1.0f - wo) * C;
+    w = wo + upd;
+    if (w < 0.0f) w = 0.0f;
+    if (w > 1.0f) w = 1.0f;
+  }
+  a.w[s] = w;
+  a.ta[j] = C;
+  a.tb[j] = D;
+}
+
+// ---------------------------------------------------------------------------------
+// Monitor collection (SpikingActivityMonitor.cu:69-98): copy the spikes of one
+// population from list slots [slot0, slot0+nslots) into a bounded store.
+
+struct CollectArgs {
+  const uint32_t* lists;      // [slots][G][cap]
+  const int* counts;          // [slots][G]
+  const float* slot_time;
+  int32_t* ids;
+  float* times;
+  unsigned long long* count;
+  int64_t store_cap;
+  int slot0, nslots, G, Gs, cap, lo, hi;   // G active lists of Gs per slot; population [lo, hi)
+  float dt;
+};
+
+__global__ __launch_bounds__(256) void k_collect(CollectArgs a) {
+  // blockIdx.y = slot offset * G + g; blockIdx.x strides over the list entries
+  const int sg = blockIdx.y;
+  const int s = a.slot0 + sg / a.G;
+  const int g = sg % a.G;
+  int n = a.counts[(size_t)s * a.Gs + g];
+  if (n > a.cap) n = a.cap;
+  const uint32_t* list = a.lists + ((size_t)s * a.Gs + g) * a.cap;
+  const float tg = a.slot_time[s] + (float)g * a.dt;
+  const int nthreads = gridDim.x * blockDim.x;
+  const int n_round = ((n + kWave - 1) / kWave) * kWave;
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n_round; j += nthreads) {
+    const bool in_range = j < n;
+    const uint32_t u = in_range ? list[j] : 0u;
+    const bool pred = in_range && (int)u >= a.lo && (int)u < a.hi;
+    const uint64_t m = __ballot(pred);
+    if (m == 0) continue;
+    const int lane = (int)__lane_id();
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    unsigned long long base = 0;
+    if (lane == leader) base = atomicAdd(a.count, (unsigned long long)__popcll(m));
+    base = __shfl(base, leader);
+    if (pred) {
+      const int64_t pos = (int64_t)base + __popcll(m & ((1ull << lane) - 1ull));
+      if (pos < a.store_cap) {
+        a.ids[pos] = (int32_t)u - a.lo;
+        a.times[pos] = tg;
+      }
+    }
+  }
+}
+
+}  // namespace spk

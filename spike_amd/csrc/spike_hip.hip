@@ -1,0 +1,1035 @@
+// spike_hip.hip -- C ABI of the MI355X Spike backend (declared in include/spike_hip.h).
+//
+// A network object holds everything one SpikingModel needs on one GPU: SoA neuron state,
+// the int64 fixed-point input ring, the delay-sorted CSR of outgoing synapses, per-group
+// spike lists, plasticity traces and monitor stores.  Host code here only validates,
+// lays out and uploads data, and sequences kernels on the network's stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/spike_hip.h"
+#include "kernels.hpp"
+
+#pragma clang fp contract(off)
+
+namespace {
+
+thread_local std::string g_error;
+
+int fail(int code, const std::string& msg) {
+  g_error = msg;
+  return code;
+}
+
+#define SPK_HIP(expr)                                                                   \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      return fail(SPK_ERR_HIP, std::string(#expr) + " failed: " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <typename T>
+int dalloc(T** p, size_t n) {
+  *p = nullptr;
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc((void**)p, n * sizeof(T));
+  if (e != hipSuccess) return fail(SPK_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  return SPK_OK;
+}
+
+template <typename T>
+void dfree(T*& p) {
+  if (p) (void)hipFree((void*)p);
+  p = nullptr;
+}
+
+// Bernoulli threshold of a Poisson input with rate r: floor((1 - exp(-r*dt)) * 2^32).
+uint32_t bernoulli_threshold(float rate, float dt) {
+  double p = -std::expm1(-(double)rate * (double)dt);
+  if (!(p > 0.0)) return 0u;
+  double t = std::floor(p * 4294967296.0);
+  if (t >= 4294967295.0) return 0xFFFFFFFFu;
+  return (uint32_t)t;
+}
+
+// Inverse-CDF thresholds of a Poisson(lambda) count, C[k] = floor(F(k) 2^32), until the
+// tail is below 2^-34.
+std::vector<uint32_t> poisson_count_table(double lambda) {
+  std::vector<uint32_t> c;
+  if (!(lambda > 0.0)) return c;
+  double pmf = std::exp(-lambda), cdf = pmf;
+  for (int k = 0; k < 255; ++k) {
+    double t = std::floor(cdf * 4294967296.0);
+    c.push_back(t >= 4294967295.0 ? 0xFFFFFFFFu : (uint32_t)t);
+    if (1.0 - cdf < 5.820766091346741e-11) break;
+    pmf = pmf * lambda / (double)(k + 1);
+    cdf += pmf;
+  }
+  return c;
+}
+
+// Fraction bits of the fixed-point ring: the largest possible per-step input of one
+// (neuron, label) entry stays below 2^52, leaving 11 bits of headroom in an int64.
+int choose_frac_bits(double max_entry) {
+  if (!(max_entry > 0.0)) return 32;
+  int e = (int)std::ceil(std::log2(max_entry));
+  int k = 52 - e;
+  return std::max(0, std::min(120, k));
+}
+
+struct RuleHost {
+  int kind = 0;  // 0 WD, 1 Vogels, 2 Evans
+  std::vector<int64_t> syn;
+  spk_wd_stdp_params wd{};
+  spk_vogels_stdp_params vg{};
+  spk_evans_stdp_params ev{};
+  // device
+  int64_t* d_pos = nullptr;
+  int32_t* d_pre = nullptr;
+  float* d_ta = nullptr;
+  float* d_tb = nullptr;
+  float* d_tab = nullptr;
+  int tab_G = -1;
+  double dterm = 0.0;
+};
+
+struct MonitorHost {
+  int pop = 0;
+  int64_t capacity = 0;
+  int32_t* d_ids = nullptr;
+  float* d_times = nullptr;
+  unsigned long long* d_count = nullptr;
+  std::vector<int32_t> h_ids;  // drained by spk_run
+  std::vector<float> h_times;
+};
+
+}  // namespace
+
+struct spk_network {
+  int N = 0, Nin = 0, kind = 0, L = 1;
+  float dt = 1e-4f;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  int device = 0;
+  // host parameters
+  std::vector<float> v0, vreset, thr, c, rp;
+  float tref = 0.002f, ibg = 0.0f;
+  bool lif_set = false;
+  std::vector<float> rates;
+  int n_stim = 0, stim = 0;
+  uint64_t seed = 42;
+  float ext_rate = 0.0f, ext_w = 0.0f;
+  std::vector<float> decay_g, erev;
+  std::vector<int32_t> h_pre, h_post, h_delay, h_label;
+  std::vector<float> h_w, h_scale;
+  bool syn_set = false;
+  std::vector<RuleHost> rules;
+  std::vector<MonitorHost> mons;
+  // derived sizes
+  int G = 1, Gprep = 1, B = 1, W = 1, frac = 32, dmin = 0, dmax = 0;
+  int64_t S = 0;
+  bool prepared = false;
+  bool uniform_scale = true;
+  float scale0 = 1.0f;
+  std::vector<uint32_t> ext_cdf;
+  int64_t q_ext = 0;
+  int64_t arm_step = -1;
+  bool pending_arm = true;
+  std::vector<int64_t> perm;  // caller synapse index -> CSR position
+  // device state
+  float *d_V = nullptr, *d_v0 = nullptr, *d_vreset = nullptr, *d_thr = nullptr, *d_c = nullptr,
+        *d_rp = nullptr;
+  float *d_last_n = nullptr, *d_last_in = nullptr;
+  uint32_t *d_hist_n = nullptr, *d_hist_in = nullptr;
+  int64_t* d_ring = nullptr;
+  float* d_g = nullptr;
+  float *d_decay = nullptr, *d_erev = nullptr;
+  uint32_t* d_in_thr = nullptr;
+  uint32_t* d_ext_cdf = nullptr;
+  int64_t* d_rowptr = nullptr;
+  int32_t* d_tgt = nullptr;
+  float* d_w = nullptr;
+  float* d_scale = nullptr;
+  uint16_t* d_delay = nullptr;
+  uint8_t* d_label = nullptr;
+  // spike lists: [slots][G][cap], counts [slots][G]
+  int slots = 1, cap = 1;
+  uint32_t* d_lists = nullptr;
+  int* d_counts = nullptr;
+  float* d_slot_time = nullptr;
+  int* d_slot_tidx = nullptr;
+  unsigned long long* d_stats = nullptr;  // [0] events, [1] external events
+  // per-phase bookkeeping
+  int cur_tidx = -1;
+  int cur_slot = 0;
+  // timing
+  hipEvent_t ev_a = nullptr, ev_b = nullptr;
+  std::vector<hipEvent_t> ev_kernel;
+  spk_timing timing{};
+
+  int nu() const { return N + Nin; }
+};
+
+namespace {
+
+int check_net(const spk_network* n) {
+  if (!n) return fail(SPK_ERR_ARG, "null network");
+  return SPK_OK;
+}
+
+int check_prepared(const spk_network* n) {
+  if (!n) return fail(SPK_ERR_ARG, "null network");
+  if (!n->prepared) return fail(SPK_ERR_STATE, "network not prepared (call spk_prepare)");
+  return SPK_OK;
+}
+
+void free_device(spk_network* n) {
+  dfree(n->d_V); dfree(n->d_v0); dfree(n->d_vreset); dfree(n->d_thr); dfree(n->d_c);
+  dfree(n->d_rp); dfree(n->d_last_n); dfree(n->d_last_in); dfree(n->d_hist_n);
+  dfree(n->d_hist_in); dfree(n->d_ring); dfree(n->d_g); dfree(n->d_decay); dfree(n->d_erev);
+  dfree(n->d_in_thr); dfree(n->d_ext_cdf); dfree(n->d_rowptr); dfree(n->d_tgt); dfree(n->d_w);
+  dfree(n->d_scale); dfree(n->d_delay); dfree(n->d_label); dfree(n->d_lists);
+  dfree(n->d_counts); dfree(n->d_slot_time); dfree(n->d_slot_tidx); dfree(n->d_stats);
+  for (auto& r : n->rules) {
+    dfree(r.d_pos); dfree(r.d_pre); dfree(r.d_ta); dfree(r.d_tb); dfree(r.d_tab);
+  }
+  for (auto& m : n->mons) {
+    dfree(m.d_ids); dfree(m.d_times); dfree(m.d_count);
+  }
+}
+
+template <typename T>
+int upload(T* d, const T* h, size_t n, hipStream_t s) {
+  if (n == 0) return SPK_OK;
+  SPK_HIP(hipMemcpyAsync(d, h, n * sizeof(T), hipMemcpyHostToDevice, s));
+  return SPK_OK;
+}
+
+int grid_for(int64_t n, int block) { return (int)std::max<int64_t>(1, (n + block - 1) / block); }
+
+// Per-rule device constants that depend on G.
+int refresh_rule_tables(spk_network* n, RuleHost& r) {
+  if (r.tab_G == n->G && r.d_tab) return SPK_OK;
+  std::vector<float> tab;
+  const float dt = n->dt;
+  const int G = n->G;
+  if (r.kind == 0) {
+    // host-side float expressions of WeightDependentSTDPPlasticity.cu:125-145
+    tab = {r.wd.a_minus, r.wd.a_plus, r.wd.lambda, r.wd.alpha, r.wd.w_max};
+    std::vector<float> Em(G + 1), Ep(G + 1);
+    for (int k = 0; k <= G; ++k) {
+      Em[k] = expf(-((float)k * dt) / r.wd.tau_minus);
+      Ep[k] = expf(-((float)k * dt) / r.wd.tau_plus);
+    }
+    for (int k = 0; k <= G; ++k) tab.push_back(Em[k]);
+    for (int k = 0; k <= G; ++k) tab.push_back(Ep[k]);
+    for (int k = 0; k <= G; ++k) tab.push_back(1.0f / Em[k]);
+    for (int k = 0; k <= G; ++k) tab.push_back(1.0f / Ep[k]);
+  } else if (r.kind == 1) {
+    tab = {expf(-dt / r.vg.tau_istdp), r.vg.learningrate};
+    r.dterm = (double)r.vg.learningrate * (2.0 * (double)r.vg.targetrate * (double)r.vg.tau_istdp);
+  } else {
+    tab = {1.0f - (dt / r.ev.decay_term_tau_C), 1.0f - (dt / r.ev.decay_term_tau_D),
+           dt * r.ev.synaptic_neurotransmitter_concentration_alpha_C,
+           dt * r.ev.model_parameter_alpha_D};
+  }
+  dfree(r.d_tab);
+  int rc = dalloc(&r.d_tab, tab.size());
+  if (rc) return rc;
+  SPK_HIP(hipMemcpy(r.d_tab, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice));
+  r.tab_G = n->G;
+  return SPK_OK;
+}
+
+int begin_group(spk_network* n, float t, bool force = false) {
+  const int tidx = (int)roundf(t / n->dt);
+  if (tidx == n->cur_tidx && !force) return SPK_OK;
+  n->cur_tidx = tidx;
+  n->cur_slot = (tidx / n->G) % n->slots;
+  hipLaunchKernelGGL(spk::k_begin_group, dim3(1), dim3(64), 0, n->stream,
+                     n->d_counts + (size_t)n->cur_slot * n->Gprep, n->G, n->d_slot_time,
+                     n->d_slot_tidx, n->cur_slot, t, tidx);
+  SPK_HIP(hipGetLastError());
+  return SPK_OK;
+}
+
+uint32_t* slot_lists(spk_network* n, int slot) {
+  return n->d_lists + (size_t)slot * n->Gprep * n->cap;
+}
+int* slot_counts(spk_network* n, int slot) { return n->d_counts + (size_t)slot * n->Gprep; }
+
+int launch_neurons(spk_network* n, float t, int slot) {
+  if (n->N == 0) return SPK_OK;
+  spk::LifArgs a{};
+  a.V = n->d_V; a.last = n->d_last_n; a.hist = n->d_hist_n; a.ring = n->d_ring; a.gcond = n->d_g;
+  a.v0 = n->d_v0; a.vreset = n->d_vreset; a.thr = n->d_thr; a.c = n->d_c; a.rp = n->d_rp;
+  a.decay_g = n->d_decay; a.erev = n->d_erev; a.ext_cdf = n->d_ext_cdf;
+  a.lists = slot_lists(n, slot); a.counts = slot_counts(n, slot);
+  a.ext_events = n->d_stats + 1;
+  a.N = n->N; a.L = n->L; a.G = n->G; a.B = n->B; a.W = n->W; a.cap = n->cap;
+  a.ext_k = (int)n->ext_cdf.size(); a.tidx = (int)roundf(t / n->dt);
+  a.t = t; a.dt = n->dt; a.tref = n->tref; a.ibg = n->ibg;
+  a.inv_scale = ldexpf(1.0f, -n->frac);
+  a.q_ext = n->q_ext; a.seed = n->seed;
+  const int grid = grid_for(n->N, 256);
+  if (n->kind == SPK_SYN_VOLTAGE)
+    hipLaunchKernelGGL(spk::k_lif<0>, dim3(grid), dim3(256), 0, n->stream, a);
+  else
+    hipLaunchKernelGGL(spk::k_lif<1>, dim3(grid), dim3(256), 0, n->stream, a);
+  SPK_HIP(hipGetLastError());
+  return SPK_OK;
+}
+
+int launch_inputs(spk_network* n, float t, int slot) {
+  if (n->Nin == 0) return SPK_OK;
+  const int tidx = (int)roundf(t / n->dt);
+  if (n->pending_arm) {
+    n->arm_step = tidx;
+    n->pending_arm = false;
+  }
+  spk::PoissonArgs a{};
+  a.last = n->d_last_in; a.hist = n->d_hist_in;
+  a.thr = n->d_in_thr + (size_t)n->stim * n->Nin;
+  a.lists = slot_lists(n, slot); a.counts = slot_counts(n, slot);
+  a.N = n->N; a.Nin = n->Nin; a.G = n->G; a.W = n->W; a.cap = n->cap; a.tidx = tidx;
+  a.arm_step = n->arm_step; a.t = t; a.dt = n->dt; a.seed = n->seed;
+  hipLaunchKernelGGL(spk::k_poisson, dim3(grid_for(n->Nin, 256)), dim3(256), 0, n->stream, a);
+  SPK_HIP(hipGetLastError());
+  return SPK_OK;
+}
+
+int launch_rule(spk_network* n, int rid, float t) {
+  RuleHost& r = n->rules[rid];
+  if (r.syn.empty()) return SPK_OK;
+  int rc = refresh_rule_tables(n, r);
+  if (rc) return rc;
+  spk::StdpArgs a{};
+  a.pos = r.d_pos; a.pre = r.d_pre; a.tgt = n->d_tgt; a.delay = n->d_delay; a.w = n->d_w;
+  a.ta = r.d_ta; a.tb = r.d_tb; a.last_n = n->d_last_n; a.last_in = n->d_last_in;
+  a.hist_n = n->d_hist_n; a.hist_in = n->d_hist_in; a.tab = r.d_tab; a.dterm = r.dterm;
+  a.P = (int64_t)r.syn.size(); a.N = n->N; a.Nin = n->Nin; a.G = n->G; a.W = n->W;
+  a.tidx = (int)roundf(t / n->dt); a.nearest = r.wd.nearest_spike_only; a.t = t; a.dt = n->dt;
+  const int grid = grid_for(a.P, 256);
+  if (r.kind == 0) hipLaunchKernelGGL(spk::k_stdp_wd, dim3(grid), dim3(256), 0, n->stream, a);
+  else if (r.kind == 1) hipLaunchKernelGGL(spk::k_stdp_vogels, dim3(grid), dim3(256), 0, n->stream, a);
+  else hipLaunchKernelGGL(spk::k_stdp_evans, dim3(grid), dim3(256), 0, n->stream, a);
+  SPK_HIP(hipGetLastError());
+  return SPK_OK;
+}
+
+int launch_propagate(spk_network* n, float t, int slot) {
+  if (n->S == 0) return SPK_OK;
+  spk::PropArgs a{};
+  a.lists = slot_lists(n, slot); a.counts = slot_counts(n, slot);
+  a.rowptr = n->d_rowptr; a.tgt = n->d_tgt; a.w = n->d_w;
+  a.scale = n->uniform_scale ? nullptr : n->d_scale;
+  a.delay = n->d_delay; a.label = (n->L > 1) ? n->d_label : nullptr;
+  a.ring = n->d_ring; a.events = n->d_stats;
+  a.scale0 = n->scale0; a.two_k = ldexpf(1.0f, n->frac);
+  a.N = n->N; a.L = n->L; a.G = n->G; a.B = n->B; a.cap = n->cap;
+  a.tidx = (int)roundf(t / n->dt);
+  hipLaunchKernelGGL(spk::k_propagate, dim3(1024), dim3(256), 0, n->stream, a);
+  SPK_HIP(hipGetLastError());
+  return SPK_OK;
+}
+
+int launch_collect(spk_network* n, int mid, int slot0, int nslots) {
+  MonitorHost& m = n->mons[mid];
+  spk::CollectArgs a{};
+  a.lists = n->d_lists; a.counts = n->d_counts; a.slot_time = n->d_slot_time;
+  a.ids = m.d_ids; a.times = m.d_times; a.count = m.d_count; a.store_cap = m.capacity;
+  a.slot0 = slot0; a.nslots = nslots; a.G = n->Gprep; a.cap = n->cap;
+  a.lo = m.pop == SPK_POP_NEURONS ? 0 : n->N;
+  a.hi = m.pop == SPK_POP_NEURONS ? n->N : n->nu();
+  a.dt = n->dt;
+  a.G = n->G;
+  a.Gs = n->Gprep;
+  hipLaunchKernelGGL(spk::k_collect, dim3(8, nslots * n->G), dim3(256), 0, n->stream, a);
+  SPK_HIP(hipGetLastError());
+  return SPK_OK;
+}
+
+// Drain a monitor's device store into its host vectors, sorted by (time, id).
+int drain_monitor(spk_network* n, MonitorHost& m, std::vector<int32_t>& ids,
+                  std::vector<float>& times) {
+  unsigned long long cnt = 0;
+  SPK_HIP(hipMemcpyAsync(&cnt, m.d_count, sizeof(cnt), hipMemcpyDeviceToHost, n->stream));
+  SPK_HIP(hipStreamSynchronize(n->stream));
+  if ((int64_t)cnt > m.capacity)
+    return fail(SPK_ERR_CAPACITY, "spike monitor device store overflowed (" +
+                                      std::to_string(cnt) + " > " + std::to_string(m.capacity) + ")");
+  std::vector<int32_t> bi(cnt);
+  std::vector<float> bt(cnt);
+  if (cnt) {
+    SPK_HIP(hipMemcpyAsync(bi.data(), m.d_ids, cnt * sizeof(int32_t), hipMemcpyDeviceToHost, n->stream));
+    SPK_HIP(hipMemcpyAsync(bt.data(), m.d_times, cnt * sizeof(float), hipMemcpyDeviceToHost, n->stream));
+  }
+  SPK_HIP(hipMemsetAsync(m.d_count, 0, sizeof(unsigned long long), n->stream));
+  SPK_HIP(hipStreamSynchronize(n->stream));
+  std::vector<size_t> order(cnt);
+  for (size_t i = 0; i < cnt; ++i) order[i] = i;
+  std::sort(order.begin(), order.end(), [&](size_t a, size_t b) {
+    if (bt[a] != bt[b]) return bt[a] < bt[b];
+    return bi[a] < bi[b];
+  });
+  for (size_t i : order) {
+    ids.push_back(bi[i]);
+    times.push_back(bt[i]);
+  }
+  return SPK_OK;
+}
+
+int ensure_monitor_capacity(spk_network* n, MonitorHost& m, int64_t need) {
+  if (m.capacity >= need && m.d_ids) return SPK_OK;
+  // keep what is on the device
+  std::vector<int32_t> ids;
+  std::vector<float> times;
+  if (m.d_ids) {
+    int rc = drain_monitor(n, m, m.h_ids, m.h_times);
+    if (rc) return rc;
+  }
+  dfree(m.d_ids);
+  dfree(m.d_times);
+  m.capacity = need;
+  int rc = dalloc(&m.d_ids, (size_t)need);
+  if (rc) return rc;
+  rc = dalloc(&m.d_times, (size_t)need);
+  if (rc) return rc;
+  if (!m.d_count) {
+    rc = dalloc(&m.d_count, 1);
+    if (rc) return rc;
+  }
+  SPK_HIP(hipMemsetAsync(m.d_count, 0, sizeof(unsigned long long), n->stream));
+  return SPK_OK;
+}
+
+template <typename T>
+int download(spk_network* n, T* out, const T* d, size_t cnt) {
+  if (cnt == 0) return SPK_OK;
+  if (!out) return fail(SPK_ERR_ARG, "null output");
+  SPK_HIP(hipMemcpyAsync(out, d, cnt * sizeof(T), hipMemcpyDeviceToHost, n->stream));
+  SPK_HIP(hipStreamSynchronize(n->stream));
+  return SPK_OK;
+}
+
+}  // namespace
+
+// =====================================================================================
+extern "C" {
+
+const char* spk_last_error(void) { return g_error.c_str(); }
+
+int spk_abi_version(void) { return 100; }
+
+int spk_device_count(void) {
+  int c = 0;
+  if (hipGetDeviceCount(&c) != hipSuccess) return 0;
+  return c;
+}
+
+int spk_create(int n_neurons, int n_inputs, int synapse_kind, int n_labels, float timestep,
+               spk_network** out) {
+  if (!out) return fail(SPK_ERR_ARG, "out is null");
+  *out = nullptr;
+  if (n_neurons < 0 || n_inputs < 0) return fail(SPK_ERR_ARG, "negative population size");
+  if ((int64_t)n_neurons + n_inputs >= (int64_t)1 << 30) return fail(SPK_ERR_ARG, "too many neurons");
+  if (synapse_kind != SPK_SYN_VOLTAGE && synapse_kind != SPK_SYN_CONDUCTANCE)
+    return fail(SPK_ERR_ARG, "unknown synapse kind");
+  if (n_labels < 1 || n_labels > spk::kMaxLabels)
+    return fail(SPK_ERR_ARG, "n_labels must be in [1, 8]");
+  if (!(timestep > 0.0f)) return fail(SPK_ERR_ARG, "timestep must be > 0");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return fail(SPK_ERR_NODEVICE, "no HIP device visible");
+  int dev = 0;
+  SPK_HIP(hipGetDevice(&dev));
+  hipDeviceProp_t prop;
+  SPK_HIP(hipGetDeviceProperties(&prop, dev));
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return fail(SPK_ERR_NODEVICE, std::string("device is ") + prop.gcnArchName + ", need gfx950");
+  spk_network* n = new spk_network();
+  n->N = n_neurons;
+  n->Nin = n_inputs;
+  n->kind = synapse_kind;
+  n->L = n_labels;
+  n->dt = timestep;
+  n->device = dev;
+  n->decay_g.assign(n_labels, 1.0f);
+  n->erev.assign(n_labels, 0.0f);
+  hipError_t e = hipStreamCreateWithFlags(&n->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete n;
+    return fail(SPK_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+  }
+  n->own_stream = true;
+  (void)hipEventCreate(&n->ev_a);
+  (void)hipEventCreate(&n->ev_b);
+  *out = n;
+  return SPK_OK;
+}
+
+int spk_destroy(spk_network* n) {
+  if (!n) return SPK_OK;
+  if (n->stream) (void)hipStreamSynchronize(n->stream);
+  free_device(n);
+  if (n->ev_a) (void)hipEventDestroy(n->ev_a);
+  if (n->ev_b) (void)hipEventDestroy(n->ev_b);
+  for (auto e : n->ev_kernel) (void)hipEventDestroy(e);
+  if (n->own_stream && n->stream) (void)hipStreamDestroy(n->stream);
+  delete n;
+  return SPK_OK;
+}
+
+int spk_set_stream(spk_network* n, void* s) {
+  if (int rc = check_net(n)) return rc;
+  SPK_HIP(hipStreamSynchronize(n->stream));
+  if (!s) return SPK_OK;
+  if (n->own_stream) (void)hipStreamDestroy(n->stream);
+  n->stream = (hipStream_t)s;
+  n->own_stream = false;
+  return SPK_OK;
+}
+
+int spk_set_lif(spk_network* n, const float* v0, const float* vreset, const float* threshold,
+                const float* decay_c, const float* resist, float refractory_s,
+                float background_current) {
+  if (int rc = check_net(n)) return rc;
+  if (n->prepared) return fail(SPK_ERR_STATE, "network already prepared");
+  if (n->N > 0 && (!v0 || !vreset || !threshold || !decay_c || !resist))
+    return fail(SPK_ERR_ARG, "null LIF parameter array");
+  n->v0.assign(v0, v0 + n->N);
+  n->vreset.assign(vreset, vreset + n->N);
+  n->thr.assign(threshold, threshold + n->N);
+  n->c.assign(decay_c, decay_c + n->N);
+  n->rp.assign(resist, resist + n->N);
+  n->tref = refractory_s;
+  n->ibg = background_current;
+  n->lif_set = true;
+  return SPK_OK;
+}
+
+int spk_set_poisson_rates(spk_network* n, const float* rates, int n_stimuli) {
+  if (int rc = check_net(n)) return rc;
+  if (n->prepared) return fail(SPK_ERR_STATE, "network already prepared");
+  if (n_stimuli < 1 || (n->Nin > 0 && !rates)) return fail(SPK_ERR_ARG, "bad Poisson rates");
+  n->rates.assign(rates, rates + (size_t)n_stimuli * n->Nin);
+  n->n_stim = n_stimuli;
+  for (float r : n->rates)
+    if (!(r >= 0.0f) || std::isinf(r)) return fail(SPK_ERR_ARG, "Poisson rates must be finite and >= 0");
+  return SPK_OK;
+}
+
+int spk_select_stimulus(spk_network* n, int s) {
+  if (int rc = check_net(n)) return rc;
+  if (s < 0 || s >= std::max(1, n->n_stim)) return fail(SPK_ERR_ARG, "stimulus index out of range");
+  n->stim = s;
+  if (n->prepared) return spk_reset_inputs(n);
+  return SPK_OK;
+}
+
+int spk_set_seed(spk_network* n, uint64_t seed) {
+  if (int rc = check_net(n)) return rc;
+  n->seed = seed;
+  return SPK_OK;
+}
+
+int spk_set_external_drive(spk_network* n, float rate_hz, float weight) {
+  if (int rc = check_net(n)) return rc;
+  if (n->prepared) return fail(SPK_ERR_STATE, "network already prepared");
+  if (!(rate_hz >= 0.0f) || std::isinf(rate_hz)) return fail(SPK_ERR_ARG, "bad external rate");
+  n->ext_rate = rate_hz;
+  n->ext_w = weight;
+  return SPK_OK;
+}
+
+int spk_set_labels(spk_network* n, const float* decay_g, const float* reversal) {
+  if (int rc = check_net(n)) return rc;
+  if (n->prepared) return fail(SPK_ERR_STATE, "network already prepared");
+  if (!decay_g || !reversal) return fail(SPK_ERR_ARG, "null label arrays");
+  n->decay_g.assign(decay_g, decay_g + n->L);
+  n->erev.assign(reversal, reversal + n->L);
+  return SPK_OK;
+}
+
+int spk_set_synapses(spk_network* n, int64_t S, const int32_t* pre, const int32_t* post,
+                     const float* weight, const float* scale, const int32_t* delay,
+                     const int32_t* label) {
+  if (int rc = check_net(n)) return rc;
+  if (n->prepared) return fail(SPK_ERR_STATE, "network already prepared");
+  if (S < 0) return fail(SPK_ERR_ARG, "negative synapse count");
+  if (S > 0 && (!pre || !post || !weight || !delay)) return fail(SPK_ERR_ARG, "null synapse array");
+  for (int64_t s = 0; s < S; ++s) {
+    if (post[s] < 0 || post[s] >= n->N) return fail(SPK_ERR_ARG, "postsynaptic id out of range");
+    if (pre[s] >= n->N || (pre[s] < 0 && -(int64_t)pre[s] - 1 >= n->Nin))
+      return fail(SPK_ERR_ARG, "presynaptic id out of range");
+    if (delay[s] < 1 || delay[s] > 65535) return fail(SPK_ERR_ARG, "delay must be in [1, 65535] steps");
+    if (label && (label[s] < 0 || label[s] >= n->L)) return fail(SPK_ERR_ARG, "label out of range");
+  }
+  n->h_pre.assign(pre, pre + S);
+  n->h_post.assign(post, post + S);
+  n->h_w.assign(weight, weight + S);
+  n->h_delay.assign(delay, delay + S);
+  n->h_label.assign(S, 0);
+  if (label) n->h_label.assign(label, label + S);
+  n->h_scale.assign(S, 1.0f);
+  if (scale) n->h_scale.assign(scale, scale + S);
+  n->S = S;
+  n->syn_set = true;
+  return SPK_OK;
+}
+
+static int add_rule(spk_network* n, int kind, const int64_t* syn, int64_t cnt, RuleHost&& r,
+                    int* rule_id) {
+  if (int rc = check_net(n)) return rc;
+  if (n->prepared) return fail(SPK_ERR_STATE, "network already prepared");
+  if (cnt < 0 || (cnt > 0 && !syn)) return fail(SPK_ERR_ARG, "bad plastic synapse list");
+  for (int64_t j = 0; j < cnt; ++j)
+    if (syn[j] < 0 || syn[j] >= n->S) return fail(SPK_ERR_ARG, "plastic synapse index out of range");
+  r.kind = kind;
+  r.syn.assign(syn, syn + cnt);
+  n->rules.push_back(std::move(r));
+  if (rule_id) *rule_id = (int)n->rules.size() - 1;
+  return SPK_OK;
+}
+
+int spk_add_wd_stdp(spk_network* n, const int64_t* syn, int64_t cnt, const spk_wd_stdp_params* p,
+                    int* rule_id) {
+  if (!p) return fail(SPK_ERR_ARG, "null parameters");
+  RuleHost r;
+  r.wd = *p;
+  return add_rule(n, 0, syn, cnt, std::move(r), rule_id);
+}
+
+int spk_add_vogels_stdp(spk_network* n, const int64_t* syn, int64_t cnt,
+                        const spk_vogels_stdp_params* p, int* rule_id) {
+  if (!p) return fail(SPK_ERR_ARG, "null parameters");
+  RuleHost r;
+  r.vg = *p;
+  return add_rule(n, 1, syn, cnt, std::move(r), rule_id);
+}
+
+int spk_add_evans_stdp(spk_network* n, const int64_t* syn, int64_t cnt,
+                       const spk_evans_stdp_params* p, int* rule_id) {
+  if (!p) return fail(SPK_ERR_ARG, "null parameters");
+  RuleHost r;
+  r.ev = *p;
+  return add_rule(n, 2, syn, cnt, std::move(r), rule_id);
+}
+
+int spk_add_monitor(spk_network* n, int population, int64_t capacity, int* monitor_id) {
+  if (int rc = check_net(n)) return rc;
+  if (n->prepared) return fail(SPK_ERR_STATE, "network already prepared");
+  if (population != SPK_POP_NEURONS && population != SPK_POP_INPUTS)
+    return fail(SPK_ERR_ARG, "unknown population");
+  if (capacity < 1) return fail(SPK_ERR_ARG, "monitor capacity must be >= 1");
+  MonitorHost m;
+  m.pop = population;
+  m.capacity = capacity;
+  n->mons.push_back(m);
+  if (monitor_id) *monitor_id = (int)n->mons.size() - 1;
+  return SPK_OK;
+}
+
+int spk_prepare(spk_network* n, int G) {
+  if (int rc = check_net(n)) return rc;
+  if (n->prepared) return fail(SPK_ERR_STATE, "network already prepared");
+  if (G < 1 || G > 1024) return fail(SPK_ERR_ARG, "timestep grouping must be in [1, 1024]");
+  if (n->N > 0 && !n->lif_set) return fail(SPK_ERR_STATE, "LIF parameters not set");
+  if (n->Nin > 0 && n->rates.empty()) {
+    n->rates.assign(n->Nin, 0.0f);
+    n->n_stim = 1;
+  }
+  const int64_t S = n->S;
+  n->G = n->Gprep = G;
+  n->dmin = S ? 1 << 30 : 0;
+  n->dmax = 0;
+  for (int64_t s = 0; s < S; ++s) {
+    n->dmin = std::min(n->dmin, n->h_delay[s]);
+    n->dmax = std::max(n->dmax, n->h_delay[s]);
+  }
+  n->B = n->dmax + 2 * G + 1;           // SpikingSynapses.cu:48
+  n->W = (n->dmax + 2 * G) / 32 + 1;    // >= dmax + 2G bits of history (SpikingNeurons.cu:33)
+  // scale of the synapse weights
+  n->uniform_scale = true;
+  n->scale0 = S ? n->h_scale[0] : 1.0f;
+  for (int64_t s = 0; s < S; ++s)
+    if (n->h_scale[s] != n->scale0) { n->uniform_scale = false; break; }
+  // fixed-point fraction bits
+  std::vector<double> cap(S);
+  for (int64_t s = 0; s < S; ++s) cap[s] = std::fabs((double)n->h_w[s]);
+  for (auto& r : n->rules)
+    for (int64_t s : r.syn) {
+      double c0 = r.kind == 0 ? std::fabs((double)r.wd.w_max) : (r.kind == 2 ? 1.0 : 0.0);
+      cap[s] = std::max(cap[s], c0);
+    }
+  std::vector<double> ent((size_t)n->N * n->L, 0.0);
+  for (int64_t s = 0; s < S; ++s)
+    ent[(size_t)n->h_post[s] * n->L + n->h_label[s]] += cap[s] * std::fabs((double)n->h_scale[s]);
+  double mx = 0.0;
+  for (double e : ent) mx = std::max(mx, e);
+  n->ext_cdf = poisson_count_table((double)n->ext_rate * (double)n->dt);
+  if ((int)n->ext_cdf.size() > spk::kMaxExtTable)
+    return fail(SPK_ERR_ARG, "external drive rate*dt too large for the count table");
+  if (!n->ext_cdf.empty()) mx += (double)n->ext_cdf.size() * std::fabs((double)n->ext_w);
+  n->frac = choose_frac_bits(mx);
+  n->q_ext = n->ext_cdf.empty() ? 0 : (int64_t)std::llrint(std::ldexp((double)n->ext_w, n->frac));
+  // CSR by unified presynaptic id, rows sorted by delay (stable in caller order)
+  const int U = n->nu();
+  std::vector<int64_t> rowptr(U + 1, 0);
+  auto uid = [&](int32_t pre) { return pre >= 0 ? pre : n->N + (-pre - 1); };
+  for (int64_t s = 0; s < S; ++s) rowptr[uid(n->h_pre[s]) + 1]++;
+  for (int u = 0; u < U; ++u) rowptr[u + 1] += rowptr[u];
+  std::vector<int64_t> order(S);
+  {
+    std::vector<int64_t> fill(rowptr.begin(), rowptr.end() - 1);
+    for (int64_t s = 0; s < S; ++s) order[fill[uid(n->h_pre[s])]++] = s;
+  }
+  for (int u = 0; u < U; ++u) {
+    auto b = order.begin() + rowptr[u], e = order.begin() + rowptr[u + 1];
+    std::stable_sort(b, e, [&](int64_t x, int64_t y) { return n->h_delay[x] < n->h_delay[y]; });
+  }
+  n->perm.assign(S, 0);
+  std::vector<int32_t> tgt(S);
+  std::vector<float> w(S), sc(S);
+  std::vector<uint16_t> dl(S);
+  std::vector<uint8_t> lb(S);
+  for (int64_t k = 0; k < S; ++k) {
+    const int64_t s = order[k];
+    n->perm[s] = k;
+    tgt[k] = n->h_post[s];
+    w[k] = n->h_w[s];
+    sc[k] = n->h_scale[s];
+    dl[k] = (uint16_t)n->h_delay[s];
+    lb[k] = (uint8_t)n->h_label[s];
+  }
+  // list capacity: every neuron of both populations may spike once per sub-step
+  n->cap = std::max(1, U);
+  n->slots = 1;
+  // allocations
+  int rc;
+  const size_t N = n->N, Nin = n->Nin, L = n->L;
+#define ALLOC(p, cnt) if ((rc = dalloc(&(p), (cnt)))) return rc
+  ALLOC(n->d_V, N); ALLOC(n->d_v0, N); ALLOC(n->d_vreset, N); ALLOC(n->d_thr, N);
+  ALLOC(n->d_c, N); ALLOC(n->d_rp, N); ALLOC(n->d_last_n, N); ALLOC(n->d_last_in, Nin);
+  ALLOC(n->d_hist_n, (size_t)n->W * N); ALLOC(n->d_hist_in, (size_t)n->W * Nin);
+  ALLOC(n->d_ring, (size_t)n->B * N * L); ALLOC(n->d_g, N * L);
+  ALLOC(n->d_decay, L); ALLOC(n->d_erev, L);
+  ALLOC(n->d_in_thr, (size_t)std::max(1, n->n_stim) * Nin);
+  ALLOC(n->d_ext_cdf, std::max<size_t>(1, n->ext_cdf.size()));
+  ALLOC(n->d_rowptr, (size_t)U + 1); ALLOC(n->d_tgt, S); ALLOC(n->d_w, S);
+  ALLOC(n->d_delay, S);
+  if (!n->uniform_scale) ALLOC(n->d_scale, S);
+  if (n->L > 1) ALLOC(n->d_label, S);
+  ALLOC(n->d_lists, (size_t)n->slots * G * n->cap);
+  ALLOC(n->d_counts, (size_t)n->slots * G);
+  ALLOC(n->d_slot_time, n->slots); ALLOC(n->d_slot_tidx, n->slots);
+  ALLOC(n->d_stats, 4);
+#undef ALLOC
+  hipStream_t st = n->stream;
+  if ((rc = upload(n->d_v0, n->v0.data(), N, st))) return rc;
+  if ((rc = upload(n->d_vreset, n->vreset.data(), N, st))) return rc;
+  if ((rc = upload(n->d_thr, n->thr.data(), N, st))) return rc;
+  if ((rc = upload(n->d_c, n->c.data(), N, st))) return rc;
+  if ((rc = upload(n->d_rp, n->rp.data(), N, st))) return rc;
+  if ((rc = upload(n->d_decay, n->decay_g.data(), L, st))) return rc;
+  if ((rc = upload(n->d_erev, n->erev.data(), L, st))) return rc;
+  {
+    std::vector<uint32_t> thr(n->rates.size());
+    for (size_t i = 0; i < thr.size(); ++i) thr[i] = bernoulli_threshold(n->rates[i], n->dt);
+    if ((rc = upload(n->d_in_thr, thr.data(), thr.size(), st))) return rc;
+  }
+  if ((rc = upload(n->d_ext_cdf, n->ext_cdf.data(), n->ext_cdf.size(), st))) return rc;
+  if ((rc = upload(n->d_rowptr, rowptr.data(), (size_t)U + 1, st))) return rc;
+  if ((rc = upload(n->d_tgt, tgt.data(), S, st))) return rc;
+  if ((rc = upload(n->d_w, w.data(), S, st))) return rc;
+  if ((rc = upload(n->d_delay, dl.data(), S, st))) return rc;
+  if (n->d_scale && (rc = upload(n->d_scale, sc.data(), S, st))) return rc;
+  if (n->d_label && (rc = upload(n->d_label, lb.data(), S, st))) return rc;
+  SPK_HIP(hipMemsetAsync(n->d_counts, 0, sizeof(int) * n->slots * G, st));
+  SPK_HIP(hipMemsetAsync(n->d_stats, 0, sizeof(unsigned long long) * 4, st));
+  // rules
+  for (auto& r : n->rules) {
+    const size_t P = r.syn.size();
+    if ((rc = dalloc(&r.d_pos, P))) return rc;
+    if ((rc = dalloc(&r.d_pre, P))) return rc;
+    if ((rc = dalloc(&r.d_ta, P))) return rc;
+    if ((rc = dalloc(&r.d_tb, P))) return rc;
+    std::vector<int64_t> pos(P);
+    std::vector<int32_t> pre(P);
+    for (size_t j = 0; j < P; ++j) {
+      pos[j] = n->perm[r.syn[j]];
+      pre[j] = uid(n->h_pre[r.syn[j]]);
+    }
+    if ((rc = upload(r.d_pos, pos.data(), P, st))) return rc;
+    if ((rc = upload(r.d_pre, pre.data(), P, st))) return rc;
+  }
+  // monitors: device store of the requested capacity
+  for (auto& m : n->mons) {
+    if ((rc = dalloc(&m.d_ids, (size_t)m.capacity))) return rc;
+    if ((rc = dalloc(&m.d_times, (size_t)m.capacity))) return rc;
+    if ((rc = dalloc(&m.d_count, 1))) return rc;
+    SPK_HIP(hipMemsetAsync(m.d_count, 0, sizeof(unsigned long long), st));
+  }
+  SPK_HIP(hipStreamSynchronize(st));
+  n->prepared = true;
+  return spk_reset_state(n);
+}
+
+int spk_set_timestep_grouping(spk_network* n, int G) {
+  if (int rc = check_prepared(n)) return rc;
+  if (G < 1 || G > n->Gprep) return fail(SPK_ERR_ARG, "grouping must be in [1, prepared grouping]");
+  if (G > n->dmin && n->S > 0) return fail(SPK_ERR_ARG, "grouping larger than the minimum delay");
+  n->G = G;
+  n->cur_tidx = -1;
+  return SPK_OK;
+}
+
+int spk_get_info(const spk_network* n, spk_info* info) {
+  if (int rc = check_net(n)) return rc;
+  if (!info) return fail(SPK_ERR_ARG, "null info");
+  info->n_neurons = n->N;
+  info->n_inputs = n->Nin;
+  info->n_labels = n->L;
+  info->synapse_kind = n->kind;
+  info->timestep_grouping = n->G;
+  info->ring_slots = n->B;
+  info->history_words = n->W;
+  info->frac_bits = n->frac;
+  info->min_delay = n->dmin;
+  info->max_delay = n->dmax;
+  info->n_synapses = n->S;
+  info->n_rules = (int)n->rules.size();
+  info->n_monitors = (int)n->mons.size();
+  return SPK_OK;
+}
+
+int spk_reset_inputs(spk_network* n) {
+  if (int rc = check_prepared(n)) return rc;
+  if (n->Nin > 0) {
+    std::vector<float> ls(n->Nin, -1000.0f);
+    if (int rc = upload(n->d_last_in, ls.data(), ls.size(), n->stream)) return rc;
+    SPK_HIP(hipMemsetAsync(n->d_hist_in, 0, sizeof(uint32_t) * n->W * n->Nin, n->stream));
+  }
+  n->pending_arm = true;
+  n->arm_step = -1;
+  SPK_HIP(hipStreamSynchronize(n->stream));
+  return SPK_OK;
+}
+
+int spk_reset_state(spk_network* n) {
+  if (int rc = check_prepared(n)) return rc;
+  hipStream_t st = n->stream;
+  if (n->N > 0) {
+    if (int rc = upload(n->d_V, n->v0.data(), (size_t)n->N, st)) return rc;
+    std::vector<float> ls(n->N, -1000.0f);
+    if (int rc = upload(n->d_last_n, ls.data(), ls.size(), st)) return rc;
+    SPK_HIP(hipMemsetAsync(n->d_hist_n, 0, sizeof(uint32_t) * n->W * n->N, st));
+    SPK_HIP(hipMemsetAsync(n->d_ring, 0, sizeof(int64_t) * n->B * n->N * n->L, st));
+    SPK_HIP(hipMemsetAsync(n->d_g, 0, sizeof(float) * n->N * n->L, st));
+    SPK_HIP(hipStreamSynchronize(st));
+  }
+  for (auto& r : n->rules) {
+    const size_t P = r.syn.size();
+    if (P) {
+      SPK_HIP(hipMemsetAsync(r.d_ta, 0, sizeof(float) * P, st));
+      SPK_HIP(hipMemsetAsync(r.d_tb, 0, sizeof(float) * P, st));
+    }
+  }
+  n->cur_tidx = -1;
+  return spk_reset_inputs(n);
+}
+
+int spk_update_neurons(spk_network* n, float t) {
+  if (int rc = check_prepared(n)) return rc;
+  if (int rc = begin_group(n, t, true)) return rc;
+  return launch_neurons(n, t, n->cur_slot);
+}
+
+int spk_update_inputs(spk_network* n, float t) {
+  if (int rc = check_prepared(n)) return rc;
+  if (int rc = begin_group(n, t)) return rc;
+  return launch_inputs(n, t, n->cur_slot);
+}
+
+int spk_update_plasticity(spk_network* n, int rid, float t) {
+  if (int rc = check_prepared(n)) return rc;
+  if (rid < 0 || rid >= (int)n->rules.size()) return fail(SPK_ERR_ARG, "rule id out of range");
+  return launch_rule(n, rid, t);
+}
+
+int spk_update_synapses(spk_network* n, float t) {
+  if (int rc = check_prepared(n)) return rc;
+  if (int rc = begin_group(n, t)) return rc;
+  return launch_propagate(n, t, n->cur_slot);
+}
+
+int spk_collect_spikes(spk_network* n, int mid, float t) {
+  if (int rc = check_prepared(n)) return rc;
+  if (mid < 0 || mid >= (int)n->mons.size()) return fail(SPK_ERR_ARG, "monitor id out of range");
+  if (int rc = begin_group(n, t)) return rc;
+  return launch_collect(n, mid, n->cur_slot, 1);
+}
+
+int spk_run(spk_network* n, float start, int64_t ngroups, int flags) {
+  if (int rc = check_prepared(n)) return rc;
+  if (ngroups < 0) return fail(SPK_ERR_ARG, "negative group count");
+  std::memset(&n->timing, 0, sizeof(n->timing));
+  SPK_HIP(hipMemsetAsync(n->d_stats, 0, sizeof(unsigned long long) * 4, n->stream));
+  // each monitor store must hold one group of its population
+  for (auto& m : n->mons) {
+    const int64_t pop = m.pop == SPK_POP_NEURONS ? n->N : n->Nin;
+    if (int rc = ensure_monitor_capacity(n, m, std::max<int64_t>(m.capacity, pop * n->G * 64))) return rc;
+  }
+  SPK_HIP(hipEventRecord(n->ev_a, n->stream));
+  const bool plastic = (flags & SPK_RUN_PLASTICITY) != 0;
+  int since_drain = 0;
+  for (int64_t s = 0; s < ngroups; ++s) {
+    const float t = start + (float)(s * n->G) * n->dt;
+    int rc;
+    if ((rc = begin_group(n, t, true))) return rc;
+    if ((rc = launch_neurons(n, t, n->cur_slot))) return rc;
+    if ((rc = launch_inputs(n, t, n->cur_slot))) return rc;
+    if (plastic)
+      for (int r = 0; r < (int)n->rules.size(); ++r)
+        if ((rc = launch_rule(n, r, t))) return rc;
+    if ((rc = launch_propagate(n, t, n->cur_slot))) return rc;
+    for (int m = 0; m < (int)n->mons.size(); ++m)
+      if ((rc = launch_collect(n, m, n->cur_slot, 1))) return rc;
+    if (!n->mons.empty() && ++since_drain == 64) {
+      since_drain = 0;
+      for (auto& m : n->mons)
+        if ((rc = drain_monitor(n, m, m.h_ids, m.h_times))) return rc;
+    }
+  }
+  SPK_HIP(hipEventRecord(n->ev_b, n->stream));
+  for (auto& m : n->mons)
+    if (int rc = drain_monitor(n, m, m.h_ids, m.h_times)) return rc;
+  SPK_HIP(hipEventSynchronize(n->ev_b));
+  float ms = 0.0f;
+  SPK_HIP(hipEventElapsedTime(&ms, n->ev_a, n->ev_b));
+  unsigned long long st[4] = {0, 0, 0, 0};
+  SPK_HIP(hipMemcpy(st, n->d_stats, sizeof(st), hipMemcpyDeviceToHost));
+  n->timing.total_ms = ms;
+  n->timing.groups = ngroups;
+  n->timing.synaptic_events = (int64_t)st[0];
+  n->timing.external_events = (int64_t)st[1];
+  n->cur_tidx = -1;
+  return SPK_OK;
+}
+
+int spk_last_timing(const spk_network* n, spk_timing* t) {
+  if (int rc = check_net(n)) return rc;
+  if (!t) return fail(SPK_ERR_ARG, "null timing");
+  *t = n->timing;
+  return SPK_OK;
+}
+
+int spk_synchronize(spk_network* n) {
+  if (int rc = check_net(n)) return rc;
+  SPK_HIP(hipStreamSynchronize(n->stream));
+  return SPK_OK;
+}
+
+int spk_monitor_device_count(spk_network* n, int mid, int64_t* count) {
+  if (int rc = check_prepared(n)) return rc;
+  if (mid < 0 || mid >= (int)n->mons.size() || !count) return fail(SPK_ERR_ARG, "bad monitor");
+  unsigned long long c = 0;
+  SPK_HIP(hipMemcpyAsync(&c, n->mons[mid].d_count, sizeof(c), hipMemcpyDeviceToHost, n->stream));
+  SPK_HIP(hipStreamSynchronize(n->stream));
+  *count = (int64_t)c;
+  return SPK_OK;
+}
+
+int spk_monitor_drain(spk_network* n, int mid, int32_t* ids, float* times, int64_t cap, int64_t* out_n) {
+  if (int rc = check_prepared(n)) return rc;
+  if (mid < 0 || mid >= (int)n->mons.size() || !out_n) return fail(SPK_ERR_ARG, "bad monitor");
+  std::vector<int32_t> bi;
+  std::vector<float> bt;
+  if (int rc = drain_monitor(n, n->mons[mid], bi, bt)) return rc;
+  *out_n = (int64_t)bi.size();
+  if ((int64_t)bi.size() > cap) return fail(SPK_ERR_CAPACITY, "output buffer too small");
+  if (bi.size()) {
+    std::memcpy(ids, bi.data(), bi.size() * sizeof(int32_t));
+    std::memcpy(times, bt.data(), bt.size() * sizeof(float));
+  }
+  return SPK_OK;
+}
+
+int spk_monitor_take(spk_network* n, int mid, int32_t* ids, float* times, int64_t cap, int64_t* out_n) {
+  if (int rc = check_prepared(n)) return rc;
+  if (mid < 0 || mid >= (int)n->mons.size() || !out_n) return fail(SPK_ERR_ARG, "bad monitor");
+  MonitorHost& m = n->mons[mid];
+  *out_n = (int64_t)m.h_ids.size();
+  if (!ids) return SPK_OK;
+  if ((int64_t)m.h_ids.size() > cap) return fail(SPK_ERR_CAPACITY, "output buffer too small");
+  if (!m.h_ids.empty()) {
+    std::memcpy(ids, m.h_ids.data(), m.h_ids.size() * sizeof(int32_t));
+    std::memcpy(times, m.h_times.data(), m.h_times.size() * sizeof(float));
+  }
+  m.h_ids.clear();
+  m.h_times.clear();
+  return SPK_OK;
+}
+
+int spk_get_weights(spk_network* n, float* out) {
+  if (int rc = check_prepared(n)) return rc;
+  if (!out && n->S) return fail(SPK_ERR_ARG, "null output");
+  std::vector<float> w(n->S);
+  if (n->S) SPK_HIP(hipMemcpyAsync(w.data(), n->d_w, n->S * sizeof(float), hipMemcpyDeviceToHost, n->stream));
+  SPK_HIP(hipStreamSynchronize(n->stream));
+  for (int64_t s = 0; s < n->S; ++s) out[s] = w[n->perm[s]];
+  return SPK_OK;
+}
+
+int spk_set_weights(spk_network* n, const float* in) {
+  if (int rc = check_prepared(n)) return rc;
+  if (!in && n->S) return fail(SPK_ERR_ARG, "null input");
+  std::vector<float> w(n->S);
+  for (int64_t s = 0; s < n->S; ++s) w[n->perm[s]] = in[s];
+  if (n->S) SPK_HIP(hipMemcpyAsync(n->d_w, w.data(), n->S * sizeof(float), hipMemcpyHostToDevice, n->stream));
+  SPK_HIP(hipStreamSynchronize(n->stream));
+  return SPK_OK;
+}
+
+int spk_get_membrane(spk_network* n, float* v) {
+  if (int rc = check_prepared(n)) return rc;
+  return download(n, v, n->d_V, (size_t)n->N);
+}
+
+int spk_get_last_spike(spk_network* n, int pop, float* ls) {
+  if (int rc = check_prepared(n)) return rc;
+  if (pop == SPK_POP_NEURONS) return download(n, ls, n->d_last_n, (size_t)n->N);
+  return download(n, ls, n->d_last_in, (size_t)n->Nin);
+}
+
+int spk_get_conductances(spk_network* n, float* g) {
+  if (int rc = check_prepared(n)) return rc;
+  return download(n, g, n->d_g, (size_t)n->N * n->L);
+}
+
+int spk_get_ring(spk_network* n, int64_t* ring) {
+  if (int rc = check_prepared(n)) return rc;
+  return download(n, ring, n->d_ring, (size_t)n->B * n->N * n->L);
+}
+
+int spk_get_history(spk_network* n, int pop, uint32_t* words) {
+  if (int rc = check_prepared(n)) return rc;
+  if (pop == SPK_POP_NEURONS) return download(n, words, n->d_hist_n, (size_t)n->W * n->N);
+  return download(n, words, n->d_hist_in, (size_t)n->W * n->Nin);
+}
+
+int spk_get_traces(spk_network* n, int rid, float* a, float* b) {
+  if (int rc = check_prepared(n)) return rc;
+  if (rid < 0 || rid >= (int)n->rules.size()) return fail(SPK_ERR_ARG, "rule id out of range");
+  RuleHost& r = n->rules[rid];
+  if (int rc = download(n, a, r.d_ta, r.syn.size())) return rc;
+  return download(n, b, r.d_tb, r.syn.size());
+}
+
+}  // extern "C"

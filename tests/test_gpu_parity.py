@@ -1,0 +1,181 @@
+"""Bit-exact parity of the HIP backend against the FP32 CPU oracle.
+
+Both run the same NetDesc; spike rasters (id, time), membrane potentials, the input
+ring, spike history, conductances, weights and plasticity traces must be identical.
+"""
+import numpy as np
+import pytest
+
+import oracle_py
+from spike_amd import abi, builders
+from spike_amd.netdesc import SYN_CONDUCTANCE, Rule, RULE_WD, RULE_VOGELS, RULE_EVANS
+
+pytestmark = pytest.mark.gpu
+
+
+def _compare(gpu, ora, d, tag=""):
+    for m in range(len(d.monitors)):
+        gi, gt = gpu.take_spikes(m)
+        oi, ot = ora.take_spikes(m)
+        assert len(gi) == len(oi), f"{tag} monitor {m}: {len(gi)} vs {len(oi)} spikes"
+        assert np.array_equal(gi, oi), f"{tag} monitor {m} ids differ"
+        assert np.array_equal(gt, ot), f"{tag} monitor {m} times differ"
+    assert np.array_equal(gpu.membrane().astype(np.float64), ora.membrane()), f"{tag} V differs"
+    assert np.array_equal(gpu.last_spike(0), ora.last_spike(0)), f"{tag} last spike differs"
+    if d.n_inputs:
+        assert np.array_equal(gpu.last_spike(1), ora.last_spike(1)), f"{tag} input last spike"
+        assert np.array_equal(gpu.history(1), ora.history(1)), f"{tag} input history differs"
+    assert np.array_equal(gpu.history(0), ora.history(0)), f"{tag} history differs"
+    assert np.array_equal(gpu.ring(), ora.ring()), f"{tag} input ring differs"
+    if d.kind == SYN_CONDUCTANCE:
+        assert np.array_equal(gpu.conductances().astype(np.float64), ora.conductances()), f"{tag} g"
+    if d.n_synapses:
+        assert np.array_equal(gpu.weights().astype(np.float64), ora.weights()), f"{tag} weights"
+    for r in range(len(d.rules)):
+        ga, gb = gpu.traces(r)
+        oa, ob = ora.traces(r)
+        assert np.array_equal(ga.astype(np.float64), oa) and np.array_equal(gb.astype(np.float64), ob), \
+            f"{tag} traces of rule {r}"
+
+
+def _run_both(d, ngroups, fused=True, start=0.0):
+    print(f"[parity] {d.name} N={d.n_neurons} Nin={d.n_inputs} S={d.n_synapses} G={d.timestep_grouping()} "
+          f"groups={ngroups} fused={fused}", flush=True)
+    gpu = abi.Network(d)
+    ora = oracle_py.OracleNetwork(d, precision=32)
+    gi, oi = gpu.info(), ora.info()
+    assert (gi.timestep_grouping, gi.ring_slots, gi.history_words, gi.frac_bits) == \
+        (oi["G"], oi["B"], oi["W"], oi["frac"])
+    if fused:
+        gpu.run(start, ngroups)
+        ora.run(start, ngroups)
+    else:
+        for s in range(ngroups):
+            t = d.group_time(start, s)
+            gpu.step_phases(t)
+            ora.step_phases(t)
+    return gpu, ora
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+@pytest.mark.parametrize("fused", [True, False])
+def test_voltage_random(seed, fused):
+    d = builders.random_net(seed=seed, input_rate=400.0, whi=4e-3)
+    gpu, ora = _run_both(d, 300, fused=fused)
+    _compare(gpu, ora, d, f"voltage seed={seed}")
+
+
+def test_voltage_grouped_external():
+    d = builders.random_net(seed=5, dmin=4, dmax=9, input_rate=300.0, ext_rate=3000.0,
+                            ext_w=1.5e-3, whi=3e-3)
+    assert d.timestep_grouping() == 4
+    gpu, ora = _run_both(d, 200)
+    _compare(gpu, ora, d, "grouped+external")
+
+
+@pytest.mark.parametrize("labels", [1, 2, 3])
+def test_conductance_random(labels):
+    d = builders.random_net(seed=11 + labels, kind=SYN_CONDUCTANCE, n_labels=labels, whi=1.0,
+                            scale=lambda rng, S: rng.uniform(0.5e-9, 2e-9, S), input_rate=500.0,
+                            dmin=2, dmax=5)
+    gpu, ora = _run_both(d, 250)
+    _compare(gpu, ora, d, f"conductance L={labels}")
+
+
+def test_brunel_small_1000_steps():
+    d = builders.brunel(n_exc=800, n_inh=200, seed=3, monitors=(0,))
+    ng = 1000 // d.timestep_grouping() + 1
+    gpu, ora = _run_both(d, ng)
+    _compare(gpu, ora, d, "brunel small")
+
+
+def test_brunel_reference_form_small():
+    d = builders.brunel(n_exc=400, n_inh=100, seed=4, form="reference", monitors=(0, 1))
+    d.rates[:] = 40.0
+    gpu, ora = _run_both(d, 40)
+    _compare(gpu, ora, d, "brunel reference form")
+
+
+def test_stimulus_switch_and_reset():
+    d = builders.random_net(seed=21, n_stimuli=3, input_rate=600.0)
+    gpu = abi.Network(d)
+    ora = oracle_py.OracleNetwork(d, precision=32)
+    gpu.run(0.0, 50)
+    ora.run(0.0, 50)
+    _compare(gpu, ora, d, "before switch")
+    for lib, h, fn in ((gpu.lib, gpu.h, "spk_select_stimulus"), (ora.lib, ora.h, "or_select_stimulus")):
+        assert getattr(lib, fn)(h, 2) == 0
+    t = d.group_time(0.0, 50)
+    gpu.run(float(t), 50)
+    ora.run(float(t), 50)
+    _compare(gpu, ora, d, "after switch")
+    gpu.reset_state()
+    ora.reset_state()
+    gpu.run(0.0, 30)
+    ora.run(0.0, 30)
+    _compare(gpu, ora, d, "after reset")
+
+
+def test_grouping_change_to_one():
+    """VogelsAbbottNet --NOTG: timestep_grouping forced to 1 after finalise."""
+    d = builders.random_net(seed=8, dmin=3, dmax=7, input_rate=400.0)
+    gpu = abi.Network(d)
+    ora = oracle_py.OracleNetwork(d, precision=32)
+    gpu.set_grouping(1)
+    ora.set_grouping(1)
+    gpu.run(0.0, 300)
+    ora.run(0.0, 300)
+    _compare(gpu, ora, d, "G=1")
+
+
+def _plastic_net(kind, seed):
+    d = builders.random_net(seed=seed, n_neurons=96, n_inputs=48, p=0.25, input_rate=500.0,
+                            whi=3e-3, dmin=2, dmax=6)
+    S = d.n_synapses
+    rng = np.random.default_rng(seed)
+    syn = np.sort(rng.choice(S, size=S // 2, replace=False)).astype(np.int64)
+    if kind == RULE_WD:
+        p = builders.wd_params(lam=1e-2, alpha=2.02, w_max=4e-3)
+    elif kind == RULE_VOGELS:
+        p = builders.vogels_params(lr=1e-5, rate=10.0)
+    else:
+        d.weight = rng.uniform(0.0, 1.0, S).astype(np.float32)
+        d.scale = np.full(S, np.float32(3e-3), np.float32)
+        p = builders.evans_params()
+    d.rules.append(Rule(kind, syn, p))
+    return d
+
+
+@pytest.mark.parametrize("kind", [RULE_WD, RULE_VOGELS, RULE_EVANS])
+@pytest.mark.parametrize("fused", [True, False])
+def test_plasticity(kind, fused):
+    d = _plastic_net(kind, 30 + kind)
+    gpu, ora = _run_both(d, 200, fused=fused)
+    _compare(gpu, ora, d, f"rule {kind}")
+
+
+def test_wd_nearest_spike():
+    d = _plastic_net(RULE_WD, 40)
+    d.rules[0].params["nearest_spike_only"] = 1
+    gpu, ora = _run_both(d, 200)
+    _compare(gpu, ora, d, "wd nearest")
+
+
+def test_coba_vogels_small():
+    d = builders.coba_vogels(n_exc=320, n_inh=80, p=0.1, seed=5)
+    gpu, ora = _run_both(d, 100)
+    _compare(gpu, ora, d, "coba")
+
+
+def test_weights_roundtrip_and_errors():
+    d = builders.random_net(seed=9)
+    gpu = abi.Network(d)
+    w = gpu.weights()
+    assert np.array_equal(w, d.weight)
+    w2 = (w * 2).astype(np.float32)
+    gpu.set_weights(w2)
+    assert np.array_equal(gpu.weights(), w2)
+    with pytest.raises(abi.SpikeError):
+        gpu.set_grouping(100)
+    with pytest.raises(abi.SpikeError):
+        abi.check(gpu.lib.spk_update_plasticity(gpu.h, 5, 0.0))
