@@ -1,0 +1,149 @@
+#!/usr/bin/env python3
+"""Brunel10K benchmark: synaptic events per second on one MI355X.
+
+Workload (the Brunel balanced network): 8000 excitatory + 2000 inhibitory LIF neurons
+(tau_m 20 ms, theta 20 mV, V_reset 10 mV, 2 ms refractory), delta synapses J = 0.1 mV,
+g = 5, fixed in-degree C_E = 800 / C_I = 200, delay 1.5 ms (G = 15 steps), external
+Poisson drive nu_ext = 2 nu_thr per neuron (20 kHz x J), dt = 0.1 ms, FP32, synthetic
+connectivity (seed 42).  One bench step = one simulated second (666 groups x 15 steps,
+the reference's truncated group count).  Synaptic events = recurrent deliveries
+(spikes x out-degree) + external Poisson events.
+
+Prints one JSON line with the metric, the roofline of the dominant kernel and the CPU
+baseline (the FP32 oracle on the host cores, on a bounded sample of the same network).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(n_neurons, steps, spikes, rec_events, b_w=4):
+    """Algorithmic HBM bytes of the hot path:
+    per neuron per step 24 B (V r/w 8, refractory state r/w 8, ring slot read+clear 8);
+    per spike 16 B (list write 4, list read 4, row pointers 8);
+    per recurrent delivery 4 B target + b_w B weight.
+    External Poisson events are generated in registers (0 B)."""
+    return n_neurons * steps * 24 + spikes * 16 + rec_events * (4 + b_w)
+
+
+def cpu_baseline(desc, groups, threads):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_py
+    o = oracle_py.OracleNetwork(desc, precision=32, threads=threads)
+    t0 = time.perf_counter()
+    o.run(0.0, groups)
+    dt = time.perf_counter() - t0
+    c = o.counters()
+    ev = c["synaptic_events"] + c["external_events"]
+    o.close()
+    return ev / dt, dt, ev
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--form", default="config", choices=["config", "reference"])
+    ap.add_argument("--path", default="fused", choices=["fused", "launches"])
+    ap.add_argument("--cpu-groups", type=int, default=100,
+                    help="timestep groups of the CPU-baseline sample (15 steps each)")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(the per-window spike exchange for neuron-range sharding is designed, not built)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    from spike_amd import abi, builders
+
+    desc = builders.brunel(seed=42, form=args.form)
+    G = desc.timestep_grouping()
+    groups = desc.groups_for(1.0)
+    net = abi.Network(desc)
+    info = net.info()
+    force = args.path == "launches"
+    # warmup: run the network into its steady state
+    t = 0.0
+    for _ in range(args.warmup):
+        net.run(t, groups, plasticity=False, force_launches=force)
+        t = float(desc.group_time(t, groups))
+    tot_ms = 0.0
+    kern_ms = 0.0
+    launches = 0
+    events = 0
+    rec = 0
+    spikes = 0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        tm = net.run(t, groups, plasticity=False, force_launches=force)
+        t = float(desc.group_time(t, groups))
+        tot_ms += tm.total_ms
+        kern_ms += tm.kernel_ms
+        launches += tm.kernel_launches
+        events += tm.synaptic_events + tm.external_events
+        rec += tm.synaptic_events
+        spikes += tm.spikes_neurons + tm.spikes_inputs
+    wall = time.perf_counter() - t0
+    steps_sim = groups * G * args.steps
+    value = events / (tot_ms / 1e3)
+    rate_hz = spikes / desc.n_neurons / (steps_sim * desc.dt)
+    # roofline of the dominant kernel (the fused group kernel)
+    bytes_alg = algorithmic_bytes(desc.n_neurons, steps_sim, spikes, rec)
+    if launches > 0 and kern_ms > 0:
+        avg_launch_ms = kern_ms / launches
+        per_launch_bytes = bytes_alg / launches
+        achieved = per_launch_bytes / (avg_launch_ms / 1e3) / 1e9
+    else:
+        avg_launch_ms = tot_ms / max(1, args.steps)
+        achieved = bytes_alg / (tot_ms / 1e3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "avg_launch_ms": round(avg_launch_ms, 5)}
+    cpu = None
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        v, secs, ev = cpu_baseline(desc, args.cpu_groups, threads)
+        cpu = {"value": round(v, 1), "unit": "syn events/s", "cores": threads, "kind": "port",
+               "sample": f"FP32 oracle, Brunel10K {args.form} form, first {args.cpu_groups} groups "
+                         f"({args.cpu_groups * G} steps) from reset, {secs:.2f} s, {ev} events"}
+    out = {
+        "metric": "syn events/s (Brunel10K)",
+        "value": round(value, 1),
+        "unit": "syn events/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(tot_ms / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (Brunel network, seed 42)",
+        "config": {"workload": f"Brunel10K {args.form} form: 8000E+2000I LIF, C_E=800/C_I=200, "
+                               f"J=0.1 mV g=5, delay 1.5 ms, nu_ext=2nu_thr, dt 0.1 ms, 1 s per step",
+                   "n_neurons": desc.n_neurons, "n_synapses": desc.n_synapses,
+                   "timestep_grouping": G, "groups_per_step": groups, "path": args.path},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "rate_hz": round(rate_hz, 3),
+        "events_per_step": events // max(1, args.steps),
+        "wall_s": round(wall, 3),
+    }
+    if cpu:
+        out["speedup_vs_cpu"] = round(value / cpu["value"], 2)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -89,6 +89,8 @@ typedef struct {
   int min_delay, max_delay;  /* in timesteps */
   int64_t n_synapses;
   int n_rules, n_monitors;
+  int partitions;            /* workgroups of the persistent kernel (neuron partitions) */
+  int fused;                 /* 1 when spk_run uses the persistent kernel */
 } spk_info;
 
 /* Per-run timing of the last spk_run call, from HIP events on the network's stream. */
@@ -180,6 +182,22 @@ int spk_prepare(spk_network* net, int timestep_grouping);
  * the value given to spk_prepare. */
 int spk_set_timestep_grouping(spk_network* net, int timestep_grouping);
 int spk_get_info(const spk_network* net, spk_info* info);
+/* Number of neuron partitions (= workgroups) of the persistent kernel; 0 picks
+ * min(CU count, ceil(N/8)).  Call before spk_prepare.  (No reference counterpart: it
+ * tunes the launch shape, like the reference's threads-per-block context parameter,
+ * Spike/Backend/Context.hpp:8-12.) */
+int spk_set_partitions(spk_network* net, int partitions);
+/* Phase profile of the persistent kernel (no reference counterpart; the reference has
+ * only a clock() timer around run(), Examples/Brunel10K.cpp:255-257).  When on, spk_run
+ * accumulates workgroup 0's wall-clock time per phase: 0 Poisson inputs and the LIF
+ * input read, 1 LIF recurrence and spike append, 2 publication, 3 barrier arrival and the
+ * next group's random draws, 4 barrier wait, 5 plasticity, 6 delivery heads, 7 count scan
+ * and entry staging, 8 synapse delivery.  Values in microseconds summed over the last run
+ * (up to 12 entries). */
+int spk_set_profiling(spk_network* net, int on);
+int spk_get_phase_profile(const spk_network* net, double* us, int count);
+/* Empty string when spk_run uses the persistent kernel, else the reason it cannot. */
+const char* spk_fused_status(const spk_network* net);
 
 /* reset_state of SpikingModel (SpikingModel.cpp:181-190): V=v0, last spike=-1000,
  * spike history, input ring, conductances and plasticity traces cleared, Poisson

@@ -47,7 +47,8 @@ class Info(C.Structure):
                 ("synapse_kind", C.c_int), ("timestep_grouping", C.c_int),
                 ("ring_slots", C.c_int), ("history_words", C.c_int), ("frac_bits", C.c_int),
                 ("min_delay", C.c_int), ("max_delay", C.c_int), ("n_synapses", C.c_int64),
-                ("n_rules", C.c_int), ("n_monitors", C.c_int)]
+                ("n_rules", C.c_int), ("n_monitors", C.c_int), ("partitions", C.c_int),
+                ("fused", C.c_int)]
 
 
 class Timing(C.Structure):
@@ -84,6 +85,10 @@ SIGNATURES = {
     "spk_add_monitor": (C.c_int, [P, C.c_int, C.c_int64, C.POINTER(C.c_int)]),
     "spk_prepare": (C.c_int, [P, C.c_int]),
     "spk_set_timestep_grouping": (C.c_int, [P, C.c_int]),
+    "spk_set_partitions": (C.c_int, [P, C.c_int]),
+    "spk_fused_status": (C.c_char_p, [P]),
+    "spk_set_profiling": (C.c_int, [P, C.c_int]),
+    "spk_get_phase_profile": (C.c_int, [P, C.POINTER(C.c_double), C.c_int]),
     "spk_get_info": (C.c_int, [P, C.POINTER(Info)]),
     "spk_reset_state": (C.c_int, [P]),
     "spk_reset_inputs": (C.c_int, [P]),
@@ -142,7 +147,7 @@ def check(rc: int):
 class Network:
     """A network on the GPU built from a NetDesc through the C ABI."""
 
-    def __init__(self, d: NetDesc, monitor_capacity: Optional[int] = None):
+    def __init__(self, d: NetDesc, monitor_capacity: Optional[int] = None, partitions: int = 0):
         lib = load()
         self.lib = lib
         self.desc = d
@@ -150,6 +155,8 @@ class Network:
         check(lib.spk_create(d.n_neurons, d.n_inputs, d.kind, d.n_labels, float(d.dt), C.byref(h)))
         self.h = h
         try:
+            if partitions:
+                check(lib.spk_set_partitions(h, int(partitions)))
             self._build(d, monitor_capacity)
         except Exception:
             lib.spk_destroy(self.h)
@@ -225,6 +232,19 @@ class Network:
 
     def reset_state(self):
         check(self.lib.spk_reset_state(self.h))
+
+    def set_profiling(self, on: bool = True):
+        check(self.lib.spk_set_profiling(self.h, 1 if on else 0))
+
+    def phase_profile(self):
+        out = (C.c_double * 12)()
+        check(self.lib.spk_get_phase_profile(self.h, out, 12))
+        names = ["inputs_lifinput", "lif", "publish", "arrive_draw", "wait", "plasticity",
+                 "dlv_heads", "dlv_scan_stage", "dlv_synapses", "s9", "s10", "s11"]
+        return dict(zip(names, list(out)))
+
+    def fused_status(self) -> str:
+        return self.lib.spk_fused_status(self.h).decode()
 
     def set_grouping(self, g: int):
         check(self.lib.spk_set_timestep_grouping(self.h, int(g)))

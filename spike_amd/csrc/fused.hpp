@@ -1,0 +1,677 @@
+// fused.hpp -- persistent single-network kernel: the whole timestep-group loop of
+// SpikingModel::run (SpikingModel.cpp:266-305) in one cooperative launch.
+//
+// Owner computes.  The LIF neurons are split into P contiguous partitions, one per
+// workgroup (the Poisson inputs likewise).  A workgroup keeps its neurons' slice of the
+// input ring in LDS for the whole launch and is the only writer of it, so delivery uses
+// LDS atomics and no global atomics.  Per timestep group:
+//   1. neuron phase: external Poisson counts and input spikes drawn in parallel over
+//      (neuron, sub-step); then one lane per neuron runs the G-step LIF recurrence
+//      (LIFSpikingNeurons.cu:91-195) on the LDS ring; spikes are compacted with a
+//      wavefront ballot into the workgroup's exchange list (id | g << 24);
+//   2. one grid barrier publishes every workgroup's list;
+//   3. plasticity of the plastic synapses whose postsynaptic neuron the workgroup owns
+//      (they are the only weights its own delivery reads);
+//   4. delivery: every spike's row segment that targets this partition
+//      (CSR rows sorted by (partition, delay)) is added into the LDS ring.
+// The arithmetic of every phase is identical to the per-phase kernels in kernels.hpp.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "kernels.hpp"
+#include "philox.hpp"
+
+#pragma clang fp contract(off)
+
+namespace spk {
+
+constexpr int kFusedBlock = 1024;
+constexpr uint32_t kIdMask = 0x00FFFFFFu;
+constexpr int kBarGroups = 8;          // level-1 barrier counters (blockIdx % 8)
+constexpr int kBarStride = 32;         // uints between counters (128 B)
+constexpr int kProfPhases = 12;
+constexpr int kXcds = 8;               // MI355X: 8 accelerator complex dies, blocks dispatched round-robin
+constexpr int kPrefetch = 16;          // sub-steps whose inputs are read ahead
+
+struct FusedArgs {
+  // partitions: neurons [nlo[p], nlo[p+1]), inputs [ilo[p], ilo[p+1])
+  const int* nlo;
+  const int* ilo;
+  // LIF state and constants
+  float* V;
+  float* last_n;
+  uint32_t* hist_n;
+  float* gcond;
+  int64_t* ring;            // global copy [B][N*L], loaded/stored at launch edges
+  const float* v0;
+  const float* vreset;
+  const float* thr;
+  const float* c;
+  const float* rp;
+  const float* decay_g;
+  const float* erev;
+  const uint32_t* ext_cdf;
+  // inputs
+  float* last_in;
+  uint32_t* hist_in;
+  const uint32_t* in_thr;   // selected stimulus
+  // synapses
+  const uint32_t* seg;      // [U][P+1] absolute segment offsets
+  const uint2* syn;         // per synapse {local target | delay << 20 | label << 28, weight bits}
+  const float* scale;       // nullptr: uniform scale0
+  // plasticity (rules grouped by postsynaptic partition)
+  StdpArgs rule[4];
+  int rule_kind[4];
+  const int64_t* rule_part[4];   // [P+1] offsets into the rule's synapse order
+  int n_rules;
+  // exchange blocks [2][P][xstride]: {count, entries (id | g << 24)...}
+  uint32_t* xlist;
+  // two-level grid barrier: counters bar[g * kBarStride] (g < 8), bar[8 * kBarStride]
+  // (level 2) and bar[9 * kBarStride] (release generation)
+  unsigned int* bar;
+  int* error;
+  // raster store: entries (unified id, step within launch)
+  uint32_t* rast_id;
+  int32_t* rast_step;
+  unsigned long long* rast_count;
+  int64_t rast_cap;
+  unsigned long long* stats;    // [0] deliveries [1] external [2] LIF spikes [3] input spikes
+  int64_t q_ext;
+  int64_t arm_step;
+  uint64_t seed;
+  long long spin_limit;
+  float scale0, two_k, inv_scale;
+  float start, dt, tref, ibg;
+  int64_t group0;             // index of the first group of this launch within the run
+  int N, Nin, L, G, B, W, P, xcap, xstride, nown_max, iown_max, ext_k, ngroups;
+  int plastic, record;
+  unsigned long long* prof;   // optional: wall-clock ticks per phase of workgroup 0
+};
+
+// Phase clock of workgroup 0 (no effect on the computation): thread 0 accumulates laps
+// in LDS and writes them once when the launch ends.
+struct PhaseClock {
+  unsigned long long t0;
+  unsigned long long* acc;   // LDS, kProfPhases entries
+  bool on;
+  __device__ void start(const FusedArgs& a, unsigned long long* lds) {
+    acc = lds;
+    on = a.prof != nullptr && blockIdx.x == 0 && threadIdx.x == 0;
+    if (on) {
+      for (int i = 0; i < kProfPhases; ++i) acc[i] = 0;
+      t0 = wall_clock64();
+    }
+  }
+  __device__ __forceinline__ void lap(const FusedArgs&, int phase) {
+    if (!on) return;
+    const unsigned long long t1 = wall_clock64();
+    acc[phase] += t1 - t0;
+    t0 = t1;
+  }
+  __device__ void flush(const FusedArgs& a) {
+    if (!on) return;
+    for (int i = 0; i < kProfPhases; ++i) a.prof[i] += acc[i];
+  }
+};
+
+// LDS layout of one workgroup: every sub-array starts on a 16-byte boundary (the input
+// spike masks take 64-bit LDS atomics, which need 8-byte aligned addresses).
+struct FusedLayout {
+  size_t ring, hist, ext, imask, inp, xbuf, head, scan, ent, counter, total;
+};
+
+constexpr int kHeadWords = 16;         // words of a source block fetched in the first trip
+constexpr int kEntCap = 4096;          // spike entries staged in LDS per delivery round
+constexpr int kQuad = 4;               // lanes cooperating on one spike's row segment
+constexpr int kSpikesPerQuad = 4;      // spikes a quad handles with its loads in flight together
+
+__host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
+
+__host__ __device__ inline FusedLayout fused_layout(int B, int L, int W, int G, int nown, int iown,
+                                                    int xcap, int P) {
+  FusedLayout f;
+  size_t o = 0;
+  f.ring = o;    o = align16(o + (size_t)B * nown * L * 8);
+  f.hist = o;    o = align16(o + (size_t)W * nown * 4);
+  f.ext = o;     o = align16(o + (size_t)2 * nown * G);       // double buffer
+  f.imask = o;   o = align16(o + (size_t)2 * iown * 8);       // double buffer
+  f.inp = o;     o = align16(o + (size_t)nown * G * L * 4);   // this group's inputs as float
+  f.xbuf = o;    o = align16(o + (size_t)xcap * 4);
+  f.head = o;    o = align16(o + (size_t)P * kHeadWords * 4);
+  f.scan = o;    o = align16(o + (size_t)(P + 1) * 4);
+  f.ent = o;     o = align16(o + (size_t)kEntCap * 4);
+  f.counter = o; o = align16(o + 4 * 4);
+  f.total = o;
+  return f;
+}
+
+__host__ __device__ inline size_t fused_lds_bytes(int B, int L, int W, int G, int nown, int iown,
+                                                  int xcap, int P) {
+  return fused_layout(B, L, W, G, nown, iown, xcap, P).total;
+}
+
+__host__ __device__ inline int fused_xstride(int xcap) { return ((1 + xcap + 7) / 8) * 8; }
+
+__device__ __forceinline__ unsigned int atomic_load_agent(const unsigned int* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Exchange data between workgroups is written and read with device-coherent (agent
+// scope) operations, which bypass the per-XCD L2's non-coherent copies.  The barrier then
+// needs no L2 write-back/invalidate: a wait for completed stores before the arrival
+// orders the data before the counter increment.
+__device__ __forceinline__ void store_coherent(uint32_t* p, uint32_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t load_coherent(const uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Barrier arrival (thread 0, after the workgroup's stores are complete): one increment of
+// this workgroup's level-1 counter; the last arrival of a level-1 group increments the
+// level-2 counter, and the last of those publishes generation s+1.  With plasticity the
+// rules read other partitions' state through ordinary loads, so arrival and departure
+// carry agent-scope release/acquire fences.
+__device__ __forceinline__ void barrier_arrive(const FusedArgs& a, int s) {
+  const int P = gridDim.x;
+  const int ngrp = P < kBarGroups ? P : kBarGroups;
+  const int grp = blockIdx.x % ngrp;
+  const unsigned int gsize = (unsigned int)((P - grp + ngrp - 1) / ngrp);
+  if (a.plastic) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  const unsigned int old = atomicAdd(a.bar + grp * kBarStride, 1u);
+  if (old + 1u == (unsigned int)(s + 1) * gsize) {
+    const unsigned int old2 = atomicAdd(a.bar + kBarGroups * kBarStride, 1u);
+    if (old2 + 1u == (unsigned int)(s + 1) * (unsigned int)ngrp)
+      __hip_atomic_store(a.bar + (kBarGroups + 1) * kBarStride, (unsigned int)(s + 1), __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// Barrier wait (thread 0): bounded spin on the release generation.
+__device__ __forceinline__ bool barrier_wait(const FusedArgs& a, int s) {
+  const unsigned int* rel = a.bar + (kBarGroups + 1) * kBarStride;
+  long long spins = 0;
+  while (atomic_load_agent(rel) < (unsigned int)(s + 1)) {
+    __builtin_amdgcn_s_sleep(1);
+    if (++spins > a.spin_limit ||
+        __hip_atomic_load(a.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+      atomicExch(a.error, 1);
+      return false;
+    }
+  }
+  if (a.plastic) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  return true;
+}
+
+__device__ __forceinline__ void group_time(const FusedArgs& a, int s, float& t, int& tidx) {
+  t = a.start + (float)((a.group0 + s) * (int64_t)a.G) * a.dt;
+  tidx = (int)roundf(t / a.dt);
+}
+
+// Random draws of group s: external Poisson counts per (neuron, sub-step) and the spike
+// masks of the Poisson inputs (Bernoulli per sub-step).  Independent of network state,
+// so they are computed while the workgroup waits at the previous group's barrier.
+__device__ __forceinline__ void draw_group(const FusedArgs& a, int s, uint8_t* ext,
+                                           unsigned long long* imask, int nlo, int nown, int ilo,
+                                           int iown, unsigned long long& st_ext) {
+  float t;
+  int tidx;
+  group_time(a, s, t, tidx);
+  const int G = a.G;
+  const int tid = threadIdx.x;
+  if (a.ext_k > 0) {
+    for (int k = tid; k < nown * G; k += kFusedBlock) {
+      const int i = k / G, g = k - i * G;
+      const uint32_t u = draw_u32(a.seed, (int64_t)tidx + g, (uint32_t)(nlo + i), kStreamExternal);
+      int cnt = 0;
+      for (int j = 0; j < a.ext_k; ++j) cnt += (u >= a.ext_cdf[j]) ? 1 : 0;
+      ext[k] = (uint8_t)cnt;
+      st_ext += (unsigned long long)cnt;
+    }
+  }
+  if (iown > 0) {
+    for (int k = tid; k < iown; k += kFusedBlock) imask[k] = 0ull;
+    __syncthreads();
+    for (int k = tid; k < iown * G; k += kFusedBlock) {
+      const int i = k / G, g = k - i * G;
+      const int64_t step = (int64_t)tidx + g;
+      const uint32_t u = draw_u32(a.seed, step, (uint32_t)(ilo + i), kStreamInput);
+      if ((u < a.in_thr[ilo + i]) && (step != a.arm_step)) atomicOr(&imask[i], 1ull << g);
+    }
+  }
+}
+
+// One quad's share of kSpikesPerQuad spikes: every lane loads its synapses of each spike's
+// segment (lane q takes s0+q, s0+q+4, ...) with all loads issued before the LDS atomics.
+template <bool PER_SYN_SCALE>
+__device__ __forceinline__ void deliver_quad(const FusedArgs& a, const uint32_t* ents, int n, int q,
+                                             int p, int slot0, int B, int nownL, int L, int64_t* sring,
+                                             unsigned long long& st_del) {
+  uint32_t s0[kSpikesPerQuad], s1[kSpikesPerQuad];
+  int gs[kSpikesPerQuad];
+#pragma unroll
+  for (int r = 0; r < kSpikesPerQuad; ++r) {
+    s0[r] = 0u;
+    s1[r] = 0u;
+    gs[r] = 0;
+    if (r < n) {
+      const uint32_t e = ents[r];
+      const size_t sb = (size_t)(e & kIdMask) * (a.P + 1) + p;
+      s0[r] = a.seg[sb];
+      s1[r] = a.seg[sb + 1];
+      gs[r] = slot0 + (int)(e >> 24);
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < kSpikesPerQuad; ++r)
+    if (q == 0) st_del += (unsigned long long)(s1[r] - s0[r]);
+  // first two synapses per lane of every segment, loads in flight together
+  uint2 rec[kSpikesPerQuad][2];
+  float scl[kSpikesPerQuad][2];
+#pragma unroll
+  for (int r = 0; r < kSpikesPerQuad; ++r) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const uint32_t k = s0[r] + (uint32_t)(q + h * kQuad);
+      rec[r][h] = make_uint2(0u, 0u);
+      scl[r][h] = a.scale0;
+      if (k < s1[r]) {
+        rec[r][h] = a.syn[k];
+        if (PER_SYN_SCALE) scl[r][h] = a.scale[k];
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < kSpikesPerQuad; ++r) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const uint32_t k = s0[r] + (uint32_t)(q + h * kQuad);
+      if (k < s1[r]) {
+        const uint32_t pk = rec[r][h].x;
+        const float wv = __uint_as_float(rec[r][h].y) * scl[r][h];
+        const long long qv = __float2ll_rn(wv * a.two_k);
+        int slot = gs[r] + (int)((pk >> 20) & 0xFFu);
+        while (slot >= B) slot -= B;
+        atomicAdd((unsigned long long*)&sring[(size_t)slot * nownL + (size_t)(pk & 0xFFFFFu) * L + (pk >> 28)],
+                  (unsigned long long)qv);
+      }
+    }
+    // longer segments: the rest, one synapse per lane per step
+    for (uint32_t k = s0[r] + (uint32_t)(q + 2 * kQuad); k < s1[r]; k += kQuad) {
+      const uint2 rc = a.syn[k];
+      const float wv = __uint_as_float(rc.y) * (PER_SYN_SCALE ? a.scale[k] : a.scale0);
+      const long long qv = __float2ll_rn(wv * a.two_k);
+      int slot = gs[r] + (int)((rc.x >> 20) & 0xFFu);
+      while (slot >= B) slot -= B;
+      atomicAdd((unsigned long long*)&sring[(size_t)slot * nownL + (size_t)(rc.x & 0xFFFFFu) * L + (rc.x >> 28)],
+                (unsigned long long)qv);
+    }
+  }
+}
+
+// Partition of a workgroup.  Workgroups are dispatched to the XCDs round-robin, so block
+// b runs on XCD b % 8; mapping the blocks of one XCD to consecutive partitions keeps that
+// XCD's synapse segments contiguous inside every CSR row (its L2 then fetches only its own
+// part of a spiking neuron's row instead of every line of it).
+__device__ __forceinline__ int partition_of_block(int b, int P) {
+  if (P % kXcds != 0) return b;
+  return (b % kXcds) * (P / kXcds) + b / kXcds;
+}
+
+template <int KIND, bool L1>
+__global__ __launch_bounds__(kFusedBlock) void k_fused(FusedArgs a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int p = partition_of_block(blockIdx.x, gridDim.x);
+  const int tid = threadIdx.x;
+  const int lane = (int)__lane_id();
+  const int nlo = a.nlo[p], nhi = a.nlo[p + 1];
+  const int ilo = a.ilo[p], ihi = a.ilo[p + 1];
+  const int nown = nhi - nlo, iown = ihi - ilo;
+  const int L = L1 ? 1 : a.L, G = a.G, B = a.B, W = a.W;
+  const FusedLayout f = fused_layout(B, L, W, G, a.nown_max, a.iown_max, a.xcap, a.P);
+  int64_t* sring = (int64_t*)(smem + f.ring);
+  uint32_t* shist = (uint32_t*)(smem + f.hist);
+  uint8_t* sext = (uint8_t*)(smem + f.ext);
+  unsigned long long* simask = (unsigned long long*)(smem + f.imask);
+  float* sinp = (float*)(smem + f.inp);
+  uint32_t* sxbuf = (uint32_t*)(smem + f.xbuf);
+  uint32_t* shead = (uint32_t*)(smem + f.head);
+  int* sscan = (int*)(smem + f.scan);
+  uint32_t* sent = (uint32_t*)(smem + f.ent);
+  int* scounter = (int*)(smem + f.counter);
+  __shared__ int s_ok;
+  __shared__ unsigned long long s_base;
+  __shared__ unsigned long long s_prof[kProfPhases];
+
+  const int NL = a.N * L;
+  const int nownL = nown * L;
+  // load the owned ring slice and spike history
+  for (int s = 0; s < B; ++s)
+    for (int j = tid; j < nownL; j += kFusedBlock)
+      sring[(size_t)s * nownL + j] = a.ring[(size_t)s * NL + (size_t)nlo * L + j];
+  for (int k = tid; k < W * nown; k += kFusedBlock) {
+    const int wd = k / nown, i = k - wd * nown;
+    shist[k] = a.hist_n[(size_t)wd * a.N + nlo + i];
+  }
+  // per-lane neuron state (lane i owns neuron nlo+i; nown <= kFusedBlock)
+  const bool own = tid < nown;
+  const int gi = nlo + (own ? tid : 0);
+  float V = 0.f, last = 0.f, cv0 = 0.f, omc = 0.f, cib = 0.f, thr = 0.f, vr = 0.f, rp = 0.f;
+  float gl[kMaxLabels];
+#pragma unroll
+  for (int l = 0; l < kMaxLabels; ++l) gl[l] = 0.f;
+  if (own) {
+    V = a.V[gi];
+    last = a.last_n[gi];
+    const float cc = a.c[gi];
+    cv0 = cc * a.v0[gi];
+    omc = 1.0f - cc;
+    cib = cc * a.ibg;
+    thr = a.thr[gi];
+    vr = a.vreset[gi];
+    rp = a.rp[gi];
+    if (KIND == 1)
+      for (int l = 0; l < L; ++l) gl[l] = a.gcond[(size_t)gi * L + l];
+  }
+  unsigned long long st_del = 0, st_ext = 0, st_spk = 0, st_in = 0;
+  const float half_dt = 0.5f * a.dt;
+  const int bits = W * 32;
+  const int nown_waves = ((nown + kWave - 1) / kWave) * kWave;
+  if (tid == 0) scounter[0] = 0;
+  draw_group(a, 0, sext, simask, nlo, nown, ilo, iown, st_ext);
+  __syncthreads();
+  PhaseClock clk;
+  clk.start(a, s_prof);
+
+  for (int s = 0; s < a.ngroups; ++s) {
+    float t;
+    int tidx;
+    group_time(a, s, t, tidx);
+    const int par = s & 1;
+    const uint8_t* ext = sext + (size_t)par * a.nown_max * G;
+    const unsigned long long* imask = simask + (size_t)par * a.iown_max;
+    const int b0 = tidx % bits;
+    const int slot0 = tidx % B;
+    // -- A. Poisson inputs: append this group's spikes, update histories ------------
+    for (int i = tid; i < iown; i += kFusedBlock) {
+      const unsigned long long m = imask[i];
+      const int c = __popcll(m);
+      int pos = c ? atomicAdd(scounter, c) : 0;
+      unsigned long long mm = m;
+      while (mm) {
+        const int g = __ffsll((unsigned long long)mm) - 1;
+        mm &= mm - 1ull;
+        if (pos < a.xcap) sxbuf[pos] = ((uint32_t)g << 24) | (uint32_t)(a.N + ilo + i);
+        ++pos;
+      }
+      for (int g = 0; g < G; ++g) {
+        int b = b0 + g;
+        if (b >= bits) b -= bits;
+        uint32_t* wp = a.hist_in + (size_t)(b >> 5) * a.Nin + ilo + i;
+        uint32_t word = *wp & ~(1u << (b & 31));
+        if ((m >> g) & 1ull) word |= (1u << (b & 31));
+        *wp = word;
+      }
+      if (m) a.last_in[ilo + i] = t + (float)(63 - __clzll((long long)m)) * a.dt;
+      st_in += (unsigned long long)c;
+    }
+    // -- B1. this group's synaptic input per (neuron, sub-step[, label]) as float ------
+    // The ring slots of the group's steps are complete (every delay is >= G): read and
+    // clear them in parallel.  Voltage synapses keep the label sum, conductance synapses
+    // the per-label values.
+    for (int k = tid; k < nown * G; k += kFusedBlock) {
+      const int i = k / G, g = k - i * G;
+      int slot = slot0 + g;
+      if (slot >= B) slot -= B;
+      int64_t* rs = sring + (size_t)slot * nownL + (size_t)i * L;
+      const int64_t ext_q = (a.ext_k > 0) ? (int64_t)ext[i * G + g] * a.q_ext : 0;
+      if (KIND == 0) {
+        float total = 0.0f;
+        for (int l = 0; l < L; ++l) {
+          int64_t q = rs[l];
+          if (l == 0) q += ext_q;
+          rs[l] = 0;
+          total += (float)q * a.inv_scale;
+        }
+        sinp[k] = total;
+      } else {
+        for (int l = 0; l < L; ++l) {
+          int64_t q = rs[l];
+          if (l == 0) q += ext_q;
+          rs[l] = 0;
+          sinp[(size_t)k * L + l] = (float)q * a.inv_scale;
+        }
+      }
+    }
+    __syncthreads();
+    clk.lap(a, 0);
+    // -- B2. LIF recurrence, one lane per neuron (LIFSpikingNeurons.cu:115-190) -------
+    if (tid < nown_waves) {
+      unsigned long long smask = 0ull;
+      if (own) {
+        for (int g0 = 0; g0 < G; g0 += kPrefetch) {
+          const int gn = min(kPrefetch, G - g0);
+          if (KIND == 0) {
+            float dvs[kPrefetch];
+#pragma unroll
+            for (int x = 0; x < kPrefetch; ++x) dvs[x] = (x < gn) ? sinp[tid * G + g0 + x] : 0.0f;
+#pragma unroll
+            for (int x = 0; x < kPrefetch; ++x) {
+              if (x < gn) {
+                const int g = g0 + x;
+                const float tg = t + (float)g * a.dt;
+                const bool ready = ((tg - last) - a.tref) > half_dt;
+                const float vn = ((cv0 + omc * V) + cib) + dvs[x];
+                const bool fire = ready && (vn >= thr);
+                V = ready ? (fire ? vr : vn) : V;
+                last = fire ? tg : last;
+                if (fire) smask |= 1ull << g;
+              }
+            }
+          } else {
+            for (int x = 0; x < gn; ++x) {
+              const int g = g0 + x;
+              float total = 0.0f;
+#pragma unroll
+              for (int l = 0; l < kMaxLabels; ++l) {
+                if (l < L) {
+                  const float in = sinp[((size_t)tid * G + g) * L + l];
+                  gl[l] = a.decay_g[l] * gl[l] + in;
+                  total += gl[l] * (a.erev[l] - V);
+                }
+              }
+              const float dV = total * rp;
+              const float tg = t + (float)g * a.dt;
+              if (((tg - last) - a.tref) > half_dt) {
+                V = ((cv0 + omc * V) + cib) + dV;
+                if (V >= thr) {
+                  last = tg;
+                  V = vr;
+                  smask |= 1ull << g;
+                }
+              }
+            }
+          }
+        }
+        // spike history: clear this group's G bits, set those of the spikes
+        int cur_w = -1;
+        uint32_t word = 0;
+        for (int g = 0; g < G; ++g) {
+          int b = b0 + g;
+          if (b >= bits) b -= bits;
+          const int wd = b >> 5;
+          if (wd != cur_w) {
+            if (cur_w >= 0) shist[cur_w * nown + tid] = word;
+            word = shist[wd * nown + tid];
+            cur_w = wd;
+          }
+          word &= ~(1u << (b & 31));
+          if ((smask >> g) & 1ull) word |= (1u << (b & 31));
+        }
+        if (cur_w >= 0) shist[cur_w * nown + tid] = word;
+      }
+      if (smask) {
+        const int c = __popcll(smask);
+        int pos = atomicAdd(scounter, c);
+        st_spk += (unsigned long long)c;
+        unsigned long long mm = smask;
+        while (mm) {
+          const int g = __ffsll((unsigned long long)mm) - 1;
+          mm &= mm - 1ull;
+          if (pos < a.xcap) sxbuf[pos] = ((uint32_t)g << 24) | (uint32_t)(nlo + tid);
+          ++pos;
+        }
+      }
+    }
+    __syncthreads();
+    clk.lap(a, 1);
+    // -- C. publish: {count, entries} block, and state other workgroups may read ------
+    const int nx = min(scounter[0], a.xcap);
+    uint32_t* xl = a.xlist + ((size_t)par * a.P + p) * a.xstride;
+    for (int k = tid; k < nx; k += kFusedBlock) store_coherent(xl + 1 + k, sxbuf[k]);
+    if (tid == 0) store_coherent(xl, (uint32_t)nx);
+    if (a.plastic) {
+      if (own) a.last_n[gi] = last;
+      for (int k = tid; k < W * nown; k += kFusedBlock) {
+        const int wd = k / nown, i = k - wd * nown;
+        a.hist_n[(size_t)wd * a.N + nlo + i] = shist[k];
+      }
+    }
+    if (a.record && nx > 0) {
+      if (tid == 0) s_base = atomicAdd(a.rast_count, (unsigned long long)nx);
+      __syncthreads();
+      const unsigned long long base = s_base;
+      for (int k = tid; k < nx; k += kFusedBlock) {
+        const unsigned long long pos = base + k;
+        if ((int64_t)pos < a.rast_cap) {
+          const uint32_t e = sxbuf[k];
+          a.rast_id[pos] = e & kIdMask;
+          a.rast_step[pos] = s * G + (int)(e >> 24);
+        }
+      }
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    clk.lap(a, 2);
+    // -- D. barrier: arrive, draw the next group while others finish, wait ----------
+    if (tid == 0) {
+      barrier_arrive(a, s);
+      scounter[0] = 0;
+    }
+    if (s + 1 < a.ngroups)
+      draw_group(a, s + 1, sext + (size_t)(par ^ 1) * a.nown_max * G,
+                 simask + (size_t)(par ^ 1) * a.iown_max, nlo, nown, ilo, iown, st_ext);
+    clk.lap(a, 3);
+    if (tid == 0) s_ok = barrier_wait(a, s) ? 1 : 0;
+    __syncthreads();
+    if (!s_ok) return;
+    clk.lap(a, 4);
+    // -- E. plasticity of the synapses onto this partition ---------------------------
+    if (a.plastic) {
+      for (int r = 0; r < a.n_rules; ++r) {
+        StdpArgs ra = a.rule[r];
+        ra.tidx = tidx;
+        ra.t = t;
+        const int64_t j0 = a.rule_part[r][p], j1 = a.rule_part[r][p + 1];
+        for (int64_t j = j0 + tid; j < j1; j += kFusedBlock) {
+          if (a.rule_kind[r] == 0) stdp_wd_one(ra, j);
+          else if (a.rule_kind[r] == 1) stdp_vogels_one(ra, j);
+          else stdp_evans_one(ra, j);
+        }
+      }
+      __syncthreads();
+    }
+    clk.lap(a, 5);
+    // -- F. delivery into the LDS ring ------------------------------------------------
+    {
+      const uint32_t* xbase = a.xlist + (size_t)par * a.P * a.xstride;
+      // (1) heads of all source blocks: count and first entries, one coalesced trip
+      for (int k = tid; k < a.P * kHeadWords; k += kFusedBlock) {
+        const int src = k / kHeadWords, wd = k - src * kHeadWords;
+        shead[k] = (wd <= a.xcap) ? load_coherent(xbase + (size_t)src * a.xstride + wd) : 0u;
+      }
+      __syncthreads();
+      clk.lap(a, 6);
+      // (2) wavefront 0: exclusive scan of the counts (each lane a run of sources)
+      const int cps = (a.P + kWave - 1) / kWave;
+      if (tid < kWave) {
+        int run = 0;
+        for (int k = 0; k < cps; ++k) {
+          const int src = tid * cps + k;
+          if (src < a.P) run += min((int)shead[src * kHeadWords], a.xcap);
+        }
+        int incl = run;
+#pragma unroll
+        for (int off = 1; off < kWave; off <<= 1) {
+          const int y = __shfl_up(incl, off);
+          if (lane >= off) incl += y;
+        }
+        int off = incl - run;
+        for (int k = 0; k < cps; ++k) {
+          const int src = tid * cps + k;
+          if (src < a.P) {
+            sscan[src] = off;
+            off += min((int)shead[src * kHeadWords], a.xcap);
+          }
+        }
+        if (tid == kWave - 1) sscan[a.P] = incl;
+      }
+      __syncthreads();
+      const int total = sscan[a.P];
+      for (int r0 = 0; r0 < total; r0 += kEntCap) {
+        // (3) stage entries [r0, r0 + kEntCap) of the concatenated lists
+        for (int src = tid; src < a.P; src += kFusedBlock) {
+          const int cnt = min((int)shead[src * kHeadWords], a.xcap);
+          const int off = sscan[src];
+          const int k_lo = max(0, r0 - off), k_hi = min(cnt, r0 + kEntCap - off);
+          for (int k = k_lo; k < k_hi; ++k) {
+            const uint32_t e = (k + 1 < kHeadWords) ? shead[src * kHeadWords + 1 + k]
+                                                    : load_coherent(xbase + (size_t)src * a.xstride + 1 + k);
+            sent[off + k - r0] = e;
+          }
+        }
+        __syncthreads();
+        clk.lap(a, 7);
+        // (4) quads of lanes over the spikes, kSpikesPerQuad at a time
+        const int nr = min(kEntCap, total - r0);
+        const int quad = tid / kQuad, q = tid % kQuad;
+        const int nquads = kFusedBlock / kQuad;
+        for (int j0 = quad * kSpikesPerQuad; j0 < nr; j0 += nquads * kSpikesPerQuad) {
+          const int n = min(kSpikesPerQuad, nr - j0);
+          if (a.scale == nullptr)
+            deliver_quad<false>(a, sent + j0, n, q, p, slot0, B, nownL, L, sring, st_del);
+          else
+            deliver_quad<true>(a, sent + j0, n, q, p, slot0, B, nownL, L, sring, st_del);
+        }
+        __syncthreads();
+        clk.lap(a, 8);
+      }
+    }
+  }
+  clk.flush(a);
+  // write back state
+  if (own) {
+    a.V[gi] = V;
+    a.last_n[gi] = last;
+    if (KIND == 1)
+      for (int l = 0; l < L; ++l) a.gcond[(size_t)gi * L + l] = gl[l];
+  }
+  for (int s = 0; s < B; ++s)
+    for (int j = tid; j < nownL; j += kFusedBlock)
+      a.ring[(size_t)s * NL + (size_t)nlo * L + j] = sring[(size_t)s * nownL + j];
+  for (int k = tid; k < W * nown; k += kFusedBlock) {
+    const int wd = k / nown, i = k - wd * nown;
+    a.hist_n[(size_t)wd * a.N + nlo + i] = shist[k];
+  }
+  // statistics: one atomic per wavefront
+  unsigned long long v4[4] = {st_del, st_ext, st_spk, st_in};
+#pragma unroll
+  for (int qq = 0; qq < 4; ++qq) {
+    unsigned long long v = v4[qq];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    if (__lane_id() == 0 && v) atomicAdd(a.stats + qq, v);
+  }
+}
+
+}  // namespace spk

@@ -14,6 +14,9 @@
 namespace spk {
 
 constexpr int kWave = 64;
+// Synapse records are {packed target/delay/label, weight} pairs: weight k is w[k * kWStride]
+// of a float view offset by one word.
+constexpr int kWStride = 2;
 constexpr int kMaxLabels = 8;
 constexpr int kMaxExtTable = 64;
 
@@ -36,13 +39,17 @@ __device__ __forceinline__ bool hist_get(const uint32_t* __restrict__ hist, int 
 // Append `id` to list (when `pred`) with one atomic per wavefront.  Every lane of the
 // wavefront must call this (the ballot spans the whole wavefront).
 __device__ __forceinline__ void wave_append(bool pred, uint32_t id, uint32_t* __restrict__ list,
-                                            int* __restrict__ count, int cap) {
+                                            int* __restrict__ count, int cap,
+                                            unsigned long long* __restrict__ stat) {
   const uint64_t m = __ballot(pred);
   if (m == 0) return;
   const int lane = (int)__lane_id();
   const int leader = __ffsll((unsigned long long)m) - 1;
   int base = 0;
-  if (lane == leader) base = atomicAdd(count, __popcll(m));
+  if (lane == leader) {
+    base = atomicAdd(count, __popcll(m));
+    atomicAdd(stat, (unsigned long long)__popcll(m));
+  }
   base = __shfl(base, leader);
   if (pred) {
     const int pos = base + __popcll(m & ((1ull << lane) - 1ull));
@@ -83,6 +90,7 @@ struct LifArgs {
   uint32_t* lists;            // [G][cap]
   int* counts;                // [G]
   unsigned long long* ext_events;
+  unsigned long long* spike_stat;
   int N, L, G, B, W, cap, ext_k, tidx;
   float t, dt, tref, ibg, inv_scale;
   int64_t q_ext;              // fixed-point value of one external event
@@ -171,7 +179,7 @@ __global__ __launch_bounds__(256) void k_lif(LifArgs a) {
     }
     spk = spk && active;
     if (spk) word |= (1u << (b & 31));
-    wave_append(spk, (uint32_t)i, a.lists + (size_t)g * a.cap, a.counts + g, a.cap);
+    wave_append(spk, (uint32_t)i, a.lists + (size_t)g * a.cap, a.counts + g, a.cap, a.spike_stat);
   }
   if (active) {
     if (cur_w >= 0) a.hist[(size_t)cur_w * a.N + i] = word;
@@ -198,6 +206,7 @@ struct PoissonArgs {
   const uint32_t* thr;   // [Nin] of the selected stimulus
   uint32_t* lists;
   int* counts;
+  unsigned long long* spike_stat;
   int N, Nin, G, W, cap, tidx;
   int64_t arm_step;
   float t, dt;
@@ -228,7 +237,8 @@ __global__ __launch_bounds__(256) void k_poisson(PoissonArgs a) {
       word |= (1u << (b & 31));
       last = a.t + (float)g * a.dt;
     }
-    wave_append(spk, (uint32_t)(a.N + i), a.lists + (size_t)g * a.cap, a.counts + g, a.cap);
+    wave_append(spk, (uint32_t)(a.N + i), a.lists + (size_t)g * a.cap, a.counts + g, a.cap,
+                a.spike_stat);
   }
   if (active) {
     if (cur_w >= 0) a.hist[(size_t)cur_w * a.Nin + i] = word;
@@ -274,7 +284,7 @@ __global__ __launch_bounds__(256) void k_propagate(PropArgs a) {
         const int post = a.tgt[s];
         const int d = a.delay[s];
         const float sc = a.scale ? a.scale[s] : a.scale0;
-        const float wv = a.w[s] * sc;
+        const float wv = a.w[(size_t)s * kWStride] * sc;
         const long long q = __float2ll_rn(wv * a.two_k);
         const int slot = (base_slot + d + g) % a.B;
         const int l = a.label ? (int)a.label[s] : 0;
@@ -310,9 +320,7 @@ struct StdpArgs {
 
 // WeightDependentSTDPPlasticity.cu:67-163. tab = {am, ap, lambda, alpha, wmax,
 // Em[0..G], Ep[0..G], iEm[0..G], iEp[0..G]}
-__global__ __launch_bounds__(256) void k_stdp_wd(StdpArgs a) {
-  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= a.P) return;
+__device__ __forceinline__ void stdp_wd_one(const StdpArgs& a, int64_t j) {
   const int64_t s = a.pos[j];
   const int u = a.pre[j];
   const int post = a.tgt[s];
@@ -327,7 +335,7 @@ __global__ __launch_bounds__(256) void k_stdp_wd(StdpArgs a) {
   int pg = (int)roundf((lpre - a.t) / a.dt);
   const int qg = (int)roundf((a.last_n[post] - a.t) / a.dt);
   if (pg >= G) pg *= -1;
-  const float w = a.w[s];
+  const float w = a.w[(size_t)s * kWStride];
   xpost *= Em[G];
   xpre *= Ep[G];
   if (a.nearest) {
@@ -352,9 +360,14 @@ __global__ __launch_bounds__(256) void k_stdp_wd(StdpArgs a) {
   }
   float nw = w + upd;
   if (nw < 0.0f) nw = 0.0f;
-  a.w[s] = nw;
+  a.w[(size_t)s * kWStride] = nw;
   a.ta[j] = xpre;
   a.tb[j] = xpost;
+}
+
+__global__ __launch_bounds__(256) void k_stdp_wd(StdpArgs a) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < a.P) stdp_wd_one(a, j);
 }
 
 __device__ __forceinline__ bool pre_bit(const StdpArgs& a, int u, int64_t step) {
@@ -363,16 +376,14 @@ __device__ __forceinline__ bool pre_bit(const StdpArgs& a, int u, int64_t step) 
 }
 
 // InhibitorySTDPPlasticity.cu:238-321. tab = {decay, lr}; dterm = lr*(2*rate*tau) (double)
-__global__ __launch_bounds__(256) void k_stdp_vogels(StdpArgs a) {
-  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= a.P) return;
+__device__ __forceinline__ void stdp_vogels_one(const StdpArgs& a, int64_t j) {
   const int64_t s = a.pos[j];
   const int u = a.pre[j];
   const int post = a.tgt[s];
   const int d = a.delay[s];
   const float decay = a.tab[0], lr = a.tab[1];
   float xpre = a.ta[j], xpost = a.tb[j];
-  float w = a.w[s];
+  float w = a.w[(size_t)s * kWStride];
   for (int g = 0; g < a.G; ++g) {
     xpre *= decay;
     xpost *= decay;
@@ -390,22 +401,25 @@ __global__ __launch_bounds__(256) void k_stdp_vogels(StdpArgs a) {
     w += upd;
     if (w < 0.0f) w = 0.0f;
   }
-  a.w[s] = w;
+  a.w[(size_t)s * kWStride] = w;
   a.ta[j] = xpre;
   a.tb[j] = xpost;
 }
 
-// EvansSTDPPlasticity.cu:59-143. tab = {dC, dD, aC, aD}
-__global__ __launch_bounds__(256) void k_stdp_evans(StdpArgs a) {
+__global__ __launch_bounds__(256) void k_stdp_vogels(StdpArgs a) {
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= a.P) return;
+  if (j < a.P) stdp_vogels_one(a, j);
+}
+
+// EvansSTDPPlasticity.cu:59-143. tab = {dC, dD, aC, aD}
+__device__ __forceinline__ void stdp_evans_one(const StdpArgs& a, int64_t j) {
   const int64_t s = a.pos[j];
   const int u = a.pre[j];
   const int post = a.tgt[s];
   const int d = a.delay[s];
   const float dC = a.tab[0], dD = a.tab[1], aC = a.tab[2], aD = a.tab[3];
   float C = a.ta[j], D = a.tb[j];
-  float w = a.w[s];
+  float w = a.w[(size_t)s * kWStride];
   for (int g = 0; g < a.G; ++g) {
     C = dC * C;
     D = dD * D;
@@ -422,9 +436,14 @@ __global__ __launch_bounds__(256) void k_stdp_evans(StdpArgs a) {
     if (w < 0.0f) w = 0.0f;
     if (w > 1.0f) w = 1.0f;
   }
-  a.w[s] = w;
+  a.w[(size_t)s * kWStride] = w;
   a.ta[j] = C;
   a.tb[j] = D;
+}
+
+__global__ __launch_bounds__(256) void k_stdp_evans(StdpArgs a) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < a.P) stdp_evans_one(a, j);
 }
 
 // ---------------------------------------------------------------------------------

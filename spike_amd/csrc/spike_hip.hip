@@ -15,6 +15,7 @@
 
 #include "../../include/spike_hip.h"
 #include "kernels.hpp"
+#include "fused.hpp"
 
 #pragma clang fp contract(off)
 
@@ -97,6 +98,9 @@ struct RuleHost {
   float* d_tab = nullptr;
   int tab_G = -1;
   double dterm = 0.0;
+  std::vector<int64_t> order;     // device order j -> index into syn (grouped by post partition)
+  std::vector<int64_t> part;      // [P+1] offsets of each postsynaptic partition
+  int64_t* d_part = nullptr;
 };
 
 struct MonitorHost {
@@ -154,7 +158,7 @@ struct spk_network {
   uint32_t* d_ext_cdf = nullptr;
   int64_t* d_rowptr = nullptr;
   int32_t* d_tgt = nullptr;
-  float* d_w = nullptr;
+  uint2* d_syn = nullptr;   // per synapse {packed local target/delay/label, weight bits}
   float* d_scale = nullptr;
   uint16_t* d_delay = nullptr;
   uint8_t* d_label = nullptr;
@@ -165,6 +169,26 @@ struct spk_network {
   float* d_slot_time = nullptr;
   int* d_slot_tidx = nullptr;
   unsigned long long* d_stats = nullptr;  // [0] events, [1] external events
+  // fused persistent path
+  int P = 1, nown_max = 0, iown_max = 0, xcap = 0;
+  int req_partitions = 0;
+  bool fused_ok = false;
+  std::string fused_why;
+  size_t fused_lds = 0;
+  int cu_count = 0;
+  std::vector<int> h_nlo, h_ilo;
+  int *d_nlo = nullptr, *d_ilo = nullptr;
+  uint32_t* d_seg = nullptr;
+  uint32_t* d_xlist = nullptr;
+  unsigned int* d_bar = nullptr;
+  int* d_err = nullptr;
+  uint32_t* d_rast_id = nullptr;
+  int32_t* d_rast_step = nullptr;
+  unsigned long long* d_rast_count = nullptr;
+  int64_t rast_cap = 0;
+  unsigned long long* d_prof = nullptr;
+  bool profiling = false;
+  double prof_us[spk::kProfPhases] = {0};
   // per-phase bookkeeping
   int cur_tidx = -1;
   int cur_slot = 0;
@@ -177,6 +201,9 @@ struct spk_network {
 };
 
 namespace {
+
+// Weight k of the synapse records is at float index 2k+1.
+float* weights_view(spk_network* n) { return reinterpret_cast<float*>(n->d_syn) + 1; }
 
 int check_net(const spk_network* n) {
   if (!n) return fail(SPK_ERR_ARG, "null network");
@@ -193,12 +220,15 @@ void free_device(spk_network* n) {
   dfree(n->d_V); dfree(n->d_v0); dfree(n->d_vreset); dfree(n->d_thr); dfree(n->d_c);
   dfree(n->d_rp); dfree(n->d_last_n); dfree(n->d_last_in); dfree(n->d_hist_n);
   dfree(n->d_hist_in); dfree(n->d_ring); dfree(n->d_g); dfree(n->d_decay); dfree(n->d_erev);
-  dfree(n->d_in_thr); dfree(n->d_ext_cdf); dfree(n->d_rowptr); dfree(n->d_tgt); dfree(n->d_w);
+  dfree(n->d_in_thr); dfree(n->d_ext_cdf); dfree(n->d_rowptr); dfree(n->d_tgt); dfree(n->d_syn);
   dfree(n->d_scale); dfree(n->d_delay); dfree(n->d_label); dfree(n->d_lists);
   dfree(n->d_counts); dfree(n->d_slot_time); dfree(n->d_slot_tidx); dfree(n->d_stats);
   for (auto& r : n->rules) {
-    dfree(r.d_pos); dfree(r.d_pre); dfree(r.d_ta); dfree(r.d_tb); dfree(r.d_tab);
+    dfree(r.d_pos); dfree(r.d_pre); dfree(r.d_ta); dfree(r.d_tb); dfree(r.d_tab); dfree(r.d_part);
   }
+  dfree(n->d_nlo); dfree(n->d_ilo); dfree(n->d_seg); dfree(n->d_xlist);
+  dfree(n->d_bar); dfree(n->d_err); dfree(n->d_rast_id);
+  dfree(n->d_rast_step); dfree(n->d_rast_count); dfree(n->d_prof);
   for (auto& m : n->mons) {
     dfree(m.d_ids); dfree(m.d_times); dfree(m.d_count);
   }
@@ -272,6 +302,7 @@ int launch_neurons(spk_network* n, float t, int slot) {
   a.decay_g = n->d_decay; a.erev = n->d_erev; a.ext_cdf = n->d_ext_cdf;
   a.lists = slot_lists(n, slot); a.counts = slot_counts(n, slot);
   a.ext_events = n->d_stats + 1;
+  a.spike_stat = n->d_stats + 2;
   a.N = n->N; a.L = n->L; a.G = n->G; a.B = n->B; a.W = n->W; a.cap = n->cap;
   a.ext_k = (int)n->ext_cdf.size(); a.tidx = (int)roundf(t / n->dt);
   a.t = t; a.dt = n->dt; a.tref = n->tref; a.ibg = n->ibg;
@@ -297,6 +328,7 @@ int launch_inputs(spk_network* n, float t, int slot) {
   a.last = n->d_last_in; a.hist = n->d_hist_in;
   a.thr = n->d_in_thr + (size_t)n->stim * n->Nin;
   a.lists = slot_lists(n, slot); a.counts = slot_counts(n, slot);
+  a.spike_stat = n->d_stats + 3;
   a.N = n->N; a.Nin = n->Nin; a.G = n->G; a.W = n->W; a.cap = n->cap; a.tidx = tidx;
   a.arm_step = n->arm_step; a.t = t; a.dt = n->dt; a.seed = n->seed;
   hipLaunchKernelGGL(spk::k_poisson, dim3(grid_for(n->Nin, 256)), dim3(256), 0, n->stream, a);
@@ -310,7 +342,7 @@ int launch_rule(spk_network* n, int rid, float t) {
   int rc = refresh_rule_tables(n, r);
   if (rc) return rc;
   spk::StdpArgs a{};
-  a.pos = r.d_pos; a.pre = r.d_pre; a.tgt = n->d_tgt; a.delay = n->d_delay; a.w = n->d_w;
+  a.pos = r.d_pos; a.pre = r.d_pre; a.tgt = n->d_tgt; a.delay = n->d_delay; a.w = weights_view(n);
   a.ta = r.d_ta; a.tb = r.d_tb; a.last_n = n->d_last_n; a.last_in = n->d_last_in;
   a.hist_n = n->d_hist_n; a.hist_in = n->d_hist_in; a.tab = r.d_tab; a.dterm = r.dterm;
   a.P = (int64_t)r.syn.size(); a.N = n->N; a.Nin = n->Nin; a.G = n->G; a.W = n->W;
@@ -327,7 +359,7 @@ int launch_propagate(spk_network* n, float t, int slot) {
   if (n->S == 0) return SPK_OK;
   spk::PropArgs a{};
   a.lists = slot_lists(n, slot); a.counts = slot_counts(n, slot);
-  a.rowptr = n->d_rowptr; a.tgt = n->d_tgt; a.w = n->d_w;
+  a.rowptr = n->d_rowptr; a.tgt = n->d_tgt; a.w = weights_view(n);
   a.scale = n->uniform_scale ? nullptr : n->d_scale;
   a.delay = n->d_delay; a.label = (n->L > 1) ? n->d_label : nullptr;
   a.ring = n->d_ring; a.events = n->d_stats;
@@ -415,6 +447,165 @@ int download(spk_network* n, T* out, const T* d, size_t cnt) {
   if (!out) return fail(SPK_ERR_ARG, "null output");
   SPK_HIP(hipMemcpyAsync(out, d, cnt * sizeof(T), hipMemcpyDeviceToHost, n->stream));
   SPK_HIP(hipStreamSynchronize(n->stream));
+  return SPK_OK;
+}
+
+
+// The persistent kernel instance for a network (synapse kind, single label or not).
+const void* fused_kernel(const spk_network* n) {
+  const bool l1 = n->L == 1;
+  if (n->kind == SPK_SYN_VOLTAGE)
+    return l1 ? (const void*)spk::k_fused<0, true> : (const void*)spk::k_fused<0, false>;
+  return l1 ? (const void*)spk::k_fused<1, true> : (const void*)spk::k_fused<1, false>;
+}
+
+// The persistent kernel over n_groups groups from `start`, in chunks of bounded length.
+int run_fused(spk_network* n, float start, int64_t ngroups, int flags) {
+  const bool plastic = (flags & SPK_RUN_PLASTICITY) != 0 && !n->rules.empty();
+  for (auto& r : n->rules)
+    if (int rc = refresh_rule_tables(n, r)) return rc;
+  const int U = n->nu();
+  const int G = n->G;
+  const bool record = !n->mons.empty();
+  const int64_t per_group = (int64_t)G * U;
+  int64_t chunk = 1000;  // bounds the length of one launch
+  if (record) {
+    const int64_t want = std::min<int64_t>((int64_t)1 << 24, std::max<int64_t>(per_group * 8, per_group));
+    if (n->rast_cap < want) {
+      dfree(n->d_rast_id);
+      dfree(n->d_rast_step);
+      int rc;
+      if ((rc = dalloc(&n->d_rast_id, (size_t)want))) return rc;
+      if ((rc = dalloc(&n->d_rast_step, (size_t)want))) return rc;
+      if (!n->d_rast_count && (rc = dalloc(&n->d_rast_count, 1))) return rc;
+      n->rast_cap = want;
+    }
+    chunk = std::max<int64_t>(1, std::min<int64_t>(chunk, n->rast_cap / per_group));
+  }
+  if (n->Nin > 0 && n->pending_arm) {
+    n->arm_step = (int)roundf(start / n->dt);
+    n->pending_arm = false;
+  }
+  spk::FusedArgs a{};
+  a.nlo = n->d_nlo; a.ilo = n->d_ilo;
+  a.V = n->d_V; a.last_n = n->d_last_n; a.hist_n = n->d_hist_n; a.gcond = n->d_g; a.ring = n->d_ring;
+  a.v0 = n->d_v0; a.vreset = n->d_vreset; a.thr = n->d_thr; a.c = n->d_c; a.rp = n->d_rp;
+  a.decay_g = n->d_decay; a.erev = n->d_erev; a.ext_cdf = n->d_ext_cdf;
+  a.last_in = n->d_last_in; a.hist_in = n->d_hist_in;
+  a.in_thr = n->d_in_thr + (size_t)n->stim * n->Nin;
+  a.seg = n->d_seg; a.syn = n->d_syn;
+  a.scale = n->uniform_scale ? nullptr : n->d_scale;
+  a.n_rules = plastic ? (int)n->rules.size() : 0;
+  if (a.n_rules > 4) return fail(SPK_ERR_ARG, "the persistent kernel takes at most 4 plasticity rules");
+  for (int r = 0; r < a.n_rules; ++r) {
+    RuleHost& rh = n->rules[r];
+    spk::StdpArgs& ra = a.rule[r];
+    ra.pos = rh.d_pos; ra.pre = rh.d_pre; ra.tgt = n->d_tgt; ra.delay = n->d_delay; ra.w = weights_view(n);
+    ra.ta = rh.d_ta; ra.tb = rh.d_tb; ra.last_n = n->d_last_n; ra.last_in = n->d_last_in;
+    ra.hist_n = n->d_hist_n; ra.hist_in = n->d_hist_in; ra.tab = rh.d_tab; ra.dterm = rh.dterm;
+    ra.P = (int64_t)rh.syn.size(); ra.N = n->N; ra.Nin = n->Nin; ra.G = G; ra.W = n->W;
+    ra.nearest = rh.wd.nearest_spike_only; ra.dt = n->dt;
+    a.rule_kind[r] = rh.kind;
+    a.rule_part[r] = rh.d_part;
+  }
+  a.xlist = n->d_xlist; a.bar = n->d_bar; a.error = n->d_err;
+  a.rast_id = n->d_rast_id; a.rast_step = n->d_rast_step; a.rast_count = n->d_rast_count;
+  a.rast_cap = n->rast_cap;
+  a.stats = n->d_stats;
+  a.q_ext = n->q_ext; a.arm_step = n->Nin > 0 ? n->arm_step : -1; a.seed = n->seed;
+  a.spin_limit = (long long)1 << 22;
+  a.scale0 = n->scale0; a.two_k = ldexpf(1.0f, n->frac); a.inv_scale = ldexpf(1.0f, -n->frac);
+  a.start = start; a.dt = n->dt; a.tref = n->tref; a.ibg = n->ibg;
+  a.N = n->N; a.Nin = n->Nin; a.L = n->L; a.G = G; a.B = n->B; a.W = n->W; a.P = n->P;
+  a.xcap = n->xcap; a.xstride = spk::fused_xstride(n->xcap);
+  a.nown_max = n->nown_max; a.iown_max = n->iown_max;
+  a.ext_k = (int)n->ext_cdf.size();
+  a.plastic = plastic ? 1 : 0;
+  a.record = record ? 1 : 0;
+  a.prof = nullptr;
+  if (n->profiling) {
+    if (!n->d_prof) {
+      if (int rc = dalloc(&n->d_prof, spk::kProfPhases)) return rc;
+    }
+    SPK_HIP(hipMemsetAsync(n->d_prof, 0, sizeof(unsigned long long) * spk::kProfPhases, n->stream));
+    a.prof = n->d_prof;
+  }
+  const void* fn = fused_kernel(n);
+  if (n->fused_lds > 65536)
+    SPK_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)n->fused_lds));
+  const int64_t nlaunch = (ngroups + chunk - 1) / chunk;
+  while ((int64_t)n->ev_kernel.size() < 2 * nlaunch) {
+    hipEvent_t e;
+    SPK_HIP(hipEventCreate(&e));
+    n->ev_kernel.push_back(e);
+  }
+  std::vector<uint32_t> rid;
+  std::vector<int32_t> rstep;
+  int64_t li = 0;
+  for (int64_t s0 = 0; s0 < ngroups; s0 += chunk, ++li) {
+    const int ng = (int)std::min<int64_t>(chunk, ngroups - s0);
+    a.group0 = s0;
+    a.ngroups = ng;
+    SPK_HIP(hipMemsetAsync(n->d_bar, 0, sizeof(unsigned int) * (spk::kBarGroups + 2) * spk::kBarStride, n->stream));
+    SPK_HIP(hipMemsetAsync(n->d_err, 0, sizeof(int), n->stream));
+    if (record) SPK_HIP(hipMemsetAsync(n->d_rast_count, 0, sizeof(unsigned long long), n->stream));
+    void* kargs[] = {&a};
+    SPK_HIP(hipEventRecord(n->ev_kernel[2 * li], n->stream));
+    SPK_HIP(hipLaunchCooperativeKernel(fn, dim3(n->P), dim3(spk::kFusedBlock), kargs,
+                                       (unsigned int)n->fused_lds, n->stream));
+    SPK_HIP(hipEventRecord(n->ev_kernel[2 * li + 1], n->stream));
+    if (record) {
+      unsigned long long cnt = 0;
+      SPK_HIP(hipMemcpyAsync(&cnt, n->d_rast_count, sizeof(cnt), hipMemcpyDeviceToHost, n->stream));
+      SPK_HIP(hipStreamSynchronize(n->stream));
+      if ((int64_t)cnt > n->rast_cap) return fail(SPK_ERR_CAPACITY, "raster store overflow");
+      rid.resize(cnt);
+      rstep.resize(cnt);
+      if (cnt) {
+        SPK_HIP(hipMemcpyAsync(rid.data(), n->d_rast_id, cnt * 4, hipMemcpyDeviceToHost, n->stream));
+        SPK_HIP(hipMemcpyAsync(rstep.data(), n->d_rast_step, cnt * 4, hipMemcpyDeviceToHost, n->stream));
+        SPK_HIP(hipStreamSynchronize(n->stream));
+      }
+      std::vector<size_t> ord(cnt);
+      for (size_t i = 0; i < cnt; ++i) ord[i] = i;
+      std::sort(ord.begin(), ord.end(), [&](size_t x, size_t y) {
+        if (rstep[x] != rstep[y]) return rstep[x] < rstep[y];
+        return rid[x] < rid[y];
+      });
+      for (auto& m : n->mons) {
+        const uint32_t lo = m.pop == SPK_POP_NEURONS ? 0u : (uint32_t)n->N;
+        const uint32_t hi = m.pop == SPK_POP_NEURONS ? (uint32_t)n->N : (uint32_t)U;
+        for (size_t i : ord) {
+          if (rid[i] < lo || rid[i] >= hi) continue;
+          const int64_t S = s0 + rstep[i] / G;
+          const int g = rstep[i] % G;
+          const float t = start + (float)(S * G) * n->dt;
+          m.h_ids.push_back((int32_t)(rid[i] - lo));
+          m.h_times.push_back(t + (float)g * n->dt);
+        }
+      }
+    }
+  }
+  SPK_HIP(hipStreamSynchronize(n->stream));
+  int err = 0;
+  SPK_HIP(hipMemcpy(&err, n->d_err, sizeof(int), hipMemcpyDeviceToHost));
+  if (err) return fail(SPK_ERR_HIP, "persistent kernel: grid barrier wait exceeded its spin limit");
+  double kms = 0.0;
+  for (int64_t i = 0; i < nlaunch; ++i) {
+    float ms = 0.0f;
+    SPK_HIP(hipEventElapsedTime(&ms, n->ev_kernel[2 * i], n->ev_kernel[2 * i + 1]));
+    kms += ms;
+  }
+  n->timing.kernel_ms = kms;
+  n->timing.kernel_launches = nlaunch;
+  if (n->profiling) {
+    unsigned long long ticks[spk::kProfPhases];
+    SPK_HIP(hipMemcpy(ticks, n->d_prof, sizeof(ticks), hipMemcpyDeviceToHost));
+    int rate_khz = 0;
+    SPK_HIP(hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, n->device));
+    for (int i = 0; i < spk::kProfPhases; ++i)
+      n->prof_us[i] = rate_khz > 0 ? (double)ticks[i] * 1e3 / (double)rate_khz : 0.0;
+  }
   return SPK_OK;
 }
 
@@ -679,7 +870,35 @@ int spk_prepare(spk_network* n, int G) {
   if (!n->ext_cdf.empty()) mx += (double)n->ext_cdf.size() * std::fabs((double)n->ext_w);
   n->frac = choose_frac_bits(mx);
   n->q_ext = n->ext_cdf.empty() ? 0 : (int64_t)std::llrint(std::ldexp((double)n->ext_w, n->frac));
-  // CSR by unified presynaptic id, rows sorted by delay (stable in caller order)
+  // ---- partitions of the persistent kernel: contiguous neuron / input ranges ----
+  {
+    int dev_cu = 0;
+    hipDeviceProp_t prop;
+    SPK_HIP(hipGetDeviceProperties(&prop, n->device));
+    dev_cu = prop.multiProcessorCount;
+    n->cu_count = dev_cu;
+    int P = n->req_partitions > 0 ? n->req_partitions : std::min(dev_cu, std::max(1, (n->N + 7) / 8));
+    P = std::max(1, std::min(P, std::max(1, n->N)));
+    n->P = P;
+    n->h_nlo.assign(P + 1, 0);
+    n->h_ilo.assign(P + 1, 0);
+    for (int q = 0; q <= P; ++q) {
+      n->h_nlo[q] = (int)((int64_t)n->N * q / P);
+      n->h_ilo[q] = (int)((int64_t)n->Nin * q / P);
+    }
+    n->nown_max = 0;
+    n->iown_max = 0;
+    for (int q = 0; q < P; ++q) {
+      n->nown_max = std::max(n->nown_max, n->h_nlo[q + 1] - n->h_nlo[q]);
+      n->iown_max = std::max(n->iown_max, n->h_ilo[q + 1] - n->h_ilo[q]);
+    }
+    n->xcap = std::max(1, (n->nown_max + n->iown_max) * G);
+  }
+  std::vector<int> owner(n->N, 0);
+  for (int q = 0; q < n->P; ++q)
+    for (int i = n->h_nlo[q]; i < n->h_nlo[q + 1]; ++i) owner[i] = q;
+  // ---- CSR by unified presynaptic id; each row sorted by (postsynaptic partition,
+  //      delay, caller order) so that a partition's synapses form one segment ----
   const int U = n->nu();
   std::vector<int64_t> rowptr(U + 1, 0);
   auto uid = [&](int32_t pre) { return pre >= 0 ? pre : n->N + (-pre - 1); };
@@ -692,7 +911,11 @@ int spk_prepare(spk_network* n, int G) {
   }
   for (int u = 0; u < U; ++u) {
     auto b = order.begin() + rowptr[u], e = order.begin() + rowptr[u + 1];
-    std::stable_sort(b, e, [&](int64_t x, int64_t y) { return n->h_delay[x] < n->h_delay[y]; });
+    std::stable_sort(b, e, [&](int64_t x, int64_t y) {
+      const int ox = owner[n->h_post[x]], oy = owner[n->h_post[y]];
+      if (ox != oy) return ox < oy;
+      return n->h_delay[x] < n->h_delay[y];
+    });
   }
   n->perm.assign(S, 0);
   std::vector<int32_t> tgt(S);
@@ -708,6 +931,40 @@ int spk_prepare(spk_network* n, int G) {
     dl[k] = (uint16_t)n->h_delay[s];
     lb[k] = (uint8_t)n->h_label[s];
   }
+  // ---- eligibility of the persistent kernel ----
+  n->fused_ok = true;
+  n->fused_why.clear();
+  auto no_fused = [&](const char* why) { if (n->fused_ok) { n->fused_ok = false; n->fused_why = why; } };
+  if (n->N < 1) no_fused("no LIF neurons");
+  if (n->nown_max > spk::kFusedBlock) no_fused("more than 256 neurons per partition");
+  if (G > 64) no_fused("timestep grouping above 64");
+  if (n->dmax > 255) no_fused("delays above 255 steps");
+  if (n->L > 15) no_fused("more than 15 labels");
+  if ((int64_t)U >= (int64_t)spk::kIdMask) no_fused("more than 2^24 neurons");
+  if (S >= ((int64_t)1 << 32)) no_fused("2^32 or more synapses");
+  if (n->P > spk::kFusedBlock) no_fused("too many partitions");
+  n->fused_lds = spk::fused_lds_bytes(n->B, n->L, n->W, G, n->nown_max, n->iown_max, n->xcap, n->P);
+  {
+    hipDeviceProp_t prop;
+    SPK_HIP(hipGetDeviceProperties(&prop, n->device));
+    if (n->fused_lds > (size_t)prop.sharedMemPerBlock) no_fused("input ring slice does not fit in LDS");
+  }
+  std::vector<uint32_t> seg, packed;
+  if (n->fused_ok) {
+    seg.assign((size_t)U * (n->P + 1), 0);
+    for (int u = 0; u < U; ++u) {
+      int64_t k = rowptr[u];
+      for (int q = 0; q <= n->P; ++q) {
+        while (k < rowptr[u + 1] && owner[tgt[k]] < q) ++k;
+        seg[(size_t)u * (n->P + 1) + q] = (uint32_t)k;
+      }
+    }
+    packed.resize(S);
+    for (int64_t k = 0; k < S; ++k) {
+      const uint32_t loc = (uint32_t)(tgt[k] - n->h_nlo[owner[tgt[k]]]);
+      packed[k] = loc | ((uint32_t)dl[k] << 20) | ((uint32_t)lb[k] << 28);
+    }
+  }
   // list capacity: every neuron of both populations may spike once per sub-step
   n->cap = std::max(1, U);
   n->slots = 1;
@@ -722,7 +979,7 @@ int spk_prepare(spk_network* n, int G) {
   ALLOC(n->d_decay, L); ALLOC(n->d_erev, L);
   ALLOC(n->d_in_thr, (size_t)std::max(1, n->n_stim) * Nin);
   ALLOC(n->d_ext_cdf, std::max<size_t>(1, n->ext_cdf.size()));
-  ALLOC(n->d_rowptr, (size_t)U + 1); ALLOC(n->d_tgt, S); ALLOC(n->d_w, S);
+  ALLOC(n->d_rowptr, (size_t)U + 1); ALLOC(n->d_tgt, S); ALLOC(n->d_syn, S);
   ALLOC(n->d_delay, S);
   if (!n->uniform_scale) ALLOC(n->d_scale, S);
   if (n->L > 1) ALLOC(n->d_label, S);
@@ -730,6 +987,12 @@ int spk_prepare(spk_network* n, int G) {
   ALLOC(n->d_counts, (size_t)n->slots * G);
   ALLOC(n->d_slot_time, n->slots); ALLOC(n->d_slot_tidx, n->slots);
   ALLOC(n->d_stats, 4);
+  if (n->fused_ok) {
+    ALLOC(n->d_nlo, (size_t)n->P + 1); ALLOC(n->d_ilo, (size_t)n->P + 1);
+    ALLOC(n->d_seg, seg.size());
+    ALLOC(n->d_xlist, (size_t)2 * n->P * spk::fused_xstride(n->xcap));
+    ALLOC(n->d_bar, (size_t)(spk::kBarGroups + 2) * spk::kBarStride); ALLOC(n->d_err, 1);
+  }
 #undef ALLOC
   hipStream_t st = n->stream;
   if ((rc = upload(n->d_v0, n->v0.data(), N, st))) return rc;
@@ -747,27 +1010,51 @@ int spk_prepare(spk_network* n, int G) {
   if ((rc = upload(n->d_ext_cdf, n->ext_cdf.data(), n->ext_cdf.size(), st))) return rc;
   if ((rc = upload(n->d_rowptr, rowptr.data(), (size_t)U + 1, st))) return rc;
   if ((rc = upload(n->d_tgt, tgt.data(), S, st))) return rc;
-  if ((rc = upload(n->d_w, w.data(), S, st))) return rc;
+  {
+    std::vector<uint2> rec(S);
+    for (int64_t k = 0; k < S; ++k) {
+      rec[k].x = packed.empty() ? 0u : packed[k];
+      std::memcpy(&rec[k].y, &w[k], sizeof(float));
+    }
+    if ((rc = upload(n->d_syn, rec.data(), S, st))) return rc;
+  }
   if ((rc = upload(n->d_delay, dl.data(), S, st))) return rc;
   if (n->d_scale && (rc = upload(n->d_scale, sc.data(), S, st))) return rc;
   if (n->d_label && (rc = upload(n->d_label, lb.data(), S, st))) return rc;
+  if (n->fused_ok) {
+    if ((rc = upload(n->d_nlo, n->h_nlo.data(), (size_t)n->P + 1, st))) return rc;
+    if ((rc = upload(n->d_ilo, n->h_ilo.data(), (size_t)n->P + 1, st))) return rc;
+    if ((rc = upload(n->d_seg, seg.data(), seg.size(), st))) return rc;
+    SPK_HIP(hipMemsetAsync(n->d_xlist, 0, sizeof(uint32_t) * 2 * n->P * spk::fused_xstride(n->xcap), st));
+  }
   SPK_HIP(hipMemsetAsync(n->d_counts, 0, sizeof(int) * n->slots * G, st));
   SPK_HIP(hipMemsetAsync(n->d_stats, 0, sizeof(unsigned long long) * 4, st));
-  // rules
+  // rules: synapses grouped by the partition of their postsynaptic neuron
   for (auto& r : n->rules) {
     const size_t P = r.syn.size();
+    r.order.resize(P);
+    for (size_t j = 0; j < P; ++j) r.order[j] = (int64_t)j;
+    std::stable_sort(r.order.begin(), r.order.end(), [&](int64_t x, int64_t y) {
+      return owner[n->h_post[r.syn[x]]] < owner[n->h_post[r.syn[y]]];
+    });
+    r.part.assign(n->P + 1, 0);
+    for (size_t j = 0; j < P; ++j) r.part[owner[n->h_post[r.syn[r.order[j]]]] + 1]++;
+    for (int q = 0; q < n->P; ++q) r.part[q + 1] += r.part[q];
     if ((rc = dalloc(&r.d_pos, P))) return rc;
     if ((rc = dalloc(&r.d_pre, P))) return rc;
     if ((rc = dalloc(&r.d_ta, P))) return rc;
     if ((rc = dalloc(&r.d_tb, P))) return rc;
+    if ((rc = dalloc(&r.d_part, (size_t)n->P + 1))) return rc;
     std::vector<int64_t> pos(P);
     std::vector<int32_t> pre(P);
     for (size_t j = 0; j < P; ++j) {
-      pos[j] = n->perm[r.syn[j]];
-      pre[j] = uid(n->h_pre[r.syn[j]]);
+      const int64_t sidx = r.syn[r.order[j]];
+      pos[j] = n->perm[sidx];
+      pre[j] = uid(n->h_pre[sidx]);
     }
     if ((rc = upload(r.d_pos, pos.data(), P, st))) return rc;
     if ((rc = upload(r.d_pre, pre.data(), P, st))) return rc;
+    if ((rc = upload(r.d_part, r.part.data(), (size_t)n->P + 1, st))) return rc;
   }
   // monitors: device store of the requested capacity
   for (auto& m : n->mons) {
@@ -777,8 +1064,22 @@ int spk_prepare(spk_network* n, int G) {
     SPK_HIP(hipMemsetAsync(m.d_count, 0, sizeof(unsigned long long), st));
   }
   SPK_HIP(hipStreamSynchronize(st));
+  // the persistent grid must be co-resident: one workgroup per partition
+  if (n->fused_ok) {
+    int per_cu = 0;
+    SPK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fused_kernel(n), spk::kFusedBlock, n->fused_lds));
+    if ((int64_t)per_cu * n->cu_count < n->P) no_fused("partitions exceed co-resident workgroups");
+  }
   n->prepared = true;
   return spk_reset_state(n);
+}
+
+int spk_set_partitions(spk_network* n, int partitions) {
+  if (int rc = check_net(n)) return rc;
+  if (n->prepared) return fail(SPK_ERR_STATE, "network already prepared");
+  if (partitions < 0 || partitions > spk::kFusedBlock) return fail(SPK_ERR_ARG, "bad partition count");
+  n->req_partitions = partitions;
+  return SPK_OK;
 }
 
 int spk_set_timestep_grouping(spk_network* n, int G) {
@@ -806,6 +1107,8 @@ int spk_get_info(const spk_network* n, spk_info* info) {
   info->n_synapses = n->S;
   info->n_rules = (int)n->rules.size();
   info->n_monitors = (int)n->mons.size();
+  info->partitions = n->P;
+  info->fused = n->fused_ok ? 1 : 0;
   return SPK_OK;
 }
 
@@ -881,35 +1184,41 @@ int spk_run(spk_network* n, float start, int64_t ngroups, int flags) {
   if (ngroups < 0) return fail(SPK_ERR_ARG, "negative group count");
   std::memset(&n->timing, 0, sizeof(n->timing));
   SPK_HIP(hipMemsetAsync(n->d_stats, 0, sizeof(unsigned long long) * 4, n->stream));
-  // each monitor store must hold one group of its population
-  for (auto& m : n->mons) {
-    const int64_t pop = m.pop == SPK_POP_NEURONS ? n->N : n->Nin;
-    if (int rc = ensure_monitor_capacity(n, m, std::max<int64_t>(m.capacity, pop * n->G * 64))) return rc;
-  }
+  const bool use_fused = n->fused_ok && !(flags & SPK_RUN_FORCE_LAUNCHES) && n->G == n->Gprep &&
+                         (!(flags & SPK_RUN_PLASTICITY) || n->rules.size() <= 4);
   SPK_HIP(hipEventRecord(n->ev_a, n->stream));
-  const bool plastic = (flags & SPK_RUN_PLASTICITY) != 0;
-  int since_drain = 0;
-  for (int64_t s = 0; s < ngroups; ++s) {
-    const float t = start + (float)(s * n->G) * n->dt;
-    int rc;
-    if ((rc = begin_group(n, t, true))) return rc;
-    if ((rc = launch_neurons(n, t, n->cur_slot))) return rc;
-    if ((rc = launch_inputs(n, t, n->cur_slot))) return rc;
-    if (plastic)
-      for (int r = 0; r < (int)n->rules.size(); ++r)
-        if ((rc = launch_rule(n, r, t))) return rc;
-    if ((rc = launch_propagate(n, t, n->cur_slot))) return rc;
-    for (int m = 0; m < (int)n->mons.size(); ++m)
-      if ((rc = launch_collect(n, m, n->cur_slot, 1))) return rc;
-    if (!n->mons.empty() && ++since_drain == 64) {
-      since_drain = 0;
-      for (auto& m : n->mons)
-        if ((rc = drain_monitor(n, m, m.h_ids, m.h_times))) return rc;
+  if (use_fused) {
+    if (int rc = run_fused(n, start, ngroups, flags)) return rc;
+  } else {
+    // each monitor store must hold 64 groups of its population
+    for (auto& m : n->mons) {
+      const int64_t pop = m.pop == SPK_POP_NEURONS ? n->N : n->Nin;
+      if (int rc = ensure_monitor_capacity(n, m, std::max<int64_t>(m.capacity, pop * n->G * 64))) return rc;
     }
+    const bool plastic = (flags & SPK_RUN_PLASTICITY) != 0;
+    int since_drain = 0;
+    for (int64_t s = 0; s < ngroups; ++s) {
+      const float t = start + (float)(s * n->G) * n->dt;
+      int rc;
+      if ((rc = begin_group(n, t, true))) return rc;
+      if ((rc = launch_neurons(n, t, n->cur_slot))) return rc;
+      if ((rc = launch_inputs(n, t, n->cur_slot))) return rc;
+      if (plastic)
+        for (int r = 0; r < (int)n->rules.size(); ++r)
+          if ((rc = launch_rule(n, r, t))) return rc;
+      if ((rc = launch_propagate(n, t, n->cur_slot))) return rc;
+      for (int m = 0; m < (int)n->mons.size(); ++m)
+        if ((rc = launch_collect(n, m, n->cur_slot, 1))) return rc;
+      if (!n->mons.empty() && ++since_drain == 64) {
+        since_drain = 0;
+        for (auto& m : n->mons)
+          if ((rc = drain_monitor(n, m, m.h_ids, m.h_times))) return rc;
+      }
+    }
+    for (auto& m : n->mons)
+      if (int rc = drain_monitor(n, m, m.h_ids, m.h_times)) return rc;
   }
   SPK_HIP(hipEventRecord(n->ev_b, n->stream));
-  for (auto& m : n->mons)
-    if (int rc = drain_monitor(n, m, m.h_ids, m.h_times)) return rc;
   SPK_HIP(hipEventSynchronize(n->ev_b));
   float ms = 0.0f;
   SPK_HIP(hipEventElapsedTime(&ms, n->ev_a, n->ev_b));
@@ -919,8 +1228,28 @@ int spk_run(spk_network* n, float start, int64_t ngroups, int flags) {
   n->timing.groups = ngroups;
   n->timing.synaptic_events = (int64_t)st[0];
   n->timing.external_events = (int64_t)st[1];
+  n->timing.spikes_neurons = (int64_t)st[2];
+  n->timing.spikes_inputs = (int64_t)st[3];
   n->cur_tidx = -1;
   return SPK_OK;
+}
+
+int spk_set_profiling(spk_network* n, int on) {
+  if (int rc = check_net(n)) return rc;
+  n->profiling = on != 0;
+  return SPK_OK;
+}
+
+int spk_get_phase_profile(const spk_network* n, double* us, int count) {
+  if (int rc = check_net(n)) return rc;
+  if (!us || count < 0) return fail(SPK_ERR_ARG, "bad output");
+  for (int i = 0; i < count && i < spk::kProfPhases; ++i) us[i] = n->prof_us[i];
+  return SPK_OK;
+}
+
+const char* spk_fused_status(const spk_network* n) {
+  if (!n) return "null network";
+  return n->fused_ok ? "" : n->fused_why.c_str();
 }
 
 int spk_last_timing(const spk_network* n, spk_timing* t) {
@@ -980,19 +1309,21 @@ int spk_monitor_take(spk_network* n, int mid, int32_t* ids, float* times, int64_
 int spk_get_weights(spk_network* n, float* out) {
   if (int rc = check_prepared(n)) return rc;
   if (!out && n->S) return fail(SPK_ERR_ARG, "null output");
-  std::vector<float> w(n->S);
-  if (n->S) SPK_HIP(hipMemcpyAsync(w.data(), n->d_w, n->S * sizeof(float), hipMemcpyDeviceToHost, n->stream));
+  std::vector<uint2> rec(n->S);
+  if (n->S) SPK_HIP(hipMemcpyAsync(rec.data(), n->d_syn, n->S * sizeof(uint2), hipMemcpyDeviceToHost, n->stream));
   SPK_HIP(hipStreamSynchronize(n->stream));
-  for (int64_t s = 0; s < n->S; ++s) out[s] = w[n->perm[s]];
+  for (int64_t s = 0; s < n->S; ++s) std::memcpy(&out[s], &rec[n->perm[s]].y, sizeof(float));
   return SPK_OK;
 }
 
 int spk_set_weights(spk_network* n, const float* in) {
   if (int rc = check_prepared(n)) return rc;
   if (!in && n->S) return fail(SPK_ERR_ARG, "null input");
-  std::vector<float> w(n->S);
-  for (int64_t s = 0; s < n->S; ++s) w[n->perm[s]] = in[s];
-  if (n->S) SPK_HIP(hipMemcpyAsync(n->d_w, w.data(), n->S * sizeof(float), hipMemcpyHostToDevice, n->stream));
+  std::vector<uint2> rec(n->S);
+  if (n->S) SPK_HIP(hipMemcpyAsync(rec.data(), n->d_syn, n->S * sizeof(uint2), hipMemcpyDeviceToHost, n->stream));
+  SPK_HIP(hipStreamSynchronize(n->stream));
+  for (int64_t s = 0; s < n->S; ++s) std::memcpy(&rec[n->perm[s]].y, &in[s], sizeof(float));
+  if (n->S) SPK_HIP(hipMemcpyAsync(n->d_syn, rec.data(), n->S * sizeof(uint2), hipMemcpyHostToDevice, n->stream));
   SPK_HIP(hipStreamSynchronize(n->stream));
   return SPK_OK;
 }
@@ -1028,8 +1359,15 @@ int spk_get_traces(spk_network* n, int rid, float* a, float* b) {
   if (int rc = check_prepared(n)) return rc;
   if (rid < 0 || rid >= (int)n->rules.size()) return fail(SPK_ERR_ARG, "rule id out of range");
   RuleHost& r = n->rules[rid];
-  if (int rc = download(n, a, r.d_ta, r.syn.size())) return rc;
-  return download(n, b, r.d_tb, r.syn.size());
+  const size_t P = r.syn.size();
+  std::vector<float> ta(P), tb(P);
+  if (int rc = download(n, ta.data(), r.d_ta, P)) return rc;
+  if (int rc = download(n, tb.data(), r.d_tb, P)) return rc;
+  for (size_t j = 0; j < P; ++j) {
+    a[r.order[j]] = ta[j];
+    b[r.order[j]] = tb[j];
+  }
+  return SPK_OK;
 }
 
 }  // extern "C"

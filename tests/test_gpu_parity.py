@@ -38,16 +38,18 @@ def _compare(gpu, ora, d, tag=""):
             f"{tag} traces of rule {r}"
 
 
-def _run_both(d, ngroups, fused=True, start=0.0):
+def _run_both(d, ngroups, fused=True, start=0.0, partitions=0, launches=False, expect_fused=None):
     print(f"[parity] {d.name} N={d.n_neurons} Nin={d.n_inputs} S={d.n_synapses} G={d.timestep_grouping()} "
-          f"groups={ngroups} fused={fused}", flush=True)
-    gpu = abi.Network(d)
+          f"groups={ngroups} fused={fused} P={partitions} launches={launches}", flush=True)
+    gpu = abi.Network(d, partitions=partitions)
+    if expect_fused is not None:
+        assert gpu.info().fused == int(expect_fused), gpu.fused_status()
     ora = oracle_py.OracleNetwork(d, precision=32)
     gi, oi = gpu.info(), ora.info()
     assert (gi.timestep_grouping, gi.ring_slots, gi.history_words, gi.frac_bits) == \
         (oi["G"], oi["B"], oi["W"], oi["frac"])
     if fused:
-        gpu.run(start, ngroups)
+        gpu.run(start, ngroups, force_launches=launches)
         ora.run(start, ngroups)
     else:
         for s in range(ngroups):
@@ -63,6 +65,37 @@ def test_voltage_random(seed, fused):
     d = builders.random_net(seed=seed, input_rate=400.0, whi=4e-3)
     gpu, ora = _run_both(d, 300, fused=fused)
     _compare(gpu, ora, d, f"voltage seed={seed}")
+
+
+@pytest.mark.parametrize("partitions", [1, 3, 8])
+def test_persistent_kernel_partitions(partitions):
+    d = builders.random_net(seed=12, input_rate=400.0, whi=4e-3, dmin=2, dmax=7, ext_rate=2000.0,
+                            ext_w=1e-3)
+    gpu, ora = _run_both(d, 150, partitions=partitions, expect_fused=True)
+    assert gpu.info().partitions == partitions
+    _compare(gpu, ora, d, f"persistent P={partitions}")
+
+
+def test_launch_path_through_run():
+    d = builders.random_net(seed=13, input_rate=400.0, whi=4e-3, dmin=2, dmax=7)
+    gpu, ora = _run_both(d, 150, launches=True)
+    _compare(gpu, ora, d, "launch path")
+
+
+def test_run_in_pieces_matches_one_run():
+    """Two spk_run calls continue the state exactly (chunk boundaries and restarts)."""
+    d = builders.random_net(seed=14, input_rate=500.0, whi=4e-3, dmin=3, dmax=8)
+    gpu = abi.Network(d)
+    ora = oracle_py.OracleNetwork(d, precision=32)
+    gpu.run(0.0, 70)
+    t = float(d.group_time(0.0, 70))
+    gpu.run(t, 130, force_launches=True)
+    t2 = float(d.group_time(t, 130))
+    gpu.run(t2, 1500)
+    ora.run(0.0, 70)
+    ora.run(t, 130)
+    ora.run(t2, 1500)
+    _compare(gpu, ora, d, "pieces")
 
 
 def test_voltage_grouped_external():
@@ -85,7 +118,7 @@ def test_conductance_random(labels):
 def test_brunel_small_1000_steps():
     d = builders.brunel(n_exc=800, n_inh=200, seed=3, monitors=(0,))
     ng = 1000 // d.timestep_grouping() + 1
-    gpu, ora = _run_both(d, ng)
+    gpu, ora = _run_both(d, ng, expect_fused=True)
     _compare(gpu, ora, d, "brunel small")
 
 
@@ -150,7 +183,7 @@ def _plastic_net(kind, seed):
 @pytest.mark.parametrize("fused", [True, False])
 def test_plasticity(kind, fused):
     d = _plastic_net(kind, 30 + kind)
-    gpu, ora = _run_both(d, 200, fused=fused)
+    gpu, ora = _run_both(d, 200, fused=fused, partitions=5)
     _compare(gpu, ora, d, f"rule {kind}")
 
 
