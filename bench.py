@@ -57,6 +57,8 @@ def main():
                     help="timestep groups of the CPU-baseline sample (15 steps each)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="also run a plain device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -66,6 +68,18 @@ def main():
 
     import numpy as np
     from spike_amd import abi, builders
+
+    if args.calib_copy_mib:
+        # A plain copy kernel of known size: rocprof's FETCH_SIZE / WRITE_SIZE for it give
+        # the factor between the counters and the bytes a kernel really moves.
+        import torch
+        n = args.calib_copy_mib * (1 << 20) // 4
+        x = torch.ones(n, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        del x, y
 
     desc = builders.brunel(seed=42, form=args.form)
     G = desc.timestep_grouping()

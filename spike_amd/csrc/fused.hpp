@@ -34,6 +34,13 @@ constexpr int kProfPhases = 12;
 constexpr int kXcds = 8;               // MI355X: 8 accelerator complex dies, blocks dispatched round-robin
 constexpr int kPrefetch = 16;          // sub-steps whose inputs are read ahead
 
+// One plasticity rule of the persistent kernel.
+struct FusedRule {
+  StdpArgs args;
+  const int64_t* part;   // [P+1] offsets of each partition in the rule's synapse order
+  int kind;
+};
+
 struct FusedArgs {
   // partitions: neurons [nlo[p], nlo[p+1]), inputs [ilo[p], ilo[p+1])
   const int* nlo;
@@ -60,10 +67,8 @@ struct FusedArgs {
   const uint32_t* seg;      // [U][P+1] absolute segment offsets
   const uint2* syn;         // per synapse {local target | delay << 20 | label << 28, weight bits}
   const float* scale;       // nullptr: uniform scale0
-  // plasticity (rules grouped by postsynaptic partition)
-  StdpArgs rule[4];
-  int rule_kind[4];
-  const int64_t* rule_part[4];   // [P+1] offsets into the rule's synapse order
+  // plasticity (rules grouped by postsynaptic partition), descriptors in device memory
+  const FusedRule* rules;
   int n_rules;
   // exchange blocks [2][P][xstride]: {count, entries (id | g << 24)...}
   uint32_t* xlist;
@@ -310,6 +315,23 @@ __device__ __forceinline__ void deliver_quad(const FusedArgs& a, const uint32_t*
   }
 }
 
+// Plasticity of the synapses onto partition p (one group).
+__device__ __forceinline__ void fused_plasticity(const FusedRule* rules, int n_rules, int p, int tidx,
+                                              float t) {
+  for (int r = 0; r < n_rules; ++r) {
+    StdpArgs ra = rules[r].args;
+    ra.tidx = tidx;
+    ra.t = t;
+    const int kind = rules[r].kind;
+    const int64_t j0 = rules[r].part[p], j1 = rules[r].part[p + 1];
+    for (int64_t j = j0 + threadIdx.x; j < j1; j += kFusedBlock) {
+      if (kind == 0) stdp_wd_one(ra, j);
+      else if (kind == 1) stdp_vogels_one(ra, j);
+      else stdp_evans_one(ra, j);
+    }
+  }
+}
+
 // Partition of a workgroup.  Workgroups are dispatched to the XCDs round-robin, so block
 // b runs on XCD b % 8; mapping the blocks of one XCD to consecutive partitions keeps that
 // XCD's synapse segments contiguous inside every CSR row (its L2 then fetches only its own
@@ -319,7 +341,7 @@ __device__ __forceinline__ int partition_of_block(int b, int P) {
   return (b % kXcds) * (P / kXcds) + b / kXcds;
 }
 
-template <int KIND, bool L1>
+template <int KIND, bool L1, bool PLASTIC>
 __global__ __launch_bounds__(kFusedBlock) void k_fused(FusedArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int p = partition_of_block(blockIdx.x, gridDim.x);
@@ -456,19 +478,22 @@ __global__ __launch_bounds__(kFusedBlock) void k_fused(FusedArgs a) {
             float dvs[kPrefetch];
 #pragma unroll
             for (int x = 0; x < kPrefetch; ++x) dvs[x] = (x < gn) ? sinp[tid * G + g0 + x] : 0.0f;
+            const float tref = a.tref, dt = a.dt, tg0 = t;
+            unsigned long long mk = 0ull;
 #pragma unroll
             for (int x = 0; x < kPrefetch; ++x) {
               if (x < gn) {
                 const int g = g0 + x;
-                const float tg = t + (float)g * a.dt;
-                const bool ready = ((tg - last) - a.tref) > half_dt;
+                const float tg = tg0 + (float)g * dt;
+                const bool ready = ((tg - last) - tref) > half_dt;
                 const float vn = ((cv0 + omc * V) + cib) + dvs[x];
                 const bool fire = ready && (vn >= thr);
                 V = ready ? (fire ? vr : vn) : V;
                 last = fire ? tg : last;
-                if (fire) smask |= 1ull << g;
+                mk |= (fire ? 1ull : 0ull) << g;
               }
             }
+            smask |= mk;
           } else {
             for (int x = 0; x < gn; ++x) {
               const int g = g0 + x;
@@ -531,7 +556,7 @@ __global__ __launch_bounds__(kFusedBlock) void k_fused(FusedArgs a) {
     uint32_t* xl = a.xlist + ((size_t)par * a.P + p) * a.xstride;
     for (int k = tid; k < nx; k += kFusedBlock) store_coherent(xl + 1 + k, sxbuf[k]);
     if (tid == 0) store_coherent(xl, (uint32_t)nx);
-    if (a.plastic) {
+    if (PLASTIC) {
       if (own) a.last_n[gi] = last;
       for (int k = tid; k < W * nown; k += kFusedBlock) {
         const int wd = k / nown, i = k - wd * nown;
@@ -568,18 +593,8 @@ __global__ __launch_bounds__(kFusedBlock) void k_fused(FusedArgs a) {
     if (!s_ok) return;
     clk.lap(a, 4);
     // -- E. plasticity of the synapses onto this partition ---------------------------
-    if (a.plastic) {
-      for (int r = 0; r < a.n_rules; ++r) {
-        StdpArgs ra = a.rule[r];
-        ra.tidx = tidx;
-        ra.t = t;
-        const int64_t j0 = a.rule_part[r][p], j1 = a.rule_part[r][p + 1];
-        for (int64_t j = j0 + tid; j < j1; j += kFusedBlock) {
-          if (a.rule_kind[r] == 0) stdp_wd_one(ra, j);
-          else if (a.rule_kind[r] == 1) stdp_vogels_one(ra, j);
-          else stdp_evans_one(ra, j);
-        }
-      }
+    if (PLASTIC) {
+      fused_plasticity(a.rules, a.n_rules, p, tidx, t);
       __syncthreads();
     }
     clk.lap(a, 5);
@@ -620,15 +635,19 @@ __global__ __launch_bounds__(kFusedBlock) void k_fused(FusedArgs a) {
       __syncthreads();
       const int total = sscan[a.P];
       for (int r0 = 0; r0 < total; r0 += kEntCap) {
-        // (3) stage entries [r0, r0 + kEntCap) of the concatenated lists
-        for (int src = tid; src < a.P; src += kFusedBlock) {
+        // (3) stage entries [r0, r0 + kEntCap) of the concatenated lists: the first
+        //     kHeadWords-1 entries of a source come from its head, 16 lanes per source
+        for (int k = tid; k < a.P * kHeadWords; k += kFusedBlock) {
+          const int src = k / kHeadWords, j = k - src * kHeadWords;
           const int cnt = min((int)shead[src * kHeadWords], a.xcap);
-          const int off = sscan[src];
-          const int k_lo = max(0, r0 - off), k_hi = min(cnt, r0 + kEntCap - off);
-          for (int k = k_lo; k < k_hi; ++k) {
-            const uint32_t e = (k + 1 < kHeadWords) ? shead[src * kHeadWords + 1 + k]
-                                                    : load_coherent(xbase + (size_t)src * a.xstride + 1 + k);
-            sent[off + k - r0] = e;
+          const int pos = sscan[src] + j - r0;
+          if (j + 1 < kHeadWords && j < cnt && pos >= 0 && pos < kEntCap)
+            sent[pos] = shead[src * kHeadWords + 1 + j];
+          if (j == 0) {
+            for (int kk = kHeadWords - 1; kk < cnt; ++kk) {
+              const int ps = sscan[src] + kk - r0;
+              if (ps >= 0 && ps < kEntCap) sent[ps] = load_coherent(xbase + (size_t)src * a.xstride + 1 + kk);
+            }
           }
         }
         __syncthreads();

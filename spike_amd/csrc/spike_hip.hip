@@ -187,6 +187,7 @@ struct spk_network {
   unsigned long long* d_rast_count = nullptr;
   int64_t rast_cap = 0;
   unsigned long long* d_prof = nullptr;
+  spk::FusedRule* d_rules = nullptr;
   bool profiling = false;
   double prof_us[spk::kProfPhases] = {0};
   // per-phase bookkeeping
@@ -228,7 +229,7 @@ void free_device(spk_network* n) {
   }
   dfree(n->d_nlo); dfree(n->d_ilo); dfree(n->d_seg); dfree(n->d_xlist);
   dfree(n->d_bar); dfree(n->d_err); dfree(n->d_rast_id);
-  dfree(n->d_rast_step); dfree(n->d_rast_count); dfree(n->d_prof);
+  dfree(n->d_rast_step); dfree(n->d_rast_count); dfree(n->d_prof); dfree(n->d_rules);
   for (auto& m : n->mons) {
     dfree(m.d_ids); dfree(m.d_times); dfree(m.d_count);
   }
@@ -451,12 +452,16 @@ int download(spk_network* n, T* out, const T* d, size_t cnt) {
 }
 
 
-// The persistent kernel instance for a network (synapse kind, single label or not).
-const void* fused_kernel(const spk_network* n) {
+// The persistent kernel instance for a network (synapse kind, single label or not,
+// plasticity applied or not).
+const void* fused_kernel(const spk_network* n, bool plastic) {
   const bool l1 = n->L == 1;
-  if (n->kind == SPK_SYN_VOLTAGE)
-    return l1 ? (const void*)spk::k_fused<0, true> : (const void*)spk::k_fused<0, false>;
-  return l1 ? (const void*)spk::k_fused<1, true> : (const void*)spk::k_fused<1, false>;
+#define SPK_FUSED_PICK(K)                                                                  \
+  (plastic ? (l1 ? (const void*)spk::k_fused<K, true, true> : (const void*)spk::k_fused<K, false, true>) \
+           : (l1 ? (const void*)spk::k_fused<K, true, false> : (const void*)spk::k_fused<K, false, false>))
+  if (n->kind == SPK_SYN_VOLTAGE) return SPK_FUSED_PICK(0);
+  return SPK_FUSED_PICK(1);
+#undef SPK_FUSED_PICK
 }
 
 // The persistent kernel over n_groups groups from `start`, in chunks of bounded length.
@@ -496,18 +501,27 @@ int run_fused(spk_network* n, float start, int64_t ngroups, int flags) {
   a.seg = n->d_seg; a.syn = n->d_syn;
   a.scale = n->uniform_scale ? nullptr : n->d_scale;
   a.n_rules = plastic ? (int)n->rules.size() : 0;
-  if (a.n_rules > 4) return fail(SPK_ERR_ARG, "the persistent kernel takes at most 4 plasticity rules");
-  for (int r = 0; r < a.n_rules; ++r) {
-    RuleHost& rh = n->rules[r];
-    spk::StdpArgs& ra = a.rule[r];
-    ra.pos = rh.d_pos; ra.pre = rh.d_pre; ra.tgt = n->d_tgt; ra.delay = n->d_delay; ra.w = weights_view(n);
-    ra.ta = rh.d_ta; ra.tb = rh.d_tb; ra.last_n = n->d_last_n; ra.last_in = n->d_last_in;
-    ra.hist_n = n->d_hist_n; ra.hist_in = n->d_hist_in; ra.tab = rh.d_tab; ra.dterm = rh.dterm;
-    ra.P = (int64_t)rh.syn.size(); ra.N = n->N; ra.Nin = n->Nin; ra.G = G; ra.W = n->W;
-    ra.nearest = rh.wd.nearest_spike_only; ra.dt = n->dt;
-    a.rule_kind[r] = rh.kind;
-    a.rule_part[r] = rh.d_part;
+  if (a.n_rules > 0) {
+    std::vector<spk::FusedRule> fr(a.n_rules);
+    for (int r = 0; r < a.n_rules; ++r) {
+      RuleHost& rh = n->rules[r];
+      spk::StdpArgs& ra = fr[r].args;
+      ra = spk::StdpArgs{};
+      ra.pos = rh.d_pos; ra.pre = rh.d_pre; ra.tgt = n->d_tgt; ra.delay = n->d_delay; ra.w = weights_view(n);
+      ra.ta = rh.d_ta; ra.tb = rh.d_tb; ra.last_n = n->d_last_n; ra.last_in = n->d_last_in;
+      ra.hist_n = n->d_hist_n; ra.hist_in = n->d_hist_in; ra.tab = rh.d_tab; ra.dterm = rh.dterm;
+      ra.P = (int64_t)rh.syn.size(); ra.N = n->N; ra.Nin = n->Nin; ra.G = G; ra.W = n->W;
+      ra.nearest = rh.wd.nearest_spike_only; ra.dt = n->dt;
+      fr[r].kind = rh.kind;
+      fr[r].part = rh.d_part;
+    }
+    if (!n->d_rules) {
+      if (int rc = dalloc(&n->d_rules, n->rules.size())) return rc;
+    }
+    SPK_HIP(hipMemcpyAsync(n->d_rules, fr.data(), fr.size() * sizeof(spk::FusedRule),
+                           hipMemcpyHostToDevice, n->stream));
   }
+  a.rules = n->d_rules;
   a.xlist = n->d_xlist; a.bar = n->d_bar; a.error = n->d_err;
   a.rast_id = n->d_rast_id; a.rast_step = n->d_rast_step; a.rast_count = n->d_rast_count;
   a.rast_cap = n->rast_cap;
@@ -530,7 +544,7 @@ int run_fused(spk_network* n, float start, int64_t ngroups, int flags) {
     SPK_HIP(hipMemsetAsync(n->d_prof, 0, sizeof(unsigned long long) * spk::kProfPhases, n->stream));
     a.prof = n->d_prof;
   }
-  const void* fn = fused_kernel(n);
+  const void* fn = fused_kernel(n, plastic);
   if (n->fused_lds > 65536)
     SPK_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)n->fused_lds));
   const int64_t nlaunch = (ngroups + chunk - 1) / chunk;
@@ -936,7 +950,7 @@ int spk_prepare(spk_network* n, int G) {
   n->fused_why.clear();
   auto no_fused = [&](const char* why) { if (n->fused_ok) { n->fused_ok = false; n->fused_why = why; } };
   if (n->N < 1) no_fused("no LIF neurons");
-  if (n->nown_max > spk::kFusedBlock) no_fused("more than 256 neurons per partition");
+  if (n->nown_max > spk::kFusedBlock) no_fused("more neurons per partition than threads per workgroup");
   if (G > 64) no_fused("timestep grouping above 64");
   if (n->dmax > 255) no_fused("delays above 255 steps");
   if (n->L > 15) no_fused("more than 15 labels");
@@ -1067,7 +1081,10 @@ int spk_prepare(spk_network* n, int G) {
   // the persistent grid must be co-resident: one workgroup per partition
   if (n->fused_ok) {
     int per_cu = 0;
-    SPK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fused_kernel(n), spk::kFusedBlock, n->fused_lds));
+    int per_cu_p = 0;
+    SPK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fused_kernel(n, false), spk::kFusedBlock, n->fused_lds));
+    SPK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_p, fused_kernel(n, true), spk::kFusedBlock, n->fused_lds));
+    per_cu = std::min(per_cu, per_cu_p);
     if ((int64_t)per_cu * n->cu_count < n->P) no_fused("partitions exceed co-resident workgroups");
   }
   n->prepared = true;
@@ -1184,8 +1201,7 @@ int spk_run(spk_network* n, float start, int64_t ngroups, int flags) {
   if (ngroups < 0) return fail(SPK_ERR_ARG, "negative group count");
   std::memset(&n->timing, 0, sizeof(n->timing));
   SPK_HIP(hipMemsetAsync(n->d_stats, 0, sizeof(unsigned long long) * 4, n->stream));
-  const bool use_fused = n->fused_ok && !(flags & SPK_RUN_FORCE_LAUNCHES) && n->G == n->Gprep &&
-                         (!(flags & SPK_RUN_PLASTICITY) || n->rules.size() <= 4);
+  const bool use_fused = n->fused_ok && !(flags & SPK_RUN_FORCE_LAUNCHES) && n->G == n->Gprep;
   SPK_HIP(hipEventRecord(n->ev_a, n->stream));
   if (use_fused) {
     if (int rc = run_fused(n, start, ngroups, flags)) return rc;
