@@ -33,6 +33,21 @@ def algorithmic_bytes(n_neurons, steps, spikes, rec_events, b_w=4):
     return n_neurons * steps * 24 + spikes * 16 + rec_events * (4 + b_w)
 
 
+def measured_traffic(form, path):
+    """HBM bytes per launch of the fused kernel, from the committed rocprofv3 PMC summary
+    (FETCH_SIZE and WRITE_SIZE, each scaled by the factor a 1 GiB copy gives); None when
+    the summary for this workload is absent."""
+    if form != "config" or path != "fused":
+        return None
+    f = os.path.join(ROOT, "profiles", "k_fused_brunel10k_config.txt")
+    if not os.path.exists(f):
+        return None
+    for line in open(f):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(float(line.split(":")[1]))
+    return None
+
+
 def cpu_baseline(desc, groups, threads):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle_py
@@ -122,7 +137,8 @@ def main():
         avg_launch_ms = tot_ms / max(1, args.steps)
         achieved = bytes_alg / (tot_ms / 1e3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                "traffic": measured_traffic(args.form, args.path),
                 "avg_launch_ms": round(avg_launch_ms, 5)}
     cpu = None
     if not args.no_cpu:
