@@ -229,6 +229,17 @@ int spk_collect_spikes(spk_network* net, int monitor_id, float t);
  * the run are drained into the library's host-side store (spk_monitor_take). */
 int spk_run(spk_network* net, float start_time, int64_t n_groups, int flags);
 int spk_last_timing(const spk_network* net, spk_timing* timing);
+/* One timestep group enqueued on the network's stream without waiting for it: the whole
+ * of SpikingModel::perform_per_step_model_instructions (SpikingModel.cpp:197-212) for the
+ * group starting at t -- neurons, inputs, plasticity (SPK_RUN_PLASTICITY), propagation
+ * and every monitor's collection into its device store.  This is what the backend
+ * state_update calls of one group map to; like the reference's kernel launches it returns
+ * before the group has run.  Monitor records stay on the device (spk_monitor_device_count,
+ * spk_monitor_drain). */
+int spk_step_group(spk_network* net, float t, int flags);
+/* Cumulative event counters since spk_prepare / the last spk_run (spikes, deliveries,
+ * external events; the time fields are zero).  Synchronizes. */
+int spk_get_counters(spk_network* net, spk_timing* counters);
 int spk_synchronize(spk_network* net);
 
 /* ---------------------------------------------------------------------------- */
@@ -237,6 +248,9 @@ int spk_synchronize(spk_network* net);
 /* Records currently in the device store of a monitor
  * (SpikingActivityMonitor.cu:35-39).  Overflow is reported as a count > capacity. */
 int spk_monitor_device_count(spk_network* net, int monitor_id, int64_t* count);
+/* Clear a monitor's device store and host store (SpikingActivityMonitor.cu reset_state,
+ * which zeroes the device spike count). */
+int spk_monitor_reset(spk_network* net, int monitor_id);
 /* Copy device records (SpikingActivityMonitor.cu:41-50) and clear the device store. */
 int spk_monitor_drain(spk_network* net, int monitor_id, int32_t* ids, float* times,
                       int64_t cap, int64_t* n);

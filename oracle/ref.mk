@@ -26,10 +26,19 @@ FRONT_OBJS := $(patsubst $(REF)/%.cpp,$(OBJ)/%.o,$(FRONT_SRCS))
 
 HIP_LIB := $(ROOT)/spike_amd/_lib
 BACKEND_LIB := $(HIP_LIB)/libspike_hip_backend.so
+BACKEND_SRC := $(ROOT)/spike_amd/backend/HIPBackend.cpp
 
 all: $(OUT)/libspike_front.so $(OUT)/context_dummy.o $(OUT)/context_hip.o $(OUT)/gen_connectivity
 
-hip: all $(OUT)/Brunel10K_hip $(OUT)/frontend_net_hip
+hip: all $(BACKEND_LIB) $(OUT)/Brunel10K_hip $(OUT)/frontend_net_hip
+
+# The HIP backend (spike_amd/backend, this project's code) compiled against the Spike
+# headers; it depends on the front-end library so that the factory maps it registers into
+# are constructed before its registrars run.
+$(BACKEND_LIB): $(BACKEND_SRC) $(ROOT)/spike_amd/backend/HIPBackend.hpp $(ROOT)/include/spike_hip.h $(OUT)/libspike_front.so $(HIP_LIB)/libspike_hip.so
+	$(CXX) -std=c++14 -O2 -fPIC -shared -DVERBOSE=1 -I$(REF) -I$(ROOT)/include $(BACKEND_SRC) \
+	  -L$(OUT) -lspike_front -L$(HIP_LIB) -lspike_hip \
+	  -Wl,-rpath,'$$ORIGIN' -Wl,-rpath,'$$ORIGIN/../../oracle/_ref' -o $@
 
 $(OBJ)/%.o: $(REF)/%.cpp
 	@mkdir -p $(dir $@)
@@ -52,12 +61,12 @@ $(OUT)/gen_connectivity: $(ROOT)/tests/golden/gen_connectivity.cpp $(OUT)/libspi
 
 $(OUT)/Brunel10K_hip: $(REF)/Examples/Brunel10K.cpp $(OUT)/libspike_front.so $(OUT)/context_hip.o $(BACKEND_LIB)
 	$(CXX) $(CXXFLAGS) -I$(REF)/Examples $< $(OUT)/context_hip.o -L$(OUT) -lspike_front \
-	  -L$(HIP_LIB) -lspike_hip_backend -Wl,--no-as-needed \
+	  -L$(HIP_LIB) -Wl,--no-as-needed -lspike_hip_backend -Wl,--as-needed \
 	  -Wl,-rpath,'$$ORIGIN' -Wl,-rpath,'$$ORIGIN/../../spike_amd/_lib' -o $@
 
 $(OUT)/frontend_net_hip: $(ROOT)/tests/frontend/frontend_net.cpp $(OUT)/libspike_front.so $(OUT)/context_hip.o $(BACKEND_LIB)
 	$(CXX) $(CXXFLAGS) -I$(REF)/Examples $< $(OUT)/context_hip.o -L$(OUT) -lspike_front \
-	  -L$(HIP_LIB) -lspike_hip_backend -Wl,--no-as-needed \
+	  -L$(HIP_LIB) -Wl,--no-as-needed -lspike_hip_backend -Wl,--as-needed \
 	  -Wl,-rpath,'$$ORIGIN' -Wl,-rpath,'$$ORIGIN/../../spike_amd/_lib' -o $@
 
 .PHONY: all hip

@@ -99,6 +99,9 @@ SIGNATURES = {
     "spk_collect_spikes": (C.c_int, [P, C.c_int, C.c_float]),
     "spk_run": (C.c_int, [P, C.c_float, C.c_int64, C.c_int]),
     "spk_last_timing": (C.c_int, [P, C.POINTER(Timing)]),
+    "spk_step_group": (C.c_int, [P, C.c_float, C.c_int]),
+    "spk_get_counters": (C.c_int, [P, C.POINTER(Timing)]),
+    "spk_monitor_reset": (C.c_int, [P, C.c_int]),
     "spk_synchronize": (C.c_int, [P]),
     "spk_monitor_device_count": (C.c_int, [P, C.c_int, lp]),
     "spk_monitor_drain": (C.c_int, [P, C.c_int, ip, fp, C.c_int64, lp]),
@@ -255,6 +258,19 @@ class Network:
         t = Timing()
         check(self.lib.spk_last_timing(self.h, C.byref(t)))
         return t
+
+    def step_group(self, t: float, plasticity: bool = True, force_launches=False):
+        """One group enqueued without waiting (the backend's per-group call)."""
+        flags = (SPK_RUN_PLASTICITY if plasticity else 0) | (SPK_RUN_FORCE_LAUNCHES if force_launches else 0)
+        check(self.lib.spk_step_group(self.h, float(t), flags))
+
+    def counters(self) -> Timing:
+        t = Timing()
+        check(self.lib.spk_get_counters(self.h, C.byref(t)))
+        return t
+
+    def synchronize(self):
+        check(self.lib.spk_synchronize(self.h))
 
     def step_phases(self, t: float, plasticity: bool = True):
         """One group through the per-phase entry points, in SpikingModel order."""

@@ -34,6 +34,18 @@ constexpr int kProfPhases = 12;
 constexpr int kXcds = 8;               // MI355X: 8 accelerator complex dies, blocks dispatched round-robin
 constexpr int kPrefetch = 16;          // sub-steps whose inputs are read ahead
 
+constexpr int kMaxFusedMons = 4;      // spike monitors the persistent kernel records for
+
+// Device store of one spike monitor (SpikingActivityMonitor.cu:22-33): records beyond cap
+// are counted but not written, so the host sees the overflow.
+struct FusedMon {
+  int32_t* ids;
+  float* times;
+  unsigned long long* count;
+  long long cap;
+  unsigned int lo, hi;
+};
+
 // One plasticity rule of the persistent kernel.
 struct FusedRule {
   StdpArgs args;
@@ -76,11 +88,11 @@ struct FusedArgs {
   // (level 2) and bar[9 * kBarStride] (release generation)
   unsigned int* bar;
   int* error;
-  // raster store: entries (unified id, step within launch)
-  uint32_t* rast_id;
-  int32_t* rast_step;
-  unsigned long long* rast_count;
-  int64_t rast_cap;
+  // spike monitors: each appends (local id, time) records of units [lo, hi) to its store
+  FusedMon mon[kMaxFusedMons];
+  int n_mons;
+  // barrier generations completed on the counters before this launch
+  unsigned int bar_base;
   unsigned long long* stats;    // [0] deliveries [1] external [2] LIF spikes [3] input spikes
   int64_t q_ext;
   int64_t arm_step;
@@ -90,7 +102,7 @@ struct FusedArgs {
   float start, dt, tref, ibg;
   int64_t group0;             // index of the first group of this launch within the run
   int N, Nin, L, G, B, W, P, xcap, xstride, nown_max, iown_max, ext_k, ngroups;
-  int plastic, record;
+  int plastic;
   unsigned long long* prof;   // optional: wall-clock ticks per phase of workgroup 0
 };
 
@@ -184,11 +196,12 @@ __device__ __forceinline__ void barrier_arrive(const FusedArgs& a, int s) {
   const int grp = blockIdx.x % ngrp;
   const unsigned int gsize = (unsigned int)((P - grp + ngrp - 1) / ngrp);
   if (a.plastic) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  const unsigned int gen = a.bar_base + (unsigned int)(s + 1);
   const unsigned int old = atomicAdd(a.bar + grp * kBarStride, 1u);
-  if (old + 1u == (unsigned int)(s + 1) * gsize) {
+  if (old + 1u == gen * gsize) {
     const unsigned int old2 = atomicAdd(a.bar + kBarGroups * kBarStride, 1u);
-    if (old2 + 1u == (unsigned int)(s + 1) * (unsigned int)ngrp)
-      __hip_atomic_store(a.bar + (kBarGroups + 1) * kBarStride, (unsigned int)(s + 1), __ATOMIC_RELAXED,
+    if (old2 + 1u == gen * (unsigned int)ngrp)
+      __hip_atomic_store(a.bar + (kBarGroups + 1) * kBarStride, gen, __ATOMIC_RELAXED,
                          __HIP_MEMORY_SCOPE_AGENT);
   }
 }
@@ -197,7 +210,8 @@ __device__ __forceinline__ void barrier_arrive(const FusedArgs& a, int s) {
 __device__ __forceinline__ bool barrier_wait(const FusedArgs& a, int s) {
   const unsigned int* rel = a.bar + (kBarGroups + 1) * kBarStride;
   long long spins = 0;
-  while (atomic_load_agent(rel) < (unsigned int)(s + 1)) {
+  const unsigned int gen = a.bar_base + (unsigned int)(s + 1);
+  while (atomic_load_agent(rel) < gen) {
     __builtin_amdgcn_s_sleep(1);
     if (++spins > a.spin_limit ||
         __hip_atomic_load(a.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
@@ -363,7 +377,6 @@ __global__ __launch_bounds__(kFusedBlock) void k_fused(FusedArgs a) {
   uint32_t* sent = (uint32_t*)(smem + f.ent);
   int* scounter = (int*)(smem + f.counter);
   __shared__ int s_ok;
-  __shared__ unsigned long long s_base;
   __shared__ unsigned long long s_prof[kProfPhases];
 
   const int NL = a.N * L;
@@ -563,16 +576,27 @@ __global__ __launch_bounds__(kFusedBlock) void k_fused(FusedArgs a) {
         a.hist_n[(size_t)wd * a.N + nlo + i] = shist[k];
       }
     }
-    if (a.record && nx > 0) {
-      if (tid == 0) s_base = atomicAdd(a.rast_count, (unsigned long long)nx);
-      __syncthreads();
-      const unsigned long long base = s_base;
-      for (int k = tid; k < nx; k += kFusedBlock) {
-        const unsigned long long pos = base + k;
-        if ((int64_t)pos < a.rast_cap) {
-          const uint32_t e = sxbuf[k];
-          a.rast_id[pos] = e & kIdMask;
-          a.rast_step[pos] = s * G + (int)(e >> 24);
+    // spike monitors (SpikingActivityMonitor.cu:69-98): one store slot per wavefront
+    // ballot; time = group time + g * dt as the reference's collection kernel computes it
+#pragma unroll
+    for (int mi = 0; mi < kMaxFusedMons; ++mi) {
+      if (mi >= a.n_mons) break;
+      const FusedMon& m = a.mon[mi];
+      for (int k0 = 0; k0 < nx; k0 += kFusedBlock) {
+        const int k = k0 + tid;
+        const uint32_t e = k < nx ? sxbuf[k] : 0u;
+        const uint32_t id = e & kIdMask;
+        const bool in = k < nx && id >= m.lo && id < m.hi;
+        const unsigned long long bal = __ballot(in);
+        unsigned long long base = 0;
+        if (lane == 0 && bal) base = atomicAdd(m.count, (unsigned long long)__popcll(bal));
+        base = __shfl(base, 0);
+        if (in) {
+          const unsigned long long pos = base + (unsigned long long)__popcll(bal & ((1ull << lane) - 1ull));
+          if (pos < (unsigned long long)m.cap) {
+            m.ids[pos] = (int32_t)(id - m.lo);
+            m.times[pos] = t + (float)(e >> 24) * a.dt;
+          }
         }
       }
     }

@@ -182,10 +182,10 @@ struct spk_network {
   uint32_t* d_xlist = nullptr;
   unsigned int* d_bar = nullptr;
   int* d_err = nullptr;
-  uint32_t* d_rast_id = nullptr;
-  int32_t* d_rast_step = nullptr;
-  unsigned long long* d_rast_count = nullptr;
-  int64_t rast_cap = 0;
+  unsigned int bar_gen = 0;      // barrier generations completed since the counters were cleared
+  bool bar_clear = true;         // counters (and the error flag) must be cleared first
+  bool rules_uploaded = false;   // device descriptors of the rules are current
+  bool lds_attr[2] = {false, false};  // dynamic LDS limit raised for the non-plastic/plastic kernel
   unsigned long long* d_prof = nullptr;
   spk::FusedRule* d_rules = nullptr;
   bool profiling = false;
@@ -228,8 +228,7 @@ void free_device(spk_network* n) {
     dfree(r.d_pos); dfree(r.d_pre); dfree(r.d_ta); dfree(r.d_tb); dfree(r.d_tab); dfree(r.d_part);
   }
   dfree(n->d_nlo); dfree(n->d_ilo); dfree(n->d_seg); dfree(n->d_xlist);
-  dfree(n->d_bar); dfree(n->d_err); dfree(n->d_rast_id);
-  dfree(n->d_rast_step); dfree(n->d_rast_count); dfree(n->d_prof); dfree(n->d_rules);
+  dfree(n->d_bar); dfree(n->d_err); dfree(n->d_prof); dfree(n->d_rules);
   for (auto& m : n->mons) {
     dfree(m.d_ids); dfree(m.d_times); dfree(m.d_count);
   }
@@ -271,6 +270,7 @@ int refresh_rule_tables(spk_network* n, RuleHost& r) {
            dt * r.ev.model_parameter_alpha_D};
   }
   dfree(r.d_tab);
+  n->rules_uploaded = false;
   int rc = dalloc(&r.d_tab, tab.size());
   if (rc) return rc;
   SPK_HIP(hipMemcpy(r.d_tab, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice));
@@ -464,34 +464,12 @@ const void* fused_kernel(const spk_network* n, bool plastic) {
 #undef SPK_FUSED_PICK
 }
 
-// The persistent kernel over n_groups groups from `start`, in chunks of bounded length.
-int run_fused(spk_network* n, float start, int64_t ngroups, int flags) {
-  const bool plastic = (flags & SPK_RUN_PLASTICITY) != 0 && !n->rules.empty();
+// Kernel arguments of the persistent kernel (all but the run position).
+int fill_fused_args(spk_network* n, bool plastic, spk::FusedArgs& a) {
+  const int G = n->G;
   for (auto& r : n->rules)
     if (int rc = refresh_rule_tables(n, r)) return rc;
-  const int U = n->nu();
-  const int G = n->G;
-  const bool record = !n->mons.empty();
-  const int64_t per_group = (int64_t)G * U;
-  int64_t chunk = 1000;  // bounds the length of one launch
-  if (record) {
-    const int64_t want = std::min<int64_t>((int64_t)1 << 24, std::max<int64_t>(per_group * 8, per_group));
-    if (n->rast_cap < want) {
-      dfree(n->d_rast_id);
-      dfree(n->d_rast_step);
-      int rc;
-      if ((rc = dalloc(&n->d_rast_id, (size_t)want))) return rc;
-      if ((rc = dalloc(&n->d_rast_step, (size_t)want))) return rc;
-      if (!n->d_rast_count && (rc = dalloc(&n->d_rast_count, 1))) return rc;
-      n->rast_cap = want;
-    }
-    chunk = std::max<int64_t>(1, std::min<int64_t>(chunk, n->rast_cap / per_group));
-  }
-  if (n->Nin > 0 && n->pending_arm) {
-    n->arm_step = (int)roundf(start / n->dt);
-    n->pending_arm = false;
-  }
-  spk::FusedArgs a{};
+  a = spk::FusedArgs{};
   a.nlo = n->d_nlo; a.ilo = n->d_ilo;
   a.V = n->d_V; a.last_n = n->d_last_n; a.hist_n = n->d_hist_n; a.gcond = n->d_g; a.ring = n->d_ring;
   a.v0 = n->d_v0; a.vreset = n->d_vreset; a.thr = n->d_thr; a.c = n->d_c; a.rp = n->d_rp;
@@ -501,7 +479,7 @@ int run_fused(spk_network* n, float start, int64_t ngroups, int flags) {
   a.seg = n->d_seg; a.syn = n->d_syn;
   a.scale = n->uniform_scale ? nullptr : n->d_scale;
   a.n_rules = plastic ? (int)n->rules.size() : 0;
-  if (a.n_rules > 0) {
+  if (a.n_rules > 0 && !n->rules_uploaded) {
     std::vector<spk::FusedRule> fr(a.n_rules);
     for (int r = 0; r < a.n_rules; ++r) {
       RuleHost& rh = n->rules[r];
@@ -518,25 +496,102 @@ int run_fused(spk_network* n, float start, int64_t ngroups, int flags) {
     if (!n->d_rules) {
       if (int rc = dalloc(&n->d_rules, n->rules.size())) return rc;
     }
-    SPK_HIP(hipMemcpyAsync(n->d_rules, fr.data(), fr.size() * sizeof(spk::FusedRule),
-                           hipMemcpyHostToDevice, n->stream));
+    SPK_HIP(hipMemcpy(n->d_rules, fr.data(), fr.size() * sizeof(spk::FusedRule), hipMemcpyHostToDevice));
+    n->rules_uploaded = true;
   }
   a.rules = n->d_rules;
   a.xlist = n->d_xlist; a.bar = n->d_bar; a.error = n->d_err;
-  a.rast_id = n->d_rast_id; a.rast_step = n->d_rast_step; a.rast_count = n->d_rast_count;
-  a.rast_cap = n->rast_cap;
+  a.n_mons = (int)n->mons.size();
+  for (int m = 0; m < a.n_mons; ++m) {
+    const MonitorHost& mh = n->mons[m];
+    a.mon[m].ids = mh.d_ids;
+    a.mon[m].times = mh.d_times;
+    a.mon[m].count = mh.d_count;
+    a.mon[m].cap = mh.capacity;
+    a.mon[m].lo = mh.pop == SPK_POP_NEURONS ? 0u : (unsigned)n->N;
+    a.mon[m].hi = mh.pop == SPK_POP_NEURONS ? (unsigned)n->N : (unsigned)n->nu();
+  }
   a.stats = n->d_stats;
   a.q_ext = n->q_ext; a.arm_step = n->Nin > 0 ? n->arm_step : -1; a.seed = n->seed;
   a.spin_limit = (long long)1 << 22;
   a.scale0 = n->scale0; a.two_k = ldexpf(1.0f, n->frac); a.inv_scale = ldexpf(1.0f, -n->frac);
-  a.start = start; a.dt = n->dt; a.tref = n->tref; a.ibg = n->ibg;
+  a.dt = n->dt; a.tref = n->tref; a.ibg = n->ibg;
   a.N = n->N; a.Nin = n->Nin; a.L = n->L; a.G = G; a.B = n->B; a.W = n->W; a.P = n->P;
   a.xcap = n->xcap; a.xstride = spk::fused_xstride(n->xcap);
   a.nown_max = n->nown_max; a.iown_max = n->iown_max;
   a.ext_k = (int)n->ext_cdf.size();
   a.plastic = plastic ? 1 : 0;
-  a.record = record ? 1 : 0;
   a.prof = nullptr;
+  return SPK_OK;
+}
+
+// Barrier generations one launch may use before the level-1 counters (up to
+// ceil(P/8) arrivals per generation) could wrap.
+constexpr unsigned int kBarGenLimit = 1u << 24;
+
+// One cooperative launch of the persistent kernel over `ng` groups starting at group
+// index group0 of the run that began at `start`; enqueued only.
+int launch_fused(spk_network* n, spk::FusedArgs& a, float start, int64_t group0, int ng,
+                 hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr) {
+  if (n->bar_clear || n->bar_gen + (unsigned int)ng > kBarGenLimit) {
+    SPK_HIP(hipMemsetAsync(n->d_bar, 0, sizeof(unsigned int) * (spk::kBarGroups + 2) * spk::kBarStride, n->stream));
+    SPK_HIP(hipMemsetAsync(n->d_err, 0, sizeof(int), n->stream));
+    n->bar_gen = 0;
+    n->bar_clear = false;
+  }
+  if (n->Nin > 0 && n->pending_arm) {
+    n->arm_step = (int)roundf(start / n->dt);
+    n->pending_arm = false;
+  }
+  a.arm_step = n->Nin > 0 ? n->arm_step : -1;
+  a.start = start;
+  a.group0 = group0;
+  a.ngroups = ng;
+  a.bar_base = n->bar_gen;
+  const void* fn = fused_kernel(n, a.plastic != 0);
+  if (n->fused_lds > 65536 && !n->lds_attr[a.plastic != 0]) {
+    SPK_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)n->fused_lds));
+    n->lds_attr[a.plastic != 0] = true;
+  }
+  void* kargs[] = {&a};
+  if (ev0) SPK_HIP(hipEventRecord(ev0, n->stream));
+  SPK_HIP(hipLaunchCooperativeKernel(fn, dim3(n->P), dim3(spk::kFusedBlock), kargs,
+                                     (unsigned int)n->fused_lds, n->stream));
+  if (ev1) SPK_HIP(hipEventRecord(ev1, n->stream));
+  n->bar_gen += (unsigned int)ng;
+  return SPK_OK;
+}
+
+// Report (and clear) a barrier time-out of the persistent kernel; synchronizes.
+int check_fused_error(spk_network* n) {
+  if (!n->d_err) return SPK_OK;
+  int err = 0;
+  SPK_HIP(hipMemcpyAsync(&err, n->d_err, sizeof(int), hipMemcpyDeviceToHost, n->stream));
+  SPK_HIP(hipStreamSynchronize(n->stream));
+  if (err) {
+    n->bar_clear = true;
+    return fail(SPK_ERR_HIP, "persistent kernel: grid barrier wait exceeded its spin limit");
+  }
+  return SPK_OK;
+}
+
+// The persistent kernel over n_groups groups from `start`, in launches of bounded length;
+// monitor records are drained into the host stores after every launch.
+int run_fused(spk_network* n, float start, int64_t ngroups, int flags) {
+  const bool plastic = (flags & SPK_RUN_PLASTICITY) != 0 && !n->rules.empty();
+  const int G = n->G;
+  int64_t chunk = 1000;  // bounds the length of one launch
+  // every monitor store holds the records of a whole launch (each unit spikes at most once
+  // per sub-step); larger stores allow longer launches
+  for (auto& m : n->mons) {
+    const int64_t pop = std::max<int64_t>(1, m.pop == SPK_POP_NEURONS ? n->N : n->Nin);
+    const int64_t per_group = pop * G;
+    const int64_t want = std::max<int64_t>(m.capacity, std::min<int64_t>(per_group * 256, (int64_t)1 << 27));
+    if (int rc = ensure_monitor_capacity(n, m, std::max<int64_t>(want, per_group))) return rc;
+    chunk = std::max<int64_t>(1, std::min<int64_t>(chunk, m.capacity / per_group));
+  }
+  spk::FusedArgs a;
+  if (int rc = fill_fused_args(n, plastic, a)) return rc;
   if (n->profiling) {
     if (!n->d_prof) {
       if (int rc = dalloc(&n->d_prof, spk::kProfPhases)) return rc;
@@ -544,66 +599,20 @@ int run_fused(spk_network* n, float start, int64_t ngroups, int flags) {
     SPK_HIP(hipMemsetAsync(n->d_prof, 0, sizeof(unsigned long long) * spk::kProfPhases, n->stream));
     a.prof = n->d_prof;
   }
-  const void* fn = fused_kernel(n, plastic);
-  if (n->fused_lds > 65536)
-    SPK_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)n->fused_lds));
   const int64_t nlaunch = (ngroups + chunk - 1) / chunk;
   while ((int64_t)n->ev_kernel.size() < 2 * nlaunch) {
     hipEvent_t e;
     SPK_HIP(hipEventCreate(&e));
     n->ev_kernel.push_back(e);
   }
-  std::vector<uint32_t> rid;
-  std::vector<int32_t> rstep;
   int64_t li = 0;
   for (int64_t s0 = 0; s0 < ngroups; s0 += chunk, ++li) {
     const int ng = (int)std::min<int64_t>(chunk, ngroups - s0);
-    a.group0 = s0;
-    a.ngroups = ng;
-    SPK_HIP(hipMemsetAsync(n->d_bar, 0, sizeof(unsigned int) * (spk::kBarGroups + 2) * spk::kBarStride, n->stream));
-    SPK_HIP(hipMemsetAsync(n->d_err, 0, sizeof(int), n->stream));
-    if (record) SPK_HIP(hipMemsetAsync(n->d_rast_count, 0, sizeof(unsigned long long), n->stream));
-    void* kargs[] = {&a};
-    SPK_HIP(hipEventRecord(n->ev_kernel[2 * li], n->stream));
-    SPK_HIP(hipLaunchCooperativeKernel(fn, dim3(n->P), dim3(spk::kFusedBlock), kargs,
-                                       (unsigned int)n->fused_lds, n->stream));
-    SPK_HIP(hipEventRecord(n->ev_kernel[2 * li + 1], n->stream));
-    if (record) {
-      unsigned long long cnt = 0;
-      SPK_HIP(hipMemcpyAsync(&cnt, n->d_rast_count, sizeof(cnt), hipMemcpyDeviceToHost, n->stream));
-      SPK_HIP(hipStreamSynchronize(n->stream));
-      if ((int64_t)cnt > n->rast_cap) return fail(SPK_ERR_CAPACITY, "raster store overflow");
-      rid.resize(cnt);
-      rstep.resize(cnt);
-      if (cnt) {
-        SPK_HIP(hipMemcpyAsync(rid.data(), n->d_rast_id, cnt * 4, hipMemcpyDeviceToHost, n->stream));
-        SPK_HIP(hipMemcpyAsync(rstep.data(), n->d_rast_step, cnt * 4, hipMemcpyDeviceToHost, n->stream));
-        SPK_HIP(hipStreamSynchronize(n->stream));
-      }
-      std::vector<size_t> ord(cnt);
-      for (size_t i = 0; i < cnt; ++i) ord[i] = i;
-      std::sort(ord.begin(), ord.end(), [&](size_t x, size_t y) {
-        if (rstep[x] != rstep[y]) return rstep[x] < rstep[y];
-        return rid[x] < rid[y];
-      });
-      for (auto& m : n->mons) {
-        const uint32_t lo = m.pop == SPK_POP_NEURONS ? 0u : (uint32_t)n->N;
-        const uint32_t hi = m.pop == SPK_POP_NEURONS ? (uint32_t)n->N : (uint32_t)U;
-        for (size_t i : ord) {
-          if (rid[i] < lo || rid[i] >= hi) continue;
-          const int64_t S = s0 + rstep[i] / G;
-          const int g = rstep[i] % G;
-          const float t = start + (float)(S * G) * n->dt;
-          m.h_ids.push_back((int32_t)(rid[i] - lo));
-          m.h_times.push_back(t + (float)g * n->dt);
-        }
-      }
-    }
+    if (int rc = launch_fused(n, a, start, s0, ng, n->ev_kernel[2 * li], n->ev_kernel[2 * li + 1])) return rc;
+    for (auto& m : n->mons)
+      if (int rc = drain_monitor(n, m, m.h_ids, m.h_times)) return rc;
   }
-  SPK_HIP(hipStreamSynchronize(n->stream));
-  int err = 0;
-  SPK_HIP(hipMemcpy(&err, n->d_err, sizeof(int), hipMemcpyDeviceToHost));
-  if (err) return fail(SPK_ERR_HIP, "persistent kernel: grid barrier wait exceeded its spin limit");
+  if (int rc = check_fused_error(n)) return rc;
   double kms = 0.0;
   for (int64_t i = 0; i < nlaunch; ++i) {
     float ms = 0.0f;
@@ -957,6 +966,7 @@ int spk_prepare(spk_network* n, int G) {
   if ((int64_t)U >= (int64_t)spk::kIdMask) no_fused("more than 2^24 neurons");
   if (S >= ((int64_t)1 << 32)) no_fused("2^32 or more synapses");
   if (n->P > spk::kFusedBlock) no_fused("too many partitions");
+  if ((int)n->mons.size() > spk::kMaxFusedMons) no_fused("more than 4 spike monitors");
   n->fused_lds = spk::fused_lds_bytes(n->B, n->L, n->W, G, n->nown_max, n->iown_max, n->xcap, n->P);
   {
     hipDeviceProp_t prop;
@@ -1250,6 +1260,42 @@ int spk_run(spk_network* n, float start, int64_t ngroups, int flags) {
   return SPK_OK;
 }
 
+int spk_step_group(spk_network* n, float t, int flags) {
+  if (int rc = check_prepared(n)) return rc;
+  const bool use_fused = n->fused_ok && !(flags & SPK_RUN_FORCE_LAUNCHES) && n->G == n->Gprep;
+  if (use_fused) {
+    const bool plastic = (flags & SPK_RUN_PLASTICITY) != 0 && !n->rules.empty();
+    spk::FusedArgs a;
+    if (int rc = fill_fused_args(n, plastic, a)) return rc;
+    return launch_fused(n, a, t, 0, 1);
+  }
+  int rc;
+  if ((rc = begin_group(n, t, true))) return rc;
+  if ((rc = launch_neurons(n, t, n->cur_slot))) return rc;
+  if ((rc = launch_inputs(n, t, n->cur_slot))) return rc;
+  if (flags & SPK_RUN_PLASTICITY)
+    for (int r = 0; r < (int)n->rules.size(); ++r)
+      if ((rc = launch_rule(n, r, t))) return rc;
+  if ((rc = launch_propagate(n, t, n->cur_slot))) return rc;
+  for (int m = 0; m < (int)n->mons.size(); ++m)
+    if ((rc = launch_collect(n, m, n->cur_slot, 1))) return rc;
+  return SPK_OK;
+}
+
+int spk_get_counters(spk_network* n, spk_timing* t) {
+  if (int rc = check_prepared(n)) return rc;
+  if (!t) return fail(SPK_ERR_ARG, "null counters");
+  if (int rc = check_fused_error(n)) return rc;
+  unsigned long long st[4] = {0, 0, 0, 0};
+  SPK_HIP(hipMemcpy(st, n->d_stats, sizeof(st), hipMemcpyDeviceToHost));
+  std::memset(t, 0, sizeof(*t));
+  t->synaptic_events = (int64_t)st[0];
+  t->external_events = (int64_t)st[1];
+  t->spikes_neurons = (int64_t)st[2];
+  t->spikes_inputs = (int64_t)st[3];
+  return SPK_OK;
+}
+
 int spk_set_profiling(spk_network* n, int on) {
   if (int rc = check_net(n)) return rc;
   n->profiling = on != 0;
@@ -1278,6 +1324,7 @@ int spk_last_timing(const spk_network* n, spk_timing* t) {
 int spk_synchronize(spk_network* n) {
   if (int rc = check_net(n)) return rc;
   SPK_HIP(hipStreamSynchronize(n->stream));
+  if (n->prepared) return check_fused_error(n);
   return SPK_OK;
 }
 
@@ -1288,6 +1335,16 @@ int spk_monitor_device_count(spk_network* n, int mid, int64_t* count) {
   SPK_HIP(hipMemcpyAsync(&c, n->mons[mid].d_count, sizeof(c), hipMemcpyDeviceToHost, n->stream));
   SPK_HIP(hipStreamSynchronize(n->stream));
   *count = (int64_t)c;
+  return check_fused_error(n);
+}
+
+int spk_monitor_reset(spk_network* n, int mid) {
+  if (int rc = check_prepared(n)) return rc;
+  if (mid < 0 || mid >= (int)n->mons.size()) return fail(SPK_ERR_ARG, "bad monitor");
+  MonitorHost& m = n->mons[mid];
+  SPK_HIP(hipMemsetAsync(m.d_count, 0, sizeof(unsigned long long), n->stream));
+  m.h_ids.clear();
+  m.h_times.clear();
   return SPK_OK;
 }
 

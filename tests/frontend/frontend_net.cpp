@@ -1,0 +1,270 @@
+// frontend_net.cpp -- a network built and run through Spike's unchanged C++ front-end.
+//
+// Linked with the front-end library and the HIP backend (oracle/ref.mk, default backend
+// "HIP"), it builds one of a few small networks with the public API the Examples use,
+// runs SpikingModel::run, and writes what the parity test needs to replay the same
+// network on the CPU oracle:
+//   syn_{pre,post,w,scale,delay,label}.bin  the synapse arrays after finalise (sorted order,
+//                                           the order the backend receives them)
+//   lif_{v0,vreset,thr,tau,R}.bin, rates.bin, decay_g.bin (expf(-dt/tau_g) per label),
+//   plastic_<r>.bin, weights_after.bin (weights copied back after the run),
+//   <pop>_SpikeIDs.bin / <pop>_SpikeTimes.bin (SpikingActivityMonitor::save_spikes_as_binary),
+//   meta.json.
+//
+//   frontend_net_hip <case> <out_dir> <simtime_s> [plastic]
+//   cases: brunel (voltage synapses, Poisson inputs, optional WD STDP),
+//          coba (conductance synapses, two labels, Vogels + Evans STDP),
+//          plain (Poisson -> LIF, RANDOM conductance synapses, delays 1-10 steps)
+#include "Spike/Spike.hpp"
+#include "UtilityFunctions.hpp"
+
+#include <sys/stat.h>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <string>
+#include <vector>
+
+namespace {
+
+std::string g_out;
+
+template <typename T>
+void put(const std::string& name, const T* p, size_t n) {
+  std::ofstream f(g_out + "/" + name + ".bin", std::ios::binary);
+  if (n) f.write(reinterpret_cast<const char*>(p), (std::streamsize)(n * sizeof(T)));
+}
+
+void lif(lif_spiking_neuron_parameters_struct* p, float cm, float g0, float v0, float vr, float thr,
+         float tref, int n) {
+  p->group_shape[0] = 1;
+  p->group_shape[1] = n;
+  p->somatic_capacitance_Cm = cm;
+  p->somatic_leakage_conductance_g0 = g0;
+  p->resting_potential_v0 = v0;
+  p->after_spike_reset_potential_vreset = vr;
+  p->threshold_for_action_potential_spike = thr;
+  p->absolute_refractory_period = tref;
+  p->background_current = 0.0f;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  if (argc < 4) {
+    fprintf(stderr, "usage: %s <brunel|coba|plain> <out_dir> <simtime_s> [plastic]\n", argv[0]);
+    return 2;
+  }
+  const std::string which = argv[1];
+  g_out = argv[2];
+  const float simtime = (float)atof(argv[3]);
+  const bool plastic = argc > 4 && std::string(argv[4]) == "plastic";
+  mkdir(g_out.c_str(), 0755);
+
+  SpikingModel* model = new SpikingModel();
+  const float dt = 0.0001f;
+  model->SetTimestep(dt);
+  LIFSpikingNeurons* neurons = new LIFSpikingNeurons();
+  PoissonInputSpikingNeurons* inputs = new PoissonInputSpikingNeurons();
+  model->spiking_neurons = neurons;
+  model->input_spiking_neurons = inputs;
+  std::vector<STDPPlasticity*> rules;
+  std::vector<std::string> rule_kinds;
+
+  if (which == "brunel") {
+    // Examples/Brunel10K.cpp:69-231 at 1/50 of its size, with a stronger external drive
+    VoltageSpikingSynapses* syn = new VoltageSpikingSynapses(42);
+    model->spiking_synapses = syn;
+    weightdependent_stdp_plasticity_parameters_struct* wd = new weightdependent_stdp_plasticity_parameters_struct;
+    wd->a_plus = 1.0; wd->a_minus = 1.0; wd->tau_plus = 0.02; wd->tau_minus = 0.02;
+    wd->lambda = 1.0e-2f; wd->alpha = 2.02; wd->w_max = 0.3e-3f;
+    WeightDependentSTDPPlasticity* rule = new WeightDependentSTDPPlasticity(
+        (SpikingSynapses*)syn, (SpikingNeurons*)neurons, (SpikingNeurons*)inputs, (stdp_plasticity_parameters_struct*)wd);
+    if (plastic) {
+      model->AddPlasticityRule(rule);
+      rules.push_back(rule);
+      rule_kinds.push_back("wd");
+    }
+    lif_spiking_neuron_parameters_struct* E = new lif_spiking_neuron_parameters_struct();
+    lif_spiking_neuron_parameters_struct* I = new lif_spiking_neuron_parameters_struct();
+    lif(E, 200e-12f, 10e-9f, 0.0f, 0.0f, 20e-3f, 2e-3f, 160);
+    lif(I, 200e-12f, 10e-9f, 0.0f, 0.0f, 20e-3f, 2e-3f, 40);
+    poisson_input_spiking_neuron_parameters_struct* ip = new poisson_input_spiking_neuron_parameters_struct();
+    ip->group_shape[0] = 1;
+    ip->group_shape[1] = 200;
+    ip->rate = 400.0f;
+    const int in_id = model->AddInputNeuronGroup(ip);
+    const int e_id = model->AddNeuronGroup(E);
+    const int i_id = model->AddNeuronGroup(I);
+    voltage_spiking_synapse_parameters_struct* EX = new voltage_spiking_synapse_parameters_struct();
+    voltage_spiking_synapse_parameters_struct* IN = new voltage_spiking_synapse_parameters_struct();
+    voltage_spiking_synapse_parameters_struct* IP = new voltage_spiking_synapse_parameters_struct();
+    for (auto* p : {EX, IN, IP}) {
+      p->delay_range[0] = 1.5e-3f;
+      p->delay_range[1] = 1.5e-3f;
+      p->weight_scaling_constant = 1.0;
+    }
+    EX->weight_range[0] = EX->weight_range[1] = 0.2e-3f;
+    IN->weight_range[0] = IN->weight_range[1] = -1.0e-3f;
+    IP->weight_range[0] = IP->weight_range[1] = 1.0e-3f;
+    connect_with_sparsity(in_id, e_id, ip, E, IP, 0.1f, model);
+    connect_with_sparsity(in_id, i_id, ip, I, IP, 0.1f, model);
+    connect_with_sparsity(e_id, i_id, E, I, EX, 0.1f, model);
+    if (plastic) EX->plasticity_vec.push_back(rule);
+    connect_with_sparsity(e_id, e_id, E, E, EX, 0.1f, model);
+    connect_with_sparsity(i_id, e_id, I, E, IN, 0.1f, model);
+    connect_with_sparsity(i_id, i_id, I, I, IN, 0.1f, model);
+  } else if (which == "coba" || which == "plain") {
+    const bool coba = which == "coba";
+    ConductanceSpikingSynapses* syn = new ConductanceSpikingSynapses(coba ? 7 : 42);
+    model->spiking_synapses = syn;
+    lif_spiking_neuron_parameters_struct* E = new lif_spiking_neuron_parameters_struct();
+    lif_spiking_neuron_parameters_struct* I = new lif_spiking_neuron_parameters_struct();
+    lif(E, 200e-12f, 10e-9f, -0.060f, -0.060f, -0.050f, coba ? 5e-3f : 2e-3f, coba ? 160 : 200);
+    lif(I, 200e-12f, 10e-9f, -0.060f, -0.060f, -0.050f, coba ? 5e-3f : 2e-3f, 40);
+    poisson_input_spiking_neuron_parameters_struct* ip = new poisson_input_spiking_neuron_parameters_struct();
+    ip->group_shape[0] = 1;
+    ip->group_shape[1] = coba ? 80 : 200;
+    ip->rate = coba ? 60.0f : 40.0f;
+    const int in_id = model->AddInputNeuronGroup(ip);
+    const int e_id = model->AddNeuronGroup(E);
+    const int i_id = coba ? model->AddNeuronGroup(I) : -1;
+    conductance_spiking_synapse_parameters_struct* xi = new conductance_spiking_synapse_parameters_struct();
+    xi->connectivity_type = CONNECTIVITY_TYPE_RANDOM;
+    xi->random_connectivity_probability = 0.1f;
+    xi->weight_scaling_constant = 1e-9f;
+    xi->reversal_potential_Vhat = 0.0f;
+    if (coba) {
+      inhibitory_stdp_plasticity_parameters_struct* vp = new inhibitory_stdp_plasticity_parameters_struct;
+      vp->tau_istdp = 0.02f;
+      vp->learningrate = 0.001f;
+      vp->targetrate = 5.0f;
+      InhibitorySTDPPlasticity* vog = new InhibitorySTDPPlasticity(
+          (SpikingSynapses*)syn, (SpikingNeurons*)neurons, (SpikingNeurons*)inputs, (stdp_plasticity_parameters_struct*)vp);
+      evans_stdp_plasticity_parameters_struct* ep = new evans_stdp_plasticity_parameters_struct;
+      EvansSTDPPlasticity* ev = new EvansSTDPPlasticity(
+          (SpikingSynapses*)syn, (SpikingNeurons*)neurons, (SpikingNeurons*)inputs, (stdp_plasticity_parameters_struct*)ep);
+      if (plastic) {
+        model->AddPlasticityRule(vog);
+        model->AddPlasticityRule(ev);
+        rules.push_back(vog);
+        rules.push_back(ev);
+        rule_kinds.push_back("vogels");
+        rule_kinds.push_back("evans");
+      }
+      conductance_spiking_synapse_parameters_struct* ex = new conductance_spiking_synapse_parameters_struct();
+      conductance_spiking_synapse_parameters_struct* ih = new conductance_spiking_synapse_parameters_struct();
+      for (auto* c : {xi, ex, ih}) {
+        c->delay_range[0] = 0.0008f;
+        c->delay_range[1] = 0.0008f;
+        c->connectivity_type = CONNECTIVITY_TYPE_RANDOM;
+        c->random_connectivity_probability = 0.1f;
+        c->weight_scaling_constant = 1e-9f;
+      }
+      xi->weight_range[0] = 1.0f;
+      xi->weight_range[1] = 3.0f;
+      xi->decay_term_tau_g = 0.005f;
+      ex->weight_range[0] = ex->weight_range[1] = 0.6f;
+      ex->reversal_potential_Vhat = 0.0f;
+      ex->decay_term_tau_g = 0.005f;
+      ih->weight_range[0] = ih->weight_range[1] = 6.7f;
+      ih->reversal_potential_Vhat = -0.080f;
+      ih->decay_term_tau_g = 0.010f;
+      if (plastic) xi->plasticity_vec.push_back(ev);
+      model->AddSynapseGroup(in_id, e_id, xi);
+      model->AddSynapseGroup(e_id, e_id, ex);
+      model->AddSynapseGroup(e_id, i_id, ex);
+      if (plastic) ih->plasticity_vec.push_back(vog);
+      model->AddSynapseGroup(i_id, e_id, ih);
+      ih->plasticity_vec.clear();
+      model->AddSynapseGroup(i_id, i_id, ih);
+    } else {
+      xi->delay_range[0] = 0.0001f;
+      xi->delay_range[1] = 0.0010f;
+      xi->weight_range[0] = 0.0f;
+      xi->weight_range[1] = 3.0f;
+      xi->decay_term_tau_g = 0.001f;
+      model->AddSynapseGroup(in_id, e_id, xi);
+    }
+  } else {
+    fprintf(stderr, "unknown case %s\n", which.c_str());
+    return 2;
+  }
+
+  SpikingActivityMonitor* mon_n = new SpikingActivityMonitor(neurons);
+  SpikingActivityMonitor* mon_i = new SpikingActivityMonitor(inputs);
+  model->AddActivityMonitor(mon_n);
+  model->AddActivityMonitor(mon_i);
+
+  model->finalise_model();
+  SpikingSynapses* s = model->spiking_synapses;
+  const int S = s->total_number_of_synapses;
+  put("syn_pre", s->presynaptic_neuron_indices, S);
+  put("syn_post", s->postsynaptic_neuron_indices, S);
+  put("syn_w", s->synaptic_efficacies_or_weights, S);
+  put("syn_scale", s->weight_scaling_constants, S);
+  put("syn_delay", s->delays, S);
+  put("syn_label", s->syn_labels, S);
+  const int N = neurons->total_number_of_neurons;
+  put("lif_v0", neurons->resting_potentials_v0, N);
+  put("lif_vreset", neurons->after_spike_reset_potentials_vreset, N);
+  put("lif_thr", neurons->thresholds_for_action_potential_spikes, N);
+  put("lif_tau", neurons->membrane_time_constants_tau_m, N);
+  put("lif_R", neurons->membrane_resistances_R, N);
+  put("rates", inputs->rates, inputs->total_number_of_neurons);
+  std::vector<float> decay;
+  std::vector<float> rev;
+  if (auto* cs = dynamic_cast<ConductanceSpikingSynapses*>(s)) {
+    for (size_t l = 0; l < cs->decay_terms_tau_g.size(); ++l) {
+      decay.push_back(expf(-dt / cs->decay_terms_tau_g[l]));
+      rev.push_back(cs->reversal_potentials_Vhat[l]);
+    }
+  }
+  put("decay_g", decay.data(), decay.size());
+  put("reversal", rev.data(), rev.size());
+  for (size_t r = 0; r < rules.size(); ++r)
+    put("plastic_" + std::to_string(r), rules[r]->plastic_synapses.data(), rules[r]->plastic_synapses.size());
+
+  model->run(simtime, plastic);
+
+  s->backend()->copy_weights_to_host();
+  put("weights_after", s->synaptic_efficacies_or_weights, S);
+  mon_n->save_spikes_as_binary(g_out, "neurons_");
+  mon_i->save_spikes_as_binary(g_out, "inputs_");
+
+  std::ofstream j(g_out + "/meta.json");
+  j.precision(9);
+  j << "{\"case\": \"" << which << "\", \"n_neurons\": " << N << ", \"n_inputs\": "
+    << inputs->total_number_of_neurons << ", \"n_synapses\": " << S << ", \"timestep\": " << dt
+    << ", \"timestep_grouping\": " << model->timestep_grouping << ", \"refractory\": "
+    << neurons->refractory_period_in_seconds << ", \"background_current\": " << neurons->background_current
+    << ", \"num_syn_labels\": " << s->num_syn_labels << ", \"conductance\": "
+    << (dynamic_cast<ConductanceSpikingSynapses*>(s) ? 1 : 0) << ", \"simtime\": " << simtime
+    << ", \"plastic\": " << (plastic ? 1 : 0) << ", \"rule_kinds\": [";
+  for (size_t r = 0; r < rule_kinds.size(); ++r) j << (r ? ", " : "") << "\"" << rule_kinds[r] << "\"";
+  j << "], \"rule_params\": [";
+  for (size_t r = 0; r < rules.size(); ++r) {
+    j << (r ? ", " : "") << "{";
+    if (auto* wd = dynamic_cast<WeightDependentSTDPPlasticity*>(rules[r])) {
+      auto* q = wd->stdp_params;
+      j << "\"a_minus\": " << q->a_minus << ", \"a_plus\": " << q->a_plus << ", \"tau_minus\": " << q->tau_minus
+        << ", \"tau_plus\": " << q->tau_plus << ", \"lambda\": " << q->lambda << ", \"alpha\": " << q->alpha
+        << ", \"w_max\": " << q->w_max << ", \"nearest_spike_only\": " << (q->nearest_spike_only ? 1 : 0);
+    } else if (auto* vg = dynamic_cast<InhibitorySTDPPlasticity*>(rules[r])) {
+      auto* q = vg->stdp_params;
+      j << "\"tau_istdp\": " << q->tau_istdp << ", \"learningrate\": " << q->learningrate
+        << ", \"targetrate\": " << q->targetrate << ", \"momentumrate\": " << q->momentumrate;
+    } else if (auto* ev = dynamic_cast<EvansSTDPPlasticity*>(rules[r])) {
+      auto* q = ev->stdp_params;
+      j << "\"decay_term_tau_D\": " << q->decay_term_tau_D << ", \"model_parameter_alpha_D\": "
+        << q->model_parameter_alpha_D << ", \"synaptic_neurotransmitter_concentration_alpha_C\": "
+        << q->synaptic_neurotransmitter_concentration_alpha_C << ", \"decay_term_tau_C\": " << q->decay_term_tau_C
+        << ", \"learning_rate_rho\": " << q->learning_rate_rho;
+    }
+    j << "}";
+  }
+  j << "], \"spikes_neurons\": " << mon_n->total_number_of_spikes_stored_on_host
+    << ", \"spikes_inputs\": " << mon_i->total_number_of_spikes_stored_on_host << "}\n";
+  return 0;
+}

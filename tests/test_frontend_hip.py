@@ -1,0 +1,124 @@
+"""Drop-in path: Spike's unchanged C++ front-end running on the HIP backend.
+
+oracle/_ref/frontend_net_hip (tests/frontend/frontend_net.cpp linked with the reference's
+front-end library and spike_amd/backend, default backend "HIP") builds a network through
+SpikingModel / neuron groups / synapse groups / plasticity rules / SpikingActivityMonitor,
+runs SpikingModel::run and saves the monitors' rasters with save_spikes_as_binary.  The
+same network (its finalised arrays, dumped by the program) is replayed on the FP32 oracle;
+rasters and plastic weights must match bit for bit.
+"""
+import json
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+from spike_amd import netdesc  # noqa: E402
+
+PROG = os.path.join(ROOT, "oracle", "_ref", "frontend_net_hip")
+F32 = np.float32
+
+pytestmark = pytest.mark.skipif(not os.path.exists(PROG), reason="front-end programs not built (needs the Spike source tree)")
+
+
+def run_program(tmp_path, case, simtime, plastic):
+    out = str(tmp_path / case)
+    args = [PROG, case, out, str(simtime)] + (["plastic"] if plastic else [])
+    print("frontend_net_hip", " ".join(args[1:]), flush=True)
+    r = subprocess.run(args, capture_output=True, text=True, timeout=180)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    meta = json.load(open(os.path.join(out, "meta.json")))
+
+    def arr(name, dt):
+        return np.fromfile(os.path.join(out, name + ".bin"), dtype=dt)
+
+    return meta, arr, r.stdout
+
+
+def desc_from_dump(meta, arr):
+    dt = F32(meta["timestep"])
+    tau = arr("lif_tau", F32)
+    R = arr("lif_R", F32)
+    c = (dt / tau).astype(F32)
+    rp = (c * R).astype(F32)
+    n_in = meta["n_inputs"]
+    cond = meta["conductance"] == 1
+    kinds = {"wd": netdesc.RULE_WD, "vogels": netdesc.RULE_VOGELS, "evans": netdesc.RULE_EVANS}
+    rules = []
+    for r, (k, p) in enumerate(zip(meta["rule_kinds"], meta["rule_params"])):
+        rules.append(netdesc.Rule(kinds[k], arr("plastic_%d" % r, np.int32).astype(np.int64), p))
+    d = netdesc.NetDesc(
+        n_neurons=meta["n_neurons"], n_inputs=n_in,
+        kind=netdesc.SYN_CONDUCTANCE if cond else netdesc.SYN_VOLTAGE,
+        n_labels=max(1, meta["num_syn_labels"]) if cond else 1, dt=float(dt),
+        v0=arr("lif_v0", F32), vreset=arr("lif_vreset", F32), threshold=arr("lif_thr", F32),
+        decay_c=c, resist=rp, refractory=meta["refractory"], background=meta["background_current"],
+        rates=arr("rates", F32).reshape(1, n_in), seed=1,
+        decay_g=arr("decay_g", F32) if cond else None, reversal=arr("reversal", F32) if cond else None,
+        pre=arr("syn_pre", np.int32), post=arr("syn_post", np.int32), weight=arr("syn_w", F32),
+        scale=arr("syn_scale", F32), delay=arr("syn_delay", np.int32), label=arr("syn_label", np.int32),
+        rules=rules, monitors=[netdesc.POP_NEURONS, netdesc.POP_INPUTS],
+        grouping=meta["timestep_grouping"], name="frontend_" + meta["case"])
+    return d
+
+
+def oracle_run(d, simtime, plastic):
+    import oracle_py
+    o = oracle_py.OracleNetwork(d, precision=32, threads=4)
+    o.run(0.0, d.groups_for(simtime), plasticity=plastic)
+    n_ids, n_t = o.take_spikes(0)
+    i_ids, i_t = o.take_spikes(1)
+    w = o.weights().astype(np.float32)   # the FP32 oracle's weights, returned widened
+    o.close()
+    return (n_ids, n_t), (i_ids, i_t), w
+
+
+def sort_raster(ids, times):
+    order = np.lexsort((ids, times))
+    return ids[order], times[order]
+
+
+CASES = [("brunel", 0.1, False), ("brunel", 0.1, True), ("coba", 0.1, True), ("plain", 0.05, False)]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case,simtime,plastic", CASES)
+def test_frontend_run_matches_oracle(tmp_path, case, simtime, plastic):
+    meta, arr, stdout = run_program(tmp_path, case, simtime, plastic)
+    d = desc_from_dump(meta, arr)
+    (on_ids, on_t), (oi_ids, oi_t), ow = oracle_run(d, simtime, plastic)
+    hn_ids, hn_t = sort_raster(arr("neurons_SpikeIDs", np.int32), arr("neurons_SpikeTimes", F32))
+    hi_ids, hi_t = sort_raster(arr("inputs_SpikeIDs", np.int32), arr("inputs_SpikeTimes", F32))
+    on_ids, on_t = sort_raster(on_ids, on_t)
+    oi_ids, oi_t = sort_raster(oi_ids, oi_t)
+    assert len(on_ids) > 0, "network is silent"
+    assert meta["spikes_neurons"] == len(hn_ids)
+    np.testing.assert_array_equal(hi_ids, oi_ids)
+    np.testing.assert_array_equal(hi_t.view(np.uint32), oi_t.view(np.uint32))
+    np.testing.assert_array_equal(hn_ids, on_ids)
+    np.testing.assert_array_equal(hn_t.view(np.uint32), on_t.view(np.uint32))
+    w = arr("weights_after", F32)
+    np.testing.assert_array_equal(w.view(np.uint32), ow.view(np.uint32))
+    if plastic:
+        assert not np.array_equal(w, arr("syn_w", F32)), "plasticity changed no weight"
+
+
+@pytest.mark.gpu
+def test_brunel_example_unchanged_runs():
+    """Examples/Brunel10K.cpp, compiled unchanged against the HIP backend, runs 0.2 s of
+    the 10,000-neuron network with its spike monitor."""
+    prog = os.path.join(ROOT, "oracle", "_ref", "Brunel10K_hip")
+    print("Brunel10K_hip --simtime 0.2", flush=True)
+    r = subprocess.run([prog, "--simtime", "0.2"], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    lines = [l for l in r.stdout.splitlines() if "Number of Spikes Recorded" in l]
+    assert lines, r.stdout[-2000:]
+    n = int(lines[-1].split(":")[1])
+    # 10,000 neurons over 0.2 s: a silent or runaway network is a failure
+    assert 0 < n < 10000 * 0.2 * 500

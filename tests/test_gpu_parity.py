@@ -212,3 +212,39 @@ def test_weights_roundtrip_and_errors():
         gpu.set_grouping(100)
     with pytest.raises(abi.SpikeError):
         abi.check(gpu.lib.spk_update_plasticity(gpu.h, 5, 0.0))
+
+
+@pytest.mark.parametrize("launches", [False, True])
+def test_step_group_async(launches):
+    """Groups enqueued one at a time (the backend's per-group call) give the oracle's run;
+    monitor records stay in the device stores until drained."""
+    d = _plastic_net(RULE_WD, 50)
+    d.monitors = [0, 1]
+    ng = 120
+    print(f"[parity] step_group {d.name} groups={ng} launches={launches}", flush=True)
+    gpu = abi.Network(d, partitions=5)
+    ora = oracle_py.OracleNetwork(d, precision=32)
+    for s in range(ng):
+        gpu.step_group(d.group_time(0.0, s), plasticity=True, force_launches=launches)
+    gpu.synchronize()
+    ora.run(0.0, ng)
+    c = gpu.counters()
+    oc = ora.counters()
+    assert c.spikes_neurons == oc["spikes_neurons"] and c.synaptic_events == oc["synaptic_events"]
+    _compare(gpu, ora, d, f"step_group launches={launches}")
+
+
+def test_run_then_steps_then_run():
+    """spk_run, single-group steps and spk_run again continue one trajectory."""
+    d = builders.random_net(seed=21, input_rate=400.0, whi=4e-3, dmin=2, dmax=7, ext_rate=2000.0,
+                            ext_w=1e-3)
+    print(f"[parity] run/steps/run {d.name}", flush=True)
+    gpu = abi.Network(d, partitions=4)
+    ora = oracle_py.OracleNetwork(d, precision=32)
+    gpu.run(0.0, 40)
+    for s in range(40, 70):
+        gpu.step_group(d.group_time(0.0, s))
+    gpu.run(d.group_time(0.0, 70), 50)
+    ora.run(0.0, 70)
+    ora.run(d.group_time(0.0, 70), 50)
+    _compare(gpu, ora, d, "run/steps/run")
