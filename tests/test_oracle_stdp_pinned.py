@@ -1,0 +1,200 @@
+"""STDP rules of the oracle against event-driven textbook restatements.
+
+Two LIF neurons with a constant background current fire periodically at different rates
+(neuron 0 is presynaptic, neuron 1 postsynaptic); one plastic synapse joins them with
+scale 0, so it learns without driving the postsynaptic neuron.  The oracle (FP64) runs the
+reference's per-group / per-step trace algorithms; the expected weights below are computed
+in float64 from the spike steps alone, with the textbook forms of each rule:
+
+* weight-dependent STDP (WeightDependentSTDPPlasticity.cu:93-158): pair-based, all-to-all
+  (or nearest-spike) exponential windows; potentiation lambda*(w_max-w)*A+*exp(-dt/tau+)
+  at each postsynaptic spike, depression lambda*alpha*w*A-*exp(-dt/tau-) at each
+  presynaptic spike, one update per timestep group from the group's starting weight,
+  no delays, clamp at 0;
+* Vogels inhibitory STDP (InhibitorySTDPPlasticity.cu:109-144): symmetric window with
+  unit traces, the target term eta*2*rho0*tau at every (delayed) presynaptic arrival;
+* Evans STDP (EvansSTDPPlasticity.cu:96-131): saturating traces with geometric decay
+  (1-dt/tau)^n across gaps, w clamped to [0, 1].
+
+A dropped term, a swapped A+/A- or tau, a wrong sign or a wrong window index changes the
+weight change by O(1); the tolerance only absorbs the oracle's float32 decay constants.
+"""
+import math
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+import oracle_py  # noqa: E402
+from spike_amd import netdesc  # noqa: E402
+from spike_amd.netdesc import NetDesc, Rule, RULE_WD, RULE_VOGELS, RULE_EVANS  # noqa: E402
+
+F32 = np.float32
+DT = 1e-4
+
+
+def pair_net(rule, w0, delay, thresholds=(10e-3, 12.5e-3), ibg=30e-3, tref=2e-3):
+    c, rp = netdesc.lif_constants(2, 200e-12, 10e-9, DT)
+    return NetDesc(
+        n_neurons=2, n_inputs=0, dt=DT,
+        v0=np.zeros(2, F32), vreset=np.zeros(2, F32), threshold=np.array(thresholds, F32),
+        decay_c=c, resist=rp, refractory=tref, background=ibg,
+        pre=np.array([0], np.int32), post=np.array([1], np.int32), weight=np.array([w0], F32),
+        scale=np.array([0.0], F32), delay=np.array([delay], np.int32), label=np.array([0], np.int32),
+        rules=[rule], monitors=[netdesc.POP_NEURONS], name="pair")
+
+
+def run_oracle(d, ngroups, precision=64):
+    o = oracle_py.OracleNetwork(d, precision=precision)
+    o.run(0.0, ngroups)
+    ids, t = o.take_spikes(0)
+    w = float(o.weights()[0])
+    o.close()
+    steps = np.rint(t.astype(np.float64) / DT).astype(np.int64)
+    return sorted(steps[ids == 0].tolist()), sorted(steps[ids == 1].tolist()), w
+
+
+def wd_expected(pre, post, w0, p, G, ngroups, nearest):
+    ap, am, tp, tm = p["a_plus"], p["a_minus"], p["tau_plus"], p["tau_minus"]
+    lam, alpha, wmax = p["lambda"], p["alpha"], p["w_max"]
+    w = w0
+    for g in range(ngroups):
+        lo, hi = g * G, (g + 1) * G
+        pre_g = [s for s in pre if lo <= s < hi]
+        post_g = [s for s in post if lo <= s < hi]
+        upd = 0.0
+        if pre_g:
+            t = pre_g[-1]
+            earlier = [q for q in post if q <= t]
+            if nearest:
+                earlier = earlier[-1:]
+            upd -= lam * alpha * w * sum(am * math.exp(-(t - q) * DT / tm) for q in earlier)
+        if post_g:
+            t = post_g[-1]
+            earlier = [q for q in pre if q <= t]
+            if nearest:
+                earlier = earlier[-1:]
+            upd += lam * (wmax - w) * sum(ap * math.exp(-(t - q) * DT / tp) for q in earlier)
+        w = max(0.0, w + upd)
+    return w
+
+
+def vogels_expected(pre, post, w0, p, delay, nsteps):
+    tau, eta, rho = p["tau_istdp"], p["learningrate"], p["targetrate"]
+    arr = [s + delay for s in pre if s + delay < nsteps]
+    w = w0
+    for t in sorted(set(arr) | set(s for s in post if s < nsteps)):
+        xpre = sum(math.exp(-(t - a) * DT / tau) for a in arr if a <= t)
+        xpost = sum(math.exp(-(t - q) * DT / tau) for q in post if q <= t)
+        dw = 0.0
+        if t in arr:
+            dw += eta * xpost - eta * 2.0 * rho * tau
+        if t in post:
+            dw += eta * xpre
+        w = max(0.0, w + dw)
+    return w
+
+
+def evans_expected(pre, post, w0, p, delay, nsteps):
+    dC = 1.0 - DT / p["decay_term_tau_C"]
+    dD = 1.0 - DT / p["decay_term_tau_D"]
+    aC = DT * p["synaptic_neurotransmitter_concentration_alpha_C"]
+    aD = DT * p["model_parameter_alpha_D"]
+    arr = [s + delay for s in pre if s + delay < nsteps]
+    C = D = 0.0
+    w = w0
+    last = -1
+    for t in sorted(set(arr) | set(s for s in post if s < nsteps)):
+        C *= dC ** (t - last)
+        D *= dD ** (t - last)
+        last = t
+        isp, isq = t in arr, t in post
+        if isp:
+            C += aC * (1.0 - C)
+        if isq:
+            D += aD * (1.0 - D)
+        dw = (-w * D if isp else 0.0) + ((1.0 - w) * C if isq else 0.0)
+        w = min(1.0, max(0.0, w + dw))
+    return w
+
+
+def close(got, exp, w0, rtol=3e-5):
+    """Weight changes agree to rtol (and the change is not trivially zero)."""
+    dg, de = got - w0, exp - w0
+    assert abs(de) > 0.0
+    assert abs(dg - de) <= rtol * abs(de) + 1e-15, (got, exp, w0)
+
+
+@pytest.mark.parametrize("G,nearest", [(3, 0), (1, 0), (1, 1), (5, 0)])
+def test_wd_pair_protocol(G, nearest):
+    p = dict(a_plus=1.0, a_minus=0.8, tau_plus=0.0168, tau_minus=0.0337, **{"lambda": 0.01},
+             alpha=1.6, w_max=1.0, nearest_spike_only=nearest)
+    w0 = 0.4
+    d = pair_net(Rule(RULE_WD, np.array([0], np.int64), p), w0, delay=G)
+    ng = 3000 // G
+    pre, post, w = run_oracle(d, ng)
+    assert len(pre) > 20 and len(post) > 20
+    close(w, wd_expected(pre, post, w0, p, G, ng, nearest), w0)
+
+
+def test_wd_depression_clamps_at_zero():
+    p = dict(a_plus=0.0, a_minus=1.0, tau_plus=0.02, tau_minus=0.02, **{"lambda": 1.0},
+             alpha=5.0, w_max=1.0, nearest_spike_only=0)
+    d = pair_net(Rule(RULE_WD, np.array([0], np.int64), p), 0.3, delay=2)
+    pre, post, w = run_oracle(d, 1000)
+    assert w == 0.0 and wd_expected(pre, post, 0.3, p, 2, 1000, False) == 0.0
+
+
+@pytest.mark.parametrize("delay", [1, 4])
+def test_vogels_protocol(delay):
+    p = dict(tau_istdp=0.02, learningrate=1e-3, targetrate=5.0, momentumrate=0.0)
+    w0 = 0.5
+    d = pair_net(Rule(RULE_VOGELS, np.array([0], np.int64), p), w0, delay=delay)
+    ng = 3000 // delay
+    pre, post, w = run_oracle(d, ng)
+    close(w, vogels_expected(pre, post, w0, p, delay, ng * delay), w0)
+
+
+def test_vogels_target_term_clamps():
+    p = dict(tau_istdp=0.02, learningrate=0.5, targetrate=400.0, momentumrate=0.0)
+    d = pair_net(Rule(RULE_VOGELS, np.array([0], np.int64), p), 0.5, delay=2)
+    pre, post, w = run_oracle(d, 1000)
+    assert w == 0.0 and vogels_expected(pre, post, 0.5, p, 2, 2000) == 0.0
+
+
+@pytest.mark.parametrize("delay,w0", [(2, 0.5), (6, 0.05), (3, 0.97)])
+def test_evans_protocol(delay, w0):
+    p = dict(decay_term_tau_D=0.005, model_parameter_alpha_D=0.5,
+             synaptic_neurotransmitter_concentration_alpha_C=0.5, decay_term_tau_C=0.004,
+             learning_rate_rho=0.1)
+    # faster firing (shorter refractory) so both traces stay large
+    d = pair_net(Rule(RULE_EVANS, np.array([0], np.int64), p), w0, delay=delay, thresholds=(5e-3, 6e-3),
+                 tref=1e-3)
+    ng = 3000 // delay
+    pre, post, w = run_oracle(d, ng)
+    close(w, evans_expected(pre, post, w0, p, delay, ng * delay), w0)
+
+
+def test_evans_clamp_to_unit_interval():
+    p = dict(decay_term_tau_D=0.005, model_parameter_alpha_D=400.0,
+             synaptic_neurotransmitter_concentration_alpha_C=400.0, decay_term_tau_C=0.004,
+             learning_rate_rho=0.1)
+    d = pair_net(Rule(RULE_EVANS, np.array([0], np.int64), p), 0.5, delay=2, thresholds=(5e-3, 6e-3), tref=1e-3)
+    pre, post, w = run_oracle(d, 1000)
+    exp = evans_expected(pre, post, 0.5, p, 2, 2000)
+    assert 0.0 <= w <= 1.0
+    assert abs(w - exp) <= 1e-6
+
+
+def test_fp32_oracle_follows_fp64_on_pair_protocol():
+    p = dict(a_plus=1.0, a_minus=0.8, tau_plus=0.0168, tau_minus=0.0337, **{"lambda": 0.01},
+             alpha=1.6, w_max=1.0, nearest_spike_only=0)
+    d = pair_net(Rule(RULE_WD, np.array([0], np.int64), p), 0.4, delay=3)
+    _, _, w64 = run_oracle(d, 1000, precision=64)
+    _, _, w32 = run_oracle(d, 1000, precision=32)
+    assert abs(w32 - w64) <= 1e-5 * abs(w64 - 0.4)
