@@ -243,6 +243,24 @@ int spk_get_counters(spk_network* net, spk_timing* counters);
 int spk_synchronize(spk_network* net);
 
 /* ---------------------------------------------------------------------------- */
+/* FP64 spike statistics of the LIF population (no reference counterpart: the reference
+ * leaves analysis to scripts over SpikeIDs.bin / SpikeTimes.bin, SpikingActivityMonitor.cpp:
+ * 140-153).  Accumulated on the device where spikes are made, over the window of
+ * `max_steps` steps that starts at the first step simulated after spk_stats_begin, and
+ * reduced in FP64 on the device:
+ *   rate_hz        spikes / (N * steps * dt)
+ *   isi_cv         mean over neurons with >= 2 in-window intervals of std/mean of them
+ *   synchrony      variance / mean of the population spike count per bin of bin_steps
+ *   mean_bin_count mean population spike count per bin */
+typedef struct {
+  double rate_hz, isi_cv, synchrony, mean_bin_count;
+  int64_t spikes, cv_neurons, bins, steps;
+} spk_stats;
+int spk_stats_begin(spk_network* net, int bin_steps, int64_t max_steps);
+int spk_stats_get(spk_network* net, spk_stats* stats);
+int spk_stats_end(spk_network* net);
+
+/* ---------------------------------------------------------------------------- */
 /* read-out */
 
 /* Records currently in the device store of a monitor

@@ -65,6 +65,13 @@ lp = C.POINTER(C.c_int64)
 up = C.POINTER(C.c_uint32)
 
 # name -> (restype, argtypes)
+class Stats(C.Structure):
+    """spk_stats of include/spike_hip.h."""
+    _fields_ = [("rate_hz", C.c_double), ("isi_cv", C.c_double), ("synchrony", C.c_double),
+                ("mean_bin_count", C.c_double), ("spikes", C.c_int64), ("cv_neurons", C.c_int64),
+                ("bins", C.c_int64), ("steps", C.c_int64)]
+
+
 SIGNATURES = {
     "spk_last_error": (C.c_char_p, []),
     "spk_abi_version": (C.c_int, []),
@@ -102,6 +109,9 @@ SIGNATURES = {
     "spk_step_group": (C.c_int, [P, C.c_float, C.c_int]),
     "spk_get_counters": (C.c_int, [P, C.POINTER(Timing)]),
     "spk_monitor_reset": (C.c_int, [P, C.c_int]),
+    "spk_stats_begin": (C.c_int, [P, C.c_int, C.c_int64]),
+    "spk_stats_get": (C.c_int, [P, C.POINTER(Stats)]),
+    "spk_stats_end": (C.c_int, [P]),
     "spk_synchronize": (C.c_int, [P]),
     "spk_monitor_device_count": (C.c_int, [P, C.c_int, lp]),
     "spk_monitor_drain": (C.c_int, [P, C.c_int, ip, fp, C.c_int64, lp]),
@@ -263,6 +273,17 @@ class Network:
         """One group enqueued without waiting (the backend's per-group call)."""
         flags = (SPK_RUN_PLASTICITY if plasticity else 0) | (SPK_RUN_FORCE_LAUNCHES if force_launches else 0)
         check(self.lib.spk_step_group(self.h, float(t), flags))
+
+    def stats_begin(self, bin_steps: int = 10, max_steps: int = 1 << 20):
+        check(self.lib.spk_stats_begin(self.h, int(bin_steps), int(max_steps)))
+
+    def stats(self) -> Stats:
+        st = Stats()
+        check(self.lib.spk_stats_get(self.h, C.byref(st)))
+        return st
+
+    def stats_end(self):
+        check(self.lib.spk_stats_end(self.h))
 
     def counters(self) -> Timing:
         t = Timing()

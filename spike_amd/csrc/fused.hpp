@@ -93,6 +93,7 @@ struct FusedArgs {
   int n_mons;
   // barrier generations completed on the counters before this launch
   unsigned int bar_base;
+  StatsArgs st;                 // window statistics of the LIF population (st.on = 0: off)
   unsigned long long* stats;    // [0] deliveries [1] external [2] LIF spikes [3] input spikes
   int64_t q_ext;
   int64_t arm_step;
@@ -135,7 +136,8 @@ struct PhaseClock {
 // LDS layout of one workgroup: every sub-array starts on a 16-byte boundary (the input
 // spike masks take 64-bit LDS atomics, which need 8-byte aligned addresses).
 struct FusedLayout {
-  size_t ring, hist, ext, imask, inp, xbuf, head, scan, ent, counter, total;
+  size_t ring, hist, ext, imask, inp, xbuf, head, scan, ent, counter, st_cnt, st_nisi, st_prev, st_s1,
+      st_s2, st_g, total;
 };
 
 constexpr int kHeadWords = 16;         // words of a source block fetched in the first trip
@@ -159,6 +161,13 @@ __host__ __device__ inline FusedLayout fused_layout(int B, int L, int W, int G, 
   f.scan = o;    o = align16(o + (size_t)(P + 1) * 4);
   f.ent = o;     o = align16(o + (size_t)kEntCap * 4);
   f.counter = o; o = align16(o + 4 * 4);
+  // window statistics of the owned neurons, and spikes per sub-step of the group
+  f.st_cnt = o;  o = align16(o + (size_t)nown * 4);
+  f.st_nisi = o; o = align16(o + (size_t)nown * 4);
+  f.st_prev = o; o = align16(o + (size_t)nown * 4);
+  f.st_s1 = o;   o = align16(o + (size_t)nown * 8);
+  f.st_s2 = o;   o = align16(o + (size_t)nown * 8);
+  f.st_g = o;    o = align16(o + 64 * 4);
   f.total = o;
   return f;
 }
@@ -376,6 +385,12 @@ __global__ __launch_bounds__(kFusedBlock) void k_fused(FusedArgs a) {
   int* sscan = (int*)(smem + f.scan);
   uint32_t* sent = (uint32_t*)(smem + f.ent);
   int* scounter = (int*)(smem + f.counter);
+  unsigned int* sst_cnt = (unsigned int*)(smem + f.st_cnt);
+  unsigned int* sst_nisi = (unsigned int*)(smem + f.st_nisi);
+  float* sst_prev = (float*)(smem + f.st_prev);
+  double* sst_s1 = (double*)(smem + f.st_s1);
+  double* sst_s2 = (double*)(smem + f.st_s2);
+  unsigned int* sst_g = (unsigned int*)(smem + f.st_g);
   __shared__ int s_ok;
   __shared__ unsigned long long s_prof[kProfPhases];
 
@@ -388,6 +403,16 @@ __global__ __launch_bounds__(kFusedBlock) void k_fused(FusedArgs a) {
   for (int k = tid; k < W * nown; k += kFusedBlock) {
     const int wd = k / nown, i = k - wd * nown;
     shist[k] = a.hist_n[(size_t)wd * a.N + nlo + i];
+  }
+  if (a.st.on) {
+    for (int i = tid; i < nown; i += kFusedBlock) {
+      sst_cnt[i] = a.st.count[nlo + i];
+      sst_nisi[i] = a.st.nisi[nlo + i];
+      sst_prev[i] = a.st.prev[nlo + i];
+      sst_s1[i] = a.st.s1[nlo + i];
+      sst_s2[i] = a.st.s2[nlo + i];
+    }
+    if (tid < 64) sst_g[tid] = 0u;
   }
   // per-lane neuron state (lane i owns neuron nlo+i; nown <= kFusedBlock)
   const bool own = tid < nown;
@@ -548,6 +573,21 @@ __global__ __launch_bounds__(kFusedBlock) void k_fused(FusedArgs a) {
           if ((smask >> g) & 1ull) word |= (1u << (b & 31));
         }
         if (cur_w >= 0) shist[cur_w * nown + tid] = word;
+        if (a.st.on && smask) {
+          // window statistics of this neuron's spikes, in time order
+          unsigned int c0 = sst_cnt[tid], c1 = sst_nisi[tid];
+          float pv = sst_prev[tid];
+          double d1 = sst_s1[tid], d2 = sst_s2[tid];
+          unsigned long long mm = smask;
+          while (mm) {
+            const int g = __ffsll((unsigned long long)mm) - 1;
+            mm &= mm - 1ull;
+            stats_spike(a.st, (int64_t)tidx + g, t + (float)g * a.dt, c0, c1, pv, d1, d2);
+            atomicAdd(&sst_g[g], 1u);
+          }
+          sst_cnt[tid] = c0; sst_nisi[tid] = c1; sst_prev[tid] = pv;
+          sst_s1[tid] = d1; sst_s2[tid] = d2;
+        }
       }
       if (smask) {
         const int c = __popcll(smask);
@@ -563,6 +603,14 @@ __global__ __launch_bounds__(kFusedBlock) void k_fused(FusedArgs a) {
       }
     }
     __syncthreads();
+    if (a.st.on && tid < G) {
+      // population spikes of this partition per sub-step into the window's bins
+      const unsigned int c = sst_g[tid];
+      const int64_t step = (int64_t)tidx + tid;
+      if (c && step >= a.st.t0 && step < a.st.t0 + a.st.nsteps)
+        atomicAdd(a.st.hist + (step - a.st.t0) / a.st.bin, c);
+      sst_g[tid] = 0u;
+    }
     clk.lap(a, 1);
     // -- C. publish: {count, entries} block, and state other workgroups may read ------
     const int nx = min(scounter[0], a.xcap);
@@ -706,6 +754,15 @@ __global__ __launch_bounds__(kFusedBlock) void k_fused(FusedArgs a) {
   for (int k = tid; k < W * nown; k += kFusedBlock) {
     const int wd = k / nown, i = k - wd * nown;
     a.hist_n[(size_t)wd * a.N + nlo + i] = shist[k];
+  }
+  if (a.st.on) {
+    for (int i = tid; i < nown; i += kFusedBlock) {
+      a.st.count[nlo + i] = sst_cnt[i];
+      a.st.nisi[nlo + i] = sst_nisi[i];
+      a.st.prev[nlo + i] = sst_prev[i];
+      a.st.s1[nlo + i] = sst_s1[i];
+      a.st.s2[nlo + i] = sst_s2[i];
+    }
   }
   // statistics: one atomic per wavefront
   unsigned long long v4[4] = {st_del, st_ext, st_spk, st_in};

@@ -73,6 +73,36 @@ __global__ void k_begin_group(int* __restrict__ counts, int G, float* __restrict
 // ---------------------------------------------------------------------------------
 // LIF neurons (LIFSpikingNeurons.cu:91-195; injection InlineDeviceFunctions.hpp:22-111)
 
+// ---------------------------------------------------------------------------------
+// Spike statistics of the LIF population over a window of steps [t0, t0 + nsteps):
+// per neuron the spike count and the sums of its inter-spike intervals (FP64), and the
+// population spike count per bin of `bin` steps.  Accumulated where the spikes are made,
+// reduced in FP64 by k_stats_reduce.
+struct StatsArgs {
+  unsigned int* count;   // [N] spikes in the window
+  unsigned int* nisi;    // [N] intervals between consecutive in-window spikes
+  float* prev;           // [N] time of the neuron's last in-window spike
+  double* s1;            // [N] sum of intervals (s)
+  double* s2;            // [N] sum of squared intervals
+  unsigned int* hist;    // [nbins] population spikes per bin
+  int64_t t0, nsteps;
+  int bin, on;
+};
+
+// One spike of neuron i at step `step`, time tg: update its running statistics.
+__device__ __forceinline__ void stats_spike(const StatsArgs& st, int64_t step, float tg, unsigned int& cnt,
+                                            unsigned int& nisi, float& prev, double& s1, double& s2) {
+  if (step < st.t0 || step >= st.t0 + st.nsteps) return;
+  if (cnt > 0u) {
+    const double isi = (double)tg - (double)prev;
+    s1 += isi;
+    s2 += isi * isi;
+    ++nisi;
+  }
+  ++cnt;
+  prev = tg;
+}
+
 struct LifArgs {
   float* V;
   float* last;
@@ -95,6 +125,7 @@ struct LifArgs {
   float t, dt, tref, ibg, inv_scale;
   int64_t q_ext;              // fixed-point value of one external event
   uint64_t seed;
+  StatsArgs st;
 };
 
 template <int KIND>
@@ -121,6 +152,13 @@ __global__ __launch_bounds__(256) void k_lif(LifArgs a) {
   uint32_t word = 0;
   unsigned long long ext_sum = 0;
   const float half_dt = 0.5f * a.dt;
+  unsigned int st_cnt = 0u, st_nisi = 0u;
+  float st_prev = 0.0f;
+  double st_s1 = 0.0, st_s2 = 0.0;
+  if (a.st.on && active) {
+    st_cnt = a.st.count[i]; st_nisi = a.st.nisi[i]; st_prev = a.st.prev[i];
+    st_s1 = a.st.s1[i]; st_s2 = a.st.s2[i];
+  }
   for (int g = 0; g < a.G; ++g) {
     const int64_t step = (int64_t)a.tidx + g;
     const int b = hist_bitpos(step, a.W);
@@ -179,6 +217,10 @@ __global__ __launch_bounds__(256) void k_lif(LifArgs a) {
     }
     spk = spk && active;
     if (spk) word |= (1u << (b & 31));
+    if (spk && a.st.on) {
+      stats_spike(a.st, step, tg, st_cnt, st_nisi, st_prev, st_s1, st_s2);
+      if (step >= a.st.t0 && step < a.st.t0 + a.st.nsteps) atomicAdd(a.st.hist + (step - a.st.t0) / a.st.bin, 1u);
+    }
     wave_append(spk, (uint32_t)i, a.lists + (size_t)g * a.cap, a.counts + g, a.cap, a.spike_stat);
   }
   if (active) {
@@ -187,6 +229,10 @@ __global__ __launch_bounds__(256) void k_lif(LifArgs a) {
     a.last[i] = last;
     if (KIND == 1) {
       for (int l = 0; l < a.L; ++l) a.gcond[(size_t)i * a.L + l] = gl[l];
+    }
+    if (a.st.on) {
+      a.st.count[i] = st_cnt; a.st.nisi[i] = st_nisi; a.st.prev[i] = st_prev;
+      a.st.s1[i] = st_s1; a.st.s2[i] = st_s2;
     }
   }
   if (a.ext_k > 0) {
@@ -491,6 +537,75 @@ __global__ __launch_bounds__(256) void k_collect(CollectArgs a) {
         a.times[pos] = tg;
       }
     }
+  }
+}
+
+// ---------------------------------------------------------------------------------
+// FP64 reduction of the window statistics (one workgroup; fixed summation order, so the
+// result is reproducible):
+//   rate      = spikes / (N * nsteps * dt)
+//   isi_cv    = mean over neurons with >= 2 intervals of sqrt(var)/mean of their intervals
+//               (population variance s2/n - (s1/n)^2)
+//   synchrony = variance / mean of the population spike count per bin (Fano factor)
+struct StatsOut {
+  double rate_hz, isi_cv, synchrony, mean_bin_count;
+  long long spikes, cv_neurons, bins, steps;
+};
+
+constexpr int kStatsBlock = 1024;
+
+__device__ double block_sum_f64(double v, double* sh) {
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int off = kStatsBlock / 2; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+    __syncthreads();
+  }
+  const double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+__global__ __launch_bounds__(kStatsBlock) void k_stats_reduce(StatsArgs st, int N, int64_t steps, float dt,
+                                                              StatsOut* out) {
+  __shared__ double sh[kStatsBlock];
+  double spikes = 0.0, cv = 0.0, ncv = 0.0;
+  for (int i = threadIdx.x; i < N; i += kStatsBlock) {
+    spikes += (double)st.count[i];
+    const unsigned int n = st.nisi[i];
+    if (n >= 2u) {
+      const double m = st.s1[i] / (double)n;
+      double var = st.s2[i] / (double)n - m * m;
+      if (var < 0.0) var = 0.0;
+      cv += sqrt(var) / m;
+      ncv += 1.0;
+    }
+  }
+  const int64_t nb = steps > 0 ? (steps + st.bin - 1) / st.bin : 0;
+  double hs = 0.0;
+  for (int64_t b = threadIdx.x; b < nb; b += kStatsBlock) hs += (double)st.hist[b];
+  const double S = block_sum_f64(spikes, sh);
+  const double CV = block_sum_f64(cv, sh);
+  const double NCV = block_sum_f64(ncv, sh);
+  const double HS = block_sum_f64(hs, sh);
+  const double mean = nb > 0 ? HS / (double)nb : 0.0;
+  double hv = 0.0;
+  for (int64_t b = threadIdx.x; b < nb; b += kStatsBlock) {
+    const double d = (double)st.hist[b] - mean;
+    hv += d * d;
+  }
+  const double HV = block_sum_f64(hv, sh);
+  if (threadIdx.x == 0) {
+    StatsOut o;
+    o.spikes = (long long)S;
+    o.cv_neurons = (long long)NCV;
+    o.bins = (long long)nb;
+    o.steps = (long long)steps;
+    o.rate_hz = (N > 0 && steps > 0) ? S / ((double)N * (double)steps * (double)dt) : 0.0;
+    o.isi_cv = NCV > 0.0 ? CV / NCV : 0.0;
+    o.mean_bin_count = mean;
+    o.synchrony = (nb > 0 && mean > 0.0) ? (HV / (double)nb) / mean : 0.0;
+    *out = o;
   }
 }
 

@@ -186,6 +186,14 @@ struct spk_network {
   bool bar_clear = true;         // counters (and the error flag) must be cleared first
   bool rules_uploaded = false;   // device descriptors of the rules are current
   bool lds_attr[2] = {false, false};  // dynamic LDS limit raised for the non-plastic/plastic kernel
+  // window statistics (spk_stats_begin)
+  unsigned int *d_st_cnt = nullptr, *d_st_nisi = nullptr, *d_st_hist = nullptr;
+  float* d_st_prev = nullptr;
+  double *d_st_s1 = nullptr, *d_st_s2 = nullptr;
+  spk::StatsOut* d_st_out = nullptr;
+  bool st_on = false, st_pending = false;
+  int st_bin = 10;
+  int64_t st_t0 = 0, st_max = 0, st_steps = 0, st_nbins = 0;
   unsigned long long* d_prof = nullptr;
   spk::FusedRule* d_rules = nullptr;
   bool profiling = false;
@@ -229,6 +237,8 @@ void free_device(spk_network* n) {
   }
   dfree(n->d_nlo); dfree(n->d_ilo); dfree(n->d_seg); dfree(n->d_xlist);
   dfree(n->d_bar); dfree(n->d_err); dfree(n->d_prof); dfree(n->d_rules);
+  dfree(n->d_st_cnt); dfree(n->d_st_nisi); dfree(n->d_st_hist); dfree(n->d_st_prev);
+  dfree(n->d_st_s1); dfree(n->d_st_s2); dfree(n->d_st_out);
   for (auto& m : n->mons) {
     dfree(m.d_ids); dfree(m.d_times); dfree(m.d_count);
   }
@@ -295,6 +305,29 @@ uint32_t* slot_lists(spk_network* n, int slot) {
 }
 int* slot_counts(spk_network* n, int slot) { return n->d_counts + (size_t)slot * n->Gprep; }
 
+// Statistics arguments of the kernels; the window starts at the first step processed
+// after spk_stats_begin.
+spk::StatsArgs stats_args(spk_network* n, float start) {
+  spk::StatsArgs st{};
+  if (!n->st_on) return st;
+  if (n->st_pending) {
+    n->st_t0 = (int64_t)roundf(start / n->dt);
+    n->st_pending = false;
+  }
+  st.count = n->d_st_cnt; st.nisi = n->d_st_nisi; st.prev = n->d_st_prev;
+  st.s1 = n->d_st_s1; st.s2 = n->d_st_s2; st.hist = n->d_st_hist;
+  st.t0 = n->st_t0; st.nsteps = n->st_max; st.bin = n->st_bin; st.on = 1;
+  return st;
+}
+
+// Steps of the statistics window covered once the groups starting at step `first`
+// (count `ng`) have run.
+void stats_advance(spk_network* n, int64_t first, int64_t ng) {
+  if (!n->st_on) return;
+  const int64_t end = first + ng * n->G - n->st_t0;
+  n->st_steps = std::max(n->st_steps, std::min(end, n->st_max));
+}
+
 int launch_neurons(spk_network* n, float t, int slot) {
   if (n->N == 0) return SPK_OK;
   spk::LifArgs a{};
@@ -309,6 +342,8 @@ int launch_neurons(spk_network* n, float t, int slot) {
   a.t = t; a.dt = n->dt; a.tref = n->tref; a.ibg = n->ibg;
   a.inv_scale = ldexpf(1.0f, -n->frac);
   a.q_ext = n->q_ext; a.seed = n->seed;
+  a.st = stats_args(n, t);
+  stats_advance(n, a.tidx, 1);
   const int grid = grid_for(n->N, 256);
   if (n->kind == SPK_SYN_VOLTAGE)
     hipLaunchKernelGGL(spk::k_lif<0>, dim3(grid), dim3(256), 0, n->stream, a);
@@ -544,6 +579,8 @@ int launch_fused(spk_network* n, spk::FusedArgs& a, float start, int64_t group0,
     n->pending_arm = false;
   }
   a.arm_step = n->Nin > 0 ? n->arm_step : -1;
+  a.st = stats_args(n, start);
+  stats_advance(n, (int64_t)roundf(start / n->dt), group0 + ng);
   a.start = start;
   a.group0 = group0;
   a.ngroups = ng;
@@ -1293,6 +1330,68 @@ int spk_get_counters(spk_network* n, spk_timing* t) {
   t->external_events = (int64_t)st[1];
   t->spikes_neurons = (int64_t)st[2];
   t->spikes_inputs = (int64_t)st[3];
+  return SPK_OK;
+}
+
+int spk_stats_begin(spk_network* n, int bin_steps, int64_t max_steps) {
+  if (int rc = check_prepared(n)) return rc;
+  if (bin_steps < 1 || max_steps < 1) return fail(SPK_ERR_ARG, "bin and window must be positive");
+  const int64_t nb = (max_steps + bin_steps - 1) / bin_steps;
+  if (nb > ((int64_t)1 << 31)) return fail(SPK_ERR_ARG, "statistics window too long");
+  const size_t N = (size_t)std::max(1, n->N);
+  if (!n->d_st_cnt) {
+    int rc;
+    if ((rc = dalloc(&n->d_st_cnt, N)) || (rc = dalloc(&n->d_st_nisi, N)) || (rc = dalloc(&n->d_st_prev, N)) ||
+        (rc = dalloc(&n->d_st_s1, N)) || (rc = dalloc(&n->d_st_s2, N)) || (rc = dalloc(&n->d_st_out, 1)))
+      return rc;
+  }
+  if (nb > n->st_nbins) {
+    dfree(n->d_st_hist);
+    if (int rc = dalloc(&n->d_st_hist, (size_t)nb)) return rc;
+    n->st_nbins = nb;
+  }
+  SPK_HIP(hipMemsetAsync(n->d_st_cnt, 0, N * sizeof(unsigned int), n->stream));
+  SPK_HIP(hipMemsetAsync(n->d_st_nisi, 0, N * sizeof(unsigned int), n->stream));
+  SPK_HIP(hipMemsetAsync(n->d_st_prev, 0, N * sizeof(float), n->stream));
+  SPK_HIP(hipMemsetAsync(n->d_st_s1, 0, N * sizeof(double), n->stream));
+  SPK_HIP(hipMemsetAsync(n->d_st_s2, 0, N * sizeof(double), n->stream));
+  SPK_HIP(hipMemsetAsync(n->d_st_hist, 0, (size_t)nb * sizeof(unsigned int), n->stream));
+  n->st_on = true;
+  n->st_pending = true;
+  n->st_bin = bin_steps;
+  n->st_max = max_steps;
+  n->st_steps = 0;
+  return SPK_OK;
+}
+
+int spk_stats_get(spk_network* n, spk_stats* out) {
+  if (int rc = check_prepared(n)) return rc;
+  if (!out) return fail(SPK_ERR_ARG, "null output");
+  if (!n->d_st_cnt) return fail(SPK_ERR_STATE, "statistics were never started");
+  spk::StatsArgs st{};
+  st.count = n->d_st_cnt; st.nisi = n->d_st_nisi; st.prev = n->d_st_prev;
+  st.s1 = n->d_st_s1; st.s2 = n->d_st_s2; st.hist = n->d_st_hist;
+  st.t0 = n->st_t0; st.nsteps = n->st_max; st.bin = n->st_bin; st.on = 1;
+  hipLaunchKernelGGL(spk::k_stats_reduce, dim3(1), dim3(spk::kStatsBlock), 0, n->stream, st, n->N,
+                     n->st_steps, n->dt, n->d_st_out);
+  SPK_HIP(hipGetLastError());
+  spk::StatsOut o;
+  SPK_HIP(hipMemcpyAsync(&o, n->d_st_out, sizeof(o), hipMemcpyDeviceToHost, n->stream));
+  SPK_HIP(hipStreamSynchronize(n->stream));
+  out->rate_hz = o.rate_hz;
+  out->isi_cv = o.isi_cv;
+  out->synchrony = o.synchrony;
+  out->mean_bin_count = o.mean_bin_count;
+  out->spikes = o.spikes;
+  out->cv_neurons = o.cv_neurons;
+  out->bins = o.bins;
+  out->steps = o.steps;
+  return SPK_OK;
+}
+
+int spk_stats_end(spk_network* n) {
+  if (int rc = check_net(n)) return rc;
+  n->st_on = false;
   return SPK_OK;
 }
 
