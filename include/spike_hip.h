@@ -192,8 +192,9 @@ int spk_set_partitions(spk_network* net, int partitions);
  * accumulates workgroup 0's wall-clock time per phase: 0 Poisson inputs and the LIF
  * input read, 1 LIF recurrence and spike append, 2 publication, 3 barrier arrival and the
  * next group's random draws, 4 barrier wait, 5 plasticity, 6 delivery heads, 7 count scan
- * and entry staging, 8 synapse delivery.  Values in microseconds summed over the last run
- * (up to 12 entries). */
+ * and entry staging, 8 synapse delivery; 10 and 11 are the wall-clock start and end of the
+ * last launch's group loop (workgroup 0; in a batch, the network's first workgroup).
+ * Values in microseconds summed over the last run (up to 12 entries). */
 int spk_set_profiling(spk_network* net, int on);
 int spk_get_phase_profile(const spk_network* net, double* us, int count);
 /* Empty string when spk_run uses the persistent kernel, else the reason it cannot. */
@@ -291,6 +292,47 @@ int spk_get_ring(spk_network* net, int64_t* ring);
 int spk_get_history(spk_network* net, int population, uint32_t* words);
 /* Plasticity traces of a rule in its synapse order (two arrays of n). */
 int spk_get_traces(spk_network* net, int rule_id, float* trace_a, float* trace_b);
+
+/* ---------------------------------------------------------------------------- */
+/* Brunel networks generated on the device (no reference counterpart: the reference builds
+ * connectivity on the host, Examples/UtilityFunctions.hpp:51-82 and Synapses.cpp:102-388,
+ * which takes minutes for 1e9 synapses).  N = n_exc + n_inh LIF neurons of one parameter
+ * set; every neuron receives ce synapses from the excitatory and ci from the inhibitory
+ * population, sources drawn with replacement by Philox4x32-10 (key = seed, counter =
+ * (slot, neuron, 3)): source = floor(u*n_exc/2^32) or n_exc + floor(u*n_inh/2^32).  All
+ * delays are delay_steps (= G).  Such a network runs on the persistent kernel only
+ * (spk_run, spk_step_group, statistics, monitors; no plasticity, no weight access). */
+typedef struct {
+  int n_exc, n_inh, ce, ci, delay_steps;
+  float weight_exc, weight_inh;                 /* J and -g*J */
+  float v0, vreset, threshold, decay_c, resist, refractory_s;
+  float ext_rate_hz, ext_weight;                /* per-neuron external Poisson drive */
+  float timestep;
+  uint64_t seed;
+} spk_brunel_params;
+int spk_create_brunel(const spk_brunel_params* params, int partitions, spk_network** out);
+
+/* Batches: many prepared networks (same synapse kind; each able to use the persistent
+ * kernel) run in one cooperative launch of `workgroups` workgroups (0 = as many as are
+ * co-resident).  Each network keeps its own partitions (spk_set_partitions /
+ * spk_create_brunel); a list scheduler assigns every network's partitions to the
+ * workgroups that become free first, with the cost model
+ *   ngroups * (fixed_us + per_neuron_us * neurons per partition
+ *              + per_synapse_us * synapses per partition).
+ * No reference counterpart (parameter sweeps run one model at a time in the reference). */
+typedef struct spk_batch spk_batch;
+int spk_batch_create(spk_network* const* nets, int n_nets, int workgroups, spk_batch** out);
+int spk_batch_destroy(spk_batch* batch);
+int spk_batch_set_cost_model(spk_batch* batch, double fixed_us, double per_neuron_us,
+                             double per_synapse_us);
+/* Per-network cost of one group in microseconds (batch order), e.g. measured by a probe
+ * run; replaces the model above for scheduling.  NULL returns to the model. */
+int spk_batch_set_costs(spk_batch* batch, const double* cost_per_group_us);
+int spk_batch_workgroups(const spk_batch* batch);
+/* Network k runs ngroups[k] groups from starts[k]; returns when all have finished. */
+int spk_batch_run(spk_batch* batch, const float* starts, const int64_t* ngroups, int flags);
+/* Kernel time of the last run (HIP events) and the scheduler's estimate. */
+int spk_batch_last_ms(const spk_batch* batch, double* kernel_ms, double* estimated_ms);
 
 #ifdef __cplusplus
 }

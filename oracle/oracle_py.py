@@ -64,6 +64,8 @@ SIG = {
     "or_get_history": (C.c_int, [P, C.c_int, up]),
     "or_get_traces": (C.c_int, [P, C.c_int, dp, dp]),
     "or_philox4x32_10": (None, [up, up, up]),
+    "or_brunel_connectivity": (C.c_int, [C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                         C.POINTER(C.c_int32)]),
     "or_bernoulli_threshold": (C.c_uint32, [C.c_float, C.c_float]),
     "or_poisson_count_table": (C.c_int, [C.c_double, up, C.c_int]),
     "or_choose_frac_bits": (C.c_int, [C.c_double]),
@@ -96,6 +98,15 @@ def _p(a, t):
 def check(rc):
     if rc != 0:
         raise RuntimeError("oracle: " + load().or_last_error().decode())
+
+
+def brunel_connectivity(seed, ne, ni, ce, ci):
+    """Presynaptic ids [N, ce+ci] of the Philox fixed in-degree rule."""
+    lib = load()
+    n = ne + ni
+    pre = np.empty((n, ce + ci), np.int32)
+    check(lib.or_brunel_connectivity(int(seed), ne, ni, ce, ci, n, _p(pre, C.c_int32)))
+    return pre
 
 
 def philox(ctr, key):

@@ -851,4 +851,18 @@ int or_poisson_count_table(double lambda, uint32_t* out, int cap) {
 
 int or_choose_frac_bits(double m) { return choose_frac_bits(m); }
 
+// Fixed in-degree Brunel connectivity drawn from Philox (the rule the library generates on
+// the device): neuron i, slot j < ce + ci draws u = philox(ctr = {j, 0, i, 3}, key = seed).x;
+// the source is floor(u*ne/2^32) for j < ce, else ne + floor(u*ni/2^32).  pre[i*(ce+ci)+j].
+int or_brunel_connectivity(uint64_t seed, int ne, int ni, int ce, int ci, int n, int32_t* pre) {
+  const int C = ce + ci;
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < C; ++j) {
+      const uint32_t u = Philox::draw(seed, (int64_t)j, (uint32_t)i, 3u);
+      pre[(size_t)i * C + j] = j < ce ? (int32_t)(((uint64_t)u * (uint64_t)ne) >> 32)
+                                      : (int32_t)(ne + (((uint64_t)u * (uint64_t)ni) >> 32));
+    }
+  return 0;
+}
+
 }  // extern "C"

@@ -65,6 +65,16 @@ lp = C.POINTER(C.c_int64)
 up = C.POINTER(C.c_uint32)
 
 # name -> (restype, argtypes)
+class BrunelParams(C.Structure):
+    """spk_brunel_params of include/spike_hip.h."""
+    _fields_ = [("n_exc", C.c_int), ("n_inh", C.c_int), ("ce", C.c_int), ("ci", C.c_int),
+                ("delay_steps", C.c_int), ("weight_exc", C.c_float), ("weight_inh", C.c_float),
+                ("v0", C.c_float), ("vreset", C.c_float), ("threshold", C.c_float),
+                ("decay_c", C.c_float), ("resist", C.c_float), ("refractory_s", C.c_float),
+                ("ext_rate_hz", C.c_float), ("ext_weight", C.c_float), ("timestep", C.c_float),
+                ("seed", C.c_uint64)]
+
+
 class Stats(C.Structure):
     """spk_stats of include/spike_hip.h."""
     _fields_ = [("rate_hz", C.c_double), ("isi_cv", C.c_double), ("synchrony", C.c_double),
@@ -110,6 +120,14 @@ SIGNATURES = {
     "spk_get_counters": (C.c_int, [P, C.POINTER(Timing)]),
     "spk_monitor_reset": (C.c_int, [P, C.c_int]),
     "spk_stats_begin": (C.c_int, [P, C.c_int, C.c_int64]),
+    "spk_create_brunel": (C.c_int, [C.POINTER(BrunelParams), C.c_int, C.POINTER(P)]),
+    "spk_batch_create": (C.c_int, [C.POINTER(P), C.c_int, C.c_int, C.POINTER(P)]),
+    "spk_batch_destroy": (C.c_int, [P]),
+    "spk_batch_set_cost_model": (C.c_int, [P, C.c_double, C.c_double, C.c_double]),
+    "spk_batch_workgroups": (C.c_int, [P]),
+    "spk_batch_set_costs": (C.c_int, [P, C.POINTER(C.c_double)]),
+    "spk_batch_run": (C.c_int, [P, C.POINTER(C.c_float), C.POINTER(C.c_int64), C.c_int]),
+    "spk_batch_last_ms": (C.c_int, [P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "spk_stats_get": (C.c_int, [P, C.POINTER(Stats)]),
     "spk_stats_end": (C.c_int, [P]),
     "spk_synchronize": (C.c_int, [P]),
@@ -378,3 +396,67 @@ class Network:
         check(self.lib.spk_monitor_drain(self.h, monitor, _ptr(ids, C.c_int32), _ptr(times, C.c_float),
                                          cap, C.byref(n)))
         return ids[: n.value].copy(), times[: n.value].copy()
+
+
+def generated_brunel(params: "BrunelParams", partitions: int = 0) -> Network:
+    """A Brunel network whose connectivity the library generates on the device
+    (spk_create_brunel); it runs on the persistent kernel only."""
+    lib = load()
+    h = P()
+    check(lib.spk_create_brunel(C.byref(params), int(partitions), C.byref(h)))
+    net = Network.__new__(Network)
+    net.lib = lib
+    net.h = h
+    n = params.n_exc + params.n_inh
+    net.desc = NetDesc(n_neurons=n, n_inputs=0, dt=float(params.timestep), grouping=params.delay_steps,
+                       name="brunel_generated")
+    net.rule_ids = []
+    net.monitor_ids = []
+    net._keep = []
+    return net
+
+
+class Batch:
+    """Many networks in one persistent launch (spk_batch_*)."""
+
+    def __init__(self, nets, workgroups: int = 0):
+        self.lib = load()
+        self.nets = list(nets)
+        arr = (P * len(self.nets))(*[n.h for n in self.nets])
+        h = P()
+        check(self.lib.spk_batch_create(arr, len(self.nets), int(workgroups), C.byref(h)))
+        self.h = h
+
+    @property
+    def workgroups(self) -> int:
+        return self.lib.spk_batch_workgroups(self.h)
+
+    def set_cost_model(self, fixed_us: float, per_neuron_us: float, per_synapse_us: float):
+        check(self.lib.spk_batch_set_cost_model(self.h, fixed_us, per_neuron_us, per_synapse_us))
+
+    def set_costs(self, cost_per_group_us):
+        if cost_per_group_us is None:
+            check(self.lib.spk_batch_set_costs(self.h, None))
+            return
+        c = np.ascontiguousarray(cost_per_group_us, dtype=np.float64)
+        self._costs = c
+        check(self.lib.spk_batch_set_costs(self.h, _ptr(c, C.c_double)))
+
+    def run(self, starts, ngroups):
+        st = np.ascontiguousarray(starts, dtype=np.float32)
+        ng = np.ascontiguousarray(ngroups, dtype=np.int64)
+        check(self.lib.spk_batch_run(self.h, _ptr(st, C.c_float), _ptr(ng, C.c_int64), 0))
+        km, em = C.c_double(), C.c_double()
+        check(self.lib.spk_batch_last_ms(self.h, C.byref(km), C.byref(em)))
+        return km.value, em.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.spk_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

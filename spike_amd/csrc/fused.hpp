@@ -103,6 +103,7 @@ struct FusedArgs {
   float start, dt, tref, ibg;
   int64_t group0;             // index of the first group of this launch within the run
   int N, Nin, L, G, B, W, P, xcap, xstride, nown_max, iown_max, ext_k, ngroups;
+  int BL;                       // slots of the LDS ring: max(largest delay, G)
   int plastic;
   unsigned long long* prof;   // optional: wall-clock ticks per phase of workgroup 0
 };
@@ -113,9 +114,9 @@ struct PhaseClock {
   unsigned long long t0;
   unsigned long long* acc;   // LDS, kProfPhases entries
   bool on;
-  __device__ void start(const FusedArgs& a, unsigned long long* lds) {
+  __device__ __forceinline__ void start(const FusedArgs& a, unsigned long long* lds, int vb) {
     acc = lds;
-    on = a.prof != nullptr && blockIdx.x == 0 && threadIdx.x == 0;
+    on = a.prof != nullptr && vb == 0 && threadIdx.x == 0;
     if (on) {
       for (int i = 0; i < kProfPhases; ++i) acc[i] = 0;
       t0 = wall_clock64();
@@ -127,7 +128,7 @@ struct PhaseClock {
     acc[phase] += t1 - t0;
     t0 = t1;
   }
-  __device__ void flush(const FusedArgs& a) {
+  __device__ __forceinline__ void flush(const FusedArgs& a) {
     if (!on) return;
     for (int i = 0; i < kProfPhases; ++i) a.prof[i] += acc[i];
   }
@@ -147,11 +148,12 @@ constexpr int kSpikesPerQuad = 4;      // spikes a quad handles with its loads i
 
 __host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
 
-__host__ __device__ inline FusedLayout fused_layout(int B, int L, int W, int G, int nown, int iown,
+// `BL` is the number of ring slots kept in LDS (see k_fused: max(largest delay, G)).
+__host__ __device__ inline FusedLayout fused_layout(int BL, int L, int W, int G, int nown, int iown,
                                                     int xcap, int P) {
   FusedLayout f;
   size_t o = 0;
-  f.ring = o;    o = align16(o + (size_t)B * nown * L * 8);
+  f.ring = o;    o = align16(o + (size_t)BL * nown * L * 8);
   f.hist = o;    o = align16(o + (size_t)W * nown * 4);
   f.ext = o;     o = align16(o + (size_t)2 * nown * G);       // double buffer
   f.imask = o;   o = align16(o + (size_t)2 * iown * 8);       // double buffer
@@ -172,9 +174,9 @@ __host__ __device__ inline FusedLayout fused_layout(int B, int L, int W, int G, 
   return f;
 }
 
-__host__ __device__ inline size_t fused_lds_bytes(int B, int L, int W, int G, int nown, int iown,
+__host__ __device__ inline size_t fused_lds_bytes(int BL, int L, int W, int G, int nown, int iown,
                                                   int xcap, int P) {
-  return fused_layout(B, L, W, G, nown, iown, xcap, P).total;
+  return fused_layout(BL, L, W, G, nown, iown, xcap, P).total;
 }
 
 __host__ __device__ inline int fused_xstride(int xcap) { return ((1 + xcap + 7) / 8) * 8; }
@@ -195,14 +197,14 @@ __device__ __forceinline__ uint32_t load_coherent(const uint32_t* p) {
 }
 
 // Barrier arrival (thread 0, after the workgroup's stores are complete): one increment of
-// this workgroup's level-1 counter; the last arrival of a level-1 group increments the
+// the level-1 counter of this workgroup's member index vb (0..P-1); the last arrival of a level-1 group increments the
 // level-2 counter, and the last of those publishes generation s+1.  With plasticity the
 // rules read other partitions' state through ordinary loads, so arrival and departure
 // carry agent-scope release/acquire fences.
-__device__ __forceinline__ void barrier_arrive(const FusedArgs& a, int s) {
-  const int P = gridDim.x;
+__device__ __forceinline__ void barrier_arrive(const FusedArgs& a, int s, int vb) {
+  const int P = a.P;
   const int ngrp = P < kBarGroups ? P : kBarGroups;
-  const int grp = blockIdx.x % ngrp;
+  const int grp = vb % ngrp;
   const unsigned int gsize = (unsigned int)((P - grp + ngrp - 1) / ngrp);
   if (a.plastic) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
   const unsigned int gen = a.bar_base + (unsigned int)(s + 1);
@@ -224,7 +226,7 @@ __device__ __forceinline__ bool barrier_wait(const FusedArgs& a, int s) {
     __builtin_amdgcn_s_sleep(1);
     if (++spins > a.spin_limit ||
         __hip_atomic_load(a.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
-      atomicExch(a.error, 1);
+      atomicCAS(a.error, 0, 1);   // keep an earlier error code
       return false;
     }
   }
@@ -325,15 +327,33 @@ __device__ __forceinline__ void deliver_quad(const FusedArgs& a, const uint32_t*
                   (unsigned long long)qv);
       }
     }
-    // longer segments: the rest, one synapse per lane per step
-    for (uint32_t k = s0[r] + (uint32_t)(q + 2 * kQuad); k < s1[r]; k += kQuad) {
-      const uint2 rc = a.syn[k];
-      const float wv = __uint_as_float(rc.y) * (PER_SYN_SCALE ? a.scale[k] : a.scale0);
-      const long long qv = __float2ll_rn(wv * a.two_k);
-      int slot = gs[r] + (int)((rc.x >> 20) & 0xFFu);
-      while (slot >= B) slot -= B;
-      atomicAdd((unsigned long long*)&sring[(size_t)slot * nownL + (size_t)(rc.x & 0xFFFFFu) * L + (rc.x >> 28)],
-                (unsigned long long)qv);
+    // longer segments: the rest, four synapses per lane per step with the loads in flight
+    // together
+    for (uint32_t k0 = s0[r] + (uint32_t)(q + 2 * kQuad); k0 < s1[r]; k0 += 4 * kQuad) {
+      uint2 rr[4];
+      float sc[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const uint32_t k = k0 + (uint32_t)(u * kQuad);
+        rr[u] = make_uint2(0u, 0u);
+        sc[u] = a.scale0;
+        if (k < s1[r]) {
+          rr[u] = a.syn[k];
+          if (PER_SYN_SCALE) sc[u] = a.scale[k];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const uint32_t k = k0 + (uint32_t)(u * kQuad);
+        if (k < s1[r]) {
+          const float wv = __uint_as_float(rr[u].y) * sc[u];
+          const long long qv = __float2ll_rn(wv * a.two_k);
+          int slot = gs[r] + (int)((rr[u].x >> 20) & 0xFFu);
+          while (slot >= B) slot -= B;
+          atomicAdd((unsigned long long*)&sring[(size_t)slot * nownL + (size_t)(rr[u].x & 0xFFFFFu) * L + (rr[u].x >> 28)],
+                    (unsigned long long)qv);
+        }
+      }
     }
   }
 }
@@ -364,17 +384,17 @@ __device__ __forceinline__ int partition_of_block(int b, int P) {
   return (b % kXcds) * (P / kXcds) + b / kXcds;
 }
 
+// The group loop of one network for partition p, run by the workgroup that is member vb of
+// the network's P workgroups.
 template <int KIND, bool L1, bool PLASTIC>
-__global__ __launch_bounds__(kFusedBlock) void k_fused(FusedArgs a) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  const int p = partition_of_block(blockIdx.x, gridDim.x);
+__device__ __forceinline__ void fused_body(const FusedArgs& a, const int p, const int vb, unsigned char* smem) {
   const int tid = threadIdx.x;
   const int lane = (int)__lane_id();
   const int nlo = a.nlo[p], nhi = a.nlo[p + 1];
   const int ilo = a.ilo[p], ihi = a.ilo[p + 1];
   const int nown = nhi - nlo, iown = ihi - ilo;
-  const int L = L1 ? 1 : a.L, G = a.G, B = a.B, W = a.W;
-  const FusedLayout f = fused_layout(B, L, W, G, a.nown_max, a.iown_max, a.xcap, a.P);
+  const int L = L1 ? 1 : a.L, G = a.G, B = a.B, BL = a.BL, W = a.W;
+  const FusedLayout f = fused_layout(BL, L, W, G, a.nown_max, a.iown_max, a.xcap, a.P);
   int64_t* sring = (int64_t*)(smem + f.ring);
   uint32_t* shist = (uint32_t*)(smem + f.hist);
   uint8_t* sext = (uint8_t*)(smem + f.ext);
@@ -396,10 +416,21 @@ __global__ __launch_bounds__(kFusedBlock) void k_fused(FusedArgs a) {
 
   const int NL = a.N * L;
   const int nownL = nown * L;
-  // load the owned ring slice and spike history
-  for (int s = 0; s < B; ++s)
-    for (int j = tid; j < nownL; j += kFusedBlock)
-      sring[(size_t)s * nownL + j] = a.ring[(size_t)s * NL + (size_t)nlo * L + j];
+  // load the owned ring slice.  The LDS ring keeps BL = max(largest delay, G) slots: when a
+  // group starts at step t every pending input is for a step in [t, t + largest delay), so
+  // step s lives in LDS slot s % BL, and in slot s % B of the global ring (B = the
+  // reference's D + 2G + 1 slots, SpikingSynapses.cu:48), which holds zeros elsewhere.
+  {
+    float tl;
+    int t0;
+    group_time(a, 0, tl, t0);
+    const int r0 = t0 % BL;
+    for (int j = 0; j < BL; ++j) {
+      const int s = t0 + (j - r0 + BL) % BL;
+      const int64_t* src = a.ring + (size_t)(s % B) * NL + (size_t)nlo * L;
+      for (int k = tid; k < nownL; k += kFusedBlock) sring[(size_t)j * nownL + k] = src[k];
+    }
+  }
   for (int k = tid; k < W * nown; k += kFusedBlock) {
     const int wd = k / nown, i = k - wd * nown;
     shist[k] = a.hist_n[(size_t)wd * a.N + nlo + i];
@@ -442,7 +473,9 @@ __global__ __launch_bounds__(kFusedBlock) void k_fused(FusedArgs a) {
   draw_group(a, 0, sext, simask, nlo, nown, ilo, iown, st_ext);
   __syncthreads();
   PhaseClock clk;
-  clk.start(a, s_prof);
+  clk.start(a, s_prof, vb);
+  // start and end of this group loop (wall clock of workgroup 0 of the network)
+  if (a.prof != nullptr && vb == 0 && tid == 0) a.prof[kProfPhases - 2] = wall_clock64();
 
   for (int s = 0; s < a.ngroups; ++s) {
     float t;
@@ -452,7 +485,7 @@ __global__ __launch_bounds__(kFusedBlock) void k_fused(FusedArgs a) {
     const uint8_t* ext = sext + (size_t)par * a.nown_max * G;
     const unsigned long long* imask = simask + (size_t)par * a.iown_max;
     const int b0 = tidx % bits;
-    const int slot0 = tidx % B;
+    const int slot0 = tidx % BL;
     // -- A. Poisson inputs: append this group's spikes, update histories ------------
     for (int i = tid; i < iown; i += kFusedBlock) {
       const unsigned long long m = imask[i];
@@ -483,7 +516,7 @@ __global__ __launch_bounds__(kFusedBlock) void k_fused(FusedArgs a) {
     for (int k = tid; k < nown * G; k += kFusedBlock) {
       const int i = k / G, g = k - i * G;
       int slot = slot0 + g;
-      if (slot >= B) slot -= B;
+      if (slot >= BL) slot -= BL;
       int64_t* rs = sring + (size_t)slot * nownL + (size_t)i * L;
       const int64_t ext_q = (a.ext_k > 0) ? (int64_t)ext[i * G + g] * a.q_ext : 0;
       if (KIND == 0) {
@@ -614,6 +647,8 @@ __global__ __launch_bounds__(kFusedBlock) void k_fused(FusedArgs a) {
     clk.lap(a, 1);
     // -- C. publish: {count, entries} block, and state other workgroups may read ------
     const int nx = min(scounter[0], a.xcap);
+    // more spikes than the exchange list holds: stop the launch with error 2
+    if (tid == 0 && scounter[0] > a.xcap) atomicExch(a.error, 2);
     uint32_t* xl = a.xlist + ((size_t)par * a.P + p) * a.xstride;
     for (int k = tid; k < nx; k += kFusedBlock) store_coherent(xl + 1 + k, sxbuf[k]);
     if (tid == 0) store_coherent(xl, (uint32_t)nx);
@@ -628,8 +663,8 @@ __global__ __launch_bounds__(kFusedBlock) void k_fused(FusedArgs a) {
     // ballot; time = group time + g * dt as the reference's collection kernel computes it
 #pragma unroll
     for (int mi = 0; mi < kMaxFusedMons; ++mi) {
-      if (mi >= a.n_mons) break;
-      const FusedMon& m = a.mon[mi];
+      if (mi >= a.n_mons) continue;
+      const FusedMon m = a.mon[mi];
       for (int k0 = 0; k0 < nx; k0 += kFusedBlock) {
         const int k = k0 + tid;
         const uint32_t e = k < nx ? sxbuf[k] : 0u;
@@ -653,7 +688,7 @@ __global__ __launch_bounds__(kFusedBlock) void k_fused(FusedArgs a) {
     clk.lap(a, 2);
     // -- D. barrier: arrive, draw the next group while others finish, wait ----------
     if (tid == 0) {
-      barrier_arrive(a, s);
+      barrier_arrive(a, s, vb);
       scounter[0] = 0;
     }
     if (s + 1 < a.ngroups)
@@ -707,20 +742,22 @@ __global__ __launch_bounds__(kFusedBlock) void k_fused(FusedArgs a) {
       __syncthreads();
       const int total = sscan[a.P];
       for (int r0 = 0; r0 < total; r0 += kEntCap) {
-        // (3) stage entries [r0, r0 + kEntCap) of the concatenated lists: the first
-        //     kHeadWords-1 entries of a source come from its head, 16 lanes per source
-        for (int k = tid; k < a.P * kHeadWords; k += kFusedBlock) {
-          const int src = k / kHeadWords, j = k - src * kHeadWords;
-          const int cnt = min((int)shead[src * kHeadWords], a.xcap);
-          const int pos = sscan[src] + j - r0;
-          if (j + 1 < kHeadWords && j < cnt && pos >= 0 && pos < kEntCap)
-            sent[pos] = shead[src * kHeadWords + 1 + j];
-          if (j == 0) {
-            for (int kk = kHeadWords - 1; kk < cnt; ++kk) {
-              const int ps = sscan[src] + kk - r0;
-              if (ps >= 0 && ps < kEntCap) sent[ps] = load_coherent(xbase + (size_t)src * a.xstride + 1 + kk);
-            }
+        // (3) stage entries [r0, r0 + kEntCap) of the concatenated lists, one entry per
+        //     thread: its source is the last with sscan[src] <= index (empty sources share
+        //     their start with the next one); the first kHeadWords-1 entries of a source
+        //     come from its head in LDS
+        const int nst = min(kEntCap, total - r0);
+        for (int e = tid; e < nst; e += kFusedBlock) {
+          const int gidx = r0 + e;
+          int lo = 0, hi = a.P;
+          while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (sscan[mid] <= gidx) lo = mid;
+            else hi = mid;
           }
+          const int j = gidx - sscan[lo];
+          sent[e] = (j + 1 < kHeadWords) ? shead[lo * kHeadWords + 1 + j]
+                                         : load_coherent(xbase + (size_t)lo * a.xstride + 1 + j);
         }
         __syncthreads();
         clk.lap(a, 7);
@@ -731,9 +768,9 @@ __global__ __launch_bounds__(kFusedBlock) void k_fused(FusedArgs a) {
         for (int j0 = quad * kSpikesPerQuad; j0 < nr; j0 += nquads * kSpikesPerQuad) {
           const int n = min(kSpikesPerQuad, nr - j0);
           if (a.scale == nullptr)
-            deliver_quad<false>(a, sent + j0, n, q, p, slot0, B, nownL, L, sring, st_del);
+            deliver_quad<false>(a, sent + j0, n, q, p, slot0, BL, nownL, L, sring, st_del);
           else
-            deliver_quad<true>(a, sent + j0, n, q, p, slot0, B, nownL, L, sring, st_del);
+            deliver_quad<true>(a, sent + j0, n, q, p, slot0, BL, nownL, L, sring, st_del);
         }
         __syncthreads();
         clk.lap(a, 8);
@@ -741,6 +778,7 @@ __global__ __launch_bounds__(kFusedBlock) void k_fused(FusedArgs a) {
     }
   }
   clk.flush(a);
+  if (a.prof != nullptr && vb == 0 && tid == 0) a.prof[kProfPhases - 1] = wall_clock64();
   // write back state
   if (own) {
     a.V[gi] = V;
@@ -748,9 +786,20 @@ __global__ __launch_bounds__(kFusedBlock) void k_fused(FusedArgs a) {
     if (KIND == 1)
       for (int l = 0; l < L; ++l) a.gcond[(size_t)gi * L + l] = gl[l];
   }
-  for (int s = 0; s < B; ++s)
-    for (int j = tid; j < nownL; j += kFusedBlock)
-      a.ring[(size_t)s * NL + (size_t)nlo * L + j] = sring[(size_t)s * nownL + j];
+  // write the whole global ring slice back in its B-slot layout (zeros past the LDS window)
+  {
+    float tl;
+    int t1;
+    group_time(a, a.ngroups, tl, t1);
+    const int r1 = t1 % B;
+    for (int k = 0; k < B; ++k) {
+      const int s = t1 + (k - r1 + B) % B;
+      int64_t* dst = a.ring + (size_t)k * NL + (size_t)nlo * L;
+      const bool live = s - t1 < BL;
+      const int64_t* src = sring + (size_t)(s % BL) * nownL;
+      for (int j = tid; j < nownL; j += kFusedBlock) dst[j] = live ? src[j] : 0;
+    }
+  }
   for (int k = tid; k < W * nown; k += kFusedBlock) {
     const int wd = k / nown, i = k - wd * nown;
     a.hist_n[(size_t)wd * a.N + nlo + i] = shist[k];
@@ -771,6 +820,32 @@ __global__ __launch_bounds__(kFusedBlock) void k_fused(FusedArgs a) {
     unsigned long long v = v4[qq];
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
     if (__lane_id() == 0 && v) atomicAdd(a.stats + qq, v);
+  }
+}
+
+template <int KIND, bool L1, bool PLASTIC>
+__global__ __launch_bounds__(kFusedBlock) void k_fused(FusedArgs a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  fused_body<KIND, L1, PLASTIC>(a, partition_of_block(blockIdx.x, gridDim.x), blockIdx.x, smem);
+}
+
+// Many independent networks in one cooperative launch.  Workgroup b runs the tasks
+// tasks[task_off[b] .. task_off[b+1]) in order, each the whole group loop of one partition
+// of one network (x = network, y = partition).  Every workgroup's list is in increasing
+// network order, so a workgroup waiting at a network's barrier waits only for workgroups
+// that are finishing earlier networks: every wait ends.
+template <int KIND, bool L1, bool PLASTIC>
+__global__ __launch_bounds__(kFusedBlock) void k_fused_batch(const FusedArgs* __restrict__ nets,
+                                                             const int2* __restrict__ tasks,
+                                                             const int* __restrict__ task_off) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int j0 = __builtin_amdgcn_readfirstlane(task_off[blockIdx.x]);
+  const int j1 = __builtin_amdgcn_readfirstlane(task_off[blockIdx.x + 1]);
+  for (int j = j0; j < j1; ++j) {
+    const int k = __builtin_amdgcn_readfirstlane(tasks[j].x);
+    const int p = __builtin_amdgcn_readfirstlane(tasks[j].y);
+    fused_body<KIND, L1, PLASTIC>(nets[k], p, p, smem);
+    __syncthreads();
   }
 }
 

@@ -16,6 +16,7 @@
 #include "../../include/spike_hip.h"
 #include "kernels.hpp"
 #include "fused.hpp"
+#include "gen.hpp"
 
 #pragma clang fp contract(off)
 
@@ -73,6 +74,15 @@ std::vector<uint32_t> poisson_count_table(double lambda) {
     cdf += pmf;
   }
   return c;
+}
+
+// Spikes one LIF neuron can emit in a group of G steps: consecutive spikes are at least
+// max(1, round(tref/dt)) steps apart (the refractory test ((t - last) - tref) > dt/2 needs
+// a gap above tref/dt + 1/2 steps; one step less covers float rounding).
+int max_spikes_per_group(float tref, float dt, int G) {
+  const double r = std::floor((double)tref / (double)dt + 0.5);
+  const int gap = (int)std::max(1.0, std::min(r, (double)G));
+  return std::min(G, (G + gap - 1) / gap);
 }
 
 // Fraction bits of the fixed-point ring: the largest possible per-step input of one
@@ -137,6 +147,7 @@ struct spk_network {
   std::vector<MonitorHost> mons;
   // derived sizes
   int G = 1, Gprep = 1, B = 1, W = 1, frac = 32, dmin = 0, dmax = 0;
+  int BL = 1;   // ring slots the persistent kernel keeps in LDS: max(dmax, G)
   int64_t S = 0;
   bool prepared = false;
   bool uniform_scale = true;
@@ -185,6 +196,7 @@ struct spk_network {
   unsigned int bar_gen = 0;      // barrier generations completed since the counters were cleared
   bool bar_clear = true;         // counters (and the error flag) must be cleared first
   bool rules_uploaded = false;   // device descriptors of the rules are current
+  bool generated = false;        // connectivity generated on the device (persistent kernel only)
   bool lds_attr[2] = {false, false};  // dynamic LDS limit raised for the non-plastic/plastic kernel
   // window statistics (spk_stats_begin)
   unsigned int *d_st_cnt = nullptr, *d_st_nisi = nullptr, *d_st_hist = nullptr;
@@ -222,6 +234,14 @@ int check_net(const spk_network* n) {
 int check_prepared(const spk_network* n) {
   if (!n) return fail(SPK_ERR_ARG, "null network");
   if (!n->prepared) return fail(SPK_ERR_STATE, "network not prepared (call spk_prepare)");
+  return SPK_OK;
+}
+
+// Entry points that need the host-built CSR (per-phase kernels, weight and trace access).
+int check_listed(const spk_network* n) {
+  if (int rc = check_prepared(n)) return rc;
+  if (n->generated)
+    return fail(SPK_ERR_STATE, "network generated on the device: persistent kernel only");
   return SPK_OK;
 }
 
@@ -551,7 +571,7 @@ int fill_fused_args(spk_network* n, bool plastic, spk::FusedArgs& a) {
   a.spin_limit = (long long)1 << 22;
   a.scale0 = n->scale0; a.two_k = ldexpf(1.0f, n->frac); a.inv_scale = ldexpf(1.0f, -n->frac);
   a.dt = n->dt; a.tref = n->tref; a.ibg = n->ibg;
-  a.N = n->N; a.Nin = n->Nin; a.L = n->L; a.G = G; a.B = n->B; a.W = n->W; a.P = n->P;
+  a.N = n->N; a.Nin = n->Nin; a.L = n->L; a.G = G; a.B = n->B; a.BL = n->BL; a.W = n->W; a.P = n->P;
   a.xcap = n->xcap; a.xstride = spk::fused_xstride(n->xcap);
   a.nown_max = n->nown_max; a.iown_max = n->iown_max;
   a.ext_k = (int)n->ext_cdf.size();
@@ -564,13 +584,13 @@ int fill_fused_args(spk_network* n, bool plastic, spk::FusedArgs& a) {
 // ceil(P/8) arrivals per generation) could wrap.
 constexpr unsigned int kBarGenLimit = 1u << 24;
 
-// One cooperative launch of the persistent kernel over `ng` groups starting at group
-// index group0 of the run that began at `start`; enqueued only.
-int launch_fused(spk_network* n, spk::FusedArgs& a, float start, int64_t group0, int ng,
-                 hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr) {
+// Run position of one launch (groups [group0, group0 + ng) of the run that began at
+// `start`) and the barrier bookkeeping that goes with it; clears on `st` what must start
+// at zero.
+int stage_fused(spk_network* n, spk::FusedArgs& a, float start, int64_t group0, int ng, hipStream_t st) {
   if (n->bar_clear || n->bar_gen + (unsigned int)ng > kBarGenLimit) {
-    SPK_HIP(hipMemsetAsync(n->d_bar, 0, sizeof(unsigned int) * (spk::kBarGroups + 2) * spk::kBarStride, n->stream));
-    SPK_HIP(hipMemsetAsync(n->d_err, 0, sizeof(int), n->stream));
+    SPK_HIP(hipMemsetAsync(n->d_bar, 0, sizeof(unsigned int) * (spk::kBarGroups + 2) * spk::kBarStride, st));
+    SPK_HIP(hipMemsetAsync(n->d_err, 0, sizeof(int), st));
     n->bar_gen = 0;
     n->bar_clear = false;
   }
@@ -585,6 +605,15 @@ int launch_fused(spk_network* n, spk::FusedArgs& a, float start, int64_t group0,
   a.group0 = group0;
   a.ngroups = ng;
   a.bar_base = n->bar_gen;
+  n->bar_gen += (unsigned int)ng;
+  return SPK_OK;
+}
+
+// One cooperative launch of the persistent kernel over `ng` groups starting at group
+// index group0 of the run that began at `start`; enqueued only.
+int launch_fused(spk_network* n, spk::FusedArgs& a, float start, int64_t group0, int ng,
+                 hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr) {
+  if (int rc = stage_fused(n, a, start, group0, ng, n->stream)) return rc;
   const void* fn = fused_kernel(n, a.plastic != 0);
   if (n->fused_lds > 65536 && !n->lds_attr[a.plastic != 0]) {
     SPK_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)n->fused_lds));
@@ -595,7 +624,6 @@ int launch_fused(spk_network* n, spk::FusedArgs& a, float start, int64_t group0,
   SPK_HIP(hipLaunchCooperativeKernel(fn, dim3(n->P), dim3(spk::kFusedBlock), kargs,
                                      (unsigned int)n->fused_lds, n->stream));
   if (ev1) SPK_HIP(hipEventRecord(ev1, n->stream));
-  n->bar_gen += (unsigned int)ng;
   return SPK_OK;
 }
 
@@ -607,6 +635,7 @@ int check_fused_error(spk_network* n) {
   SPK_HIP(hipStreamSynchronize(n->stream));
   if (err) {
     n->bar_clear = true;
+    if (err == 2) return fail(SPK_ERR_CAPACITY, "persistent kernel: spikes of a group exceeded the exchange list");
     return fail(SPK_ERR_HIP, "persistent kernel: grid barrier wait exceeded its spin limit");
   }
   return SPK_OK;
@@ -952,7 +981,7 @@ int spk_prepare(spk_network* n, int G) {
       n->nown_max = std::max(n->nown_max, n->h_nlo[q + 1] - n->h_nlo[q]);
       n->iown_max = std::max(n->iown_max, n->h_ilo[q + 1] - n->h_ilo[q]);
     }
-    n->xcap = std::max(1, (n->nown_max + n->iown_max) * G);
+    n->xcap = std::max(1, n->nown_max * max_spikes_per_group(n->tref, n->dt, G) + n->iown_max * G);
   }
   std::vector<int> owner(n->N, 0);
   for (int q = 0; q < n->P; ++q)
@@ -1004,7 +1033,8 @@ int spk_prepare(spk_network* n, int G) {
   if (S >= ((int64_t)1 << 32)) no_fused("2^32 or more synapses");
   if (n->P > spk::kFusedBlock) no_fused("too many partitions");
   if ((int)n->mons.size() > spk::kMaxFusedMons) no_fused("more than 4 spike monitors");
-  n->fused_lds = spk::fused_lds_bytes(n->B, n->L, n->W, G, n->nown_max, n->iown_max, n->xcap, n->P);
+  n->BL = std::max(n->dmax, G);
+  n->fused_lds = spk::fused_lds_bytes(n->BL, n->L, n->W, G, n->nown_max, n->iown_max, n->xcap, n->P);
   {
     hipDeviceProp_t prop;
     SPK_HIP(hipGetDeviceProperties(&prop, n->device));
@@ -1213,31 +1243,31 @@ int spk_reset_state(spk_network* n) {
 }
 
 int spk_update_neurons(spk_network* n, float t) {
-  if (int rc = check_prepared(n)) return rc;
+  if (int rc = check_listed(n)) return rc;
   if (int rc = begin_group(n, t, true)) return rc;
   return launch_neurons(n, t, n->cur_slot);
 }
 
 int spk_update_inputs(spk_network* n, float t) {
-  if (int rc = check_prepared(n)) return rc;
+  if (int rc = check_listed(n)) return rc;
   if (int rc = begin_group(n, t)) return rc;
   return launch_inputs(n, t, n->cur_slot);
 }
 
 int spk_update_plasticity(spk_network* n, int rid, float t) {
-  if (int rc = check_prepared(n)) return rc;
+  if (int rc = check_listed(n)) return rc;
   if (rid < 0 || rid >= (int)n->rules.size()) return fail(SPK_ERR_ARG, "rule id out of range");
   return launch_rule(n, rid, t);
 }
 
 int spk_update_synapses(spk_network* n, float t) {
-  if (int rc = check_prepared(n)) return rc;
+  if (int rc = check_listed(n)) return rc;
   if (int rc = begin_group(n, t)) return rc;
   return launch_propagate(n, t, n->cur_slot);
 }
 
 int spk_collect_spikes(spk_network* n, int mid, float t) {
-  if (int rc = check_prepared(n)) return rc;
+  if (int rc = check_listed(n)) return rc;
   if (mid < 0 || mid >= (int)n->mons.size()) return fail(SPK_ERR_ARG, "monitor id out of range");
   if (int rc = begin_group(n, t)) return rc;
   return launch_collect(n, mid, n->cur_slot, 1);
@@ -1250,6 +1280,7 @@ int spk_run(spk_network* n, float start, int64_t ngroups, int flags) {
   SPK_HIP(hipMemsetAsync(n->d_stats, 0, sizeof(unsigned long long) * 4, n->stream));
   const bool use_fused = n->fused_ok && !(flags & SPK_RUN_FORCE_LAUNCHES) && n->G == n->Gprep;
   SPK_HIP(hipEventRecord(n->ev_a, n->stream));
+  if (!use_fused && n->generated) return check_listed(n);
   if (use_fused) {
     if (int rc = run_fused(n, start, ngroups, flags)) return rc;
   } else {
@@ -1306,6 +1337,7 @@ int spk_step_group(spk_network* n, float t, int flags) {
     if (int rc = fill_fused_args(n, plastic, a)) return rc;
     return launch_fused(n, a, t, 0, 1);
   }
+  if (int rc = check_listed(n)) return rc;
   int rc;
   if ((rc = begin_group(n, t, true))) return rc;
   if ((rc = launch_neurons(n, t, n->cur_slot))) return rc;
@@ -1479,7 +1511,7 @@ int spk_monitor_take(spk_network* n, int mid, int32_t* ids, float* times, int64_
 }
 
 int spk_get_weights(spk_network* n, float* out) {
-  if (int rc = check_prepared(n)) return rc;
+  if (int rc = check_listed(n)) return rc;
   if (!out && n->S) return fail(SPK_ERR_ARG, "null output");
   std::vector<uint2> rec(n->S);
   if (n->S) SPK_HIP(hipMemcpyAsync(rec.data(), n->d_syn, n->S * sizeof(uint2), hipMemcpyDeviceToHost, n->stream));
@@ -1489,7 +1521,7 @@ int spk_get_weights(spk_network* n, float* out) {
 }
 
 int spk_set_weights(spk_network* n, const float* in) {
-  if (int rc = check_prepared(n)) return rc;
+  if (int rc = check_listed(n)) return rc;
   if (!in && n->S) return fail(SPK_ERR_ARG, "null input");
   std::vector<uint2> rec(n->S);
   if (n->S) SPK_HIP(hipMemcpyAsync(rec.data(), n->d_syn, n->S * sizeof(uint2), hipMemcpyDeviceToHost, n->stream));
@@ -1528,7 +1560,7 @@ int spk_get_history(spk_network* n, int pop, uint32_t* words) {
 }
 
 int spk_get_traces(spk_network* n, int rid, float* a, float* b) {
-  if (int rc = check_prepared(n)) return rc;
+  if (int rc = check_listed(n)) return rc;
   if (rid < 0 || rid >= (int)n->rules.size()) return fail(SPK_ERR_ARG, "rule id out of range");
   RuleHost& r = n->rules[rid];
   const size_t P = r.syn.size();
@@ -1539,6 +1571,364 @@ int spk_get_traces(spk_network* n, int rid, float* a, float* b) {
     a[r.order[j]] = ta[j];
     b[r.order[j]] = tb[j];
   }
+  return SPK_OK;
+}
+
+// ---------------------------------------------------------------------------------------
+// Brunel networks generated on the device
+
+int spk_create_brunel(const spk_brunel_params* p, int partitions, spk_network** out) {
+  if (!p || !out) return fail(SPK_ERR_ARG, "null argument");
+  *out = nullptr;
+  if (p->n_exc < 1 || p->n_inh < 0 || p->ce < 0 || p->ci < 0) return fail(SPK_ERR_ARG, "bad population size or in-degree");
+  if (p->ci > 0 && p->n_inh < 1) return fail(SPK_ERR_ARG, "inhibitory in-degree without inhibitory neurons");
+  if (p->delay_steps < 1 || p->delay_steps > 64) return fail(SPK_ERR_ARG, "delay must be in [1, 64] steps");
+  const int N = p->n_exc + p->n_inh;
+  const int C = p->ce + p->ci;
+  const int64_t S = (int64_t)N * C;
+  if (S >= ((int64_t)1 << 32)) return fail(SPK_ERR_ARG, "2^32 or more synapses");
+  spk_network* n = nullptr;
+  if (int rc = spk_create(N, 0, SPK_SYN_VOLTAGE, 1, p->timestep, &n)) return rc;
+  auto bail = [&](int rc) {
+    const std::string msg = g_error;
+    spk_destroy(n);
+    g_error = msg;
+    return rc;
+  };
+  {
+    std::vector<float> v0(N, p->v0), vr(N, p->vreset), th(N, p->threshold), c(N, p->decay_c), rp(N, p->resist);
+    int rc;
+    if ((rc = spk_set_lif(n, v0.data(), vr.data(), th.data(), c.data(), rp.data(), p->refractory_s, 0.0f)) ||
+        (rc = spk_set_seed(n, p->seed)) || (rc = spk_set_external_drive(n, p->ext_rate_hz, p->ext_weight)))
+      return bail(rc);
+  }
+  n->req_partitions = partitions;
+  const int G = p->delay_steps;
+  n->S = S;
+  n->G = n->Gprep = G;
+  n->dmin = n->dmax = S ? G : 0;
+  n->B = n->dmax + 2 * G + 1;
+  n->W = (n->dmax + 2 * G) / 32 + 1;
+  n->uniform_scale = true;
+  n->scale0 = 1.0f;
+  // fraction bits: every neuron's per-step input bound, summed in synapse order (E then I)
+  double mx = 0.0;
+  for (int j = 0; j < p->ce; ++j) mx += std::fabs((double)p->weight_exc);
+  for (int j = 0; j < p->ci; ++j) mx += std::fabs((double)p->weight_inh);
+  n->ext_cdf = poisson_count_table((double)n->ext_rate * (double)n->dt);
+  if ((int)n->ext_cdf.size() > spk::kMaxExtTable) return bail(fail(SPK_ERR_ARG, "external drive too strong"));
+  if (!n->ext_cdf.empty()) mx += (double)n->ext_cdf.size() * std::fabs((double)n->ext_w);
+  n->frac = choose_frac_bits(mx);
+  n->q_ext = n->ext_cdf.empty() ? 0 : (int64_t)std::llrint(std::ldexp((double)n->ext_w, n->frac));
+  hipDeviceProp_t prop;
+  SPK_HIP(hipGetDeviceProperties(&prop, n->device));
+  n->cu_count = prop.multiProcessorCount;
+  int P = partitions > 0 ? partitions : std::min(n->cu_count, std::max(1, (N + 7) / 8));
+  P = std::max(1, std::min(P, N));
+  n->P = P;
+  n->h_nlo.assign(P + 1, 0);
+  n->h_ilo.assign(P + 1, 0);
+  for (int q = 0; q <= P; ++q) n->h_nlo[q] = (int)((int64_t)N * q / P);
+  n->nown_max = 0;
+  n->iown_max = 0;
+  for (int q = 0; q < P; ++q) n->nown_max = std::max(n->nown_max, n->h_nlo[q + 1] - n->h_nlo[q]);
+  n->xcap = std::max(1, n->nown_max * max_spikes_per_group(n->tref, n->dt, G));
+  n->BL = std::max(n->dmax, G);
+  n->fused_lds = spk::fused_lds_bytes(n->BL, 1, n->W, G, n->nown_max, 0, n->xcap, P);
+  if (n->nown_max > spk::kFusedBlock) return bail(fail(SPK_ERR_ARG, "more neurons per partition than threads per workgroup"));
+  if (P > spk::kFusedBlock) return bail(fail(SPK_ERR_ARG, "too many partitions"));
+  if (n->fused_lds > (size_t)prop.sharedMemPerBlock)
+    return bail(fail(SPK_ERR_ARG, "input ring slice does not fit in LDS: use more partitions"));
+  n->fused_ok = true;
+  n->fused_why.clear();
+  n->cap = 1;
+  n->slots = 1;
+  int rc;
+  const size_t NN = (size_t)N;
+#define ALLOC(ptr, cnt) if ((rc = dalloc(&(ptr), (cnt)))) return bail(rc)
+  ALLOC(n->d_V, NN); ALLOC(n->d_v0, NN); ALLOC(n->d_vreset, NN); ALLOC(n->d_thr, NN);
+  ALLOC(n->d_c, NN); ALLOC(n->d_rp, NN); ALLOC(n->d_last_n, NN); ALLOC(n->d_last_in, 1);
+  ALLOC(n->d_hist_n, (size_t)n->W * NN); ALLOC(n->d_hist_in, 1);
+  ALLOC(n->d_ring, (size_t)n->B * NN); ALLOC(n->d_g, NN);
+  ALLOC(n->d_decay, 1); ALLOC(n->d_erev, 1); ALLOC(n->d_in_thr, 1);
+  ALLOC(n->d_ext_cdf, std::max<size_t>(1, n->ext_cdf.size()));
+  ALLOC(n->d_syn, (size_t)std::max<int64_t>(1, S));
+  ALLOC(n->d_stats, 4);
+  ALLOC(n->d_nlo, (size_t)P + 1); ALLOC(n->d_ilo, (size_t)P + 1);
+  ALLOC(n->d_seg, (size_t)N * (P + 1));
+  ALLOC(n->d_xlist, (size_t)2 * P * spk::fused_xstride(n->xcap));
+  ALLOC(n->d_bar, (size_t)(spk::kBarGroups + 2) * spk::kBarStride); ALLOC(n->d_err, 1);
+  int* d_owner = nullptr;
+  unsigned int *d_count = nullptr, *d_off = nullptr, *d_cursor = nullptr;
+  ALLOC(d_owner, NN);
+  ALLOC(d_count, NN * P); ALLOC(d_off, NN * P); ALLOC(d_cursor, NN * P);
+#undef ALLOC
+  hipStream_t st = n->stream;
+  auto cleanup = [&]() { dfree(d_owner); dfree(d_count); dfree(d_off); dfree(d_cursor); };
+  std::vector<int> owner(N);
+  for (int q = 0; q < P; ++q)
+    for (int i = n->h_nlo[q]; i < n->h_nlo[q + 1]; ++i) owner[i] = q;
+  if ((rc = upload(n->d_v0, n->v0.data(), NN, st)) || (rc = upload(n->d_vreset, n->vreset.data(), NN, st)) ||
+      (rc = upload(n->d_thr, n->thr.data(), NN, st)) || (rc = upload(n->d_c, n->c.data(), NN, st)) ||
+      (rc = upload(n->d_rp, n->rp.data(), NN, st)) || (rc = upload(n->d_decay, n->decay_g.data(), 1, st)) ||
+      (rc = upload(n->d_erev, n->erev.data(), 1, st)) ||
+      (rc = upload(n->d_ext_cdf, n->ext_cdf.data(), n->ext_cdf.size(), st)) ||
+      (rc = upload(n->d_nlo, n->h_nlo.data(), (size_t)P + 1, st)) ||
+      (rc = upload(n->d_ilo, n->h_ilo.data(), (size_t)P + 1, st)) || (rc = upload(d_owner, owner.data(), NN, st))) {
+    cleanup();
+    return bail(rc);
+  }
+  SPK_HIP(hipMemsetAsync(n->d_xlist, 0, sizeof(uint32_t) * 2 * P * spk::fused_xstride(n->xcap), st));
+  SPK_HIP(hipMemsetAsync(n->d_stats, 0, sizeof(unsigned long long) * 4, st));
+  SPK_HIP(hipMemsetAsync(d_count, 0, sizeof(unsigned int) * NN * P, st));
+  SPK_HIP(hipMemsetAsync(d_cursor, 0, sizeof(unsigned int) * NN * P, st));
+  spk::BrunelGenArgs ga{};
+  ga.seed = n->seed; ga.ne = p->n_exc; ga.ni = p->n_inh; ga.ce = p->ce; ga.ci = p->ci; ga.P = P;
+  ga.owner = d_owner; ga.nlo = n->d_nlo; ga.count = d_count; ga.off = d_off; ga.cursor = d_cursor;
+  ga.syn = n->d_syn; ga.delay_bits = (uint32_t)G << 20; ga.we = p->weight_exc; ga.wi = p->weight_inh;
+  const int grid = (int)std::min<int64_t>(8192, std::max<int64_t>(1, (S + 255) / 256));
+  if (S > 0) {
+    hipLaunchKernelGGL(spk::k_gen_count, dim3(grid), dim3(256), 0, st, ga, S);
+    SPK_HIP(hipGetLastError());
+  }
+  // exclusive scan of the key counts: the segment table seg[u][P+1]
+  std::vector<unsigned int> cnt(NN * P), off(NN * P);
+  SPK_HIP(hipMemcpyAsync(cnt.data(), d_count, sizeof(unsigned int) * NN * P, hipMemcpyDeviceToHost, st));
+  SPK_HIP(hipStreamSynchronize(st));
+  std::vector<uint32_t> seg(NN * (P + 1));
+  uint64_t run = 0;
+  for (size_t u = 0; u < NN; ++u) {
+    for (int q = 0; q < P; ++q) {
+      off[u * P + q] = (unsigned int)run;
+      seg[u * (P + 1) + q] = (uint32_t)run;
+      run += cnt[u * P + q];
+    }
+    seg[u * (P + 1) + P] = (uint32_t)run;
+  }
+  if ((rc = upload(d_off, off.data(), NN * P, st)) || (rc = upload(n->d_seg, seg.data(), seg.size(), st))) {
+    cleanup();
+    return bail(rc);
+  }
+  if (S > 0) {
+    hipLaunchKernelGGL(spk::k_gen_scatter, dim3(grid), dim3(256), 0, st, ga, S);
+    SPK_HIP(hipGetLastError());
+  }
+  SPK_HIP(hipStreamSynchronize(st));
+  cleanup();
+  int per_cu = 0;
+  SPK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fused_kernel(n, false), spk::kFusedBlock, n->fused_lds));
+  if ((int64_t)per_cu * n->cu_count < P) return bail(fail(SPK_ERR_ARG, "partitions exceed co-resident workgroups"));
+  n->prepared = true;
+  n->generated = true;
+  if ((rc = spk_reset_state(n))) return bail(rc);
+  *out = n;
+  return SPK_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------------------
+// Batches of independent networks in one persistent launch
+
+struct spk_batch {
+  std::vector<spk_network*> nets;
+  int kind = 0;
+  bool l1 = true;
+  int nb = 0;                 // workgroups of the launch
+  size_t lds = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  spk::FusedArgs* d_args = nullptr;
+  int2* d_tasks = nullptr;
+  int* d_off = nullptr;
+  size_t tasks_cap = 0;
+  double cost_fixed_us = 8.0, cost_neuron_us = 0.004, cost_synapse_us = 2e-5;
+  std::vector<double> cost_per_group_us;   // measured per-network costs (empty: the model)
+  double last_ms = 0.0;
+  double est_ms = 0.0;
+};
+
+namespace {
+
+const void* batch_kernel(const spk_batch* b) {
+  if (b->kind == SPK_SYN_VOLTAGE)
+    return b->l1 ? (const void*)spk::k_fused_batch<0, true, false> : (const void*)spk::k_fused_batch<0, false, false>;
+  return b->l1 ? (const void*)spk::k_fused_batch<1, true, false> : (const void*)spk::k_fused_batch<1, false, false>;
+}
+
+}  // namespace
+
+extern "C" {
+
+int spk_batch_create(spk_network* const* nets, int n_nets, int workgroups, spk_batch** out) {
+  if (!nets || n_nets < 1 || !out) return fail(SPK_ERR_ARG, "bad batch arguments");
+  *out = nullptr;
+  spk_batch* b = new spk_batch();
+  for (int k = 0; k < n_nets; ++k) {
+    spk_network* n = nets[k];
+    if (int rc = check_prepared(n)) { delete b; return rc; }
+    if (!n->fused_ok) { delete b; return fail(SPK_ERR_ARG, "network " + std::to_string(k) + " cannot use the persistent kernel: " + n->fused_why); }
+    if (k == 0) { b->kind = n->kind; b->l1 = n->L == 1; }
+    if (n->kind != b->kind || (n->L == 1) != b->l1) { delete b; return fail(SPK_ERR_ARG, "networks of one batch need the same synapse kind and label count class"); }
+    if (n->device != nets[0]->device) { delete b; return fail(SPK_ERR_ARG, "networks on different devices"); }
+    b->lds = std::max(b->lds, n->fused_lds);
+    b->nets.push_back(n);
+  }
+  const void* fn = batch_kernel(b);
+  if (b->lds > 65536) SPK_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)b->lds));
+  int per_cu = 0;
+  SPK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, spk::kFusedBlock, b->lds));
+  const int cap = per_cu * nets[0]->cu_count;
+  b->nb = workgroups > 0 ? std::min(workgroups, cap) : cap;
+  int pmax = 0;
+  for (auto* n : b->nets) pmax = std::max(pmax, n->P);
+  if (b->nb < pmax || b->nb < 1) {
+    delete b;
+    return fail(SPK_ERR_ARG, "a network has more partitions than the batch has co-resident workgroups");
+  }
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) { delete b; return fail(SPK_ERR_HIP, "hipStreamCreate"); }
+  (void)hipEventCreate(&b->e0);
+  (void)hipEventCreate(&b->e1);
+  if (int rc = dalloc(&b->d_args, b->nets.size())) { delete b; return rc; }
+  *out = b;
+  return SPK_OK;
+}
+
+int spk_batch_destroy(spk_batch* b) {
+  if (!b) return SPK_OK;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  dfree(b->d_args);
+  dfree(b->d_tasks);
+  dfree(b->d_off);
+  if (b->e0) (void)hipEventDestroy(b->e0);
+  if (b->e1) (void)hipEventDestroy(b->e1);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return SPK_OK;
+}
+
+int spk_batch_set_cost_model(spk_batch* b, double fixed_us, double per_neuron_us, double per_synapse_us) {
+  if (!b) return fail(SPK_ERR_ARG, "null batch");
+  b->cost_fixed_us = fixed_us;
+  b->cost_neuron_us = per_neuron_us;
+  b->cost_synapse_us = per_synapse_us;
+  return SPK_OK;
+}
+
+int spk_batch_set_costs(spk_batch* b, const double* cost_per_group_us) {
+  if (!b) return fail(SPK_ERR_ARG, "null batch");
+  if (!cost_per_group_us) {
+    b->cost_per_group_us.clear();
+    return SPK_OK;
+  }
+  b->cost_per_group_us.assign(cost_per_group_us, cost_per_group_us + b->nets.size());
+  return SPK_OK;
+}
+
+int spk_batch_workgroups(const spk_batch* b) { return b ? b->nb : 0; }
+
+int spk_batch_run(spk_batch* b, const float* starts, const int64_t* ngroups, int flags) {
+  if (!b || !starts || !ngroups) return fail(SPK_ERR_ARG, "null argument");
+  if (flags & SPK_RUN_PLASTICITY) return fail(SPK_ERR_ARG, "batches run without plasticity");
+  const int K = (int)b->nets.size();
+  for (int k = 0; k < K; ++k) {
+    if (ngroups[k] < 0 || ngroups[k] > 0x7FFFFFFF) return fail(SPK_ERR_ARG, "bad group count");
+    SPK_HIP(hipStreamSynchronize(b->nets[k]->stream));   // earlier work of every network is done
+  }
+  // list scheduling in network order: each network takes the P workgroups that are free
+  // first; every workgroup's task list is then in network order
+  std::vector<double> avail(b->nb, 0.0);
+  std::vector<std::vector<int2>> per(b->nb);
+  std::vector<int> idx(b->nb);
+  double makespan = 0.0;
+  for (int k = 0; k < K; ++k) {
+    spk_network* n = b->nets[k];
+    const double per_group = b->cost_per_group_us.empty()
+        ? b->cost_fixed_us + b->cost_neuron_us * n->nown_max + b->cost_synapse_us * (double)n->S / n->P
+        : b->cost_per_group_us[k];
+    const double cost = (double)ngroups[k] * per_group;
+    for (int i = 0; i < b->nb; ++i) idx[i] = i;
+    std::partial_sort(idx.begin(), idx.begin() + n->P, idx.end(), [&](int x, int y) {
+      return avail[x] != avail[y] ? avail[x] < avail[y] : x < y;
+    });
+    std::sort(idx.begin(), idx.begin() + n->P);
+    double start = 0.0;
+    for (int q = 0; q < n->P; ++q) start = std::max(start, avail[idx[q]]);
+    for (int q = 0; q < n->P; ++q) {
+      avail[idx[q]] = start + cost;
+      per[idx[q]].push_back(make_int2(k, q));
+    }
+    makespan = std::max(makespan, start + cost);
+  }
+  b->est_ms = makespan * 1e-3;
+  std::vector<int2> tasks;
+  std::vector<int> off(b->nb + 1, 0);
+  for (int i = 0; i < b->nb; ++i) {
+    off[i] = (int)tasks.size();
+    tasks.insert(tasks.end(), per[i].begin(), per[i].end());
+  }
+  off[b->nb] = (int)tasks.size();
+  if (tasks.size() > b->tasks_cap) {
+    dfree(b->d_tasks);
+    if (int rc = dalloc(&b->d_tasks, tasks.size())) return rc;
+    b->tasks_cap = tasks.size();
+  }
+  if (!b->d_off) {
+    if (int rc = dalloc(&b->d_off, (size_t)b->nb + 1)) return rc;
+  }
+  std::vector<spk::FusedArgs> args(K);
+  for (int k = 0; k < K; ++k) {
+    spk_network* n = b->nets[k];
+    if (int rc = fill_fused_args(n, false, args[k])) return rc;
+    if (int rc = stage_fused(n, args[k], starts[k], 0, (int)ngroups[k], b->stream)) return rc;
+    if (n->profiling) {
+      if (!n->d_prof) {
+        if (int rc = dalloc(&n->d_prof, spk::kProfPhases)) return rc;
+      }
+      SPK_HIP(hipMemsetAsync(n->d_prof, 0, sizeof(unsigned long long) * spk::kProfPhases, b->stream));
+      args[k].prof = n->d_prof;
+    }
+  }
+  SPK_HIP(hipMemcpyAsync(b->d_tasks, tasks.data(), tasks.size() * sizeof(int2), hipMemcpyHostToDevice, b->stream));
+  SPK_HIP(hipMemcpyAsync(b->d_off, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice, b->stream));
+  SPK_HIP(hipMemcpyAsync(b->d_args, args.data(), args.size() * sizeof(spk::FusedArgs), hipMemcpyHostToDevice, b->stream));
+  const void* fn = batch_kernel(b);
+  const spk::FusedArgs* pa = b->d_args;
+  const int2* pt = b->d_tasks;
+  const int* po = b->d_off;
+  void* kargs[] = {(void*)&pa, (void*)&pt, (void*)&po};
+  SPK_HIP(hipEventRecord(b->e0, b->stream));
+  SPK_HIP(hipLaunchCooperativeKernel(fn, dim3(b->nb), dim3(spk::kFusedBlock), kargs, (unsigned int)b->lds, b->stream));
+  SPK_HIP(hipEventRecord(b->e1, b->stream));
+  SPK_HIP(hipEventSynchronize(b->e1));
+  float ms = 0.0f;
+  SPK_HIP(hipEventElapsedTime(&ms, b->e0, b->e1));
+  b->last_ms = ms;
+  int rate_khz = 0;
+  SPK_HIP(hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, b->nets[0]->device));
+  for (auto* n : b->nets) {
+    if (!n->profiling || !n->d_prof) continue;
+    unsigned long long ticks[spk::kProfPhases];
+    SPK_HIP(hipMemcpy(ticks, n->d_prof, sizeof(ticks), hipMemcpyDeviceToHost));
+    for (int i = 0; i < spk::kProfPhases; ++i)
+      n->prof_us[i] = rate_khz > 0 ? (double)ticks[i] * 1e3 / (double)rate_khz : 0.0;
+  }
+  for (auto* n : b->nets) {
+    int err = 0;
+    SPK_HIP(hipMemcpy(&err, n->d_err, sizeof(int), hipMemcpyDeviceToHost));
+    if (err) {
+      n->bar_clear = true;
+      if (err == 2) return fail(SPK_ERR_CAPACITY, "batched persistent kernel: spikes of a group exceeded the exchange list");
+      return fail(SPK_ERR_HIP, "batched persistent kernel: grid barrier wait exceeded its spin limit");
+    }
+  }
+  return SPK_OK;
+}
+
+int spk_batch_last_ms(const spk_batch* b, double* kernel_ms, double* estimated_ms) {
+  if (!b) return fail(SPK_ERR_ARG, "null batch");
+  if (kernel_ms) *kernel_ms = b->last_ms;
+  if (estimated_ms) *estimated_ms = b->est_ms;
   return SPK_OK;
 }
 

@@ -1,0 +1,254 @@
+"""Parameter sweeps of Brunel networks: plain (one network per launch) and batched (all
+networks in one persistent launch), with an autotuned workgroup-to-network tiling.
+
+A sweep point is one Brunel (2000) network: N neurons (80 % excitatory), relative
+inhibition g, external drive nu_ext/nu_thr, synaptic weight J and delay D steps; in-degree
+rule C_E = eps N_E, C_I = eps N_I with eps = 0.1.  The network is generated on the device
+(spk_create_brunel) and its statistics (rate, ISI CV, synchrony) are reduced in FP64 on the
+device (spk_stats_*).
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+import time
+from typing import Dict, List, Optional, Sequence
+
+import numpy as np
+
+from . import abi
+from .netdesc import lif_constants
+
+F32 = np.float32
+
+
+@dataclasses.dataclass
+class SweepPoint:
+    n: int                 # neurons
+    g: float               # relative inhibition
+    nu_ratio: float        # nu_ext / nu_thr
+    J: float               # EPSP amplitude (V)
+    delay_steps: int       # axonal delay D (= timestep grouping G)
+    seed: int
+    eps: float = 0.1
+    theta: float = 20e-3
+    vreset: float = 10e-3
+    tau: float = 20e-3
+    tref: float = 2e-3
+    dt: float = 1e-4
+
+    @property
+    def n_exc(self) -> int:
+        return int(round(0.8 * self.n))
+
+    @property
+    def n_inh(self) -> int:
+        return self.n - self.n_exc
+
+    @property
+    def ce(self) -> int:
+        return max(1, int(self.eps * self.n_exc))
+
+    @property
+    def ci(self) -> int:
+        return max(1, int(self.eps * self.n_inh))
+
+    def groups(self, seconds: float) -> int:
+        """SpikingModel::run's group count: ceil(T/dt) steps, truncated to whole groups."""
+        steps = int(math.ceil(F32(seconds) / F32(self.dt)))
+        return steps // self.delay_steps
+
+
+def sample_points(count: int = 256, seed: int = 2024, n_range=(1250, 12500), g_range=(3.0, 8.0),
+                  nu_range=(0.9, 4.0), J_range=(0.05e-3, 0.2e-3), delay_range=(10, 20)) -> List[SweepPoint]:
+    """N log-uniform, g, nu_ext/nu_thr and J uniform, delay uniform in whole steps; per-point
+    seeds = base seed + point index."""
+    rng = np.random.default_rng(seed)
+    n = np.exp(rng.uniform(np.log(n_range[0]), np.log(n_range[1]), count)).round().astype(int)
+    g = rng.uniform(*g_range, count)
+    nu = rng.uniform(*nu_range, count)
+    J = rng.uniform(*J_range, count)
+    d = rng.integers(delay_range[0], delay_range[1] + 1, count)
+    return [SweepPoint(int(n[k]), float(g[k]), float(nu[k]), float(J[k]), int(d[k]), seed + k)
+            for k in range(count)]
+
+
+def point_constants(p: SweepPoint) -> Dict:
+    """Every float32 constant of the network, computed as builders.brunel does (the same
+    values go to the device generator and to a host replay of the network)."""
+    cm = 200e-12
+    g0 = cm / p.tau
+    c, rp = lif_constants(1, cm, g0, p.dt)
+    nu_thr = p.theta / (p.J * p.ce * p.tau)
+    return dict(v0=F32(0.0), vreset=F32(p.vreset), threshold=F32(p.theta), decay_c=c[0], resist=rp[0],
+                refractory=F32(p.tref), weight_exc=F32(p.J), weight_inh=F32(-p.g * p.J),
+                ext_rate=F32(p.nu_ratio * nu_thr * p.ce), ext_weight=F32(p.J), dt=F32(p.dt))
+
+
+def brunel_params(p: SweepPoint) -> abi.BrunelParams:
+    k = point_constants(p)
+    return abi.BrunelParams(
+        n_exc=p.n_exc, n_inh=p.n_inh, ce=p.ce, ci=p.ci, delay_steps=p.delay_steps,
+        weight_exc=k["weight_exc"], weight_inh=k["weight_inh"], v0=k["v0"], vreset=k["vreset"],
+        threshold=k["threshold"], decay_c=k["decay_c"], resist=k["resist"],
+        refractory_s=k["refractory"], ext_rate_hz=k["ext_rate"], ext_weight=k["ext_weight"],
+        timestep=k["dt"], seed=p.seed)
+
+
+def build(p: SweepPoint, partitions: int = 0) -> abi.Network:
+    """The point's network on the device; with partitions > 0 the count is raised until the
+    per-workgroup input ring fits in LDS."""
+    P = partitions
+    while True:
+        try:
+            return abi.generated_brunel(brunel_params(p), P)
+        except abi.SpikeError as e:
+            if "LDS" not in str(e) or P <= 0 or P >= p.n:
+                raise
+            P = min(p.n, int(P * 1.25) + 1)
+
+
+@dataclasses.dataclass
+class SweepResult:
+    ms: float                  # device time of the simulations (HIP events)
+    wall_s: float              # host wall time including network generation
+    rate: np.ndarray
+    cv: np.ndarray
+    sync: np.ndarray
+    config: Dict
+
+
+def _stats(nets, pts, seconds):
+    rate, cv, sync = [], [], []
+    for n in nets:
+        s = n.stats()
+        rate.append(s.rate_hz)
+        cv.append(s.isi_cv)
+        sync.append(s.synchrony)
+    return np.array(rate), np.array(cv), np.array(sync)
+
+
+def run_plain(points: Sequence[SweepPoint], seconds: float, bin_steps: int = 10) -> SweepResult:
+    """One network at a time on the persistent kernel with its default partitioning."""
+    t0 = time.perf_counter()
+    ms = 0.0
+    rate, cv, sync = [], [], []
+    for p in points:
+        net = build(p)
+        ng = p.groups(seconds)
+        net.stats_begin(bin_steps, ng * p.delay_steps)
+        tm = net.run(0.0, ng, plasticity=False)
+        ms += tm.total_ms
+        s = net.stats()
+        rate.append(s.rate_hz)
+        cv.append(s.isi_cv)
+        sync.append(s.synchrony)
+        net.close()
+    return SweepResult(ms, time.perf_counter() - t0, np.array(rate), np.array(cv), np.array(sync),
+                       dict(mode="plain"))
+
+
+def _build_all(points, partitions, seconds, bin_steps, profiling=False):
+    nets = []
+    for p, P in zip(points, partitions):
+        n = build(p, int(P))
+        n.stats_begin(bin_steps, p.groups(seconds) * p.delay_steps)
+        if profiling:
+            n.set_profiling(True)
+        nets.append(n)
+    return nets
+
+
+def default_plan(points: Sequence[SweepPoint], neurons_per_wg: int = 384) -> Dict:
+    """Tiling from sizes alone: ceil(N / neurons_per_wg) workgroups, largest first."""
+    parts = [max(1, -(-p.n // neurons_per_wg)) for p in points]
+    order = sorted(range(len(points)), key=lambda k: -points[k].n)
+    return dict(partitions=parts, order=order, cost=None, name=f"size/{neurons_per_wg}")
+
+
+def run_batched(points: Sequence[SweepPoint], seconds: float, plan: Optional[Dict] = None,
+                workgroups: int = 0, bin_steps: int = 10, profiling: bool = False) -> SweepResult:
+    """All networks in one persistent launch.  `plan` gives every network's workgroup count
+    (partitions), the scheduling order and optionally the per-group cost the scheduler
+    uses (microseconds); None uses default_plan."""
+    plan = plan or default_plan(points)
+    t0 = time.perf_counter()
+    order = list(plan["order"])
+    nets = _build_all(points, plan["partitions"], seconds, bin_steps, profiling)
+    batch = abi.Batch([nets[k] for k in order], workgroups)
+    if plan.get("cost") is not None:
+        batch.set_costs([plan["cost"][k] for k in order])
+    ng = [points[k].groups(seconds) for k in order]
+    ms, est = batch.run(np.zeros(len(order)), ng)
+    rate, cv, sync = _stats(nets, points, seconds)
+    extra = {}
+    if profiling:
+        ev = np.array([n.counters().synaptic_events for n in nets], dtype=np.float64)
+        dur = np.array([(lambda pr: pr["s11"] - pr["s10"])(n.phase_profile()) for n in nets])
+        extra = dict(events=ev, duration_us=dur)
+    parts = [n.info().partitions for n in nets]
+    wg = batch.workgroups
+    batch.close()
+    for n in nets:
+        n.close()
+    r = SweepResult(ms, time.perf_counter() - t0, rate, cv, sync,
+                    dict(mode="batched", plan=plan.get("name", "custom"), workgroups=wg, estimated_ms=est,
+                         partitions_total=int(sum(parts))))
+    r.extra = dict(extra, partitions=parts)
+    return r
+
+
+def plan_from_work(points, events_per_group, target_events, neurons_per_wg_max, workgroups,
+                   fixed_us, per_event_us, name=None) -> Dict:
+    """Tiling from measured work: network k gets enough workgroups that each handles about
+    `target_events` deliveries per group (at least ceil(N/neurons_per_wg_max), at most all
+    workgroups); its per-group cost is fixed_us + per_event_us * events/workgroup; the
+    longest networks are scheduled first."""
+    K = len(points)
+    parts, cost = [], []
+    for k, p in enumerate(points):
+        P = max(-(-p.n // neurons_per_wg_max), int(math.ceil(events_per_group[k] / max(1.0, target_events))))
+        P = int(max(1, min(P, workgroups, p.n)))
+        parts.append(P)
+        cost.append(fixed_us + per_event_us * events_per_group[k] / P)
+    total = [cost[k] * points[k].groups(1.0) * parts[k] for k in range(K)]
+    order = sorted(range(K), key=lambda k: -cost[k] * points[k].groups(1.0))
+    return dict(partitions=parts, order=order, cost=cost, name=name or f"work/{target_events:.0f}")
+
+
+def autotune(points: Sequence[SweepPoint], probe_seconds: float = 0.02,
+             targets=(2e4, 5e4, 1e5, 2e5), neurons_per_wg_max: int = 384, log=None) -> Dict:
+    """The project's tiling autotuner.  (1) A probe run of the whole sweep with the
+    size-only plan measures every network's deliveries per group and its time per group;
+    (2) a least-squares fit of time = fixed + per_event * events/workgroup gives the cost
+    model; (3) every candidate events-per-workgroup target is turned into a plan
+    (plan_from_work) and timed on a probe run; the fastest plan is returned."""
+    base = default_plan(points, neurons_per_wg_max)
+    r0 = run_batched(points, probe_seconds, base, profiling=True)
+    groups = np.array([p.groups(probe_seconds) for p in points], dtype=np.float64)
+    ev = r0.extra["events"] / np.maximum(groups, 1.0)
+    us = r0.extra["duration_us"] / np.maximum(groups, 1.0)
+    x = ev / np.array(r0.extra["partitions"], dtype=np.float64)
+    A = np.stack([np.ones_like(x), x], axis=1)
+    (fixed, per_ev), *_ = np.linalg.lstsq(A, us, rcond=None)
+    fixed, per_ev = float(max(fixed, 1.0)), float(max(per_ev, 1e-6))
+    wg = r0.config["workgroups"]
+    trials = [dict(plan=base["name"], ms=r0.ms)]
+    best = dict(plan=base, ms=r0.ms)
+    if log:
+        log(f"autotune probe {base['name']}: {r0.ms:.2f} ms; fit {fixed:.2f} us + {per_ev * 1e3:.3f} ns/event")
+    for tgt in targets:
+        plan = plan_from_work(points, ev, tgt, neurons_per_wg_max, wg, fixed, per_ev)
+        try:
+            r = run_batched(points, probe_seconds, plan)
+        except abi.SpikeError as e:
+            trials.append(dict(plan=plan["name"], error=str(e)))
+            continue
+        trials.append(dict(plan=plan["name"], ms=r.ms, estimated_ms=r.config["estimated_ms"]))
+        if log:
+            log(f"autotune {plan['name']}: {r.ms:.2f} ms (scheduler estimate {r.config['estimated_ms']:.2f})")
+        if r.ms < best["ms"]:
+            best = dict(plan=plan, ms=r.ms)
+    best["trials"] = trials
+    best["fit"] = dict(fixed_us=fixed, per_event_us=per_ev)
+    return best

@@ -1,0 +1,122 @@
+"""Device-generated Brunel networks and batched persistent launches.
+
+1. A network generated on the device equals, bit for bit, the FP32 oracle running the same
+   network built on the host from the oracle's restatement of the generation rule.
+2. Networks run in one batch give exactly what they give when run one at a time (the
+   batch only changes which workgroups run which partitions), also when a few workgroups
+   must run many networks one after another.
+3. Batched FP32 rates are within 2 % of the FP64 oracle's.
+"""
+import numpy as np
+import pytest
+
+import oracle_py
+from spike_amd import abi, sweep
+from spike_amd.netdesc import NetDesc, SYN_VOLTAGE
+
+pytestmark = pytest.mark.gpu
+F32 = np.float32
+
+
+def host_replay(p: sweep.SweepPoint, monitors=(0,)) -> NetDesc:
+    """The point's network as host arrays: connectivity from the oracle's restatement of
+    the device generator, constants from sweep.point_constants."""
+    k = sweep.point_constants(p)
+    N = p.n
+    pre = oracle_py.brunel_connectivity(p.seed, p.n_exc, p.n_inh, p.ce, p.ci)
+    C = p.ce + p.ci
+    post = np.repeat(np.arange(N, dtype=np.int32), C)
+    w = np.where(np.arange(C) < p.ce, k["weight_exc"], k["weight_inh"]).astype(F32)
+    d = NetDesc(n_neurons=N, n_inputs=0, kind=SYN_VOLTAGE, dt=float(k["dt"]), seed=p.seed,
+                v0=np.full(N, k["v0"], F32), vreset=np.full(N, k["vreset"], F32),
+                threshold=np.full(N, k["threshold"], F32), decay_c=np.full(N, k["decay_c"], F32),
+                resist=np.full(N, k["resist"], F32), refractory=float(k["refractory"]), background=0.0,
+                ext_rate=float(k["ext_rate"]), ext_weight=float(k["ext_weight"]),
+                pre=pre.reshape(-1), post=post, weight=np.tile(w, N),
+                delay=np.full(N * C, p.delay_steps, np.int32), monitors=list(monitors), name="replay")
+    return d
+
+
+def small_points(count, seed=7, n_range=(300, 1500)):
+    return sweep.sample_points(count, seed=seed, n_range=n_range)
+
+
+@pytest.mark.parametrize("partitions", [0, 5])
+def test_generated_network_matches_oracle(partitions):
+    p = sweep.SweepPoint(n=1000, g=5.0, nu_ratio=2.0, J=0.1e-3, delay_steps=12, seed=33)
+    ng = 150
+    print(f"[batch] generated N=1000 P={partitions}", flush=True)
+    gpu = sweep.build(p, partitions)
+    gpu.stats_begin(10, ng * p.delay_steps)
+    gpu.run(0.0, ng, plasticity=False)
+    d = host_replay(p)
+    ora = oracle_py.OracleNetwork(d, precision=32)
+    ora.run(0.0, ng)
+    gi = gpu.info()
+    assert (gi.frac_bits, gi.ring_slots, gi.history_words) == (ora.info()["frac"], ora.info()["B"], ora.info()["W"])
+    assert np.array_equal(gpu.membrane().astype(np.float64), ora.membrane())
+    assert np.array_equal(gpu.last_spike(0), ora.last_spike(0))
+    assert np.array_equal(gpu.history(0), ora.history(0))
+    assert np.array_equal(gpu.ring(), ora.ring())
+    c, oc = gpu.counters(), ora.counters()
+    assert c.spikes_neurons == oc["spikes_neurons"] > 0
+    assert c.synaptic_events == oc["synaptic_events"] and c.external_events == oc["external_events"]
+    s = gpu.stats()
+    oi, ot = ora.take_spikes(0)
+    assert s.spikes == len(oi)
+
+
+def _individual(points, ng_of, partitions_of):
+    out = []
+    for p in points:
+        n = sweep.build(p, partitions_of(p))
+        ng = ng_of(p)
+        n.stats_begin(10, ng * p.delay_steps)
+        n.run(0.0, ng, plasticity=False)
+        s = n.stats()
+        out.append((s.spikes, s.isi_cv, s.synchrony, n.membrane().copy()))
+        n.close()
+    return out
+
+
+@pytest.mark.parametrize("workgroups", [0, 6])
+def test_batch_equals_individual_runs(workgroups):
+    pts = small_points(7, n_range=(300, 600))     # at most 5 workgroups per network
+    ng_of = lambda p: 80 + p.delay_steps          # uneven run lengths
+    part_of = lambda p: max(1, -(-p.n // 128))
+    print(f"[batch] 7 networks, workgroups={workgroups}", flush=True)
+    ref = _individual(pts, ng_of, part_of)
+    nets = [sweep.build(p, part_of(p)) for p in pts]
+    for n, p in zip(nets, pts):
+        n.stats_begin(10, ng_of(p) * p.delay_steps)
+    b = abi.Batch(nets, workgroups)
+    if workgroups:
+        assert b.workgroups == workgroups
+    ms, est = b.run(np.zeros(len(pts)), [ng_of(p) for p in pts])
+    assert ms > 0
+    for k, (n, p) in enumerate(zip(nets, pts)):
+        s = n.stats()
+        assert (s.spikes, s.isi_cv, s.synchrony) == ref[k][:3], f"network {k}"
+        assert np.array_equal(n.membrane(), ref[k][3]), f"network {k} membrane"
+    # a second batch run continues every network where it stopped
+    starts = [float(F32(F32(ng_of(p) * p.delay_steps) * F32(p.dt))) for p in pts]
+    b.run(np.array(starts), [10] * len(pts))
+    b.close()
+    for n in nets:
+        n.close()
+
+
+def test_batched_rates_within_2pc_of_fp64():
+    pts = small_points(4, seed=11, n_range=(800, 1600))
+    seconds = 1.0
+    print(f"[batch] 4 networks 1 s FP32 batch vs FP64 oracle", flush=True)
+    r = sweep.run_batched(pts, seconds, sweep.default_plan(pts, 128))
+    for k, p in enumerate(pts):
+        d = host_replay(p, monitors=(0,))
+        o = oracle_py.OracleNetwork(d, precision=64, threads=8)
+        o.run(0.0, p.groups(seconds))
+        c = o.counters()
+        rate64 = c["spikes_neurons"] / (p.n * p.groups(seconds) * p.delay_steps * float(F32(p.dt)))
+        o.close()
+        print(f"point {k}: N={p.n} rate FP32 batch {r.rate[k]:.3f} FP64 {rate64:.3f}", flush=True)
+        assert abs(r.rate[k] - rate64) <= 0.02 * rate64
