@@ -298,8 +298,9 @@ int spk_get_traces(spk_network* net, int rule_id, float* trace_a, float* trace_b
  * connectivity on the host, Examples/UtilityFunctions.hpp:51-82 and Synapses.cpp:102-388,
  * which takes minutes for 1e9 synapses).  N = n_exc + n_inh LIF neurons of one parameter
  * set; every neuron receives ce synapses from the excitatory and ci from the inhibitory
- * population, sources drawn with replacement by Philox4x32-10 (key = seed, counter =
- * (slot, neuron, 3)): source = floor(u*n_exc/2^32) or n_exc + floor(u*n_inh/2^32).  All
+ * population, sources drawn with replacement by Philox4x32-10 (key = seed; draw j of
+ * neuron i is output j mod 4 of the call with counter (j/4, i, 3)):
+ * source = floor(u*n_exc/2^32) or n_exc + floor(u*n_inh/2^32).  All
  * delays are delay_steps (= G).  Such a network runs on the persistent kernel only
  * (spk_run, spk_step_group, statistics, monitors; no plasticity, no weight access). */
 typedef struct {

@@ -29,10 +29,11 @@
 //   * Poisson inputs: the reference's next-spike-time sampler (exponential waiting time,
 //     one spike per step at most, first processed step only arms the neuron) is the
 //     per-step Bernoulli process p = 1 - exp(-rate*dt) started one step after arming.
-//     It is drawn as philox4x32_10(ctr={step_lo, step_hi, input id, 1}, key=seed).x <
-//     floor(p * 2^32) (p in FP64 on the host).
+//     The draw for step n of input i is output n mod 4 of
+//     philox4x32_10(ctr={n/4 lo, n/4 hi, i, 1}, key=seed), compared < floor(p * 2^32)
+//     (p in FP64 on the host).
 //   * External per-neuron drive: count = #{k : u >= C[k]}, C[k] = floor(F(k) * 2^32), F the
-//     Poisson(rate*dt) CDF, u = philox(ctr={step_lo, step_hi, neuron, 2}, key=seed).x; the
+//     Poisson(rate*dt) CDF, u = output step mod 4 of philox(ctr={step/4, neuron, 2}, key=seed); the
 //     count times the event weight joins label 0 of that step's ring input.
 //   * All float expressions are evaluated as written, without FMA contraction
 //     (built with -ffp-contract=off).
@@ -80,12 +81,33 @@ struct Philox {
     }
     out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
   }
-  static uint32_t draw(uint64_t seed, int64_t step, uint32_t id, uint32_t stream) {
-    uint32_t ctr[4] = {(uint32_t)(uint64_t)step, (uint32_t)((uint64_t)step >> 32), id, stream};
+  // The four outputs of one call are four consecutive draws: draw n of (id, stream) is
+  // output n mod 4 of the call with counter (n / 4 as 64 bits, id, stream), key = seed.
+  static void draws4(uint64_t seed, int64_t blk, uint32_t id, uint32_t stream, uint32_t out[4]) {
+    uint32_t ctr[4] = {(uint32_t)(uint64_t)blk, (uint32_t)((uint64_t)blk >> 32), id, stream};
     uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
-    uint32_t out[4];
     block(ctr, key, out);
-    return out[0];
+  }
+  static uint32_t draw(uint64_t seed, int64_t n, uint32_t id, uint32_t stream) {
+    uint32_t out[4];
+    draws4(seed, n >> 2, id, stream, out);
+    return out[n & 3];
+  }
+};
+
+// Draws of one (id, stream) taken in increasing order: one Philox call per four draws.
+struct DrawCursor {
+  uint64_t seed;
+  uint32_t id, stream;
+  int64_t blk = -1;
+  uint32_t out[4] = {0, 0, 0, 0};
+  DrawCursor(uint64_t s, uint32_t i, uint32_t st) : seed(s), id(i), stream(st) {}
+  uint32_t operator()(int64_t n) {
+    if ((n >> 2) != blk) {
+      blk = n >> 2;
+      Philox::draws4(seed, blk, id, stream, out);
+    }
+    return out[n & 3];
   }
 };
 
@@ -303,13 +325,14 @@ struct Net {
     parallel(N, [&](int th, int a, int b) {
       for (int i = a; i < b; ++i) {
         R Vi = V[i];
+        DrawCursor ext_draw(seed, (uint32_t)i, STREAM_EXTERNAL);
         for (int g = 0; g < G; ++g) {
           int64_t step = (int64_t)tidx + g;
           set_bit(hist_n, N, W, i, step, false);
           size_t slot = (size_t)((tidx + g) % B);
           Acc ext = (Acc)0;
           if (!ext_cdf.empty()) {
-            uint32_t u = Philox::draw(seed, step, (uint32_t)i, STREAM_EXTERNAL);
+            uint32_t u = ext_draw(step);
             int k = 0;
             for (uint32_t cth : ext_cdf) k += (u >= cth) ? 1 : 0;
             ext_counts[th] += k;
@@ -371,12 +394,13 @@ struct Net {
     int tidx = (int)std::roundf(t / dt);
     if (pending_arm) { arm_step = tidx; pending_arm = false; }
     const uint32_t* thr_s = in_thr.data() + (size_t)stim * Nin;
-    for (int g = 0; g < G; ++g) {
-      int64_t step = (int64_t)tidx + g;
-      float tg = t + (float)g * dt;
-      for (int i = 0; i < Nin; ++i) {
+    for (int i = 0; i < Nin; ++i) {
+      DrawCursor in_draw(seed, (uint32_t)i, STREAM_INPUT);
+      for (int g = 0; g < G; ++g) {
+        int64_t step = (int64_t)tidx + g;
+        float tg = t + (float)g * dt;
         set_bit(hist_in, Nin, W, i, step, false);
-        uint32_t u = Philox::draw(seed, step, (uint32_t)i, STREAM_INPUT);
+        uint32_t u = in_draw(step);
         if (u < thr_s[i] && step != arm_step) {
           set_bit(hist_in, Nin, W, i, step, true);
           last_in[i] = tg;
@@ -852,7 +876,8 @@ int or_poisson_count_table(double lambda, uint32_t* out, int cap) {
 int or_choose_frac_bits(double m) { return choose_frac_bits(m); }
 
 // Fixed in-degree Brunel connectivity drawn from Philox (the rule the library generates on
-// the device): neuron i, slot j < ce + ci draws u = philox(ctr = {j, 0, i, 3}, key = seed).x;
+// the device): neuron i, slot j < ce + ci draws u = output j mod 4 of
+// philox(ctr = {j/4, 0, i, 3}, key = seed);
 // the source is floor(u*ne/2^32) for j < ce, else ne + floor(u*ni/2^32).  pre[i*(ce+ci)+j].
 int or_brunel_connectivity(uint64_t seed, int ne, int ni, int ce, int ci, int n, int32_t* pre) {
   const int C = ce + ci;

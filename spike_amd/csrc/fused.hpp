@@ -138,7 +138,7 @@ struct PhaseClock {
 // spike masks take 64-bit LDS atomics, which need 8-byte aligned addresses).
 struct FusedLayout {
   size_t ring, hist, ext, imask, inp, xbuf, head, scan, ent, counter, st_cnt, st_nisi, st_prev, st_s1,
-      st_s2, st_g, total;
+      st_s2, st_g, cdf, total;
 };
 
 constexpr int kHeadWords = 16;         // words of a source block fetched in the first trip
@@ -170,6 +170,7 @@ __host__ __device__ inline FusedLayout fused_layout(int BL, int L, int W, int G,
   f.st_s1 = o;   o = align16(o + (size_t)nown * 8);
   f.st_s2 = o;   o = align16(o + (size_t)nown * 8);
   f.st_g = o;    o = align16(o + 64 * 4);
+  f.cdf = o;     o = align16(o + (size_t)kMaxExtTable * 4);   // external count thresholds
   f.total = o;
   return f;
 }
@@ -242,20 +243,40 @@ __device__ __forceinline__ void group_time(const FusedArgs& a, int s, float& t, 
 // Random draws of group s: external Poisson counts per (neuron, sub-step) and the spike
 // masks of the Poisson inputs (Bernoulli per sub-step).  Independent of network state,
 // so they are computed while the workgroup waits at the previous group's barrier.
+// `su` is LDS scratch of nown*G words (the group's float inputs, unused at this point) and
+// `scdf` the count thresholds in LDS.
 __device__ __forceinline__ void draw_group(const FusedArgs& a, int s, uint8_t* ext,
                                            unsigned long long* imask, int nlo, int nown, int ilo,
-                                           int iown, unsigned long long& st_ext) {
+                                           int iown, unsigned long long& st_ext, uint32_t* su,
+                                           const uint32_t* scdf) {
   float t;
   int tidx;
   group_time(a, s, t, tidx);
   const int G = a.G;
   const int tid = threadIdx.x;
+  // the group's steps [tidx, tidx + G) lie in the 4-step Philox blocks b0 .. b0 + nbk - 1;
+  // one work item per (unit, block) uses all four outputs of its call
+  const int64_t b0 = (int64_t)tidx >> 2;
+  const int nbk = (int)((((int64_t)tidx + G - 1) >> 2) - b0) + 1;
   if (a.ext_k > 0) {
+    // (1) one Philox call per (neuron, block) into su[i*G + g]; (2) every (neuron, step)
+    // counts its events in parallel: the number of thresholds C[k] <= u, scanned upwards
+    // until the first C[k] > u (C is increasing)
+    for (int k = tid; k < nown * nbk; k += kFusedBlock) {
+      const int i = k / nbk;
+      const int64_t blk = b0 + (k - i * nbk);
+      const u32x4 r = draw_block(a.seed, blk, (uint32_t)(nlo + i), kStreamExternal);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const int g = (int)(blk * 4 + c - (int64_t)tidx);
+        if (g >= 0 && g < G) su[i * G + g] = pick(r, c);
+      }
+    }
+    __syncthreads();
     for (int k = tid; k < nown * G; k += kFusedBlock) {
-      const int i = k / G, g = k - i * G;
-      const uint32_t u = draw_u32(a.seed, (int64_t)tidx + g, (uint32_t)(nlo + i), kStreamExternal);
+      const uint32_t u = su[k];
       int cnt = 0;
-      for (int j = 0; j < a.ext_k; ++j) cnt += (u >= a.ext_cdf[j]) ? 1 : 0;
+      while (cnt < a.ext_k && u >= scdf[cnt]) ++cnt;
       ext[k] = (uint8_t)cnt;
       st_ext += (unsigned long long)cnt;
     }
@@ -263,11 +284,19 @@ __device__ __forceinline__ void draw_group(const FusedArgs& a, int s, uint8_t* e
   if (iown > 0) {
     for (int k = tid; k < iown; k += kFusedBlock) imask[k] = 0ull;
     __syncthreads();
-    for (int k = tid; k < iown * G; k += kFusedBlock) {
-      const int i = k / G, g = k - i * G;
-      const int64_t step = (int64_t)tidx + g;
-      const uint32_t u = draw_u32(a.seed, step, (uint32_t)(ilo + i), kStreamInput);
-      if ((u < a.in_thr[ilo + i]) && (step != a.arm_step)) atomicOr(&imask[i], 1ull << g);
+    for (int k = tid; k < iown * nbk; k += kFusedBlock) {
+      const int i = k / nbk;
+      const int64_t blk = b0 + (k - i * nbk);
+      const u32x4 r = draw_block(a.seed, blk, (uint32_t)(ilo + i), kStreamInput);
+      const uint32_t thr = a.in_thr[ilo + i];
+      unsigned long long bits = 0ull;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const int64_t step = blk * 4 + c;
+        const int g = (int)(step - (int64_t)tidx);
+        if (g >= 0 && g < G && pick(r, c) < thr && step != a.arm_step) bits |= 1ull << g;
+      }
+      if (bits) atomicOr(&imask[i], bits);
     }
   }
 }
@@ -411,6 +440,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const int p, cons
   double* sst_s1 = (double*)(smem + f.st_s1);
   double* sst_s2 = (double*)(smem + f.st_s2);
   unsigned int* sst_g = (unsigned int*)(smem + f.st_g);
+  uint32_t* scdf = (uint32_t*)(smem + f.cdf);
   __shared__ int s_ok;
   __shared__ unsigned long long s_prof[kProfPhases];
 
@@ -470,7 +500,9 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const int p, cons
   const int bits = W * 32;
   const int nown_waves = ((nown + kWave - 1) / kWave) * kWave;
   if (tid == 0) scounter[0] = 0;
-  draw_group(a, 0, sext, simask, nlo, nown, ilo, iown, st_ext);
+  for (int j = tid; j < a.ext_k; j += kFusedBlock) scdf[j] = a.ext_cdf[j];
+  __syncthreads();
+  draw_group(a, 0, sext, simask, nlo, nown, ilo, iown, st_ext, (uint32_t*)sinp, scdf);
   __syncthreads();
   PhaseClock clk;
   clk.start(a, s_prof, vb);
@@ -693,7 +725,8 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const int p, cons
     }
     if (s + 1 < a.ngroups)
       draw_group(a, s + 1, sext + (size_t)(par ^ 1) * a.nown_max * G,
-                 simask + (size_t)(par ^ 1) * a.iown_max, nlo, nown, ilo, iown, st_ext);
+                 simask + (size_t)(par ^ 1) * a.iown_max, nlo, nown, ilo, iown, st_ext,
+                 (uint32_t*)sinp, scdf);
     clk.lap(a, 3);
     if (tid == 0) s_ok = barrier_wait(a, s) ? 1 : 0;
     __syncthreads();

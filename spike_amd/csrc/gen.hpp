@@ -3,7 +3,8 @@
 // Fixed in-degree (Brunel 2000; the rule of Examples/UtilityFunctions.hpp:51-82, sources
 // drawn with replacement): every neuron i receives ce synapses from the excitatory
 // population [0, ne) and ci from the inhibitory population [ne, ne+ni).  Slot j of neuron
-// i draws u = Philox(seed, ctr = (j, i, stream 3)) and takes
+// i draws u = draw j of (i, stream 3) (output j mod 4 of Philox(seed, ctr = (j/4, i, 3))) and
+// takes
 //   pre = floor(u * ne / 2^32)            for j <  ce
 //   pre = ne + floor(u * ni / 2^32)       for j >= ce
 // The synapses are counting-sorted on the device by key (presynaptic unit, partition of

@@ -152,6 +152,8 @@ __global__ __launch_bounds__(256) void k_lif(LifArgs a) {
   uint32_t word = 0;
   unsigned long long ext_sum = 0;
   const float half_dt = 0.5f * a.dt;
+  int64_t ext_blk = -1;
+  u32x4 ext_r{0u, 0u, 0u, 0u};
   unsigned int st_cnt = 0u, st_nisi = 0u;
   float st_prev = 0.0f;
   double st_s1 = 0.0, st_s2 = 0.0;
@@ -172,7 +174,11 @@ __global__ __launch_bounds__(256) void k_lif(LifArgs a) {
     // external per-neuron Poisson drive, joined to label 0 of this step's input
     int64_t ext = 0;
     if (a.ext_k > 0) {
-      const uint32_t u = draw_u32(a.seed, step, (uint32_t)ii, kStreamExternal);
+      if ((step >> 2) != ext_blk) {
+        ext_blk = step >> 2;
+        ext_r = draw_block(a.seed, ext_blk, (uint32_t)ii, kStreamExternal);
+      }
+      const uint32_t u = pick(ext_r, (int)(step & 3));
       int k = 0;
       for (int j = 0; j < a.ext_k; ++j) k += (u >= a.ext_cdf[j]) ? 1 : 0;
       ext = (int64_t)k * a.q_ext;
@@ -265,6 +271,8 @@ __global__ __launch_bounds__(256) void k_poisson(PoissonArgs a) {
   const int ii = active ? i : 0;
   const uint32_t T = a.thr[ii];
   float last = a.last[ii];
+  int64_t in_blk = -1;
+  u32x4 in_r{0u, 0u, 0u, 0u};
   int cur_w = -1;
   uint32_t word = 0;
   for (int g = 0; g < a.G; ++g) {
@@ -277,7 +285,11 @@ __global__ __launch_bounds__(256) void k_poisson(PoissonArgs a) {
       cur_w = w;
     }
     word &= ~(1u << (b & 31));
-    const uint32_t u = draw_u32(a.seed, step, (uint32_t)ii, kStreamInput);
+    if ((step >> 2) != in_blk) {
+      in_blk = step >> 2;
+      in_r = draw_block(a.seed, in_blk, (uint32_t)ii, kStreamInput);
+    }
+    const uint32_t u = pick(in_r, (int)(step & 3));
     const bool spk = active && (u < T) && (step != a.arm_step);
     if (spk) {
       word |= (1u << (b & 31));

@@ -47,14 +47,22 @@ SPK_HD u32x4 philox4x32_10(u32x4 c, uint32_t k0, uint32_t k1) {
   return c;
 }
 
-// First 32-bit output for (seed, step, id, stream).
-SPK_HD uint32_t draw_u32(uint64_t seed, int64_t step, uint32_t id, uint32_t stream) {
+// One call yields four draws: draw n of (id, stream) is output n mod 4 of the block with
+// counter (n / 4 as 64 bits, id, stream) and key = seed.
+SPK_HD u32x4 draw_block(uint64_t seed, int64_t block, uint32_t id, uint32_t stream) {
   u32x4 c;
-  c.x = (uint32_t)(uint64_t)step;
-  c.y = (uint32_t)((uint64_t)step >> 32);
+  c.x = (uint32_t)(uint64_t)block;
+  c.y = (uint32_t)((uint64_t)block >> 32);
   c.z = id;
   c.w = stream;
-  return philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32)).x;
+  return philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+}
+
+SPK_HD uint32_t pick(const u32x4& r, int c) { return c == 0 ? r.x : (c == 1 ? r.y : (c == 2 ? r.z : r.w)); }
+
+// Draw n (n >= 0) of (seed, id, stream).
+SPK_HD uint32_t draw_u32(uint64_t seed, int64_t n, uint32_t id, uint32_t stream) {
+  return pick(draw_block(seed, n >> 2, id, stream), (int)(n & 3));
 }
 
 }  // namespace spk

@@ -107,19 +107,22 @@ def test_fp32_gpu_statistics_within_2pc_of_fp64_oracle():
 
 
 def test_fp32_plastic_weight_histogram_within_2pc_of_fp64():
-    """WD STDP on E->E for 2 s with a learning rate at which each weight averages many
-    pairings: the FP32 weight distribution (mean, spread, 20-bin histogram) stays within
-    2 % of the FP64 one although the spike trains have decorrelated."""
-    d = builders.brunel(n_exc=800, n_inh=200, seed=6)
-    ne = 800
+    """WD STDP on E->E of a 4000-neuron network for 2 s at a learning rate where each weight
+    averages many pairings: the FP32 weight distribution (mean, spread, 20-bin histogram)
+    stays within 2 % of the FP64 one although the spike trains have decorrelated.  (At
+    1000 neurons the weights of synapses sharing a neuron are too correlated for a 2 %
+    histogram comparison: there two FP64 runs that differ only in the random seed already
+    differ by about 6 %.)"""
+    d = builders.brunel(n_exc=3200, n_inh=800, seed=6)
+    ne = 3200
     exc = np.nonzero((d.pre >= 0) & (d.pre < ne) & (d.post < ne))[0].astype(np.int64)
     d.rules.append(Rule(RULE_WD, exc, builders.wd_params(lam=0.05, alpha=2.02, w_max=0.3e-3)))
     ng = d.groups_for(2.0)
-    print(f"[stats] plastic brunel N=1000 2 s FP32 GPU vs FP64 oracle weights", flush=True)
+    print(f"[stats] plastic brunel N=4000 2 s FP32 GPU vs FP64 oracle weights", flush=True)
     gpu = abi.Network(d)
     gpu.run(0.0, ng)
     wg = gpu.weights()[exc].astype(np.float64)
-    o = oracle_py.OracleNetwork(d, precision=64, threads=8)
+    o = oracle_py.OracleNetwork(d, precision=64, threads=16)
     o.run(0.0, ng)
     wo = o.weights()[exc]
     edges = np.linspace(min(wg.min(), wo.min()), max(wg.max(), wo.max()) * (1 + 1e-9), 21)
