@@ -9,6 +9,10 @@ connectivity (seed 42).  One bench step = one simulated second (666 groups x 15 
 the reference's truncated group count).  Synaptic events = recurrent deliveries
 (spikes x out-degree) + external Poisson events.
 
+Second quantity of the metric ("sweep" in the JSON line): 256 Brunel networks of uneven
+size, 1 s each, in one batched persistent launch versus one network per launch, with the
+batched FP32 rates checked against the FP64 oracle on a sample of the points.
+
 Prints one JSON line with the metric, the roofline of the dominant kernel and the CPU
 baseline (the FP32 oracle on the host cores, on a bounded sample of the same network).
 """
@@ -61,6 +65,63 @@ def cpu_baseline(desc, groups, threads):
     return ev / dt, dt, ev
 
 
+def sweep_leg(n_points, seconds, cpu_sample, threads, log):
+    """Batched sweep: `n_points` Brunel networks (N log-uniform 1250-12500, g U(3,8),
+    nu_ext/nu_thr U(0.9,4), J U(0.05,0.2) mV, delay U{10..20} steps, eps 0.1, seeds
+    2024 + index) simulated for `seconds` each, (1) in one persistent batched launch with the
+    autotuned tiling and (2) one network per launch.  Points/s = points / device seconds
+    of the simulation launches (HIP events; network generation and the autotuner's probe
+    runs are reported separately).  The CPU leg runs the FP64 oracle on every
+    (n_points // cpu_sample)-th point: its points/s is the CPU baseline of this metric and
+    its rates gate the batched FP32 rates (<= 2 %)."""
+    import numpy as np
+    from spike_amd import sweep
+    pts = sweep.sample_points(n_points)
+    t0 = time.perf_counter()
+    best = sweep.autotune(pts, log=log)
+    tune_s = time.perf_counter() - t0
+    rb = sweep.run_batched(pts, seconds, best["plan"])
+    log(f"sweep batched: {rb.ms:.2f} ms ({rb.config})")
+    rp = sweep.run_plain(pts, seconds)
+    log(f"sweep plain: {rp.ms:.2f} ms")
+    out = {
+        "points": n_points, "seconds_per_point": seconds,
+        "batched_points_per_s": round(n_points / (rb.ms / 1e3), 2),
+        "plain_points_per_s": round(n_points / (rp.ms / 1e3), 2),
+        "speedup_vs_plain": round(rp.ms / rb.ms, 3),
+        "batched_ms": round(rb.ms, 3), "plain_ms": round(rp.ms, 3),
+        "plan": best["plan"]["name"], "workgroups": rb.config["workgroups"],
+        "autotune_s": round(tune_s, 2), "cost_fit_us": best["fit"],
+        "batched_vs_plain_rate_max_rel_diff": float(np.max(np.abs(rb.rate - rp.rate) / np.maximum(rp.rate, 1e-12))),
+        "mean_rate_hz": round(float(rb.rate.mean()), 3),
+    }
+    if cpu_sample > 0:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        from sweep_replay import oracle_rate
+        idx = list(range(0, n_points, max(1, n_points // cpu_sample)))[:cpu_sample]
+        errs, floor, secs = [], [], 0.0
+        for k in idx:
+            r64, el = oracle_rate(pts[k], seconds, precision=64, threads=threads)
+            secs += el
+            errs.append(abs(rb.rate[k] - r64) / r64)
+            # the same network and precision under another realisation of the external drive
+            r64b, _ = oracle_rate(pts[k], seconds, precision=64, threads=threads,
+                                  drive_seed=pts[k].seed + 7919)
+            floor.append(abs(r64b - r64) / r64)
+            log(f"sweep point {k} N={pts[k].n}: batched FP32 {rb.rate[k]:.3f} Hz, FP64 oracle {r64:.3f} Hz, "
+                f"FP64 other drive {r64b:.3f} Hz")
+        out["rate_err_vs_fp64_max"] = round(float(max(errs)), 6)
+        out["rate_err_vs_fp64_mean"] = round(float(np.mean(errs)), 6)
+        out["fp64_drive_realisation_spread_max"] = round(float(max(floor)), 6)
+        out["fp64_drive_realisation_spread_mean"] = round(float(np.mean(floor)), 6)
+        out["rate_err_points"] = idx
+        out["cpu_baseline"] = {"value": round(len(idx) / secs, 4), "unit": "sweep points/s",
+                               "cores": threads, "kind": "port",
+                               "sample": f"FP64 oracle on {len(idx)} of the {n_points} points, {seconds} s each, "
+                                         f"{secs:.2f} s"}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -72,6 +133,11 @@ def main():
                     help="timestep groups of the CPU-baseline sample (15 steps each)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--sweep-points", type=int, default=256,
+                    help="Brunel networks in the batched-sweep measurement (0 skips it)")
+    ap.add_argument("--sweep-seconds", type=float, default=1.0)
+    ap.add_argument("--sweep-cpu-sample", type=int, default=8,
+                    help="sweep points re-run on the FP64 oracle (rate-error gate, CPU baseline)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run a plain device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -148,7 +214,7 @@ def main():
                "sample": f"FP32 oracle, Brunel10K {args.form} form, first {args.cpu_groups} groups "
                          f"({args.cpu_groups * G} steps) from reset, {secs:.2f} s, {ev} events"}
     out = {
-        "metric": "syn events/s (Brunel10K)",
+        "metric": "syn events/s (Brunel10K); batched sweep points/s vs plain at \u22642% rate err",
         "value": round(value, 1),
         "unit": "syn events/s",
         "n_gpus": 1,
@@ -172,6 +238,11 @@ def main():
     }
     if cpu:
         out["speedup_vs_cpu"] = round(value / cpu["value"], 2)
+    if args.sweep_points > 0:
+        log = lambda m: print(m, file=sys.stderr, flush=True)
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        out["sweep"] = sweep_leg(args.sweep_points, args.sweep_seconds,
+                                 0 if args.no_cpu else args.sweep_cpu_sample, threads, log)
     print(json.dumps(out))
 
 

@@ -11,30 +11,11 @@ import numpy as np
 import pytest
 
 import oracle_py
+from sweep_replay import replay_desc as host_replay, oracle_rate
 from spike_amd import abi, sweep
-from spike_amd.netdesc import NetDesc, SYN_VOLTAGE
 
 pytestmark = pytest.mark.gpu
 F32 = np.float32
-
-
-def host_replay(p: sweep.SweepPoint, monitors=(0,)) -> NetDesc:
-    """The point's network as host arrays: connectivity from the oracle's restatement of
-    the device generator, constants from sweep.point_constants."""
-    k = sweep.point_constants(p)
-    N = p.n
-    pre = oracle_py.brunel_connectivity(p.seed, p.n_exc, p.n_inh, p.ce, p.ci)
-    C = p.ce + p.ci
-    post = np.repeat(np.arange(N, dtype=np.int32), C)
-    w = np.where(np.arange(C) < p.ce, k["weight_exc"], k["weight_inh"]).astype(F32)
-    d = NetDesc(n_neurons=N, n_inputs=0, kind=SYN_VOLTAGE, dt=float(k["dt"]), seed=p.seed,
-                v0=np.full(N, k["v0"], F32), vreset=np.full(N, k["vreset"], F32),
-                threshold=np.full(N, k["threshold"], F32), decay_c=np.full(N, k["decay_c"], F32),
-                resist=np.full(N, k["resist"], F32), refractory=float(k["refractory"]), background=0.0,
-                ext_rate=float(k["ext_rate"]), ext_weight=float(k["ext_weight"]),
-                pre=pre.reshape(-1), post=post, weight=np.tile(w, N),
-                delay=np.full(N * C, p.delay_steps, np.int32), monitors=list(monitors), name="replay")
-    return d
 
 
 def small_points(count, seed=7, n_range=(300, 1500)):
@@ -112,11 +93,6 @@ def test_batched_rates_within_2pc_of_fp64():
     print(f"[batch] 4 networks 1 s FP32 batch vs FP64 oracle", flush=True)
     r = sweep.run_batched(pts, seconds, sweep.default_plan(pts, 128))
     for k, p in enumerate(pts):
-        d = host_replay(p, monitors=(0,))
-        o = oracle_py.OracleNetwork(d, precision=64, threads=8)
-        o.run(0.0, p.groups(seconds))
-        c = o.counters()
-        rate64 = c["spikes_neurons"] / (p.n * p.groups(seconds) * p.delay_steps * float(F32(p.dt)))
-        o.close()
+        rate64, _ = oracle_rate(p, seconds, precision=64, threads=8)
         print(f"point {k}: N={p.n} rate FP32 batch {r.rate[k]:.3f} FP64 {rate64:.3f}", flush=True)
         assert abs(r.rate[k] - rate64) <= 0.02 * rate64
