@@ -1,4 +1,4 @@
-"""ctypes binding of the C ABI in include/spike_hip.h (libspike_hip.so).
+"""ctypes binding of the C ABI in include/spike_hip.h and include/spike_mlp.h (libspike_hip.so).
 
 This module is plumbing for tests and the benchmark: the product's host interface is
 Spike's C++ front-end, whose HIP backend calls the same entry points.  Loading fails
@@ -142,6 +142,21 @@ SIGNATURES = {
     "spk_get_ring": (C.c_int, [P, lp]),
     "spk_get_history": (C.c_int, [P, C.c_int, up]),
     "spk_get_traces": (C.c_int, [P, C.c_int, fp, fp]),
+    # include/spike_mlp.h
+    "spk_mlp_param_count": (C.c_int, []),
+    "spk_mlp_create": (C.c_int, [C.c_int, C.c_int, C.POINTER(P)]),
+    "spk_mlp_destroy": (C.c_int, [P]),
+    "spk_mlp_set_params": (C.c_int, [P, fp]),
+    "spk_mlp_get_params": (C.c_int, [P, fp]),
+    "spk_mlp_reset_optimizer": (C.c_int, [P]),
+    "spk_mlp_set_data": (C.c_int, [P, fp, fp, C.c_int64]),
+    "spk_mlp_set_loss_weights": (C.c_int, [P, fp]),
+    "spk_mlp_gradients": (C.c_int, [P, C.c_int, fp, fp]),
+    "spk_mlp_apply_gradients": (C.c_int, [P, fp, C.c_float]),
+    "spk_mlp_train": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_uint64, C.c_float, C.c_float, C.c_int,
+                                C.c_int, fp]),
+    "spk_mlp_predict": (C.c_int, [P, fp, C.c_int64, fp]),
+    "spk_mlp_last_timing": (C.c_int, [P, C.c_void_p]),
 }
 
 _lib: Optional[C.CDLL] = None

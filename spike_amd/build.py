@@ -30,10 +30,10 @@ def run(cmd, cwd=None):
 
 def build_hip_library() -> str:
     os.makedirs(LIB_DIR, exist_ok=True)
-    src = os.path.join(HERE, "csrc", "spike_hip.hip")
+    srcs = [os.path.join(HERE, "csrc", f) for f in ("spike_hip.hip", "mlp.hip")]
     out = os.path.join(LIB_DIR, "libspike_hip.so")
     tmp = out + ".tmp"
-    run([HIPCC, *HIP_FLAGS, "-shared", "-o", tmp, src])
+    run([HIPCC, *HIP_FLAGS, "-shared", "-o", tmp, *srcs])
     os.replace(tmp, out)
     return out
 

@@ -700,6 +700,11 @@ int run_fused(spk_network* n, float start, int64_t ngroups, int flags) {
 
 }  // namespace
 
+namespace spk {
+// error message of the calling thread, shared with the surrogate's entry points (mlp.hip)
+void set_last_error(const std::string& msg) { g_error = msg; }
+}  // namespace spk
+
 // =====================================================================================
 extern "C" {
 

@@ -1,5 +1,5 @@
 """The C-ABI library loads and exports every entry point declared in include/spike_hip.h
-(no device calls: this runs without a GPU)."""
+and include/spike_mlp.h (no device calls: this runs without a GPU)."""
 import ctypes
 import os
 import re
@@ -10,14 +10,17 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def declared_symbols():
-    text = open(os.path.join(ROOT, "include", "spike_hip.h")).read()
-    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(spk_[a-z0-9_]+)\s*\(", text)))
+    syms = set()
+    for h in ("spike_hip.h", "spike_mlp.h"):
+        text = open(os.path.join(ROOT, "include", h)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        syms |= set(re.findall(r"\b(spk_[a-z0-9_]+)\s*\(", text))
+    return sorted(syms)
 
 
 def test_header_declares_entry_points():
     syms = declared_symbols()
-    assert "spk_run" in syms and "spk_create" in syms
+    assert "spk_run" in syms and "spk_create" in syms and "spk_mlp_train" in syms
     assert len(syms) >= 40
 
 
