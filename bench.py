@@ -28,13 +28,14 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
-def algorithmic_bytes(n_neurons, steps, spikes, rec_events, b_w=4):
+def algorithmic_bytes(n_neurons, steps, spikes, rec_events, record_bytes=8):
     """Algorithmic HBM bytes of the hot path:
     per neuron per step 24 B (V r/w 8, refractory state r/w 8, ring slot read+clear 8);
     per spike 16 B (list write 4, list read 4, row pointers 8);
-    per recurrent delivery 4 B target + b_w B weight.
+    per recurrent delivery the synapse record: 8 B (4 B target/delay/label + 4 B FP32
+    weight), or 2 B in the compact form (16-bit target; weight and delay per source type).
     External Poisson events are generated in registers (0 B)."""
-    return n_neurons * steps * 24 + spikes * 16 + rec_events * (4 + b_w)
+    return n_neurons * steps * 24 + spikes * 16 + rec_events * record_bytes
 
 
 def measured_traffic(form, path):
@@ -122,6 +123,54 @@ def sweep_leg(n_points, seconds, cpu_sample, threads, log):
     return out
 
 
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+
+
+def mlp_leg(log, members=8, rows=1 << 20, batch=4096, steps=200):
+    """Surrogate MLP kernels (5 -> 4 x 256 SiLU -> 3, BF16 MFMA, FP32 accumulation):
+    inference of `rows` rows by `members` ensemble members in one launch, and training
+    steps (forward, backward, weight gradient, Adam) on batches of `batch` rows, replayed as
+    a HIP graph.  Algorithmic FLOPs per row per member: 2 (5*256 + 3*256*256 + 256*3);
+    a training step counts three times that per row (forward, input gradients, weight
+    gradients).  Times are HIP events on the surrogate's stream."""
+    import numpy as np
+    from spike_amd import mlp
+    fl = 2.0 * (5 * 256 + 3 * 256 * 256 + 256 * 3)
+    rng = np.random.default_rng(3)
+    m = mlp.Mlp(members, batch)
+    m.set_params(mlp.init_params(4, members))
+    x = rng.uniform(-1.0, 1.0, (rows, 5)).astype(np.float32)
+    ms = []
+    for _ in range(5):
+        m.predict(x)
+        ms.append(m.timing().forward_ms)
+    inf_ms = float(np.mean(ms[1:]))
+    inf_tflops = members * rows * fl / (inf_ms / 1e3) / 1e12
+    data_x = rng.uniform(-1.0, 1.0, (50000, 5)).astype(np.float32)
+    data_t = rng.standard_normal((50000, 3)).astype(np.float32)
+    m.set_data(data_x, data_t)
+    m.gradients(batch)
+    k = m.timing()
+    m.train(20, batch, 1e-3, 1e-5, 10 * steps, use_graph=True, record_loss=False)
+    m.train(steps, batch, 1e-3, 1e-5, 10 * steps, use_graph=True, record_loss=False)
+    step_ms = m.timing().total_ms / steps
+    train_tflops = 3 * members * batch * fl / (step_ms / 1e3) / 1e12
+    m.close()
+    log(f"mlp: inference {inf_ms:.3f} ms ({inf_tflops:.0f} TFLOP/s), training step {step_ms * 1e3:.1f} us "
+        f"({train_tflops:.0f} TFLOP/s)")
+    return {
+        "workload": f"{members} members x {rows} rows inference; training batch {batch} x {members} members",
+        "dtype": "bf16 (fp32 accumulate)",
+        "inference_ms": round(inf_ms, 4), "inference_rows_per_s": round(members * rows / (inf_ms / 1e3), 1),
+        "roofline": {"bound": "mfma", "kernel": "k_mlp_forward", "achieved": round(inf_tflops, 1),
+                     "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(inf_tflops / MFMA_BF16_PEAK_TFLOPS, 4), "traffic": None},
+        "train_step_us": round(step_ms * 1e3, 2), "train_tflops": round(train_tflops, 1),
+        "train_kernels_us": {"forward": round(k.forward_ms * 1e3, 2), "backward": round(k.backward_ms * 1e3, 2),
+                             "wgrad": round(k.wgrad_ms * 1e3, 2), "adam": round(k.update_ms * 1e3, 2)},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -138,6 +187,7 @@ def main():
     ap.add_argument("--sweep-seconds", type=float, default=1.0)
     ap.add_argument("--sweep-cpu-sample", type=int, default=8,
                     help="sweep points re-run on the FP64 oracle (rate-error gate, CPU baseline)")
+    ap.add_argument("--no-mlp", action="store_true", help="skip the surrogate MLP kernels")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run a plain device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -194,7 +244,8 @@ def main():
     value = events / (tot_ms / 1e3)
     rate_hz = spikes / desc.n_neurons / (steps_sim * desc.dt)
     # roofline of the dominant kernel (the fused group kernel)
-    bytes_alg = algorithmic_bytes(desc.n_neurons, steps_sim, spikes, rec)
+    bytes_alg = algorithmic_bytes(desc.n_neurons, steps_sim, spikes, rec,
+                                  info.record_bytes if args.path == "fused" else 8)
     if launches > 0 and kern_ms > 0:
         avg_launch_ms = kern_ms / launches
         per_launch_bytes = bytes_alg / launches
@@ -229,7 +280,8 @@ def main():
         "config": {"workload": f"Brunel10K {args.form} form: 8000E+2000I LIF, C_E=800/C_I=200, "
                                f"J=0.1 mV g=5, delay 1.5 ms, nu_ext=2nu_thr, dt 0.1 ms, 1 s per step",
                    "n_neurons": desc.n_neurons, "n_synapses": desc.n_synapses,
-                   "timestep_grouping": G, "groups_per_step": groups, "path": args.path},
+                   "timestep_grouping": G, "groups_per_step": groups, "path": args.path,
+                   "record_bytes": info.record_bytes if args.path == "fused" else 8},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "rate_hz": round(rate_hz, 3),
@@ -243,6 +295,8 @@ def main():
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         out["sweep"] = sweep_leg(args.sweep_points, args.sweep_seconds,
                                  0 if args.no_cpu else args.sweep_cpu_sample, threads, log)
+    if not args.no_mlp:
+        out["mlp"] = mlp_leg(lambda m: print(m, file=sys.stderr, flush=True))
     print(json.dumps(out))
 
 

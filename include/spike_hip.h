@@ -91,6 +91,9 @@ typedef struct {
   int n_rules, n_monitors;
   int partitions;            /* workgroups of the persistent kernel (neuron partitions) */
   int fused;                 /* 1 when spk_run uses the persistent kernel */
+  int record_bytes;          /* bytes the persistent kernel reads per delivery: 8 for the
+                                full records {target|delay|label, weight}; 2 for the compact
+                                form (16-bit target; weight and delay per source type) */
 } spk_info;
 
 /* Per-run timing of the last spk_run call, from HIP events on the network's stream. */

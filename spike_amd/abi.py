@@ -48,7 +48,7 @@ class Info(C.Structure):
                 ("ring_slots", C.c_int), ("history_words", C.c_int), ("frac_bits", C.c_int),
                 ("min_delay", C.c_int), ("max_delay", C.c_int), ("n_synapses", C.c_int64),
                 ("n_rules", C.c_int), ("n_monitors", C.c_int), ("partitions", C.c_int),
-                ("fused", C.c_int)]
+                ("fused", C.c_int), ("record_bytes", C.c_int)]
 
 
 class Timing(C.Structure):

@@ -79,6 +79,12 @@ struct FusedArgs {
   const uint32_t* seg;      // [U][P+1] absolute segment offsets
   const uint2* syn;         // per synapse {local target | delay << 20 | label << 28, weight bits}
   const float* scale;       // nullptr: uniform scale0
+  // compact form (no plasticity, one label, uniform scale): every synapse of a source
+  // u < src_split has weight w_lo, every other one w_hi, all have delay src_delay, and the
+  // records are the 16-bit local targets alone (null: the full records above)
+  const uint16_t* syn16;
+  float w_lo, w_hi;
+  int src_split, src_delay;
   // plasticity (rules grouped by postsynaptic partition), descriptors in device memory
   const FusedRule* rules;
   int n_rules;
@@ -138,7 +144,7 @@ struct PhaseClock {
 // spike masks take 64-bit LDS atomics, which need 8-byte aligned addresses).
 struct FusedLayout {
   size_t ring, hist, ext, imask, inp, xbuf, head, scan, ent, counter, st_cnt, st_nisi, st_prev, st_s1,
-      st_s2, st_g, cdf, total;
+      st_s2, st_g, cdf, guide, total;
 };
 
 constexpr int kHeadWords = 16;         // words of a source block fetched in the first trip
@@ -171,6 +177,7 @@ __host__ __device__ inline FusedLayout fused_layout(int BL, int L, int W, int G,
   f.st_s2 = o;   o = align16(o + (size_t)nown * 8);
   f.st_g = o;    o = align16(o + 64 * 4);
   f.cdf = o;     o = align16(o + (size_t)kMaxExtTable * 4);   // external count thresholds
+  f.guide = o;   o = align16(o + 256);                          // guide table of the thresholds
   f.total = o;
   return f;
 }
@@ -243,12 +250,14 @@ __device__ __forceinline__ void group_time(const FusedArgs& a, int s, float& t, 
 // Random draws of group s: external Poisson counts per (neuron, sub-step) and the spike
 // masks of the Poisson inputs (Bernoulli per sub-step).  Independent of network state,
 // so they are computed while the workgroup waits at the previous group's barrier.
-// `su` is LDS scratch of nown*G words (the group's float inputs, unused at this point) and
-// `scdf` the count thresholds in LDS.
+// `su` is LDS scratch of nown*G words (the group's float inputs, unused at this point),
+// `scdf` the count thresholds in LDS and `sguide` their guide table: sguide[j] is the count
+// of u = j * 2^24, a lower bound of the count of every u in [j * 2^24, (j + 1) * 2^24)
+// (Chen & Asau 1974), so the upward scan from it takes about one step.
 __device__ __forceinline__ void draw_group(const FusedArgs& a, int s, uint8_t* ext,
                                            unsigned long long* imask, int nlo, int nown, int ilo,
                                            int iown, unsigned long long& st_ext, uint32_t* su,
-                                           const uint32_t* scdf) {
+                                           const uint32_t* scdf, const uint8_t* sguide) {
   float t;
   int tidx;
   group_time(a, s, t, tidx);
@@ -261,7 +270,7 @@ __device__ __forceinline__ void draw_group(const FusedArgs& a, int s, uint8_t* e
   if (a.ext_k > 0) {
     // (1) one Philox call per (neuron, block) into su[i*G + g]; (2) every (neuron, step)
     // counts its events in parallel: the number of thresholds C[k] <= u, scanned upwards
-    // until the first C[k] > u (C is increasing)
+    // from the guide entry of u until the first C[k] > u (C is non-decreasing)
     for (int k = tid; k < nown * nbk; k += kFusedBlock) {
       const int i = k / nbk;
       const int64_t blk = b0 + (k - i * nbk);
@@ -275,7 +284,7 @@ __device__ __forceinline__ void draw_group(const FusedArgs& a, int s, uint8_t* e
     __syncthreads();
     for (int k = tid; k < nown * G; k += kFusedBlock) {
       const uint32_t u = su[k];
-      int cnt = 0;
+      int cnt = sguide[u >> 24];
       while (cnt < a.ext_k && u >= scdf[cnt]) ++cnt;
       ext[k] = (uint8_t)cnt;
       st_ext += (unsigned long long)cnt;
@@ -387,6 +396,69 @@ __device__ __forceinline__ void deliver_quad(const FusedArgs& a, const uint32_t*
   }
 }
 
+// Compact form of deliver_quad: 2-byte records, the fixed-point weight and the slot are
+// those of the spike's source (q_lo below src_split, q_hi from it on), computed once per
+// spike with the same expression as the full form.
+__device__ __forceinline__ void deliver_quad_compact(const FusedArgs& a, const uint32_t* ents, int n, int q,
+                                                     int p, int slot0, int B, int nown, int64_t* sring,
+                                                     unsigned long long& st_del, long long q_lo, long long q_hi) {
+  uint32_t s0[kSpikesPerQuad], s1[kSpikesPerQuad];
+  long long qv[kSpikesPerQuad];
+  int slot[kSpikesPerQuad];
+#pragma unroll
+  for (int r = 0; r < kSpikesPerQuad; ++r) {
+    s0[r] = 0u;
+    s1[r] = 0u;
+    qv[r] = 0;
+    slot[r] = 0;
+    if (r < n) {
+      const uint32_t e = ents[r];
+      const uint32_t id = e & kIdMask;
+      const size_t sb = (size_t)id * (a.P + 1) + p;
+      s0[r] = a.seg[sb];
+      s1[r] = a.seg[sb + 1];
+      qv[r] = (int)id < a.src_split ? q_lo : q_hi;
+      int sl = slot0 + (int)(e >> 24) + a.src_delay;
+      while (sl >= B) sl -= B;
+      slot[r] = sl;
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < kSpikesPerQuad; ++r)
+    if (q == 0) st_del += (unsigned long long)(s1[r] - s0[r]);
+  // first two targets per lane of every segment, loads in flight together
+  uint16_t t0[kSpikesPerQuad][2];
+#pragma unroll
+  for (int r = 0; r < kSpikesPerQuad; ++r)
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const uint32_t k = s0[r] + (uint32_t)(q + h * kQuad);
+      t0[r][h] = k < s1[r] ? a.syn16[k] : (uint16_t)0;
+    }
+#pragma unroll
+  for (int r = 0; r < kSpikesPerQuad; ++r) {
+    int64_t* row = sring + (size_t)slot[r] * nown;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const uint32_t k = s0[r] + (uint32_t)(q + h * kQuad);
+      if (k < s1[r]) atomicAdd((unsigned long long*)&row[t0[r][h]], (unsigned long long)qv[r]);
+    }
+    for (uint32_t k0 = s0[r] + (uint32_t)(q + 2 * kQuad); k0 < s1[r]; k0 += 4 * kQuad) {
+      uint16_t tt[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const uint32_t k = k0 + (uint32_t)(u * kQuad);
+        tt[u] = k < s1[r] ? a.syn16[k] : (uint16_t)0;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const uint32_t k = k0 + (uint32_t)(u * kQuad);
+        if (k < s1[r]) atomicAdd((unsigned long long*)&row[tt[u]], (unsigned long long)qv[r]);
+      }
+    }
+  }
+}
+
 // Plasticity of the synapses onto partition p (one group).
 __device__ __forceinline__ void fused_plasticity(const FusedRule* rules, int n_rules, int p, int tidx,
                                               float t) {
@@ -441,6 +513,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const int p, cons
   double* sst_s2 = (double*)(smem + f.st_s2);
   unsigned int* sst_g = (unsigned int*)(smem + f.st_g);
   uint32_t* scdf = (uint32_t*)(smem + f.cdf);
+  uint8_t* sguide = (uint8_t*)(smem + f.guide);
   __shared__ int s_ok;
   __shared__ unsigned long long s_prof[kProfPhases];
 
@@ -497,12 +570,22 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const int p, cons
   }
   unsigned long long st_del = 0, st_ext = 0, st_spk = 0, st_in = 0;
   const float half_dt = 0.5f * a.dt;
+  // fixed-point weights of the compact form: the full form's expression for one record
+  const long long q_lo = __float2ll_rn((a.w_lo * a.scale0) * a.two_k);
+  const long long q_hi = __float2ll_rn((a.w_hi * a.scale0) * a.two_k);
   const int bits = W * 32;
   const int nown_waves = ((nown + kWave - 1) / kWave) * kWave;
   if (tid == 0) scounter[0] = 0;
   for (int j = tid; j < a.ext_k; j += kFusedBlock) scdf[j] = a.ext_cdf[j];
   __syncthreads();
-  draw_group(a, 0, sext, simask, nlo, nown, ilo, iown, st_ext, (uint32_t*)sinp, scdf);
+  if (tid < 256) {
+    const uint32_t u0 = (uint32_t)tid << 24;
+    int c = 0;
+    while (c < a.ext_k && u0 >= scdf[c]) ++c;
+    sguide[tid] = (uint8_t)c;
+  }
+  __syncthreads();
+  draw_group(a, 0, sext, simask, nlo, nown, ilo, iown, st_ext, (uint32_t*)sinp, scdf, sguide);
   __syncthreads();
   PhaseClock clk;
   clk.start(a, s_prof, vb);
@@ -726,7 +809,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const int p, cons
     if (s + 1 < a.ngroups)
       draw_group(a, s + 1, sext + (size_t)(par ^ 1) * a.nown_max * G,
                  simask + (size_t)(par ^ 1) * a.iown_max, nlo, nown, ilo, iown, st_ext,
-                 (uint32_t*)sinp, scdf);
+                 (uint32_t*)sinp, scdf, sguide);
     clk.lap(a, 3);
     if (tid == 0) s_ok = barrier_wait(a, s) ? 1 : 0;
     __syncthreads();
@@ -800,7 +883,9 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const int p, cons
         const int nquads = kFusedBlock / kQuad;
         for (int j0 = quad * kSpikesPerQuad; j0 < nr; j0 += nquads * kSpikesPerQuad) {
           const int n = min(kSpikesPerQuad, nr - j0);
-          if (a.scale == nullptr)
+          if (a.syn16 != nullptr)
+            deliver_quad_compact(a, sent + j0, n, q, p, slot0, BL, nownL, sring, st_del, q_lo, q_hi);
+          else if (a.scale == nullptr)
             deliver_quad<false>(a, sent + j0, n, q, p, slot0, BL, nownL, L, sring, st_del);
           else
             deliver_quad<true>(a, sent + j0, n, q, p, slot0, BL, nownL, L, sring, st_del);

@@ -31,6 +31,7 @@ struct BrunelGenArgs {
   const unsigned int* off;  // [N*P] exclusive scan of count
   unsigned int* cursor;     // [N*P]
   uint2* syn;               // records {local target | delay << 20, weight bits}
+  uint16_t* syn16;          // the same records' local targets (compact delivery)
   uint32_t delay_bits;      // delay << 20
   float we, wi;
 };
@@ -62,6 +63,7 @@ __global__ __launch_bounds__(256) void k_gen_scatter(BrunelGenArgs a, int64_t to
     const unsigned int pos = a.off[key] + atomicAdd(a.cursor + key, 1u);
     const float w = j < a.ce ? a.we : a.wi;
     a.syn[pos] = make_uint2((uint32_t)(i - a.nlo[q]) | a.delay_bits, __float_as_uint(w));
+    a.syn16[pos] = (uint16_t)(i - a.nlo[q]);
   }
 }
 

@@ -170,6 +170,12 @@ struct spk_network {
   int64_t* d_rowptr = nullptr;
   int32_t* d_tgt = nullptr;
   uint2* d_syn = nullptr;   // per synapse {packed local target/delay/label, weight bits}
+  // compact form of the persistent kernel's delivery (FusedArgs::syn16): 16-bit local
+  // targets; sources below csplit weigh cw_lo, the others cw_hi; one delay cdelay
+  uint16_t* d_syn16 = nullptr;
+  bool compact = false;
+  float cw_lo = 0.0f, cw_hi = 0.0f;
+  int csplit = 0, cdelay = 0;
   float* d_scale = nullptr;
   uint16_t* d_delay = nullptr;
   uint8_t* d_label = nullptr;
@@ -249,7 +255,7 @@ void free_device(spk_network* n) {
   dfree(n->d_V); dfree(n->d_v0); dfree(n->d_vreset); dfree(n->d_thr); dfree(n->d_c);
   dfree(n->d_rp); dfree(n->d_last_n); dfree(n->d_last_in); dfree(n->d_hist_n);
   dfree(n->d_hist_in); dfree(n->d_ring); dfree(n->d_g); dfree(n->d_decay); dfree(n->d_erev);
-  dfree(n->d_in_thr); dfree(n->d_ext_cdf); dfree(n->d_rowptr); dfree(n->d_tgt); dfree(n->d_syn);
+  dfree(n->d_in_thr); dfree(n->d_ext_cdf); dfree(n->d_rowptr); dfree(n->d_tgt); dfree(n->d_syn); dfree(n->d_syn16);
   dfree(n->d_scale); dfree(n->d_delay); dfree(n->d_label); dfree(n->d_lists);
   dfree(n->d_counts); dfree(n->d_slot_time); dfree(n->d_slot_tidx); dfree(n->d_stats);
   for (auto& r : n->rules) {
@@ -533,6 +539,10 @@ int fill_fused_args(spk_network* n, bool plastic, spk::FusedArgs& a) {
   a.in_thr = n->d_in_thr + (size_t)n->stim * n->Nin;
   a.seg = n->d_seg; a.syn = n->d_syn;
   a.scale = n->uniform_scale ? nullptr : n->d_scale;
+  if (n->compact && n->rules.empty()) {
+    a.syn16 = n->d_syn16;
+    a.w_lo = n->cw_lo; a.w_hi = n->cw_hi; a.src_split = n->csplit; a.src_delay = n->cdelay;
+  }
   a.n_rules = plastic ? (int)n->rules.size() : 0;
   if (a.n_rules > 0 && !n->rules_uploaded) {
     std::vector<spk::FusedRule> fr(a.n_rules);
@@ -1060,6 +1070,46 @@ int spk_prepare(spk_network* n, int G) {
       const uint32_t loc = (uint32_t)(tgt[k] - n->h_nlo[owner[tgt[k]]]);
       packed[k] = loc | ((uint32_t)dl[k] << 20) | ((uint32_t)lb[k] << 28);
     }
+    // compact form: one label, uniform scale, no plasticity, every row with one weight and
+    // one delay, one delay for all rows, and the row weights w_lo below a split source and
+    // w_hi from it on (an excitatory/inhibitory network numbered E first)
+    n->compact = false;
+    if (n->L == 1 && n->uniform_scale && n->rules.empty() && S > 0 && n->nown_max <= 65536) {
+      bool ok = true, have_lo = false, have_hi = false;
+      uint32_t blo = 0, bhi = 0;
+      int split = U, dconst = -1;
+      for (int u = 0; u < U && ok; ++u) {
+        const int64_t k0 = rowptr[u], k1 = rowptr[u + 1];
+        if (k0 == k1) continue;
+        uint32_t wb;
+        std::memcpy(&wb, &w[k0], sizeof(float));
+        if (dconst < 0) dconst = dl[k0];
+        for (int64_t k = k0; k < k1 && ok; ++k) {
+          uint32_t b;
+          std::memcpy(&b, &w[k], sizeof(float));
+          ok = b == wb && (int)dl[k] == dconst;
+        }
+        if (!have_lo) {
+          have_lo = true;
+          blo = wb;
+        } else if (!have_hi) {
+          if (wb != blo) {
+            have_hi = true;
+            bhi = wb;
+            split = u;
+          }
+        } else if (wb != bhi) {
+          ok = false;
+        }
+      }
+      if (ok) {
+        n->compact = true;
+        std::memcpy(&n->cw_lo, &blo, sizeof(float));
+        std::memcpy(&n->cw_hi, have_hi ? &bhi : &blo, sizeof(float));
+        n->csplit = split;
+        n->cdelay = dconst;
+      }
+    }
   }
   // list capacity: every neuron of both populations may spike once per sub-step
   n->cap = std::max(1, U);
@@ -1113,6 +1163,12 @@ int spk_prepare(spk_network* n, int G) {
       std::memcpy(&rec[k].y, &w[k], sizeof(float));
     }
     if ((rc = upload(n->d_syn, rec.data(), S, st))) return rc;
+  }
+  if (n->compact) {
+    std::vector<uint16_t> t16(S);
+    for (int64_t k = 0; k < S; ++k) t16[k] = (uint16_t)(packed[k] & 0xFFFFFu);
+    if ((rc = dalloc(&n->d_syn16, S))) return rc;
+    if ((rc = upload(n->d_syn16, t16.data(), S, st))) return rc;
   }
   if ((rc = upload(n->d_delay, dl.data(), S, st))) return rc;
   if (n->d_scale && (rc = upload(n->d_scale, sc.data(), S, st))) return rc;
@@ -1208,6 +1264,7 @@ int spk_get_info(const spk_network* n, spk_info* info) {
   info->n_monitors = (int)n->mons.size();
   info->partitions = n->P;
   info->fused = n->fused_ok ? 1 : 0;
+  info->record_bytes = (n->compact && n->rules.empty()) ? 2 : 8;
   return SPK_OK;
 }
 
@@ -1534,6 +1591,7 @@ int spk_set_weights(spk_network* n, const float* in) {
   for (int64_t s = 0; s < n->S; ++s) std::memcpy(&rec[n->perm[s]].y, &in[s], sizeof(float));
   if (n->S) SPK_HIP(hipMemcpyAsync(n->d_syn, rec.data(), n->S * sizeof(uint2), hipMemcpyHostToDevice, n->stream));
   SPK_HIP(hipStreamSynchronize(n->stream));
+  n->compact = false;   // caller weights: deliveries read the full records from now on
   return SPK_OK;
 }
 
@@ -1658,6 +1716,7 @@ int spk_create_brunel(const spk_brunel_params* p, int partitions, spk_network** 
   ALLOC(n->d_decay, 1); ALLOC(n->d_erev, 1); ALLOC(n->d_in_thr, 1);
   ALLOC(n->d_ext_cdf, std::max<size_t>(1, n->ext_cdf.size()));
   ALLOC(n->d_syn, (size_t)std::max<int64_t>(1, S));
+  ALLOC(n->d_syn16, (size_t)std::max<int64_t>(1, S));
   ALLOC(n->d_stats, 4);
   ALLOC(n->d_nlo, (size_t)P + 1); ALLOC(n->d_ilo, (size_t)P + 1);
   ALLOC(n->d_seg, (size_t)N * (P + 1));
@@ -1690,7 +1749,11 @@ int spk_create_brunel(const spk_brunel_params* p, int partitions, spk_network** 
   spk::BrunelGenArgs ga{};
   ga.seed = n->seed; ga.ne = p->n_exc; ga.ni = p->n_inh; ga.ce = p->ce; ga.ci = p->ci; ga.P = P;
   ga.owner = d_owner; ga.nlo = n->d_nlo; ga.count = d_count; ga.off = d_off; ga.cursor = d_cursor;
-  ga.syn = n->d_syn; ga.delay_bits = (uint32_t)G << 20; ga.we = p->weight_exc; ga.wi = p->weight_inh;
+  ga.syn = n->d_syn; ga.syn16 = n->d_syn16;
+  ga.delay_bits = (uint32_t)G << 20; ga.we = p->weight_exc; ga.wi = p->weight_inh;
+  // every excitatory source weighs weight_exc, every inhibitory one weight_inh, one delay
+  n->compact = S > 0;
+  n->cw_lo = p->weight_exc; n->cw_hi = p->weight_inh; n->csplit = p->n_exc; n->cdelay = G;
   const int grid = (int)std::min<int64_t>(8192, std::max<int64_t>(1, (S + 255) / 256));
   if (S > 0) {
     hipLaunchKernelGGL(spk::k_gen_count, dim3(grid), dim3(256), 0, st, ga, S);
