@@ -28,14 +28,15 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
-def algorithmic_bytes(n_neurons, steps, spikes, rec_events, record_bytes=8):
-    """Algorithmic HBM bytes of the hot path:
+def algorithmic_bytes(n_neurons, steps, spikes, rec_events, b_t=4, b_w=4):
+    """Algorithmic HBM bytes of the hot path, per unit of work:
     per neuron per step 24 B (V r/w 8, refractory state r/w 8, ring slot read+clear 8);
     per spike 16 B (list write 4, list read 4, row pointers 8);
-    per recurrent delivery the synapse record: 8 B (4 B target/delay/label + 4 B FP32
-    weight), or 2 B in the compact form (16-bit target; weight and delay per source type).
-    External Poisson events are generated in registers (0 B)."""
-    return n_neurons * steps * 24 + spikes * 16 + rec_events * record_bytes
+    per recurrent delivery b_t = 4 B target + b_w = 4 B FP32 weight.
+    External Poisson events are generated in registers (0 B).  These are the algorithm's
+    bytes at FP32, whatever record format the kernel reads: a kernel that reads 2-byte
+    compact records moves fewer bytes than this (see the measured traffic)."""
+    return n_neurons * steps * 24 + spikes * 16 + rec_events * (b_t + b_w)
 
 
 def measured_traffic(form, path):
@@ -244,8 +245,7 @@ def main():
     value = events / (tot_ms / 1e3)
     rate_hz = spikes / desc.n_neurons / (steps_sim * desc.dt)
     # roofline of the dominant kernel (the fused group kernel)
-    bytes_alg = algorithmic_bytes(desc.n_neurons, steps_sim, spikes, rec,
-                                  info.record_bytes if args.path == "fused" else 8)
+    bytes_alg = algorithmic_bytes(desc.n_neurons, steps_sim, spikes, rec)
     if launches > 0 and kern_ms > 0:
         avg_launch_ms = kern_ms / launches
         per_launch_bytes = bytes_alg / launches
@@ -286,6 +286,9 @@ def main():
         "cpu_baseline": cpu,
         "rate_hz": round(rate_hz, 3),
         "events_per_step": events // max(1, args.steps),
+        "recurrent_events_per_step": rec // max(1, args.steps),
+        "spikes_per_step": spikes // max(1, args.steps),
+        "algorithmic_bytes_per_step": int(bytes_alg) // max(1, args.steps),
         "wall_s": round(wall, 3),
     }
     if cpu:
