@@ -193,9 +193,10 @@ int spk_set_partitions(spk_network* net, int partitions);
 /* Phase profile of the persistent kernel (no reference counterpart; the reference has
  * only a clock() timer around run(), Examples/Brunel10K.cpp:255-257).  When on, spk_run
  * accumulates workgroup 0's wall-clock time per phase: 0 Poisson inputs and the LIF
- * input read, 1 LIF recurrence and spike append, 2 publication, 3 barrier arrival and the
- * next group's random draws, 4 barrier wait, 5 plasticity, 6 delivery heads, 7 count scan
- * and entry staging, 8 synapse delivery; 10 and 11 are the wall-clock start and end of the
+ * input read, 9 the LIF recurrence, 1 spike history, statistics and spike append,
+ * 2 publication, 3 barrier arrival and the next group's random draws, 4 barrier wait,
+ * 5 plasticity, 6 delivery heads, 7 count scan and entry staging, 8 synapse delivery;
+ * 10 and 11 are the wall-clock start and end of the
  * last launch's group loop (workgroup 0; in a batch, the network's first workgroup).
  * Values in microseconds summed over the last run (up to 12 entries). */
 int spk_set_profiling(spk_network* net, int on);

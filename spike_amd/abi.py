@@ -286,7 +286,7 @@ class Network:
         out = (C.c_double * 12)()
         check(self.lib.spk_get_phase_profile(self.h, out, 12))
         names = ["inputs_lifinput", "lif", "publish", "arrive_draw", "wait", "plasticity",
-                 "dlv_heads", "dlv_scan_stage", "dlv_synapses", "s9", "s10", "s11"]
+                 "dlv_heads", "dlv_scan_stage", "dlv_synapses", "lif_recurrence", "s10", "s11"]
         return dict(zip(names, list(out)))
 
     def fused_status(self) -> str:

@@ -26,7 +26,8 @@
 
 namespace spk {
 
-constexpr int kFusedBlock = 1024;
+constexpr int kFusedBlock = 1024;      // threads per workgroup
+constexpr int kMaxPartitions = 1024;    // workgroups of one network
 constexpr uint32_t kIdMask = 0x00FFFFFFu;
 constexpr int kBarGroups = 8;          // level-1 barrier counters (blockIdx % 8)
 constexpr int kBarStride = 32;         // uints between counters (128 B)
@@ -476,6 +477,34 @@ __device__ __forceinline__ void fused_plasticity(const FusedRule* rules, int n_r
   }
 }
 
+// G sub-steps of one neuron's LIF recurrence (LIFSpikingNeurons.cu:115-190) with voltage
+// input `in[g]`; returns the spike mask.  The same expressions, in the same order, as the
+// generic loop in fused_body.
+template <int GG>
+__device__ __forceinline__ uint32_t lif_block(const float* in, float t, float dt, float tref, float half_dt,
+                                              float cv0, float omc, float cib, float thr, float vr, float& V,
+                                              float& last) {
+  float dvs[GG];
+#pragma unroll
+  for (int x = 0; x < GG; ++x) dvs[x] = in[x];
+  // keep (float)x * dt computed here: hoisted out of the group loop, the GG products would
+  // be spilled to scratch and reloaded one per sub-step (an empty asm, no instruction)
+  float dtl = dt;
+  asm volatile("" : "+v"(dtl));
+  uint32_t mk = 0u;
+#pragma unroll
+  for (int x = 0; x < GG; ++x) {
+    const float tg = t + (float)x * dtl;
+    const bool ready = ((tg - last) - tref) > half_dt;
+    const float vn = ((cv0 + omc * V) + cib) + dvs[x];
+    const bool fire = ready && (vn >= thr);
+    V = ready ? (fire ? vr : vn) : V;
+    last = fire ? tg : last;
+    mk |= (fire ? 1u : 0u) << x;
+  }
+  return mk;
+}
+
 // Partition of a workgroup.  Workgroups are dispatched to the XCDs round-robin, so block
 // b runs on XCD b % 8; mapping the blocks of one XCD to consecutive partitions keeps that
 // XCD's synapse segments contiguous inside every CSR row (its L2 then fetches only its own
@@ -658,7 +687,22 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const int p, cons
     if (tid < nown_waves) {
       unsigned long long smask = 0ull;
       if (own) {
-        for (int g0 = 0; g0 < G; g0 += kPrefetch) {
+        if (KIND == 0 && G <= kPrefetch) {
+          // G known at compile time: straight-line code, LDS reads at immediate offsets
+          const float* in = sinp + tid * G;
+          uint32_t mk = 0u;
+          switch (G) {
+#define SPK_LIF_CASE(GG) \
+            case GG: mk = lif_block<GG>(in, t, a.dt, a.tref, half_dt, cv0, omc, cib, thr, vr, V, last); break;
+            SPK_LIF_CASE(1) SPK_LIF_CASE(2) SPK_LIF_CASE(3) SPK_LIF_CASE(4) SPK_LIF_CASE(5) SPK_LIF_CASE(6)
+            SPK_LIF_CASE(7) SPK_LIF_CASE(8) SPK_LIF_CASE(9) SPK_LIF_CASE(10) SPK_LIF_CASE(11) SPK_LIF_CASE(12)
+            SPK_LIF_CASE(13) SPK_LIF_CASE(14) SPK_LIF_CASE(15) SPK_LIF_CASE(16)
+#undef SPK_LIF_CASE
+            default: break;
+          }
+          smask = mk;
+        }
+        for (int g0 = 0; g0 < G && !(KIND == 0 && G <= kPrefetch); g0 += kPrefetch) {
           const int gn = min(kPrefetch, G - g0);
           if (KIND == 0) {
             float dvs[kPrefetch];
@@ -705,22 +749,23 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const int p, cons
             }
           }
         }
-        // spike history: clear this group's G bits, set those of the spikes
-        int cur_w = -1;
-        uint32_t word = 0;
-        for (int g = 0; g < G; ++g) {
-          int b = b0 + g;
-          if (b >= bits) b -= bits;
-          const int wd = b >> 5;
-          if (wd != cur_w) {
-            if (cur_w >= 0) shist[cur_w * nown + tid] = word;
-            word = shist[wd * nown + tid];
-            cur_w = wd;
+        clk.lap(a, 9);
+        // spike history: clear this group's G bits [b0, b0 + G) of the bit ring and set
+        // those of the spikes, one 32-bit word (or the part up to the ring's end) at a time
+        {
+          int pos = b0, done = 0;
+          while (done < G) {
+            const int off = pos & 31;
+            const int n = min(min(32 - off, G - done), bits - pos);
+            const uint32_t ones = n == 32 ? 0xFFFFFFFFu : ((1u << n) - 1u);
+            const uint32_t sp = (uint32_t)(smask >> done) & ones;
+            uint32_t* wp = shist + (pos >> 5) * nown + tid;
+            *wp = (*wp & ~(ones << off)) | (sp << off);
+            done += n;
+            pos += n;
+            if (pos >= bits) pos -= bits;
           }
-          word &= ~(1u << (b & 31));
-          if ((smask >> g) & 1ull) word |= (1u << (b & 31));
         }
-        if (cur_w >= 0) shist[cur_w * nown + tid] = word;
         if (a.st.on && smask) {
           // window statistics of this neuron's spikes, in time order
           unsigned int c0 = sst_cnt[tid], c1 = sst_nisi[tid];

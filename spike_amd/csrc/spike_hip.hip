@@ -1046,7 +1046,7 @@ int spk_prepare(spk_network* n, int G) {
   if (n->L > 15) no_fused("more than 15 labels");
   if ((int64_t)U >= (int64_t)spk::kIdMask) no_fused("more than 2^24 neurons");
   if (S >= ((int64_t)1 << 32)) no_fused("2^32 or more synapses");
-  if (n->P > spk::kFusedBlock) no_fused("too many partitions");
+  if (n->P > spk::kMaxPartitions) no_fused("too many partitions");
   if ((int)n->mons.size() > spk::kMaxFusedMons) no_fused("more than 4 spike monitors");
   n->BL = std::max(n->dmax, G);
   n->fused_lds = spk::fused_lds_bytes(n->BL, n->L, n->W, G, n->nown_max, n->iown_max, n->xcap, n->P);
@@ -1232,7 +1232,7 @@ int spk_prepare(spk_network* n, int G) {
 int spk_set_partitions(spk_network* n, int partitions) {
   if (int rc = check_net(n)) return rc;
   if (n->prepared) return fail(SPK_ERR_STATE, "network already prepared");
-  if (partitions < 0 || partitions > spk::kFusedBlock) return fail(SPK_ERR_ARG, "bad partition count");
+  if (partitions < 0 || partitions > spk::kMaxPartitions) return fail(SPK_ERR_ARG, "bad partition count");
   n->req_partitions = partitions;
   return SPK_OK;
 }
@@ -1699,7 +1699,7 @@ int spk_create_brunel(const spk_brunel_params* p, int partitions, spk_network** 
   n->BL = std::max(n->dmax, G);
   n->fused_lds = spk::fused_lds_bytes(n->BL, 1, n->W, G, n->nown_max, 0, n->xcap, P);
   if (n->nown_max > spk::kFusedBlock) return bail(fail(SPK_ERR_ARG, "more neurons per partition than threads per workgroup"));
-  if (P > spk::kFusedBlock) return bail(fail(SPK_ERR_ARG, "too many partitions"));
+  if (P > spk::kMaxPartitions) return bail(fail(SPK_ERR_ARG, "too many partitions"));
   if (n->fused_lds > (size_t)prop.sharedMemPerBlock)
     return bail(fail(SPK_ERR_ARG, "input ring slice does not fit in LDS: use more partitions"));
   n->fused_ok = true;
