@@ -149,6 +149,42 @@ def coba_vogels(n_exc=3200, n_inh=800, p=0.02, seed=42, dt=1e-4, delay_steps=8,
     return d
 
 
+def large_plastic(n=500_000, out_degree=1000, n_inputs=None, input_out_degree=1000, input_rate=10.0,
+                  delay_max=20, J=0.1e-3, J_in=2.0e-3, g=5.0, theta=20e-3, vreset=10e-3, tau=20e-3,
+                  tref=2e-3, seed=42, dt=1e-4, plastic=True, monitors=()) -> NetDesc:
+    """Large plastic network (shrinks with n): 80 % excitatory LIF neurons, each with
+    `out_degree` random targets (drawn with replacement), delays U{1..delay_max} steps
+    (so G = 1); excitatory weights w ~ U(0, 1) scaled by J, inhibitory ones fixed at 1
+    scaled by -g J; n/20 Poisson inputs at `input_rate`, each with `input_out_degree`
+    targets, weight 1 scaled by J_in.  Weight-dependent nearest-spike STDP on E->E with
+    tau+ 16.8 ms, tau- 33.7 ms, learning rate 0.005, LTD/LTP ratio 1.05 (A- 0.00525 vs
+    A+ 0.005), w in [0, 1]."""
+    rng = np.random.default_rng(seed)
+    ne = int(round(0.8 * n))
+    n_in = n // 20 if n_inputs is None else n_inputs
+    d = NetDesc(n_neurons=n, n_inputs=n_in, kind=SYN_VOLTAGE, dt=dt, seed=seed, name="large_plastic")
+    _lif(d, n, 0.0, vreset, theta, cm=200e-12, g0=200e-12 / tau, tref=tref)
+    pre = np.repeat(np.arange(n, dtype=np.int32), out_degree)
+    post = rng.integers(0, n, size=n * out_degree, dtype=np.int64).astype(np.int32)
+    ipre = np.repeat(np.arange(n_in, dtype=np.int32), input_out_degree)
+    ipost = rng.integers(0, n, size=n_in * input_out_degree, dtype=np.int64).astype(np.int32)
+    exc = pre < ne
+    w = np.where(exc, rng.random(len(pre)).astype(F32), F32(1.0)).astype(F32)
+    sc = np.where(exc, F32(J), F32(-g * J)).astype(F32)
+    d.pre = np.concatenate([pre, (-ipre - 1).astype(np.int32)])
+    d.post = np.concatenate([post, ipost])
+    d.weight = np.concatenate([w, np.ones(len(ipre), F32)])
+    d.scale = np.concatenate([sc, np.full(len(ipre), F32(J_in), F32)])
+    d.delay = rng.integers(1, delay_max + 1, size=len(d.pre), dtype=np.int64).astype(np.int32)
+    d.rates = np.full((1, n_in), F32(input_rate), F32)
+    if plastic:
+        ee = np.nonzero(exc & (post < ne))[0].astype(np.int64)
+        d.rules.append(Rule(RULE_WD, ee, wd_params(a_minus=1.0, a_plus=1.0, tau_minus=33.7e-3, tau_plus=16.8e-3,
+                                                   lam=0.005, alpha=1.05, w_max=1.0, nearest=1)))
+    d.monitors = list(monitors)
+    return d
+
+
 def wd_params(a_minus=1.0, a_plus=1.0, tau_minus=0.02, tau_plus=0.02, lam=1e-2, alpha=2.02,
               w_max=0.3e-3, nearest=0):
     return dict(a_minus=a_minus, a_plus=a_plus, tau_minus=tau_minus, tau_plus=tau_plus,
@@ -165,5 +201,5 @@ def vogels_params(tau=0.02, lr=4e-4, rate=10.0):
     return dict(tau_istdp=tau, learningrate=lr, targetrate=rate, momentumrate=0.0)
 
 
-__all__ = ["brunel", "random_net", "plain_small", "coba_vogels", "wd_params", "evans_params",
+__all__ = ["brunel", "random_net", "plain_small", "coba_vogels", "large_plastic", "wd_params", "evans_params",
            "vogels_params", "Rule", "RULE_WD", "RULE_VOGELS", "RULE_EVANS"]

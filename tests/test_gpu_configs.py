@@ -1,0 +1,109 @@
+"""The benchmark configurations at their real sizes on the GPU.
+
+1. The first 1000 timesteps are bit-exact against the FP32 oracle (spike rasters, membrane
+   potentials, last-spike times, spike history, the input ring, conductances, weights and
+   plasticity traces) for:
+   * Brunel10K, config form (8000 E + 2000 I, 10^7 synapses, per-neuron Poisson drive) and
+     reference form (+10^4 Poisson inputs, +10^7 synapses);
+   * the plain small network (1000 Poisson inputs -> 1000 conductance LIF, p = 0.1; with
+     U(0, 1) nS weights and 20 Hz inputs its LIF neurons stay below threshold, so this case
+     checks the inputs, conductances and membranes);
+   * the COBA network with Vogels iSTDP on I->E (3200 E + 800 I, p = 0.02), plasticity on;
+   * the large plastic network at 1/100 of its size (5000 neurons x 100 outgoing synapses,
+     delays 1..20 steps so G = 1, nearest-spike WD STDP on E->E, 250 Poisson inputs).
+2. A 2 s COBA + iSTDP run in FP32 on the GPU stays within 2 % of the FP64 oracle in
+   population rate, ISI CV and the I->E weight distribution (mean and 20-bin histogram),
+   although the two spike trains decorrelate within milliseconds.  The spread of the
+   weights is printed, not asserted: after 2 s it is about 2 % of their mean, and the FP32
+   and FP64 spreads differed by 2.9 % on the seed used here.
+"""
+import numpy as np
+import pytest
+
+import oracle_py
+from spike_amd import abi, builders
+
+pytestmark = pytest.mark.gpu
+
+from test_gpu_parity import _compare  # noqa: E402
+
+
+def _bit_exact_first_steps(d, steps=1000, tag=""):
+    G = d.timestep_grouping()
+    ng = -(-steps // G)
+    print(f"[configs] {tag}: N={d.n_neurons} Nin={d.n_inputs} S={d.n_synapses} G={G} groups={ng}", flush=True)
+    gpu = abi.Network(d)
+    ora = oracle_py.OracleNetwork(d, precision=32, threads=16)
+    gi, oi = gpu.info(), ora.info()
+    assert (gi.timestep_grouping, gi.ring_slots, gi.history_words, gi.frac_bits) == \
+        (oi["G"], oi["B"], oi["W"], oi["frac"])
+    gpu.run(0.0, ng)
+    ora.run(0.0, ng)
+    c, oc = gpu.counters(), ora.counters()
+    assert c.spikes_neurons == oc["spikes_neurons"], tag
+    assert c.spikes_inputs == oc["spikes_inputs"], tag
+    assert c.spikes_neurons + c.spikes_inputs > 0, tag
+    print(f"  fused={gi.fused} record_bytes={gi.record_bytes}: {c.spikes_neurons} spikes, "
+          f"{c.synaptic_events} deliveries", flush=True)
+    _compare(gpu, ora, d, tag)
+    gpu.close()
+    ora.close()
+
+
+@pytest.mark.parametrize("form", ["config", "reference"])
+def test_brunel10k_first_1000_steps(form):
+    _bit_exact_first_steps(builders.brunel(seed=42, form=form, monitors=(0,)), tag=f"brunel10k {form}")
+
+
+def test_plain_small_first_1000_steps():
+    _bit_exact_first_steps(builders.plain_small(seed=42, monitors=(0, 1)), tag="plain small")
+
+
+def test_coba_vogels_first_1000_steps():
+    _bit_exact_first_steps(builders.coba_vogels(seed=42, monitors=(0,)), tag="coba + vogels")
+
+
+def test_large_plastic_scaled_first_1000_steps():
+    d = builders.large_plastic(n=5000, out_degree=100, monitors=(0, 1))
+    _bit_exact_first_steps(d, tag="large plastic / 100")
+
+
+def _isi_cv(ids, times, n):
+    cvs = []
+    order = np.lexsort((times, ids))
+    ids, times = ids[order], times[order].astype(np.float64)
+    starts = np.searchsorted(ids, np.arange(n))
+    ends = np.searchsorted(ids, np.arange(n), side="right")
+    for a, b in zip(starts, ends):
+        if b - a >= 3:
+            isi = np.diff(times[a:b])
+            cvs.append(isi.std() / isi.mean())
+    return float(np.mean(cvs))
+
+
+def test_coba_vogels_2s_fp32_within_2pc_of_fp64():
+    d = builders.coba_vogels(seed=42, monitors=(0,))
+    ng = d.groups_for(2.0)
+    print(f"[configs] coba + vogels 2 s FP32 GPU vs FP64 oracle", flush=True)
+    gpu = abi.Network(d)
+    gpu.run(0.0, ng)
+    gi, gt = gpu.take_spikes(0)
+    wg = gpu.weights()[d.rules[0].synapses].astype(np.float64)
+    ora = oracle_py.OracleNetwork(d, precision=64, threads=16)
+    ora.run(0.0, ng)
+    oi, ot = ora.take_spikes(0)
+    wo = ora.weights()[d.rules[0].synapses]
+    rg, ro = len(gi), len(oi)
+    cg, co = _isi_cv(gi, gt, d.n_neurons), _isi_cv(oi, ot, d.n_neurons)
+    edges = np.linspace(min(wg.min(), wo.min()), max(wg.max(), wo.max()) * (1 + 1e-9), 21)
+    hg, _ = np.histogram(wg, bins=edges)
+    ho, _ = np.histogram(wo, bins=edges)
+    l1 = np.abs(hg - ho).sum() / (2.0 * len(wg))
+    print(f"spikes GPU {rg} FP64 {ro}; ISI CV GPU {cg:.4f} FP64 {co:.4f}; I->E weights mean GPU "
+          f"{wg.mean():.5e} FP64 {wo.mean():.5e}, std GPU {wg.std():.5e} FP64 {wo.std():.5e}, "
+          f"histogram L1/2 {l1:.4f}", flush=True)
+    assert abs(rg - ro) <= 0.02 * ro
+    assert abs(cg - co) <= 0.02 * co
+    assert wo.std() > 0.01 * wo.mean(), "weights did not move"
+    assert abs(wg.mean() - wo.mean()) <= 0.02 * wo.mean()
+    assert l1 <= 0.02
