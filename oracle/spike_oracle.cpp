@@ -24,7 +24,9 @@
 // Decisions shared with the product (they define the semantics both must meet):
 //   * FP32 oracle: the input ring holds exact int64 fixed-point sums with `frac_bits`
 //     fraction bits (the reference's float atomicAdd order is undefined).  A delivery adds
-//     llrint(ldexp(w*scale, frac_bits)); the neuron reads (float)sum * 2^-frac_bits.
+//     llrint(ldexp(w*scale, frac_bits)); the neuron reads (float)(double)sum * 2^-frac_bits
+//     (the same as (float)sum while |sum| <= 2^53, which the choice of frac_bits ensures;
+//     the device converts the same way because it is cheaper there).
 //     FP64 oracle: the ring holds plain double sums.
 //   * Poisson inputs: the reference's next-spike-time sampler (exponential waiting time,
 //     one spike per step at most, first processed step only arms the neuron) is the
@@ -229,7 +231,7 @@ struct Net {
     else return (Acc)v;
   }
   static R from_acc(Acc a, int k) {
-    if constexpr (kFixed) return (float)a * std::ldexp(1.0f, -k);
+    if constexpr (kFixed) return (float)(double)a * std::ldexp(1.0f, -k);
     else return (R)a;
   }
 
@@ -347,7 +349,7 @@ struct Net {
               Acc a = ring[slot * NL + (size_t)i * L + l];
               if (l == 0) a += ext;
               ring[slot * NL + (size_t)i * L + l] = (Acc)0;
-              if constexpr (kFixed) total += (float)a * inv;
+              if constexpr (kFixed) total += (float)(double)a * inv;
               else total += (R)a;
             }
             dV = total;
@@ -359,7 +361,7 @@ struct Net {
               if (l == 0) a += ext;
               ring[slot * NL + (size_t)i * L + l] = (Acc)0;
               R in;
-              if constexpr (kFixed) in = (float)a * inv; else in = (R)a;
+              if constexpr (kFixed) in = (float)(double)a * inv; else in = (R)a;
               R gl = (R)decay_g[l] * gcond[(size_t)i * L + l] + in;
               total += gl * ((R)Erev[l] - Vi);
               gcond[(size_t)i * L + l] = gl;

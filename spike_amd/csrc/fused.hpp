@@ -669,7 +669,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const int p, cons
           int64_t q = rs[l];
           if (l == 0) q += ext_q;
           rs[l] = 0;
-          total += (float)q * a.inv_scale;
+          total += fixed_to_float(q) * a.inv_scale;
         }
         sinp[k] = total;
       } else {
@@ -677,7 +677,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const int p, cons
           int64_t q = rs[l];
           if (l == 0) q += ext_q;
           rs[l] = 0;
-          sinp[(size_t)k * L + l] = (float)q * a.inv_scale;
+          sinp[(size_t)k * L + l] = fixed_to_float(q) * a.inv_scale;
         }
       }
     }

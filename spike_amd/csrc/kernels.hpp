@@ -20,6 +20,11 @@ constexpr int kWStride = 2;
 constexpr int kMaxLabels = 8;
 constexpr int kMaxExtTable = 64;
 
+// An input-ring sum (|q| <= 2^52 by the choice of fraction bits) as float: through double,
+// which holds q exactly, so the one rounding is the same as that of (float)q and the
+// conversion is a few instructions instead of a 64-bit integer-to-float sequence.
+__host__ __device__ __forceinline__ float fixed_to_float(int64_t q) { return (float)(double)q; }
+
 // ---------------------------------------------------------------------------------
 // spike history: bit (step mod 32W) of neuron i lives in word hist[(b/32)*npop + i]
 
@@ -193,7 +198,7 @@ __global__ __launch_bounds__(256) void k_lif(LifArgs a) {
         int64_t q = active ? rs[l] : 0;
         if (l == 0) q += ext;
         if (active) rs[l] = 0;
-        total += (float)q * a.inv_scale;
+        total += fixed_to_float(q) * a.inv_scale;
       }
       dV = total;
     } else {
@@ -204,7 +209,7 @@ __global__ __launch_bounds__(256) void k_lif(LifArgs a) {
           int64_t q = active ? rs[l] : 0;
           if (l == 0) q += ext;
           if (active) rs[l] = 0;
-          const float in = (float)q * a.inv_scale;
+          const float in = fixed_to_float(q) * a.inv_scale;
           gl[l] = a.decay_g[l] * gl[l] + in;
           total += gl[l] * (a.erev[l] - V);
         }
