@@ -49,6 +49,9 @@ enum { SPK_SYN_VOLTAGE = 0, SPK_SYN_CONDUCTANCE = 1 };
 /* populations */
 enum { SPK_POP_NEURONS = 0, SPK_POP_INPUTS = 1 };
 
+/* storage of synaptic weights and plasticity traces (spk_set_weight_format) */
+enum { SPK_WEIGHTS_FP32 = 0, SPK_WEIGHTS_BF16 = 1, SPK_WEIGHTS_FP16 = 2 };
+
 /* run-time switches for spk_run (bit flags) */
 enum {
   SPK_RUN_PLASTICITY = 1,   /* apply plasticity rules (SpikingModel::run plasticity_on) */
@@ -149,6 +152,16 @@ int spk_set_seed(spk_network* net, uint64_t seed);
  * external population nu_ext, generated in registers instead of through synapses).
  * Each event adds `weight` to the neuron's input for that timestep. rate_hz=0 disables. */
 int spk_set_external_drive(spk_network* net, float rate_hz, float weight);
+/* Storage of synaptic weights and plasticity traces, before spk_prepare.  The reference
+ * keeps FP32 weights (synaptic_efficacies_or_weights, Spike/Synapses/Synapses.hpp) and FP32
+ * traces; SPK_WEIGHTS_BF16 / _FP16 store the weights as bfloat16 / binary16 words (rounded
+ * to nearest even), so a delivery reads 4 B of target record + 2 B of weight, and the
+ * traces as binary16 in both modes (bfloat16's spacing is near a trace's 0.5 % decay per
+ * 0.1 ms step, which would bias its decay).  Plastic weight updates are
+ * computed in FP32 from the stored weight and added with Kahan compensation (a second
+ * 16-bit word per synapse holds what the first cannot).  Inputs, conductances and
+ * membranes stay FP32; spk_get_weights/spk_get_traces return the stored values. */
+int spk_set_weight_format(spk_network* net, int format);
 
 /* Conductance synapse labels.  Replaces ConductanceSpikingSynapses::prepare
  * (CUDA ConductanceSpikingSynapses.cu:19-87): decay_g[l] = expf(-dt/tau_g[l]). */

@@ -34,6 +34,9 @@ SIG = {
     "or_select_stimulus": (C.c_int, [P, C.c_int]),
     "or_set_seed": (C.c_int, [P, C.c_uint64]),
     "or_set_external_drive": (C.c_int, [P, C.c_float, C.c_float]),
+    "or_set_weight_format": (C.c_int, [P, C.c_int]),
+    "or_round16_bits": (C.c_uint16, [C.c_float, C.c_int]),
+    "or_decode16": (C.c_float, [C.c_uint16, C.c_int]),
     "or_set_labels": (C.c_int, [P, fp, fp]),
     "or_set_synapses": (C.c_int, [P, C.c_int64, ip, ip, fp, fp, ip, ip]),
     "or_add_wd_stdp": (C.c_int, [P, lp, C.c_int64, fp, C.c_int]),
@@ -139,6 +142,8 @@ class OracleNetwork:
             rates = f32(d.rates if d.rates is not None else np.zeros((1, d.n_inputs))).reshape(-1, d.n_inputs)
             check(lib.or_set_poisson_rates(h, _p(rates, C.c_float), rates.shape[0]))
         check(lib.or_set_seed(h, int(d.seed)))
+        if getattr(d, "weight_format", 0):
+            check(lib.or_set_weight_format(h, int(d.weight_format)))
         if d.ext_rate:
             check(lib.or_set_external_drive(h, float(d.ext_rate), float(d.ext_weight)))
         if d.decay_g is not None:

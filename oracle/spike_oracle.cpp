@@ -155,6 +155,51 @@ int choose_frac_bits(double max_entry) {
   return k;
 }
 
+// ---- 16-bit storage of weights and traces (format 1 bfloat16, 2 IEEE binary16), round to
+// nearest with ties to even.  bfloat16 by the usual bit rounding of the upper half of the
+// float; binary16 from the value itself (frexp/ldexp and nearbyint in the default
+// to-nearest mode), a different route from the device's integer conversion.
+static uint16_t o_to_bf16(float f) {
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  if ((u & 0x7FFFFFFFu) > 0x7F800000u) return (uint16_t)((u >> 16) | 0x40u);
+  return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+static float o_from_bf16(uint16_t h) {
+  uint32_t u = (uint32_t)h << 16;
+  float f;
+  std::memcpy(&f, &u, 4);
+  return f;
+}
+static uint16_t o_to_half(float f) {
+  const uint16_t sign = std::signbit(f) ? 0x8000 : 0;
+  const double a = std::fabs((double)f);
+  if (std::isnan(f)) return sign | 0x7E00;
+  if (a >= 65520.0) return sign | 0x7C00;                      // rounds past 65504: inf
+  if (a < std::ldexp(1.0, -14)) {                               // subnormal: multiples of 2^-24
+    const double q = std::nearbyint(std::ldexp(a, 24));         // 1024 is the smallest normal
+    return (uint16_t)(sign | (uint16_t)q);
+  }
+  int ex;
+  std::frexp(a, &ex);                                           // a = m 2^ex, m in [0.5, 1)
+  int e = ex - 1;                                               // a in [2^e, 2^(e+1))
+  double q = std::nearbyint(std::ldexp(a, 10 - e));             // 1024 .. 2048
+  if (q >= 2048.0) { q = 1024.0; ++e; }
+  if (e > 15) return sign | 0x7C00;
+  return (uint16_t)(sign | (uint16_t)((e + 15) << 10) | (uint16_t)((int)q - 1024));
+}
+static float o_from_half(uint16_t h) {
+  const int e = (h >> 10) & 0x1F, m = h & 0x3FF;
+  double v;
+  if (e == 0) v = std::ldexp((double)m, -24);
+  else if (e == 31) v = m ? NAN : INFINITY;
+  else v = std::ldexp((double)(1024 + m), e - 25);
+  return (float)((h & 0x8000) ? -v : v);
+}
+static float o_round16(float x, int fmt) {
+  return fmt == 1 ? o_from_bf16(o_to_bf16(x)) : o_from_half(o_to_half(x));
+}
+
 struct Rule {
   int kind;  // 0 WD, 1 Vogels, 2 Evans
   std::vector<int64_t> syn;  // caller's synapse indices
@@ -191,6 +236,10 @@ struct Net {
   std::vector<int32_t> spre, spost, sdelay, slabel;
   std::vector<R> sw;
   std::vector<float> sscale;
+  // 16-bit weight storage of the FP32 oracle: sw holds the stored 16-bit values, sc the
+  // Kahan compensation of each synapse (also a 16-bit value)
+  int wfmt = 0;
+  std::vector<float> sc;
   // per-thread sub-CSR by target range: rows[t][u] -> list of synapse ids
   std::vector<int> trange;  // nthreads+1 boundaries over posts
   std::vector<std::vector<int64_t>> rp;   // [t][u] row pointer into rs[t]
@@ -235,6 +284,35 @@ struct Net {
     else return (R)a;
   }
 
+  // Store a plastic weight moved by a rule from w0 (stored) to nw (clamped to [lo, hi]):
+  // FP32 as it is; with 16-bit storage one Kahan step of the increment, clamped.
+  void wstore(int64_t s, R w0, R nw, float lo, float hi) {
+    if constexpr (kFixed) {
+      if (wfmt) {
+        const float c = sc[s];
+        const float d = nw - w0;
+        const float y = d - c;
+        const float t = w0 + y;
+        const float v1 = o_round16(t, wfmt);
+        if (v1 < lo) { sw[s] = o_round16(lo, wfmt); sc[s] = 0.0f; }
+        else if (v1 > hi) { sw[s] = o_round16(hi, wfmt); sc[s] = 0.0f; }
+        else { sw[s] = v1; sc[s] = o_round16((v1 - w0) - y, wfmt); }
+        return;
+      }
+    }
+    (void)lo; (void)hi; (void)w0;
+    sw[s] = nw;
+  }
+  // a trace as stored: with either 16-bit weight format the traces are binary16 (a
+  // bfloat16 trace would decay in uneven steps: its spacing is near the 0.5 % decay per
+  // step of a 20 ms trace at dt = 0.1 ms)
+  double tstore(R x) const {
+    if constexpr (kFixed) {
+      if (wfmt) return (double)o_round16(x, 2);
+    }
+    return (double)x;
+  }
+
   int prepare(int g) {
     if (!lif_set && N > 0) return fail("LIF parameters not set");
     if (g < 1) return fail("timestep grouping must be >= 1");
@@ -254,6 +332,13 @@ struct Net {
     std::vector<double> ent((size_t)N * L, 0.0);
     for (int64_t s = 0; s < S; ++s)
       ent[(size_t)spost[s] * L + slabel[s]] += cap[s] * std::fabs((double)sscale[s]);
+    // 16-bit storage (FP32 oracle only; the bound above uses the caller's weights)
+    if constexpr (kFixed) {
+      if (wfmt) {
+        for (int64_t s = 0; s < S; ++s) sw[s] = o_round16(sw[s], wfmt);
+        sc.assign(S, 0.0f);
+      }
+    }
     double mx = 0.0;
     for (double e : ent) mx = std::max(mx, e);
     ext_cdf = poisson_count_table((double)ext_rate * (double)dt);
@@ -493,8 +578,8 @@ struct Net {
         }
         R nw = w + upd;
         if (nw < (R)0) nw = (R)0;
-        sw[s] = nw;
-        r.ta[j] = (double)xpre; r.tb[j] = (double)xpost;
+        wstore(s, w, nw, 0.0f, INFINITY);
+        r.ta[j] = tstore(xpre); r.tb[j] = tstore(xpost);
       }
     } else if (r.kind == 1) {
       // InhibitorySTDPPlasticity.cu:238-321 (the target term is evaluated in double)
@@ -504,7 +589,8 @@ struct Net {
       for (int64_t j = 0; j < P; ++j) {
         int64_t s = r.syn[j];
         R xpre = (R)r.ta[j], xpost = (R)r.tb[j];
-        R w = sw[s];
+        const R w0 = sw[s];
+        R w = w0;
         for (int g = 0; g < G; ++g) {
           xpre *= (R)decay; xpost *= (R)decay;
           int64_t step = (int64_t)tidx + g;
@@ -522,8 +608,8 @@ struct Net {
           w += upd;
           if (w < (R)0) w = (R)0;
         }
-        sw[s] = w;
-        r.ta[j] = (double)xpre; r.tb[j] = (double)xpost;
+        wstore(s, w0, w, 0.0f, INFINITY);
+        r.ta[j] = tstore(xpre); r.tb[j] = tstore(xpost);
       }
     } else {
       // EvansSTDPPlasticity.cu:59-143
@@ -534,7 +620,8 @@ struct Net {
       for (int64_t j = 0; j < P; ++j) {
         int64_t s = r.syn[j];
         R C = (R)r.ta[j], D = (R)r.tb[j];
-        R w = sw[s];
+        const R w0 = sw[s];
+        R w = w0;
         for (int g = 0; g < G; ++g) {
           C = (R)dC * C;
           D = (R)dD * D;
@@ -551,8 +638,8 @@ struct Net {
           if (w < (R)0) w = (R)0;
           if (w > (R)1) w = (R)1;
         }
-        sw[s] = w;
-        r.ta[j] = (double)C; r.tb[j] = (double)D;
+        wstore(s, w0, w, 0.0f, 1.0f);
+        r.ta[j] = tstore(C); r.tb[j] = tstore(D);
       }
     }
   }
@@ -659,6 +746,14 @@ int or_select_stimulus(void* p, int s) {
 int or_set_seed(void* p, uint64_t seed) {
   return dispatch((Handle*)p, [&](auto& n) { n.seed = seed; return 0; });
 }
+
+int or_set_weight_format(void* p, int fmt) {
+  if (fmt < 0 || fmt > 2) return fail("weight format must be 0, 1 or 2");
+  return dispatch((Handle*)p, [&](auto& n) { n.wfmt = fmt; return 0; });
+}
+
+uint16_t or_round16_bits(float x, int fmt) { return fmt == 1 ? o_to_bf16(x) : o_to_half(x); }
+float or_decode16(uint16_t h, int fmt) { return fmt == 1 ? o_from_bf16(h) : o_from_half(h); }
 
 int or_set_external_drive(void* p, float rate, float w) {
   return dispatch((Handle*)p, [&](auto& n) { n.ext_rate = rate; n.ext_w = w; return 0; });
@@ -804,6 +899,12 @@ int or_set_weights(void* p, const double* w) {
   return dispatch((Handle*)p, [&](auto& n) {
     using RT = typename std::remove_reference<decltype(n.sw[0])>::type;
     for (size_t s = 0; s < n.sw.size(); ++s) n.sw[s] = (RT)w[s];
+    if constexpr (std::is_same<RT, float>::value) {
+      if (n.wfmt && n.prepared) {
+        for (size_t s = 0; s < n.sw.size(); ++s) n.sw[s] = o_round16((float)w[s], n.wfmt);
+        n.sc.assign(n.sw.size(), 0.0f);
+      }
+    }
     return 0;
   });
 }

@@ -143,6 +143,7 @@ SIGNATURES = {
     "spk_get_history": (C.c_int, [P, C.c_int, up]),
     "spk_get_traces": (C.c_int, [P, C.c_int, fp, fp]),
     # include/spike_mlp.h
+    "spk_set_weight_format": (C.c_int, [P, C.c_int]),
     "spk_mlp_param_count": (C.c_int, []),
     "spk_mlp_create": (C.c_int, [C.c_int, C.c_int, C.POINTER(P)]),
     "spk_mlp_destroy": (C.c_int, [P]),
@@ -223,6 +224,8 @@ class Network:
             rates = rates.reshape(-1, d.n_inputs)
             check(lib.spk_set_poisson_rates(h, _ptr(rates, C.c_float), rates.shape[0]))
         check(lib.spk_set_seed(h, int(d.seed)))
+        if d.weight_format:
+            check(lib.spk_set_weight_format(h, int(d.weight_format)))
         if d.ext_rate:
             check(lib.spk_set_external_drive(h, float(d.ext_rate), float(d.ext_weight)))
         if d.decay_g is not None:

@@ -79,6 +79,8 @@ struct FusedArgs {
   // synapses
   const uint32_t* seg;      // [U][P+1] absolute segment offsets
   const uint2* syn;         // per synapse {local target | delay << 20 | label << 28, weight bits}
+  const uint16_t* w16;      // 16-bit weights (wfmt 1 bfloat16, 2 binary16) instead of the records'
+  int wfmt;
   const float* scale;       // nullptr: uniform scale0
   // compact form (no plasticity, one label, uniform scale): every synapse of a source
   // u < src_split has weight w_lo, every other one w_hi, all have delay src_delay, and the
@@ -313,10 +315,16 @@ __device__ __forceinline__ void draw_group(const FusedArgs& a, int s, uint8_t* e
 
 // One quad's share of kSpikesPerQuad spikes: every lane loads its synapses of each spike's
 // segment (lane q takes s0+q, s0+q+4, ...) with all loads issued before the LDS atomics.
+// With 16-bit weights only the packed word of a record is read, plus the 2-byte weight.
+__device__ __forceinline__ float syn_weight(const FusedArgs& a, const uint32_t* synw, uint32_t k) {
+  return a.wfmt ? dec16(a.w16[k], a.wfmt) : __uint_as_float(synw[(size_t)k * 2 + 1]);
+}
+
 template <bool PER_SYN_SCALE>
 __device__ __forceinline__ void deliver_quad(const FusedArgs& a, const uint32_t* ents, int n, int q,
                                              int p, int slot0, int B, int nownL, int L, int64_t* sring,
                                              unsigned long long& st_del) {
+  const uint32_t* synw = (const uint32_t*)a.syn;   // word 2k: packed, word 2k+1: FP32 weight
   uint32_t s0[kSpikesPerQuad], s1[kSpikesPerQuad];
   int gs[kSpikesPerQuad];
 #pragma unroll
@@ -336,18 +344,18 @@ __device__ __forceinline__ void deliver_quad(const FusedArgs& a, const uint32_t*
   for (int r = 0; r < kSpikesPerQuad; ++r)
     if (q == 0) st_del += (unsigned long long)(s1[r] - s0[r]);
   // first two synapses per lane of every segment, loads in flight together
-  uint2 rec[kSpikesPerQuad][2];
-  float scl[kSpikesPerQuad][2];
+  uint32_t pkh[kSpikesPerQuad][2];
+  float wvh[kSpikesPerQuad][2];
 #pragma unroll
   for (int r = 0; r < kSpikesPerQuad; ++r) {
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       const uint32_t k = s0[r] + (uint32_t)(q + h * kQuad);
-      rec[r][h] = make_uint2(0u, 0u);
-      scl[r][h] = a.scale0;
+      pkh[r][h] = 0u;
+      wvh[r][h] = 0.0f;
       if (k < s1[r]) {
-        rec[r][h] = a.syn[k];
-        if (PER_SYN_SCALE) scl[r][h] = a.scale[k];
+        pkh[r][h] = synw[(size_t)k * 2];
+        wvh[r][h] = syn_weight(a, synw, k) * (PER_SYN_SCALE ? a.scale[k] : a.scale0);
       }
     }
   }
@@ -357,9 +365,8 @@ __device__ __forceinline__ void deliver_quad(const FusedArgs& a, const uint32_t*
     for (int h = 0; h < 2; ++h) {
       const uint32_t k = s0[r] + (uint32_t)(q + h * kQuad);
       if (k < s1[r]) {
-        const uint32_t pk = rec[r][h].x;
-        const float wv = __uint_as_float(rec[r][h].y) * scl[r][h];
-        const long long qv = __float2ll_rn(wv * a.two_k);
+        const uint32_t pk = pkh[r][h];
+        const long long qv = __float2ll_rn(wvh[r][h] * a.two_k);
         int slot = gs[r] + (int)((pk >> 20) & 0xFFu);
         while (slot >= B) slot -= B;
         atomicAdd((unsigned long long*)&sring[(size_t)slot * nownL + (size_t)(pk & 0xFFFFFu) * L + (pk >> 28)],
@@ -369,27 +376,26 @@ __device__ __forceinline__ void deliver_quad(const FusedArgs& a, const uint32_t*
     // longer segments: the rest, four synapses per lane per step with the loads in flight
     // together
     for (uint32_t k0 = s0[r] + (uint32_t)(q + 2 * kQuad); k0 < s1[r]; k0 += 4 * kQuad) {
-      uint2 rr[4];
-      float sc[4];
+      uint32_t pk4[4];
+      float wv4[4];
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         const uint32_t k = k0 + (uint32_t)(u * kQuad);
-        rr[u] = make_uint2(0u, 0u);
-        sc[u] = a.scale0;
+        pk4[u] = 0u;
+        wv4[u] = 0.0f;
         if (k < s1[r]) {
-          rr[u] = a.syn[k];
-          if (PER_SYN_SCALE) sc[u] = a.scale[k];
+          pk4[u] = synw[(size_t)k * 2];
+          wv4[u] = syn_weight(a, synw, k) * (PER_SYN_SCALE ? a.scale[k] : a.scale0);
         }
       }
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         const uint32_t k = k0 + (uint32_t)(u * kQuad);
         if (k < s1[r]) {
-          const float wv = __uint_as_float(rr[u].y) * sc[u];
-          const long long qv = __float2ll_rn(wv * a.two_k);
-          int slot = gs[r] + (int)((rr[u].x >> 20) & 0xFFu);
+          const long long qv = __float2ll_rn(wv4[u] * a.two_k);
+          int slot = gs[r] + (int)((pk4[u] >> 20) & 0xFFu);
           while (slot >= B) slot -= B;
-          atomicAdd((unsigned long long*)&sring[(size_t)slot * nownL + (size_t)(rr[u].x & 0xFFFFFu) * L + (rr[u].x >> 28)],
+          atomicAdd((unsigned long long*)&sring[(size_t)slot * nownL + (size_t)(pk4[u] & 0xFFFFFu) * L + (pk4[u] >> 28)],
                     (unsigned long long)qv);
         }
       }

@@ -26,6 +26,130 @@ constexpr int kMaxExtTable = 64;
 __host__ __device__ __forceinline__ float fixed_to_float(int64_t q) { return (float)(double)q; }
 
 // ---------------------------------------------------------------------------------
+// 16-bit weight and trace storage (format 1: bfloat16, format 2: IEEE binary16), both
+// rounded to nearest, ties to even, by integer operations (no dependence on the device's
+// conversion modes); decoding is exact.
+
+__host__ __device__ __forceinline__ uint32_t f32_bits(float f) {
+  uint32_t u;
+  __builtin_memcpy(&u, &f, 4);
+  return u;
+}
+__host__ __device__ __forceinline__ float bits_f32(uint32_t u) {
+  float f;
+  __builtin_memcpy(&f, &u, 4);
+  return f;
+}
+
+__host__ __device__ __forceinline__ uint16_t f32_to_bf16(float f) {
+  const uint32_t u = f32_bits(f);
+  if ((u & 0x7FFFFFFFu) > 0x7F800000u) return (uint16_t)((u >> 16) | 0x40u);  // NaN stays NaN
+  return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+
+__host__ __device__ __forceinline__ uint16_t f32_to_f16(float f) {
+  const uint32_t x = f32_bits(f);
+  const uint32_t sign = (x >> 16) & 0x8000u;
+  const uint32_t ax = x & 0x7FFFFFFFu;
+  if (ax >= 0x7F800000u) return (uint16_t)(sign | (ax > 0x7F800000u ? 0x7E00u : 0x7C00u));
+  if (ax >= 0x477FF000u) return (uint16_t)(sign | 0x7C00u);   // rounds to 65520 or more: inf
+  if (ax <= 0x33000000u) return (uint16_t)sign;                // at most 2^-25: zero
+  const int e = (int)(ax >> 23) - 127;
+  const uint32_t m = (ax & 0x7FFFFFu) | 0x800000u;
+  uint32_t q, rem, half;
+  if (e < -14) {                                               // binary16 subnormal
+    const int sh = -(e + 1);                                   // 14..24
+    q = m >> sh;
+    rem = m & ((1u << sh) - 1u);
+    half = 1u << (sh - 1);
+  } else {
+    q = ((uint32_t)(e + 15) << 10) | ((m >> 13) & 0x3FFu);
+    rem = m & 0x1FFFu;
+    half = 0x1000u;
+  }
+  if (rem > half || (rem == half && (q & 1u))) ++q;            // a carry may raise the exponent
+  return (uint16_t)(sign | q);
+}
+
+__host__ __device__ __forceinline__ float f16_to_f32(uint16_t h) {
+  const uint32_t sign = ((uint32_t)h & 0x8000u) << 16;
+  const uint32_t e = ((uint32_t)h >> 10) & 0x1Fu, m = (uint32_t)h & 0x3FFu;
+  if (e == 0u) return bits_f32(f32_bits((float)m * 5.9604644775390625e-8f) | sign);   // m 2^-24
+  if (e == 31u) return bits_f32(sign | 0x7F800000u | (m << 13));
+  return bits_f32(sign | ((e + 112u) << 23) | (m << 13));
+}
+
+__host__ __device__ __forceinline__ uint16_t enc16(float f, int fmt) {
+  return fmt == 1 ? f32_to_bf16(f) : f32_to_f16(f);
+}
+__host__ __device__ __forceinline__ float dec16(uint16_t h, int fmt) {
+  return fmt == 1 ? bits_f32((uint32_t)h << 16) : f16_to_f32(h);
+}
+
+// Synaptic weights: FP32 in the synapse records (w[k * kWStride]), or, with fmt != 0,
+// 16-bit words w16[k] plus a 16-bit Kahan compensation c16[k] for plastic updates.
+struct Weights {
+  float* w;
+  uint16_t* w16;
+  uint16_t* c16;
+  int fmt;
+};
+
+__device__ __forceinline__ float wload(const Weights& W, int64_t k) {
+  return W.fmt ? dec16(W.w16[k], W.fmt) : W.w[(size_t)k * kWStride];
+}
+
+// Store a plastic weight that a rule moved from w0 (the stored value) to nw (already
+// clamped by the rule to [lo, hi]).  FP32: nw as it is.  16-bit: one step of Kahan
+// summation of the increment, carrying what the 16-bit word cannot hold in c16; a value
+// that the compensation would push outside [lo, hi] is clamped and the compensation
+// cleared.
+__device__ __forceinline__ void wupdate(const Weights& W, int64_t k, float w0, float nw, float lo, float hi) {
+  if (!W.fmt) {
+    W.w[(size_t)k * kWStride] = nw;
+    return;
+  }
+  const float c = dec16(W.c16[k], W.fmt);
+  const float d = nw - w0;
+  const float y = d - c;
+  const float t = w0 + y;
+  uint16_t s1 = enc16(t, W.fmt);
+  const float v1 = dec16(s1, W.fmt);
+  uint16_t c1;
+  if (v1 < lo) {
+    s1 = enc16(lo, W.fmt);
+    c1 = 0;
+  } else if (v1 > hi) {
+    s1 = enc16(hi, W.fmt);
+    c1 = 0;
+  } else {
+    c1 = enc16((v1 - w0) - y, W.fmt);
+  }
+  W.w16[k] = s1;
+  W.c16[k] = c1;
+}
+
+// Plasticity traces per plastic synapse: FP32, or 16-bit words of the weight format.
+struct Traces {
+  float* a;
+  float* b;
+  uint16_t* a16;
+  uint16_t* b16;
+  int fmt;
+};
+__device__ __forceinline__ float tload(const Traces& T, int which, int64_t j) {
+  if (!T.fmt) return which ? T.b[j] : T.a[j];
+  return dec16(which ? T.b16[j] : T.a16[j], T.fmt);
+}
+__device__ __forceinline__ void tstore(const Traces& T, int which, int64_t j, float v) {
+  if (!T.fmt) {
+    (which ? T.b : T.a)[j] = v;
+    return;
+  }
+  (which ? T.b16 : T.a16)[j] = enc16(v, T.fmt);
+}
+
+// ---------------------------------------------------------------------------------
 // spike history: bit (step mod 32W) of neuron i lives in word hist[(b/32)*npop + i]
 
 __device__ __forceinline__ int hist_bitpos(int64_t step, int W) {
@@ -318,7 +442,7 @@ struct PropArgs {
   const int* counts;
   const int64_t* rowptr;
   const int32_t* tgt;
-  const float* w;
+  Weights wts;
   const float* scale;       // nullptr: uniform scale0
   const uint16_t* delay;
   const uint8_t* label;     // nullptr: single label
@@ -347,7 +471,7 @@ __global__ __launch_bounds__(256) void k_propagate(PropArgs a) {
         const int post = a.tgt[s];
         const int d = a.delay[s];
         const float sc = a.scale ? a.scale[s] : a.scale0;
-        const float wv = a.w[(size_t)s * kWStride] * sc;
+        const float wv = wload(a.wts, s) * sc;
         const long long q = __float2ll_rn(wv * a.two_k);
         const int slot = (base_slot + d + g) % a.B;
         const int l = a.label ? (int)a.label[s] : 0;
@@ -367,9 +491,8 @@ struct StdpArgs {
   const int32_t* pre;      // unified presynaptic id (inputs at N + i)
   const int32_t* tgt;
   const uint16_t* delay;
-  float* w;
-  float* ta;
-  float* tb;
+  Weights wts;
+  Traces tr;
   const float* last_n;
   const float* last_in;
   const uint32_t* hist_n;
@@ -393,12 +516,12 @@ __device__ __forceinline__ void stdp_wd_one(const StdpArgs& a, int64_t j) {
   const float* iEm = Ep + (a.G + 1);
   const float* iEp = iEm + (a.G + 1);
   const int G = a.G;
-  float xpre = a.ta[j], xpost = a.tb[j];
+  float xpre = tload(a.tr, 0, j), xpost = tload(a.tr, 1, j);
   const float lpre = (u < a.N) ? a.last_n[u] : a.last_in[u - a.N];
   int pg = (int)roundf((lpre - a.t) / a.dt);
   const int qg = (int)roundf((a.last_n[post] - a.t) / a.dt);
   if (pg >= G) pg *= -1;
-  const float w = a.w[(size_t)s * kWStride];
+  const float w = wload(a.wts, s);
   xpost *= Em[G];
   xpre *= Ep[G];
   if (a.nearest) {
@@ -423,9 +546,9 @@ __device__ __forceinline__ void stdp_wd_one(const StdpArgs& a, int64_t j) {
   }
   float nw = w + upd;
   if (nw < 0.0f) nw = 0.0f;
-  a.w[(size_t)s * kWStride] = nw;
-  a.ta[j] = xpre;
-  a.tb[j] = xpost;
+  wupdate(a.wts, s, w, nw, 0.0f, __builtin_inff());
+  tstore(a.tr, 0, j, xpre);
+  tstore(a.tr, 1, j, xpost);
 }
 
 __global__ __launch_bounds__(256) void k_stdp_wd(StdpArgs a) {
@@ -445,8 +568,9 @@ __device__ __forceinline__ void stdp_vogels_one(const StdpArgs& a, int64_t j) {
   const int post = a.tgt[s];
   const int d = a.delay[s];
   const float decay = a.tab[0], lr = a.tab[1];
-  float xpre = a.ta[j], xpost = a.tb[j];
-  float w = a.w[(size_t)s * kWStride];
+  float xpre = tload(a.tr, 0, j), xpost = tload(a.tr, 1, j);
+  const float w0 = wload(a.wts, s);
+  float w = w0;
   for (int g = 0; g < a.G; ++g) {
     xpre *= decay;
     xpost *= decay;
@@ -464,9 +588,9 @@ __device__ __forceinline__ void stdp_vogels_one(const StdpArgs& a, int64_t j) {
     w += upd;
     if (w < 0.0f) w = 0.0f;
   }
-  a.w[(size_t)s * kWStride] = w;
-  a.ta[j] = xpre;
-  a.tb[j] = xpost;
+  wupdate(a.wts, s, w0, w, 0.0f, __builtin_inff());
+  tstore(a.tr, 0, j, xpre);
+  tstore(a.tr, 1, j, xpost);
 }
 
 __global__ __launch_bounds__(256) void k_stdp_vogels(StdpArgs a) {
@@ -481,8 +605,9 @@ __device__ __forceinline__ void stdp_evans_one(const StdpArgs& a, int64_t j) {
   const int post = a.tgt[s];
   const int d = a.delay[s];
   const float dC = a.tab[0], dD = a.tab[1], aC = a.tab[2], aD = a.tab[3];
-  float C = a.ta[j], D = a.tb[j];
-  float w = a.w[(size_t)s * kWStride];
+  float C = tload(a.tr, 0, j), D = tload(a.tr, 1, j);
+  const float w0 = wload(a.wts, s);
+  float w = w0;
   for (int g = 0; g < a.G; ++g) {
     C = dC * C;
     D = dD * D;
@@ -499,9 +624,9 @@ __device__ __forceinline__ void stdp_evans_one(const StdpArgs& a, int64_t j) {
     if (w < 0.0f) w = 0.0f;
     if (w > 1.0f) w = 1.0f;
   }
-  a.w[(size_t)s * kWStride] = w;
-  a.ta[j] = C;
-  a.tb[j] = D;
+  wupdate(a.wts, s, w0, w, 0.0f, 1.0f);
+  tstore(a.tr, 0, j, C);
+  tstore(a.tr, 1, j, D);
 }
 
 __global__ __launch_bounds__(256) void k_stdp_evans(StdpArgs a) {

@@ -105,6 +105,8 @@ struct RuleHost {
   int32_t* d_pre = nullptr;
   float* d_ta = nullptr;
   float* d_tb = nullptr;
+  uint16_t* d_ta16 = nullptr;   // traces in the network's 16-bit weight format
+  uint16_t* d_tb16 = nullptr;
   float* d_tab = nullptr;
   int tab_G = -1;
   double dterm = 0.0;
@@ -174,6 +176,10 @@ struct spk_network {
   // targets; sources below csplit weigh cw_lo, the others cw_hi; one delay cdelay
   uint16_t* d_syn16 = nullptr;
   bool compact = false;
+  // 16-bit weights (spk_set_weight_format): words and Kahan compensations in device order
+  int wfmt = SPK_WEIGHTS_FP32;
+  uint16_t* d_w16 = nullptr;
+  uint16_t* d_c16 = nullptr;
   float cw_lo = 0.0f, cw_hi = 0.0f;
   int csplit = 0, cdelay = 0;
   float* d_scale = nullptr;
@@ -232,6 +238,25 @@ namespace {
 // Weight k of the synapse records is at float index 2k+1.
 float* weights_view(spk_network* n) { return reinterpret_cast<float*>(n->d_syn) + 1; }
 
+spk::Weights weights_of(spk_network* n) {
+  spk::Weights w{};
+  w.w = weights_view(n);
+  w.w16 = n->d_w16;
+  w.c16 = n->d_c16;
+  w.fmt = n->wfmt;
+  return w;
+}
+
+spk::Traces traces_of(spk_network* n, RuleHost& r) {
+  spk::Traces t{};
+  // traces are binary16 in both 16-bit modes: a trace decays by about 0.5 % per 0.1 ms
+  // step, near the spacing of bfloat16 values, which would make bfloat16 traces decay in
+  // uneven (biased) steps; binary16's 11-bit significand keeps that error to 0.05 %
+  t.a = r.d_ta; t.b = r.d_tb; t.a16 = r.d_ta16; t.b16 = r.d_tb16;
+  t.fmt = n->wfmt ? SPK_WEIGHTS_FP16 : SPK_WEIGHTS_FP32;
+  return t;
+}
+
 int check_net(const spk_network* n) {
   if (!n) return fail(SPK_ERR_ARG, "null network");
   return SPK_OK;
@@ -256,10 +281,12 @@ void free_device(spk_network* n) {
   dfree(n->d_rp); dfree(n->d_last_n); dfree(n->d_last_in); dfree(n->d_hist_n);
   dfree(n->d_hist_in); dfree(n->d_ring); dfree(n->d_g); dfree(n->d_decay); dfree(n->d_erev);
   dfree(n->d_in_thr); dfree(n->d_ext_cdf); dfree(n->d_rowptr); dfree(n->d_tgt); dfree(n->d_syn); dfree(n->d_syn16);
+  dfree(n->d_w16); dfree(n->d_c16);
   dfree(n->d_scale); dfree(n->d_delay); dfree(n->d_label); dfree(n->d_lists);
   dfree(n->d_counts); dfree(n->d_slot_time); dfree(n->d_slot_tidx); dfree(n->d_stats);
   for (auto& r : n->rules) {
     dfree(r.d_pos); dfree(r.d_pre); dfree(r.d_ta); dfree(r.d_tb); dfree(r.d_tab); dfree(r.d_part);
+    dfree(r.d_ta16); dfree(r.d_tb16);
   }
   dfree(n->d_nlo); dfree(n->d_ilo); dfree(n->d_seg); dfree(n->d_xlist);
   dfree(n->d_bar); dfree(n->d_err); dfree(n->d_prof); dfree(n->d_rules);
@@ -404,8 +431,8 @@ int launch_rule(spk_network* n, int rid, float t) {
   int rc = refresh_rule_tables(n, r);
   if (rc) return rc;
   spk::StdpArgs a{};
-  a.pos = r.d_pos; a.pre = r.d_pre; a.tgt = n->d_tgt; a.delay = n->d_delay; a.w = weights_view(n);
-  a.ta = r.d_ta; a.tb = r.d_tb; a.last_n = n->d_last_n; a.last_in = n->d_last_in;
+  a.pos = r.d_pos; a.pre = r.d_pre; a.tgt = n->d_tgt; a.delay = n->d_delay; a.wts = weights_of(n);
+  a.tr = traces_of(n, r); a.last_n = n->d_last_n; a.last_in = n->d_last_in;
   a.hist_n = n->d_hist_n; a.hist_in = n->d_hist_in; a.tab = r.d_tab; a.dterm = r.dterm;
   a.P = (int64_t)r.syn.size(); a.N = n->N; a.Nin = n->Nin; a.G = n->G; a.W = n->W;
   a.tidx = (int)roundf(t / n->dt); a.nearest = r.wd.nearest_spike_only; a.t = t; a.dt = n->dt;
@@ -421,7 +448,7 @@ int launch_propagate(spk_network* n, float t, int slot) {
   if (n->S == 0) return SPK_OK;
   spk::PropArgs a{};
   a.lists = slot_lists(n, slot); a.counts = slot_counts(n, slot);
-  a.rowptr = n->d_rowptr; a.tgt = n->d_tgt; a.w = weights_view(n);
+  a.rowptr = n->d_rowptr; a.tgt = n->d_tgt; a.wts = weights_of(n);
   a.scale = n->uniform_scale ? nullptr : n->d_scale;
   a.delay = n->d_delay; a.label = (n->L > 1) ? n->d_label : nullptr;
   a.ring = n->d_ring; a.events = n->d_stats;
@@ -538,6 +565,7 @@ int fill_fused_args(spk_network* n, bool plastic, spk::FusedArgs& a) {
   a.last_in = n->d_last_in; a.hist_in = n->d_hist_in;
   a.in_thr = n->d_in_thr + (size_t)n->stim * n->Nin;
   a.seg = n->d_seg; a.syn = n->d_syn;
+  a.w16 = n->d_w16; a.wfmt = n->wfmt;
   a.scale = n->uniform_scale ? nullptr : n->d_scale;
   if (n->compact && n->rules.empty()) {
     a.syn16 = n->d_syn16;
@@ -550,8 +578,8 @@ int fill_fused_args(spk_network* n, bool plastic, spk::FusedArgs& a) {
       RuleHost& rh = n->rules[r];
       spk::StdpArgs& ra = fr[r].args;
       ra = spk::StdpArgs{};
-      ra.pos = rh.d_pos; ra.pre = rh.d_pre; ra.tgt = n->d_tgt; ra.delay = n->d_delay; ra.w = weights_view(n);
-      ra.ta = rh.d_ta; ra.tb = rh.d_tb; ra.last_n = n->d_last_n; ra.last_in = n->d_last_in;
+      ra.pos = rh.d_pos; ra.pre = rh.d_pre; ra.tgt = n->d_tgt; ra.delay = n->d_delay; ra.wts = weights_of(n);
+      ra.tr = traces_of(n, rh); ra.last_n = n->d_last_n; ra.last_in = n->d_last_in;
       ra.hist_n = n->d_hist_n; ra.hist_in = n->d_hist_in; ra.tab = rh.d_tab; ra.dterm = rh.dterm;
       ra.P = (int64_t)rh.syn.size(); ra.N = n->N; ra.Nin = n->Nin; ra.G = G; ra.W = n->W;
       ra.nearest = rh.wd.nearest_spike_only; ra.dt = n->dt;
@@ -834,6 +862,15 @@ int spk_set_seed(spk_network* n, uint64_t seed) {
   return SPK_OK;
 }
 
+int spk_set_weight_format(spk_network* n, int format) {
+  if (int rc = check_net(n)) return rc;
+  if (n->prepared) return fail(SPK_ERR_STATE, "network already prepared");
+  if (format != SPK_WEIGHTS_FP32 && format != SPK_WEIGHTS_BF16 && format != SPK_WEIGHTS_FP16)
+    return fail(SPK_ERR_ARG, "weight format must be SPK_WEIGHTS_FP32, _BF16 or _FP16");
+  n->wfmt = format;
+  return SPK_OK;
+}
+
 int spk_set_external_drive(spk_network* n, float rate_hz, float weight) {
   if (int rc = check_net(n)) return rc;
   if (n->prepared) return fail(SPK_ERR_STATE, "network already prepared");
@@ -1074,7 +1111,8 @@ int spk_prepare(spk_network* n, int G) {
     // one delay, one delay for all rows, and the row weights w_lo below a split source and
     // w_hi from it on (an excitatory/inhibitory network numbered E first)
     n->compact = false;
-    if (n->L == 1 && n->uniform_scale && n->rules.empty() && S > 0 && n->nown_max <= 65536) {
+    if (n->L == 1 && n->uniform_scale && n->rules.empty() && S > 0 && n->nown_max <= 65536 &&
+        n->wfmt == SPK_WEIGHTS_FP32) {
       bool ok = true, have_lo = false, have_hi = false;
       uint32_t blo = 0, bhi = 0;
       int split = U, dconst = -1;
@@ -1164,6 +1202,13 @@ int spk_prepare(spk_network* n, int G) {
     }
     if ((rc = upload(n->d_syn, rec.data(), S, st))) return rc;
   }
+  if (n->wfmt != SPK_WEIGHTS_FP32) {
+    // 16-bit weights of the records' weights, compensations zero
+    std::vector<uint16_t> w16(S), c16(S, 0);
+    for (int64_t k = 0; k < S; ++k) w16[k] = spk::enc16(w[k], n->wfmt);
+    if ((rc = dalloc(&n->d_w16, S)) || (rc = dalloc(&n->d_c16, S))) return rc;
+    if ((rc = upload(n->d_w16, w16.data(), S, st)) || (rc = upload(n->d_c16, c16.data(), S, st))) return rc;
+  }
   if (n->compact) {
     std::vector<uint16_t> t16(S);
     for (int64_t k = 0; k < S; ++k) t16[k] = (uint16_t)(packed[k] & 0xFFFFFu);
@@ -1196,6 +1241,10 @@ int spk_prepare(spk_network* n, int G) {
     if ((rc = dalloc(&r.d_pre, P))) return rc;
     if ((rc = dalloc(&r.d_ta, P))) return rc;
     if ((rc = dalloc(&r.d_tb, P))) return rc;
+    if (n->wfmt != SPK_WEIGHTS_FP32) {
+      if ((rc = dalloc(&r.d_ta16, P))) return rc;
+      if ((rc = dalloc(&r.d_tb16, P))) return rc;
+    }
     if ((rc = dalloc(&r.d_part, (size_t)n->P + 1))) return rc;
     std::vector<int64_t> pos(P);
     std::vector<int32_t> pre(P);
@@ -1298,6 +1347,10 @@ int spk_reset_state(spk_network* n) {
     if (P) {
       SPK_HIP(hipMemsetAsync(r.d_ta, 0, sizeof(float) * P, st));
       SPK_HIP(hipMemsetAsync(r.d_tb, 0, sizeof(float) * P, st));
+      if (r.d_ta16) {
+        SPK_HIP(hipMemsetAsync(r.d_ta16, 0, sizeof(uint16_t) * P, st));
+        SPK_HIP(hipMemsetAsync(r.d_tb16, 0, sizeof(uint16_t) * P, st));
+      }
     }
   }
   n->cur_tidx = -1;
@@ -1575,6 +1628,13 @@ int spk_monitor_take(spk_network* n, int mid, int32_t* ids, float* times, int64_
 int spk_get_weights(spk_network* n, float* out) {
   if (int rc = check_listed(n)) return rc;
   if (!out && n->S) return fail(SPK_ERR_ARG, "null output");
+  if (n->wfmt != SPK_WEIGHTS_FP32) {
+    std::vector<uint16_t> w16(n->S);
+    if (n->S) SPK_HIP(hipMemcpyAsync(w16.data(), n->d_w16, n->S * sizeof(uint16_t), hipMemcpyDeviceToHost, n->stream));
+    SPK_HIP(hipStreamSynchronize(n->stream));
+    for (int64_t s = 0; s < n->S; ++s) out[s] = spk::dec16(w16[n->perm[s]], n->wfmt);
+    return SPK_OK;
+  }
   std::vector<uint2> rec(n->S);
   if (n->S) SPK_HIP(hipMemcpyAsync(rec.data(), n->d_syn, n->S * sizeof(uint2), hipMemcpyDeviceToHost, n->stream));
   SPK_HIP(hipStreamSynchronize(n->stream));
@@ -1590,6 +1650,13 @@ int spk_set_weights(spk_network* n, const float* in) {
   SPK_HIP(hipStreamSynchronize(n->stream));
   for (int64_t s = 0; s < n->S; ++s) std::memcpy(&rec[n->perm[s]].y, &in[s], sizeof(float));
   if (n->S) SPK_HIP(hipMemcpyAsync(n->d_syn, rec.data(), n->S * sizeof(uint2), hipMemcpyHostToDevice, n->stream));
+  if (n->wfmt != SPK_WEIGHTS_FP32 && n->S) {
+    // the 16-bit words of the caller's weights; compensations start again from zero
+    std::vector<uint16_t> w16(n->S), c16(n->S, 0);
+    for (int64_t s = 0; s < n->S; ++s) w16[n->perm[s]] = spk::enc16(in[s], n->wfmt);
+    SPK_HIP(hipMemcpyAsync(n->d_w16, w16.data(), n->S * sizeof(uint16_t), hipMemcpyHostToDevice, n->stream));
+    SPK_HIP(hipMemcpyAsync(n->d_c16, c16.data(), n->S * sizeof(uint16_t), hipMemcpyHostToDevice, n->stream));
+  }
   SPK_HIP(hipStreamSynchronize(n->stream));
   n->compact = false;   // caller weights: deliveries read the full records from now on
   return SPK_OK;
@@ -1628,8 +1695,18 @@ int spk_get_traces(spk_network* n, int rid, float* a, float* b) {
   RuleHost& r = n->rules[rid];
   const size_t P = r.syn.size();
   std::vector<float> ta(P), tb(P);
-  if (int rc = download(n, ta.data(), r.d_ta, P)) return rc;
-  if (int rc = download(n, tb.data(), r.d_tb, P)) return rc;
+  if (r.d_ta16) {
+    std::vector<uint16_t> a16(P), b16(P);
+    if (int rc = download(n, a16.data(), r.d_ta16, P)) return rc;
+    if (int rc = download(n, b16.data(), r.d_tb16, P)) return rc;
+    for (size_t j = 0; j < P; ++j) {
+      ta[j] = spk::dec16(a16[j], SPK_WEIGHTS_FP16);
+      tb[j] = spk::dec16(b16[j], SPK_WEIGHTS_FP16);
+    }
+  } else {
+    if (int rc = download(n, ta.data(), r.d_ta, P)) return rc;
+    if (int rc = download(n, tb.data(), r.d_tb, P)) return rc;
+  }
   for (size_t j = 0; j < P; ++j) {
     a[r.order[j]] = ta[j];
     b[r.order[j]] = tb[j];

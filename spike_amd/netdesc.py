@@ -23,6 +23,7 @@ import numpy as np
 
 F32 = np.float32
 
+WEIGHTS_FP32, WEIGHTS_BF16, WEIGHTS_FP16 = 0, 1, 2
 SYN_VOLTAGE = 0
 SYN_CONDUCTANCE = 1
 POP_NEURONS = 0
@@ -74,6 +75,7 @@ class NetDesc:
     monitors: List[int] = dataclasses.field(default_factory=list)
     grouping: Optional[int] = None   # default: minimum delay (SpikingModel.cpp:98)
     name: str = "net"
+    weight_format: int = 0        # 0 FP32, 1 BF16, 2 FP16 storage of weights and traces
 
     @property
     def n_synapses(self) -> int:

@@ -8,7 +8,8 @@ import pytest
 
 import oracle_py
 from spike_amd import abi, builders
-from spike_amd.netdesc import SYN_CONDUCTANCE, Rule, RULE_WD, RULE_VOGELS, RULE_EVANS
+from spike_amd.netdesc import (SYN_CONDUCTANCE, Rule, RULE_WD, RULE_VOGELS, RULE_EVANS, WEIGHTS_BF16,
+                               WEIGHTS_FP16)
 
 pytestmark = pytest.mark.gpu
 
@@ -185,6 +186,28 @@ def test_plasticity(kind, fused):
     d = _plastic_net(kind, 30 + kind)
     gpu, ora = _run_both(d, 200, fused=fused, partitions=5)
     _compare(gpu, ora, d, f"rule {kind}")
+
+
+@pytest.mark.parametrize("fmt", [WEIGHTS_BF16, WEIGHTS_FP16])
+@pytest.mark.parametrize("kind", [RULE_WD, RULE_VOGELS, RULE_EVANS])
+@pytest.mark.parametrize("fused", [True, False])
+def test_plasticity_16bit_weights(kind, fused, fmt):
+    """16-bit weights (Kahan-compensated updates) and binary16 traces: bit-exact against the
+    FP32 oracle storing the same formats."""
+    d = _plastic_net(kind, 50 + kind)
+    d.weight_format = fmt
+    gpu, ora = _run_both(d, 200, fused=fused, partitions=5)
+    _compare(gpu, ora, d, f"rule {kind} weights format {fmt}")
+
+
+@pytest.mark.parametrize("fmt", [WEIGHTS_BF16, WEIGHTS_FP16])
+def test_conductance_16bit_weights(fmt):
+    d = builders.random_net(seed=17, kind=SYN_CONDUCTANCE, n_labels=2, whi=1.0,
+                            scale=lambda rng, S: rng.uniform(0.5e-9, 2e-9, S), input_rate=500.0,
+                            dmin=2, dmax=5)
+    d.weight_format = fmt
+    gpu, ora = _run_both(d, 250)
+    _compare(gpu, ora, d, f"conductance, weights format {fmt}")
 
 
 def test_wd_nearest_spike():
