@@ -11,7 +11,8 @@
    * the COBA network with Vogels iSTDP on I->E (3200 E + 800 I, p = 0.02), plasticity on;
    * the large plastic network at 1/100 of its size (5000 neurons x 100 outgoing synapses,
      delays 1..20 steps so G = 1, nearest-spike WD STDP on E->E, 250 Poisson inputs).
-2. A 2 s COBA + iSTDP run in FP32 on the GPU stays within 2 % of the FP64 oracle in
+2. A 2 s COBA + iSTDP run on the GPU, with FP32 or FP16 weight storage, stays within
+   2 % of the FP64 oracle (BF16: the stated tolerances of COBA_TOL) in
    population rate, ISI CV and the I->E weight distribution (mean and 20-bin histogram),
    although the two spike trains decorrelate within milliseconds.  The spread of the
    weights is printed, not asserted: after 2 s it is about 2 % of their mean, and the FP32
@@ -81,15 +82,28 @@ def _isi_cv(ids, times, n):
     return float(np.mean(cvs))
 
 
-def test_coba_vogels_2s_fp32_within_2pc_of_fp64():
+# Relative tolerances against FP64 per weight format: (rate, ISI CV, mean I->E weight,
+# 20-bin weight histogram L1/2).  FP32 and FP16 storage meet the 2 % criterion.  BF16 is
+# stated, not held to it: its 8-bit significand moves the fixed weights themselves (0.06 by
+# +0.1 %, 0.67 by -0.3 %), which shifts the dynamics (measured: rate -2.0 %, CV +2.7 %,
+# mean +0.2 %), and its spacing near 0.67 (0.0039) is a third of the weights' spread, so
+# the histogram of stored values is visibly quantised (measured L1/2 6.8 %).
+COBA_TOL = {0: (0.02, 0.02, 0.02, 0.02), 1: (0.03, 0.04, 0.005, 0.10), 2: (0.02, 0.02, 0.02, 0.02)}
+
+
+@pytest.mark.parametrize("fmt", [0, 1, 2])
+def test_coba_vogels_2s_within_2pc_of_fp64(fmt):
+    """Weights stored in FP32, BF16 (Kahan) or FP16 (Kahan), traces FP32 or binary16."""
     d = builders.coba_vogels(seed=42, monitors=(0,))
+    d.weight_format = fmt
     ng = d.groups_for(2.0)
-    print(f"[configs] coba + vogels 2 s FP32 GPU vs FP64 oracle", flush=True)
+    d64 = builders.coba_vogels(seed=42, monitors=(0,))
+    print(f"[configs] coba + vogels 2 s GPU (weights format {fmt}) vs FP64 oracle", flush=True)
     gpu = abi.Network(d)
     gpu.run(0.0, ng)
     gi, gt = gpu.take_spikes(0)
     wg = gpu.weights()[d.rules[0].synapses].astype(np.float64)
-    ora = oracle_py.OracleNetwork(d, precision=64, threads=16)
+    ora = oracle_py.OracleNetwork(d64, precision=64, threads=16)
     ora.run(0.0, ng)
     oi, ot = ora.take_spikes(0)
     wo = ora.weights()[d.rules[0].synapses]
@@ -102,8 +116,9 @@ def test_coba_vogels_2s_fp32_within_2pc_of_fp64():
     print(f"spikes GPU {rg} FP64 {ro}; ISI CV GPU {cg:.4f} FP64 {co:.4f}; I->E weights mean GPU "
           f"{wg.mean():.5e} FP64 {wo.mean():.5e}, std GPU {wg.std():.5e} FP64 {wo.std():.5e}, "
           f"histogram L1/2 {l1:.4f}", flush=True)
-    assert abs(rg - ro) <= 0.02 * ro
-    assert abs(cg - co) <= 0.02 * co
+    t_rate, t_cv, t_mean, t_hist = COBA_TOL[fmt]
+    assert abs(rg - ro) <= t_rate * ro
+    assert abs(cg - co) <= t_cv * co
     assert wo.std() > 0.01 * wo.mean(), "weights did not move"
-    assert abs(wg.mean() - wo.mean()) <= 0.02 * wo.mean()
-    assert l1 <= 0.02
+    assert abs(wg.mean() - wo.mean()) <= t_mean * wo.mean()
+    assert l1 <= t_hist
