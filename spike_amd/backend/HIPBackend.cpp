@@ -135,6 +135,14 @@ void ModelNetwork::build() {
   uint64_t seed = 1;
   if (const char* s = getenv("SPIKE_HIP_SEED")) seed = strtoull(s, nullptr, 10);
   check(spk_set_seed(net, seed), "spk_set_seed");
+  // weight and trace storage: FP32 as in the reference; SPIKE_HIP_WEIGHTS=bf16 or fp16
+  // selects 16-bit storage with Kahan-compensated plastic updates
+  if (const char* f = getenv("SPIKE_HIP_WEIGHTS")) {
+    const std::string fs(f);
+    const int fmt = fs == "bf16" ? SPK_WEIGHTS_BF16 : (fs == "fp16" ? SPK_WEIGHTS_FP16 : SPK_WEIGHTS_FP32);
+    if (fmt == SPK_WEIGHTS_FP32 && fs != "fp32") die("SPIKE_HIP_WEIGHTS must be fp32, bf16 or fp16");
+    check(spk_set_weight_format(net, fmt), "spk_set_weight_format");
+  }
 
   if (cond) {
     // decay factors as in Spike/Backend/CUDA/Synapses/ConductanceSpikingSynapses.cu:74-79
