@@ -628,6 +628,13 @@ int enqueue_step(spk_mlp* m, int batch, int mode, uint64_t seed, int apply, floa
   return SPK_OK;
 }
 
+// The captured training step holds the data pointers, the row count and the loss weights
+// as kernel arguments: any change to them drops it (it is captured again when needed).
+void drop_graph(spk_mlp* m) {
+  if (m->graph) (void)hipGraphExecDestroy(m->graph);
+  m->graph = nullptr;
+}
+
 int check_batch(spk_mlp* m, int batch) {
   if (batch <= 0 || batch % MT || batch > m->max_batch)
     return mfail(SPK_ERR_ARG, "batch must be a positive multiple of " + std::to_string(MT) + " up to max_batch " +
@@ -738,6 +745,8 @@ int spk_mlp_get_params(spk_mlp* m, float* params) {
 int spk_mlp_set_data(spk_mlp* m, const float* features, const float* targets, int64_t n) {
   if (!m || !features || !targets || n < 1 || n > ((int64_t)1 << 31) - 1)
     return mfail(SPK_ERR_ARG, "bad training set");
+  MLP_HIP(hipStreamSynchronize(m->stream));
+  drop_graph(m);
   if (n > m->cap) {
     MLP_HIP(hipStreamSynchronize(m->stream));
     free_d(m->x);
@@ -757,8 +766,10 @@ int spk_mlp_set_loss_weights(spk_mlp* m, const float* w3) {
   if (!m || !w3) return mfail(SPK_ERR_ARG, "null argument");
   for (int i = 0; i < 3; ++i) {
     if (!(w3[i] >= 0.0f) || !std::isfinite(w3[i])) return mfail(SPK_ERR_ARG, "loss weights must be finite and >= 0");
-    m->wout[i] = w3[i];
   }
+  MLP_HIP(hipStreamSynchronize(m->stream));
+  drop_graph(m);
+  for (int i = 0; i < 3; ++i) m->wout[i] = w3[i];
   return SPK_OK;
 }
 

@@ -146,3 +146,25 @@ def test_sampled_training_converges_and_graph_replay_is_identical():
     assert np.array_equal(pa, pb), "graph replay differs from launches"
     print("first", first, "final", la, flush=True)
     assert np.all(la < 0.2 * first)
+
+
+def test_graph_follows_new_training_data():
+    """A captured training step is dropped when the training set (or the loss weights)
+    changes, so graph replays after set_data train on the new data exactly as launches do."""
+    print("[mlp] graph after new data", flush=True)
+    x1, t1 = rows(71, 1024)
+    x2, t2 = rows(72, 6000)                       # larger: the data buffers are reallocated
+    params = mlp.init_params(73, 2)
+    out = []
+    for use_graph in (False, True):
+        m = mlp.Mlp(2, 256)
+        m.set_params(params)
+        m.set_data(x1, t1)
+        m.train(30, 256, 1e-3, 1e-4, 90, mode=mlp.BATCH_SAMPLED, seed=9, use_graph=use_graph, record_loss=False)
+        m.set_data(x2, t2)
+        m.train(30, 256, 1e-3, 1e-4, 90, mode=mlp.BATCH_SAMPLED, seed=9, use_graph=use_graph, record_loss=False)
+        m.set_loss_weights((1.0, 2.0, 0.5))
+        m.train(30, 256, 1e-3, 1e-4, 90, mode=mlp.BATCH_SAMPLED, seed=9, use_graph=use_graph, record_loss=False)
+        out.append(m.params())
+        m.close()
+    assert np.array_equal(out[0], out[1])
