@@ -1050,28 +1050,6 @@ int spk_prepare(spk_network* n, int G) {
     std::vector<int64_t> fill(rowptr.begin(), rowptr.end() - 1);
     for (int64_t s = 0; s < S; ++s) order[fill[uid(n->h_pre[s])]++] = s;
   }
-  for (int u = 0; u < U; ++u) {
-    auto b = order.begin() + rowptr[u], e = order.begin() + rowptr[u + 1];
-    std::stable_sort(b, e, [&](int64_t x, int64_t y) {
-      const int ox = owner[n->h_post[x]], oy = owner[n->h_post[y]];
-      if (ox != oy) return ox < oy;
-      return n->h_delay[x] < n->h_delay[y];
-    });
-  }
-  n->perm.assign(S, 0);
-  std::vector<int32_t> tgt(S);
-  std::vector<float> w(S), sc(S);
-  std::vector<uint16_t> dl(S);
-  std::vector<uint8_t> lb(S);
-  for (int64_t k = 0; k < S; ++k) {
-    const int64_t s = order[k];
-    n->perm[s] = k;
-    tgt[k] = n->h_post[s];
-    w[k] = n->h_w[s];
-    sc[k] = n->h_scale[s];
-    dl[k] = (uint16_t)n->h_delay[s];
-    lb[k] = (uint8_t)n->h_label[s];
-  }
   // ---- eligibility of the persistent kernel ----
   n->fused_ok = true;
   n->fused_why.clear();
@@ -1091,6 +1069,32 @@ int spk_prepare(spk_network* n, int G) {
     hipDeviceProp_t prop;
     SPK_HIP(hipGetDeviceProperties(&prop, n->device));
     if (n->fused_lds > (size_t)prop.sharedMemPerBlock) no_fused("input ring slice does not fit in LDS");
+  }
+  // the (partition, delay) order inside a row serves only the persistent kernel's
+  // segments; the per-phase kernels take a row in any order (the ring sums are exact)
+  if (n->fused_ok) {
+    for (int u = 0; u < U; ++u) {
+      auto b = order.begin() + rowptr[u], e = order.begin() + rowptr[u + 1];
+      std::stable_sort(b, e, [&](int64_t x, int64_t y) {
+        const int ox = owner[n->h_post[x]], oy = owner[n->h_post[y]];
+        if (ox != oy) return ox < oy;
+        return n->h_delay[x] < n->h_delay[y];
+      });
+    }
+  }
+  n->perm.assign(S, 0);
+  std::vector<int32_t> tgt(S);
+  std::vector<float> w(S), sc(S);
+  std::vector<uint16_t> dl(S);
+  std::vector<uint8_t> lb(S);
+  for (int64_t k = 0; k < S; ++k) {
+    const int64_t s = order[k];
+    n->perm[s] = k;
+    tgt[k] = n->h_post[s];
+    w[k] = n->h_w[s];
+    sc[k] = n->h_scale[s];
+    dl[k] = (uint16_t)n->h_delay[s];
+    lb[k] = (uint8_t)n->h_label[s];
   }
   std::vector<uint32_t> seg, packed;
   if (n->fused_ok) {
@@ -1229,14 +1233,25 @@ int spk_prepare(spk_network* n, int G) {
   // rules: synapses grouped by the partition of their postsynaptic neuron
   for (auto& r : n->rules) {
     const size_t P = r.syn.size();
-    r.order.resize(P);
-    for (size_t j = 0; j < P; ++j) r.order[j] = (int64_t)j;
-    std::stable_sort(r.order.begin(), r.order.end(), [&](int64_t x, int64_t y) {
-      return owner[n->h_post[r.syn[x]]] < owner[n->h_post[r.syn[y]]];
-    });
+    // persistent kernel: stable counting sort by the partition of the postsynaptic neuron
+    // (a workgroup updates the synapses onto its own neurons); per-phase kernels: CSR
+    // position order, so that the rule's reads and writes of the synapse arrays stream
     r.part.assign(n->P + 1, 0);
-    for (size_t j = 0; j < P; ++j) r.part[owner[n->h_post[r.syn[r.order[j]]]] + 1]++;
-    for (int q = 0; q < n->P; ++q) r.part[q + 1] += r.part[q];
+    r.order.resize(P);
+    if (n->fused_ok) {
+      for (size_t j = 0; j < P; ++j) r.part[owner[n->h_post[r.syn[j]]] + 1]++;
+      for (int q = 0; q < n->P; ++q) r.part[q + 1] += r.part[q];
+      std::vector<int64_t> fill(r.part.begin(), r.part.end() - 1);
+      for (size_t j = 0; j < P; ++j) r.order[fill[owner[n->h_post[r.syn[j]]]]++] = (int64_t)j;
+    } else {
+      for (size_t j = 0; j < P; ++j) r.order[j] = (int64_t)j;
+      bool sorted = true;
+      for (size_t j = 1; j < P && sorted; ++j) sorted = n->perm[r.syn[j - 1]] <= n->perm[r.syn[j]];
+      if (!sorted)
+        std::stable_sort(r.order.begin(), r.order.end(),
+                         [&](int64_t x, int64_t y) { return n->perm[r.syn[x]] < n->perm[r.syn[y]]; });
+      for (int q = 0; q < n->P; ++q) r.part[q + 1] = (int64_t)P;
+    }
     if ((rc = dalloc(&r.d_pos, P))) return rc;
     if ((rc = dalloc(&r.d_pre, P))) return rc;
     if ((rc = dalloc(&r.d_ta, P))) return rc;
