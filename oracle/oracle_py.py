@@ -236,6 +236,11 @@ class OracleNetwork:
         check(self.lib.or_get_weights(self.h, _p(w, C.c_double)))
         return w
 
+    def set_weights(self, w):
+        w = np.ascontiguousarray(w, np.float64)
+        assert w.shape == (self.desc.n_synapses,)
+        check(self.lib.or_set_weights(self.h, _p(w, C.c_double)))
+
     def conductances(self):
         g = np.empty(self.desc.n_neurons * self.desc.n_labels, np.float64)
         check(self.lib.or_get_conductances(self.h, _p(g, C.c_double)))

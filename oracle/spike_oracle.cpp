@@ -285,10 +285,11 @@ struct Net {
   }
 
   // Store a plastic weight moved by a rule from w0 (stored) to nw (clamped to [lo, hi]):
-  // FP32 as it is; with 16-bit storage one Kahan step of the increment, clamped.
+  // FP32 as it is; with 16-bit storage one Kahan step of a nonzero increment, clamped.
   void wstore(int64_t s, R w0, R nw, float lo, float hi) {
     if constexpr (kFixed) {
       if (wfmt) {
+        if (nw == w0) return;   // a zero increment leaves word and compensation as they are
         const float c = sc[s];
         const float d = nw - w0;
         const float y = d - c;
@@ -551,7 +552,8 @@ struct Net {
         int64_t s = r.syn[j];
         R xpre = (R)r.ta[j], xpost = (R)r.tb[j];
         int pg = (int)std::roundf((pre_last(spre[s]) - t) / dt);
-        int qg = (int)std::roundf((last_n[spost[s]] - t) / dt);
+        // qg is below G while time advances; bounded at G like the device's table index
+        int qg = std::min((int)std::roundf((last_n[spost[s]] - t) / dt), G);
         if (pg >= G) pg *= -1;
         R w = sw[s];
         xpost *= (R)Em[G];

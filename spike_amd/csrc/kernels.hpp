@@ -103,12 +103,14 @@ __device__ __forceinline__ float wload(const Weights& W, int64_t k) {
 // clamped by the rule to [lo, hi]).  FP32: nw as it is.  16-bit: one step of Kahan
 // summation of the increment, carrying what the 16-bit word cannot hold in c16; a value
 // that the compensation would push outside [lo, hi] is clamped and the compensation
-// cleared.
+// cleared.  A zero increment is no step at all (the weights a spike-driven rule skips
+// must be those the per-synapse form leaves).
 __device__ __forceinline__ void wupdate(const Weights& W, int64_t k, float w0, float nw, float lo, float hi) {
   if (!W.fmt) {
     W.w[(size_t)k * kWStride] = nw;
     return;
   }
+  if (nw == w0) return;   // a zero increment leaves the word and its compensation as they are
   const float c = dec16(W.c16[k], W.fmt);
   const float d = nw - w0;
   const float y = d - c;
@@ -506,31 +508,41 @@ struct StdpArgs {
 
 // WeightDependentSTDPPlasticity.cu:67-163. tab = {am, ap, lambda, alpha, wmax,
 // Em[0..G], Ep[0..G], iEm[0..G], iEp[0..G]}
-__device__ __forceinline__ void stdp_wd_one(const StdpArgs& a, int64_t j) {
-  const int64_t s = a.pos[j];
-  const int u = a.pre[j];
-  const int post = a.tgt[s];
-  const float am = a.tab[0], ap = a.tab[1], lam = a.tab[2], alp = a.tab[3], wmax = a.tab[4];
-  const float* Em = a.tab + 5;
-  const float* Ep = Em + (a.G + 1);
-  const float* iEm = Ep + (a.G + 1);
-  const float* iEp = iEm + (a.G + 1);
-  const int G = a.G;
-  float xpre = tload(a.tr, 0, j), xpost = tload(a.tr, 1, j);
-  const float lpre = (u < a.N) ? a.last_n[u] : a.last_in[u - a.N];
-  int pg = (int)roundf((lpre - a.t) / a.dt);
-  const int qg = (int)roundf((a.last_n[post] - a.t) / a.dt);
+
+// Step of the presynaptic neuron's last spike within the group starting at t; a last
+// spike G or more steps ahead is taken as before the group (WeightDependentSTDPPlasticity.cu:98-101).
+__device__ __forceinline__ int wd_pre_step(float last, float t, float dt, int G) {
+  int pg = (int)roundf((last - t) / dt);
   if (pg >= G) pg *= -1;
-  const float w = wload(a.wts, s);
-  xpost *= Em[G];
-  xpre *= Ep[G];
-  if (a.nearest) {
-    if (qg >= 0) xpost = am * Em[G - qg];
-    if (pg >= 0) xpre = ap * Ep[G - pg];
+  return pg;
+}
+// Step of the postsynaptic neuron's last spike within the group.  While time advances it
+// is below G; the bound keeps the table lookups inside [0, G] for any other input.
+__device__ __forceinline__ int wd_post_step(float last, float t, float dt, int G) {
+  return min((int)roundf((last - t) / dt), G);
+}
+
+// A trace over one group: decay by E[G], then the group's spike at step k (if k >= 0)
+// adds A*E[G-k], or with nearest-spike pairing replaces the trace by it.
+__device__ __forceinline__ float wd_trace(float x, int k, float A, const float* E, int G, int nearest) {
+  x *= E[G];
+  if (nearest) {
+    if (k >= 0) x = A * E[G - k];
   } else {
-    xpost += (qg >= 0) ? am * Em[G - qg] : 0.0f;
-    xpre += (pg >= 0) ? ap * Ep[G - pg] : 0.0f;
+    x += (k >= 0) ? A * E[G - k] : 0.0f;
   }
+  return x;
+}
+
+// New weight of a synapse from the stored weight, the group's updated traces and the
+// spike steps of its two neurons (WeightDependentSTDPPlasticity.cu:125-158), clamped at 0.
+__device__ __forceinline__ float wd_weight(float w, float xpre, float xpost, int pg, int qg,
+                                           const float* tab, int G) {
+  const float am = tab[0], ap = tab[1], lam = tab[2], alp = tab[3], wmax = tab[4];
+  const float* Em = tab + 5;
+  const float* Ep = Em + (G + 1);
+  const float* iEm = Ep + (G + 1);
+  const float* iEp = iEm + (G + 1);
   float upd = 0.0f;
   if (pg >= 0) {
     float tmp = xpost;
@@ -546,7 +558,24 @@ __device__ __forceinline__ void stdp_wd_one(const StdpArgs& a, int64_t j) {
   }
   float nw = w + upd;
   if (nw < 0.0f) nw = 0.0f;
-  wupdate(a.wts, s, w, nw, 0.0f, __builtin_inff());
+  return nw;
+}
+
+// Every plastic synapse of the rule, with its own pair of traces.
+__device__ __forceinline__ void stdp_wd_one(const StdpArgs& a, int64_t j) {
+  const int64_t s = a.pos[j];
+  const int u = a.pre[j];
+  const int post = a.tgt[s];
+  const int G = a.G;
+  const float* Em = a.tab + 5;
+  const float* Ep = Em + (G + 1);
+  const float lpre = (u < a.N) ? a.last_n[u] : a.last_in[u - a.N];
+  const int pg = wd_pre_step(lpre, a.t, a.dt, G);
+  const int qg = wd_post_step(a.last_n[post], a.t, a.dt, G);
+  const float w = wload(a.wts, s);
+  const float xpost = wd_trace(tload(a.tr, 1, j), qg, a.tab[0], Em, G, a.nearest);
+  const float xpre = wd_trace(tload(a.tr, 0, j), pg, a.tab[1], Ep, G, a.nearest);
+  wupdate(a.wts, s, w, wd_weight(w, xpre, xpost, pg, qg, a.tab, G), 0.0f, __builtin_inff());
   tstore(a.tr, 0, j, xpre);
   tstore(a.tr, 1, j, xpost);
 }
@@ -554,6 +583,150 @@ __device__ __forceinline__ void stdp_wd_one(const StdpArgs& a, int64_t j) {
 __global__ __launch_bounds__(256) void k_stdp_wd(StdpArgs a) {
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j < a.P) stdp_wd_one(a, j);
+}
+
+// ---------------------------------------------------------------------------------
+// The weight-dependent rule driven by spikes.  A synapse's two traces depend only on its
+// presynaptic and its postsynaptic neuron (every synapse of a neuron sees the same
+// decays and the same spike steps), so they are kept once per neuron; and a weight moves
+// only in a group where its presynaptic neuron has pg >= 0 or its postsynaptic neuron has
+// qg >= 0 -- otherwise the update adds exactly 0 to a weight that an earlier group has
+// already clamped at 0.  Per group: (1) one lane per neuron advances both traces and
+// lists the neurons with pg >= 0 (presynaptic side) and qg >= 0 (postsynaptic side);
+// (2) one wavefront per listed presynaptic neuron updates all its rule synapses, and
+// one per listed postsynaptic neuron the rest of its incoming rule synapses (those whose
+// presynaptic neuron is not listed), so each synapse is updated by at most one lane.
+
+struct WdEventArgs {
+  Traces trn;              // stored traces per neuron: a over unified presynaptic ids, b over neurons
+  float* xa;               // this group's traces before rounding to storage: [N + Nin], [N]
+  float* xb;
+  Weights wts;
+  const float* last_n;
+  const float* last_in;
+  const float* tab;
+  const int64_t* pos;      // rule index -> CSR position
+  const int32_t* pre;      // rule index -> unified presynaptic id
+  const int32_t* tgt;      // CSR position -> postsynaptic neuron
+  const int64_t* pre_ptr;  // [N + Nin + 1]: rule indices of presynaptic neuron u are
+  const int32_t* pre_idx;  //   pre_idx[pre_ptr[u] .. pre_ptr[u+1]) (nullptr: the range itself)
+  const int64_t* post_ptr; // [N + 1]
+  const int32_t* post_idx;
+  uint32_t* act;           // listed neurons: presynaptic ids from 0, postsynaptic from N + Nin
+  int* cnt;                // [2][2]: list lengths, double-buffered by `par`
+  int par;
+  int N, Nin, G, nearest;
+  float t, dt;
+};
+
+__device__ __forceinline__ void wave_push(bool pred, uint32_t id, uint32_t* __restrict__ list,
+                                          int* __restrict__ count, int cap) {
+  const uint64_t m = __ballot(pred);
+  if (m == 0) return;
+  const int lane = (int)__lane_id();
+  const int leader = __ffsll((unsigned long long)m) - 1;
+  int base = 0;
+  if (lane == leader) base = atomicAdd(count, __popcll(m));
+  base = __shfl(base, leader);
+  const int p = base + __popcll(m & ((1ull << lane) - 1ull));
+  if (pred && p < cap) list[p] = id;
+}
+
+__global__ __launch_bounds__(256) void k_wd_neurons(WdEventArgs a) {
+  const int U = a.N + a.Nin;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int G = a.G;
+  const float* Em = a.tab + 5;
+  const float* Ep = Em + (G + 1);
+  bool listed = false;
+  if (i < U) {
+    const float l = i < a.N ? a.last_n[i] : a.last_in[i - a.N];
+    const int pg = wd_pre_step(l, a.t, a.dt, G);
+    const float x = wd_trace(tload(a.trn, 0, i), pg, a.tab[1], Ep, G, a.nearest);
+    a.xa[i] = x;
+    tstore(a.trn, 0, i, x);
+    listed = pg >= 0 && a.pre_ptr[i + 1] > a.pre_ptr[i];
+  }
+  wave_push(listed, (uint32_t)i, a.act, a.cnt + 2 * a.par, U);
+  listed = false;
+  const int v = i - U;
+  if (v >= 0 && v < a.N) {
+    const int qg = wd_post_step(a.last_n[v], a.t, a.dt, G);
+    const float x = wd_trace(tload(a.trn, 1, v), qg, a.tab[0], Em, G, a.nearest);
+    a.xb[v] = x;
+    tstore(a.trn, 1, v, x);
+    listed = qg >= 0 && a.post_ptr[v + 1] > a.post_ptr[v];
+  }
+  wave_push(listed, (uint32_t)v, a.act + U, a.cnt + 2 * a.par + 1, a.N);
+}
+
+__global__ __launch_bounds__(256) void k_wd_events(WdEventArgs a) {
+  const int lane = (int)__lane_id();
+  const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) / kWave);
+  const int nwaves = (int)((gridDim.x * blockDim.x) / kWave);
+  const int U = a.N + a.Nin;
+  const int G = a.G;
+  const int npre = min(a.cnt[2 * a.par], U), npost = min(a.cnt[2 * a.par + 1], a.N);
+  // the other buffer is next group's: clear it (nothing reads it in this group)
+  if (blockIdx.x == 0 && threadIdx.x < 2) a.cnt[2 * (a.par ^ 1) + threadIdx.x] = 0;
+  for (int k = wave; k < npre + npost; k += nwaves) {
+    if (k < npre) {
+      const int u = (int)a.act[k];
+      const float l = u < a.N ? a.last_n[u] : a.last_in[u - a.N];
+      const int pg = wd_pre_step(l, a.t, a.dt, G);
+      const float xpre = a.xa[u];
+      const int64_t e1 = a.pre_ptr[u + 1];
+      for (int64_t e = a.pre_ptr[u] + lane; e < e1; e += kWave) {
+        const int64_t j = a.pre_idx ? (int64_t)a.pre_idx[e] : e;
+        const int64_t s = a.pos[j];
+        const int post = a.tgt[s];
+        const int qg = wd_post_step(a.last_n[post], a.t, a.dt, G);
+        const float w = wload(a.wts, s);
+        wupdate(a.wts, s, w, wd_weight(w, xpre, a.xb[post], pg, qg, a.tab, G), 0.0f, __builtin_inff());
+      }
+    } else {
+      const int v = (int)a.act[U + (k - npre)];
+      const int qg = wd_post_step(a.last_n[v], a.t, a.dt, G);
+      const float xpost = a.xb[v];
+      const int64_t e1 = a.post_ptr[v + 1];
+      for (int64_t e = a.post_ptr[v] + lane; e < e1; e += kWave) {
+        const int64_t j = a.post_idx[e];
+        const int u = a.pre[j];
+        const float l = u < a.N ? a.last_n[u] : a.last_in[u - a.N];
+        const int pg = wd_pre_step(l, a.t, a.dt, G);
+        if (pg >= 0) continue;              // updated from the presynaptic side
+        const int64_t s = a.pos[j];
+        const float w = wload(a.wts, s);
+        wupdate(a.wts, s, w, wd_weight(w, a.xa[u], xpost, pg, qg, a.tab, G), 0.0f, __builtin_inff());
+      }
+    }
+  }
+}
+
+// Traces between the two forms: per synapse (the per-synapse kernels) and per neuron.
+// Every synapse of a neuron holds that neuron's trace, so either copy is exact.
+struct TraceCopyArgs {
+  Traces syn;
+  Traces neu;
+  const int64_t* pos;
+  const int32_t* pre;
+  const int32_t* tgt;
+  int64_t P;
+  int to_neurons;
+};
+
+__global__ __launch_bounds__(256) void k_trace_copy(TraceCopyArgs a) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= a.P) return;
+  const int u = a.pre[j];
+  const int v = a.tgt[a.pos[j]];
+  if (a.to_neurons) {   // all lanes that write one neuron's entry write the same value
+    tstore(a.neu, 0, u, tload(a.syn, 0, j));
+    tstore(a.neu, 1, v, tload(a.syn, 1, j));
+  } else {
+    tstore(a.syn, 0, j, tload(a.neu, 0, u));
+    tstore(a.syn, 1, j, tload(a.neu, 1, v));
+  }
 }
 
 __device__ __forceinline__ bool pre_bit(const StdpArgs& a, int u, int64_t step) {

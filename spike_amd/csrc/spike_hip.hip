@@ -113,6 +113,28 @@ struct RuleHost {
   std::vector<int64_t> order;     // device order j -> index into syn (grouped by post partition)
   std::vector<int64_t> part;      // [P+1] offsets of each postsynaptic partition
   int64_t* d_part = nullptr;
+  // spike-driven form of the weight-dependent rule (per-phase kernels): rule indices by
+  // presynaptic and by postsynaptic neuron, traces per neuron, the listed neurons
+  bool ev_ok = false;
+  int64_t* d_ev_pre_ptr = nullptr;
+  int32_t* d_ev_pre_idx = nullptr;   // nullptr: rule order already groups by presynaptic neuron
+  int64_t* d_ev_post_ptr = nullptr;
+  int32_t* d_ev_post_idx = nullptr;
+  float* d_xn_a = nullptr;           // stored per-neuron traces (FP32 or binary16)
+  float* d_xn_b = nullptr;
+  uint16_t* d_xn_a16 = nullptr;
+  uint16_t* d_xn_b16 = nullptr;
+  float* d_xcur_a = nullptr;         // this group's traces before rounding
+  float* d_xcur_b = nullptr;
+  uint32_t* d_act = nullptr;
+  int* d_act_cnt = nullptr;
+  int par = 0;
+  // which trace form is current: per synapse (per-synapse kernels, the persistent kernel,
+  // read-out) and/or per neuron (spike-driven kernels)
+  bool syn_tr = true, neu_tr = true;
+  // the next group runs the per-synapse form: it also clamps weights that were set
+  // below 0, which the spike-driven form (touching only synapses with spikes) would not
+  bool dense_next = true;
 };
 
 struct MonitorHost {
@@ -224,6 +246,10 @@ struct spk_network {
   double prof_us[spk::kProfPhases] = {0};
   // per-phase bookkeeping
   int cur_tidx = -1;
+  // start time of the latest group simulated since spk_reset_state: time only advances
+  // (SpikingModel::run continues from the current time), so last-spike times never lie
+  // ahead of a group, which the plasticity tables rely on
+  float t_hi = -INFINITY;
   int cur_slot = 0;
   // timing
   hipEvent_t ev_a = nullptr, ev_b = nullptr;
@@ -287,6 +313,9 @@ void free_device(spk_network* n) {
   for (auto& r : n->rules) {
     dfree(r.d_pos); dfree(r.d_pre); dfree(r.d_ta); dfree(r.d_tb); dfree(r.d_tab); dfree(r.d_part);
     dfree(r.d_ta16); dfree(r.d_tb16);
+    dfree(r.d_ev_pre_ptr); dfree(r.d_ev_pre_idx); dfree(r.d_ev_post_ptr); dfree(r.d_ev_post_idx);
+    dfree(r.d_xn_a); dfree(r.d_xn_b); dfree(r.d_xn_a16); dfree(r.d_xn_b16);
+    dfree(r.d_xcur_a); dfree(r.d_xcur_b); dfree(r.d_act); dfree(r.d_act_cnt);
   }
   dfree(n->d_nlo); dfree(n->d_ilo); dfree(n->d_seg); dfree(n->d_xlist);
   dfree(n->d_bar); dfree(n->d_err); dfree(n->d_prof); dfree(n->d_rules);
@@ -341,7 +370,16 @@ int refresh_rule_tables(spk_network* n, RuleHost& r) {
   return SPK_OK;
 }
 
+int check_time(spk_network* n, float t) {
+  if (!(t >= n->t_hi - 0.5f * n->dt))
+    return fail(SPK_ERR_ARG, "group start time precedes the latest group simulated "
+                             "(spk_reset_state starts the network again at any time)");
+  n->t_hi = std::max(n->t_hi, t);
+  return SPK_OK;
+}
+
 int begin_group(spk_network* n, float t, bool force = false) {
+  if (int rc = check_time(n, t)) return rc;
   const int tidx = (int)roundf(t / n->dt);
   if (tidx == n->cur_tidx && !force) return SPK_OK;
   n->cur_tidx = tidx;
@@ -425,11 +463,69 @@ int launch_inputs(spk_network* n, float t, int slot) {
   return SPK_OK;
 }
 
+// Per-neuron traces of a weight-dependent rule (same storage format as its per-synapse ones).
+spk::Traces neuron_traces_of(spk_network* n, RuleHost& r) {
+  spk::Traces t = traces_of(n, r);
+  t.a = r.d_xn_a; t.b = r.d_xn_b; t.a16 = r.d_xn_a16; t.b16 = r.d_xn_b16;
+  return t;
+}
+
+int copy_traces(spk_network* n, RuleHost& r, bool to_neurons) {
+  spk::TraceCopyArgs a{};
+  a.syn = traces_of(n, r);
+  a.neu = neuron_traces_of(n, r);
+  a.pos = r.d_pos; a.pre = r.d_pre; a.tgt = n->d_tgt;
+  a.P = (int64_t)r.syn.size();
+  a.to_neurons = to_neurons ? 1 : 0;
+  hipLaunchKernelGGL(spk::k_trace_copy, dim3(grid_for(a.P, 256)), dim3(256), 0, n->stream, a);
+  SPK_HIP(hipGetLastError());
+  return SPK_OK;
+}
+
+// Make the per-synapse traces current (after spike-driven groups).
+int traces_per_synapse(spk_network* n, RuleHost& r) {
+  if (r.syn_tr) return SPK_OK;
+  if (int rc = copy_traces(n, r, false)) return rc;
+  r.syn_tr = true;
+  return SPK_OK;
+}
+
+// One group of the weight-dependent rule in its spike-driven form (k_wd_neurons, k_wd_events).
+int launch_wd_events(spk_network* n, RuleHost& r, float t) {
+  if (!r.neu_tr) {
+    if (int rc = copy_traces(n, r, true)) return rc;
+    r.neu_tr = true;
+  }
+  spk::WdEventArgs a{};
+  a.trn = neuron_traces_of(n, r);
+  a.xa = r.d_xcur_a; a.xb = r.d_xcur_b;
+  a.wts = weights_of(n);
+  a.last_n = n->d_last_n; a.last_in = n->d_last_in; a.tab = r.d_tab;
+  a.pos = r.d_pos; a.pre = r.d_pre; a.tgt = n->d_tgt;
+  a.pre_ptr = r.d_ev_pre_ptr; a.pre_idx = r.d_ev_pre_idx;
+  a.post_ptr = r.d_ev_post_ptr; a.post_idx = r.d_ev_post_idx;
+  a.act = r.d_act; a.cnt = r.d_act_cnt; a.par = r.par;
+  a.N = n->N; a.Nin = n->Nin; a.G = n->G; a.nearest = r.wd.nearest_spike_only;
+  a.t = t; a.dt = n->dt;
+  hipLaunchKernelGGL(spk::k_wd_neurons, dim3(grid_for((int64_t)n->nu() + n->N, 256)), dim3(256), 0,
+                     n->stream, a);
+  SPK_HIP(hipGetLastError());
+  hipLaunchKernelGGL(spk::k_wd_events, dim3(512), dim3(256), 0, n->stream, a);
+  SPK_HIP(hipGetLastError());
+  r.par ^= 1;
+  r.syn_tr = false;
+  return SPK_OK;
+}
+
 int launch_rule(spk_network* n, int rid, float t) {
   RuleHost& r = n->rules[rid];
   if (r.syn.empty()) return SPK_OK;
   int rc = refresh_rule_tables(n, r);
   if (rc) return rc;
+  if (r.kind == 0 && r.ev_ok && !r.dense_next) return launch_wd_events(n, r, t);
+  if ((rc = traces_per_synapse(n, r))) return rc;
+  r.neu_tr = false;
+  r.dense_next = false;
   spk::StdpArgs a{};
   a.pos = r.d_pos; a.pre = r.d_pre; a.tgt = n->d_tgt; a.delay = n->d_delay; a.wts = weights_of(n);
   a.tr = traces_of(n, r); a.last_n = n->d_last_n; a.last_in = n->d_last_in;
@@ -572,6 +668,12 @@ int fill_fused_args(spk_network* n, bool plastic, spk::FusedArgs& a) {
     a.w_lo = n->cw_lo; a.w_hi = n->cw_hi; a.src_split = n->csplit; a.src_delay = n->cdelay;
   }
   a.n_rules = plastic ? (int)n->rules.size() : 0;
+  for (int r = 0; r < a.n_rules; ++r) {
+    RuleHost& rh = n->rules[r];
+    if (int rc = traces_per_synapse(n, rh)) return rc;
+    rh.neu_tr = false;
+    rh.dense_next = false;
+  }
   if (a.n_rules > 0 && !n->rules_uploaded) {
     std::vector<spk::FusedRule> fr(a.n_rules);
     for (int r = 0; r < a.n_rules; ++r) {
@@ -921,8 +1023,13 @@ static int add_rule(spk_network* n, int kind, const int64_t* syn, int64_t cnt, R
   if (int rc = check_net(n)) return rc;
   if (n->prepared) return fail(SPK_ERR_STATE, "network already prepared");
   if (cnt < 0 || (cnt > 0 && !syn)) return fail(SPK_ERR_ARG, "bad plastic synapse list");
-  for (int64_t j = 0; j < cnt; ++j)
+  // a synapse listed twice would be updated by two lanes at once
+  std::vector<bool> seen((size_t)n->S, false);
+  for (int64_t j = 0; j < cnt; ++j) {
     if (syn[j] < 0 || syn[j] >= n->S) return fail(SPK_ERR_ARG, "plastic synapse index out of range");
+    if (seen[(size_t)syn[j]]) return fail(SPK_ERR_ARG, "synapse listed twice in one plastic rule");
+    seen[(size_t)syn[j]] = true;
+  }
   r.kind = kind;
   r.syn.assign(syn, syn + cnt);
   n->rules.push_back(std::move(r));
@@ -1271,6 +1378,51 @@ int spk_prepare(spk_network* n, int G) {
     if ((rc = upload(r.d_pos, pos.data(), P, st))) return rc;
     if ((rc = upload(r.d_pre, pre.data(), P, st))) return rc;
     if ((rc = upload(r.d_part, r.part.data(), (size_t)n->P + 1, st))) return rc;
+    // the weight-dependent rule's spike-driven form: its rule indices grouped by
+    // presynaptic neuron (already so in CSR order) and by postsynaptic neuron
+    r.ev_ok = r.kind == 0 && P > 0 && P < (size_t)INT32_MAX;
+    if (r.ev_ok) {
+      const int U = n->nu();
+      std::vector<int64_t> pp((size_t)U + 1, 0), qp((size_t)n->N + 1, 0);
+      bool grouped = true;
+      for (size_t j = 0; j < P; ++j) {
+        pp[(size_t)pre[j] + 1]++;
+        qp[(size_t)n->h_post[r.syn[r.order[j]]] + 1]++;
+        if (j && pre[j] < pre[j - 1]) grouped = false;
+      }
+      for (int u = 0; u < U; ++u) pp[u + 1] += pp[u];
+      for (int v = 0; v < n->N; ++v) qp[v + 1] += qp[v];
+      std::vector<int32_t> qi(P);
+      {
+        std::vector<int64_t> fill(qp.begin(), qp.end() - 1);
+        for (size_t j = 0; j < P; ++j) qi[fill[n->h_post[r.syn[r.order[j]]]]++] = (int32_t)j;
+      }
+      if ((rc = dalloc(&r.d_ev_pre_ptr, (size_t)U + 1))) return rc;
+      if ((rc = dalloc(&r.d_ev_post_ptr, (size_t)n->N + 1))) return rc;
+      if ((rc = dalloc(&r.d_ev_post_idx, P))) return rc;
+      if ((rc = upload(r.d_ev_pre_ptr, pp.data(), pp.size(), st))) return rc;
+      if ((rc = upload(r.d_ev_post_ptr, qp.data(), qp.size(), st))) return rc;
+      if ((rc = upload(r.d_ev_post_idx, qi.data(), P, st))) return rc;
+      if (!grouped) {
+        std::vector<int32_t> pi(P);
+        std::vector<int64_t> fill(pp.begin(), pp.end() - 1);
+        for (size_t j = 0; j < P; ++j) pi[fill[pre[j]]++] = (int32_t)j;
+        if ((rc = dalloc(&r.d_ev_pre_idx, P))) return rc;
+        if ((rc = upload(r.d_ev_pre_idx, pi.data(), P, st))) return rc;
+      }
+      if (n->wfmt != SPK_WEIGHTS_FP32) {
+        if ((rc = dalloc(&r.d_xn_a16, (size_t)U))) return rc;
+        if ((rc = dalloc(&r.d_xn_b16, (size_t)n->N))) return rc;
+      } else {
+        if ((rc = dalloc(&r.d_xn_a, (size_t)U))) return rc;
+        if ((rc = dalloc(&r.d_xn_b, (size_t)n->N))) return rc;
+      }
+      if ((rc = dalloc(&r.d_xcur_a, (size_t)U))) return rc;
+      if ((rc = dalloc(&r.d_xcur_b, (size_t)n->N))) return rc;
+      if ((rc = dalloc(&r.d_act, (size_t)U + n->N))) return rc;
+      if ((rc = dalloc(&r.d_act_cnt, 4))) return rc;
+      SPK_HIP(hipStreamSynchronize(st));   // the host index arrays end with this scope
+    }
   }
   // monitors: device store of the requested capacity
   for (auto& m : n->mons) {
@@ -1367,8 +1519,19 @@ int spk_reset_state(spk_network* n) {
         SPK_HIP(hipMemsetAsync(r.d_tb16, 0, sizeof(uint16_t) * P, st));
       }
     }
+    if (r.ev_ok) {
+      const size_t U = (size_t)n->nu(), N = (size_t)n->N;
+      if (r.d_xn_a) SPK_HIP(hipMemsetAsync(r.d_xn_a, 0, sizeof(float) * U, st));
+      if (r.d_xn_b) SPK_HIP(hipMemsetAsync(r.d_xn_b, 0, sizeof(float) * N, st));
+      if (r.d_xn_a16) SPK_HIP(hipMemsetAsync(r.d_xn_a16, 0, sizeof(uint16_t) * U, st));
+      if (r.d_xn_b16) SPK_HIP(hipMemsetAsync(r.d_xn_b16, 0, sizeof(uint16_t) * N, st));
+      SPK_HIP(hipMemsetAsync(r.d_act_cnt, 0, sizeof(int) * 4, st));
+      r.par = 0;
+    }
+    r.syn_tr = r.neu_tr = true;
   }
   n->cur_tidx = -1;
+  n->t_hi = -INFINITY;
   return spk_reset_inputs(n);
 }
 
@@ -1387,6 +1550,7 @@ int spk_update_inputs(spk_network* n, float t) {
 int spk_update_plasticity(spk_network* n, int rid, float t) {
   if (int rc = check_listed(n)) return rc;
   if (rid < 0 || rid >= (int)n->rules.size()) return fail(SPK_ERR_ARG, "rule id out of range");
+  if (int rc = check_time(n, t)) return rc;
   return launch_rule(n, rid, t);
 }
 
@@ -1412,6 +1576,8 @@ int spk_run(spk_network* n, float start, int64_t ngroups, int flags) {
   SPK_HIP(hipEventRecord(n->ev_a, n->stream));
   if (!use_fused && n->generated) return check_listed(n);
   if (use_fused) {
+    if (int rc = check_time(n, start)) return rc;
+    if (ngroups > 0) n->t_hi = std::max(n->t_hi, start + (float)((ngroups - 1) * n->G) * n->dt);
     if (int rc = run_fused(n, start, ngroups, flags)) return rc;
   } else {
     // each monitor store must hold 64 groups of its population
@@ -1462,6 +1628,7 @@ int spk_step_group(spk_network* n, float t, int flags) {
   if (int rc = check_prepared(n)) return rc;
   const bool use_fused = n->fused_ok && !(flags & SPK_RUN_FORCE_LAUNCHES) && n->G == n->Gprep;
   if (use_fused) {
+    if (int rc = check_time(n, t)) return rc;
     const bool plastic = (flags & SPK_RUN_PLASTICITY) != 0 && !n->rules.empty();
     spk::FusedArgs a;
     if (int rc = fill_fused_args(n, plastic, a)) return rc;
@@ -1674,6 +1841,7 @@ int spk_set_weights(spk_network* n, const float* in) {
   }
   SPK_HIP(hipStreamSynchronize(n->stream));
   n->compact = false;   // caller weights: deliveries read the full records from now on
+  for (auto& r : n->rules) r.dense_next = true;   // caller weights may lie below 0
   return SPK_OK;
 }
 
@@ -1709,6 +1877,7 @@ int spk_get_traces(spk_network* n, int rid, float* a, float* b) {
   if (rid < 0 || rid >= (int)n->rules.size()) return fail(SPK_ERR_ARG, "rule id out of range");
   RuleHost& r = n->rules[rid];
   const size_t P = r.syn.size();
+  if (int rc = traces_per_synapse(n, r)) return rc;
   std::vector<float> ta(P), tb(P);
   if (r.d_ta16) {
     std::vector<uint16_t> a16(P), b16(P);

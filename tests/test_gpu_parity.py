@@ -9,7 +9,7 @@ import pytest
 import oracle_py
 from spike_amd import abi, builders
 from spike_amd.netdesc import (SYN_CONDUCTANCE, Rule, RULE_WD, RULE_VOGELS, RULE_EVANS, WEIGHTS_BF16,
-                               WEIGHTS_FP16)
+                               WEIGHTS_FP16, WEIGHTS_FP32)
 
 pytestmark = pytest.mark.gpu
 
@@ -198,6 +198,65 @@ def test_plasticity_16bit_weights(kind, fused, fmt):
     d.weight_format = fmt
     gpu, ora = _run_both(d, 200, fused=fused, partitions=5)
     _compare(gpu, ora, d, f"rule {kind} weights format {fmt}")
+
+
+@pytest.mark.parametrize("fmt", [WEIGHTS_FP32, WEIGHTS_FP16])
+@pytest.mark.parametrize("nearest", [0, 1])
+def test_wd_spike_driven_form_across_paths(nearest, fmt):
+    """The per-phase path runs the weight-dependent rule spike-driven (traces per neuron,
+    only synapses with a pre- or postsynaptic spike in the group updated) after one
+    per-synapse group; the persistent kernel runs it per synapse.  Alternating the two,
+    reading traces and weights in between, and setting weights below 0 (which only the
+    per-synapse form clamps, so the next group is one) keeps every array bit-exact."""
+    d = _plastic_net(RULE_WD, 60 + nearest)
+    d.rules[0].params["nearest_spike_only"] = nearest
+    d.weight_format = fmt
+    print(f"[parity] wd spike-driven across paths nearest={nearest} fmt={fmt}", flush=True)
+    gpu = abi.Network(d, partitions=5)
+    ora = oracle_py.OracleNetwork(d, precision=32)
+    assert gpu.info().fused == 1
+    t, tag = 0.0, "wd paths"
+    for k, (ng, launches) in enumerate([(40, True), (30, False), (50, True), (20, True)]):
+        if k == 3:
+            w = gpu.weights()
+            w[d.rules[0].synapses[::7]] = np.float32(-1e-3)
+            gpu.set_weights(w)
+            ora.set_weights(w)
+        gpu.run(t, ng, force_launches=launches)
+        ora.run(t, ng)
+        t = float(d.group_time(t, ng))
+        _compare(gpu, ora, d, f"{tag} piece {k}")
+    assert gpu.weights()[d.rules[0].synapses].min() >= 0.0
+
+
+def test_wd_spike_driven_form_csr_order():
+    """A network that the persistent kernel cannot hold (one partition whose input-ring
+    slice exceeds LDS) keeps its plastic synapses in CSR order, which already groups them
+    by presynaptic neuron: the spike-driven kernels then index them without a permutation."""
+    d = builders.random_net(seed=71, n_neurons=2000, n_inputs=200, p=0.02, input_rate=500.0,
+                            whi=3e-3, dmin=2, dmax=6, monitors=(0,))
+    S = d.n_synapses
+    syn = np.sort(np.random.default_rng(71).choice(S, size=S // 2, replace=False)).astype(np.int64)
+    d.rules.append(Rule(RULE_WD, syn, builders.wd_params(lam=1e-2, alpha=2.02, w_max=4e-3)))
+    gpu, ora = _run_both(d, 150, partitions=1, expect_fused=False)
+    _compare(gpu, ora, d, "wd csr order")
+
+
+def test_time_only_advances_and_rules_list_synapses_once():
+    d = _plastic_net(RULE_WD, 70)
+    gpu = abi.Network(d)
+    gpu.run(0.0, 20)
+    with pytest.raises(abi.SpikeError, match="precedes"):
+        gpu.run(0.0, 5)
+    with pytest.raises(abi.SpikeError, match="precedes"):
+        gpu.step_phases(float(d.group_time(0.0, 3)))
+    gpu.reset_state()
+    gpu.run(0.0, 5)
+    gpu.close()
+    syn = d.rules[0].synapses
+    d.rules[0].synapses = np.concatenate([syn, syn[:1]])
+    with pytest.raises(abi.SpikeError, match="twice"):
+        abi.Network(d)
 
 
 @pytest.mark.parametrize("fmt", [WEIGHTS_BF16, WEIGHTS_FP16])
