@@ -11,17 +11,20 @@
    * the COBA network with Vogels iSTDP on I->E (3200 E + 800 I, p = 0.02), plasticity on;
    * the large plastic network at 1/100 of its size (5000 neurons x 100 outgoing synapses,
      delays 1..20 steps so G = 1, nearest-spike WD STDP on E->E, 250 Poisson inputs).
-2. A 2 s COBA + iSTDP run on the GPU, with FP32 or FP16 weight storage, stays within
-   2 % of the FP64 oracle (BF16: the stated tolerances of COBA_TOL) in
-   population rate, ISI CV and the I->E weight distribution (mean and 20-bin histogram),
-   although the two spike trains decorrelate within milliseconds.  The spread of the
-   weights is printed, not asserted: after 2 s it is about 2 % of their mean, and the FP32
-   and FP64 spreads differed by 2.9 % on the seed used here.
+2. 2 s COBA + iSTDP runs on the GPU, with FP32 or FP16 weight storage, stay within 2 % of
+   the FP64 oracle (BF16: the stated tolerances of COBA_TOL) in population rate, ISI CV
+   and the I->E weight distribution (mean and 20-bin histogram), as means over 8
+   connectivity seeds: single runs decorrelate within milliseconds, and the FP64 oracle's
+   own rate and CV move by 1-3 % per seed when its resting potentials move by one ulp.
 """
+import json
+import os
+
 import numpy as np
 import pytest
 
 import oracle_py
+from chaos_stats import hist_l1, histogram, isi_cv
 from spike_amd import abi, builders
 
 pytestmark = pytest.mark.gpu
@@ -69,56 +72,50 @@ def test_large_plastic_scaled_first_1000_steps():
     _bit_exact_first_steps(d, tag="large plastic / 100")
 
 
-def _isi_cv(ids, times, n):
-    cvs = []
-    order = np.lexsort((times, ids))
-    ids, times = ids[order], times[order].astype(np.float64)
-    starts = np.searchsorted(ids, np.arange(n))
-    ends = np.searchsorted(ids, np.arange(n), side="right")
-    for a, b in zip(starts, ends):
-        if b - a >= 3:
-            isi = np.diff(times[a:b])
-            cvs.append(isi.std() / isi.mean())
-    return float(np.mean(cvs))
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "coba_vogels_fp64_2s.json")
 
-
-# Relative tolerances against FP64 per weight format: (rate, ISI CV, mean I->E weight,
-# 20-bin weight histogram L1/2).  FP32 and FP16 storage meet the 2 % criterion.  BF16 is
-# stated, not held to it: its 8-bit significand moves the fixed weights themselves (0.06 by
-# +0.1 %, 0.67 by -0.3 %), which shifts the dynamics (measured: rate -2.0 %, CV +2.7 %,
-# mean +0.2 %), and its spacing near 0.67 (0.0039) is a third of the weights' spread, so
-# the histogram of stored values is visibly quantised (measured L1/2 6.8 %).
-COBA_TOL = {0: (0.02, 0.02, 0.02, 0.02), 1: (0.03, 0.04, 0.005, 0.10), 2: (0.02, 0.02, 0.02, 0.02)}
+# Relative tolerances against FP64 per weight format, on means over the fixture's seeds:
+# (rate, ISI CV, mean I->E weight, mean histogram L1/2).  FP32 and FP16 storage meet the
+# 2 % criterion.  BF16 is stated, not held to it: its 8-bit significand moves the fixed
+# weights themselves (0.06 by +0.1 %, 0.67 by -0.3 %) and biases the mean plastic weight
+# (+0.25 %), and its spacing near 0.67 (0.0039) is a third of the weights' spread, so its
+# histogram is compared with that of the FP64 weights rounded to bfloat16.
+COBA_TOL = {0: (0.02, 0.02, 0.02, 0.02), 1: (0.03, 0.03, 0.005, 0.10), 2: (0.02, 0.02, 0.02, 0.02)}
 
 
 @pytest.mark.parametrize("fmt", [0, 1, 2])
-def test_coba_vogels_2s_within_2pc_of_fp64(fmt):
-    """Weights stored in FP32, BF16 (Kahan) or FP16 (Kahan), traces FP32 or binary16."""
-    d = builders.coba_vogels(seed=42, monitors=(0,))
-    d.weight_format = fmt
-    ng = d.groups_for(2.0)
-    d64 = builders.coba_vogels(seed=42, monitors=(0,))
-    print(f"[configs] coba + vogels 2 s GPU (weights format {fmt}) vs FP64 oracle", flush=True)
-    gpu = abi.Network(d)
-    gpu.run(0.0, ng)
-    gi, gt = gpu.take_spikes(0)
-    wg = gpu.weights()[d.rules[0].synapses].astype(np.float64)
-    ora = oracle_py.OracleNetwork(d64, precision=64, threads=16)
-    ora.run(0.0, ng)
-    oi, ot = ora.take_spikes(0)
-    wo = ora.weights()[d.rules[0].synapses]
-    rg, ro = len(gi), len(oi)
-    cg, co = _isi_cv(gi, gt, d.n_neurons), _isi_cv(oi, ot, d.n_neurons)
-    edges = np.linspace(min(wg.min(), wo.min()), max(wg.max(), wo.max()) * (1 + 1e-9), 21)
-    hg, _ = np.histogram(wg, bins=edges)
-    ho, _ = np.histogram(wo, bins=edges)
-    l1 = np.abs(hg - ho).sum() / (2.0 * len(wg))
-    print(f"spikes GPU {rg} FP64 {ro}; ISI CV GPU {cg:.4f} FP64 {co:.4f}; I->E weights mean GPU "
-          f"{wg.mean():.5e} FP64 {wo.mean():.5e}, std GPU {wg.std():.5e} FP64 {wo.std():.5e}, "
+def test_coba_vogels_2s_ensemble_within_2pc_of_fp64(fmt):
+    """COBA + iSTDP, 2 s on the GPU with FP32, BF16 (Kahan) or FP16 (Kahan) weight storage
+    against FP64 oracle statistics of the same networks (tests/golden/make_coba_fp64.py),
+    over 8 connectivity seeds: a single 2 s run is one realisation of a chaotic network
+    whose rate and ISI CV move by 1-3 % under a one-ulp perturbation (chaos_stats), so the
+    criterion is on the means over the seeds; per-seed differences are printed."""
+    with open(GOLD) as f:
+        gold = json.load(f)
+    rows = []
+    for seed, g in gold["seeds"].items():
+        d = builders.coba_vogels(seed=int(seed), monitors=(0,))
+        d.weight_format = fmt
+        print(f"[configs] coba + vogels 2 s seed {seed} weights format {fmt}", flush=True)
+        gpu = abi.Network(d)
+        gpu.run(0.0, d.groups_for(gold["seconds"]))
+        ids, times = gpu.take_spikes(0)
+        w = gpu.weights()[d.rules[0].synapses].astype(np.float64)
+        gpu.close()
+        hist = histogram(w, np.asarray(g["edges"]))
+        l1 = hist_l1(hist, g["hist_bf16"] if fmt == 1 else g["hist"])
+        rows.append((len(ids), isi_cv(ids, times, d.n_neurons), w.mean(), l1, g["spikes"], g["cv"], g["w_mean"]))
+        r = rows[-1]
+        print(f"  spikes {r[0]} vs {r[4]} ({(r[0] - r[4]) / r[4]:+.2%}), CV {r[1]:.4f} vs {r[5]:.4f} "
+              f"({(r[1] - r[5]) / r[5]:+.2%}), mean weight {(r[2] - r[6]) / r[6]:+.3%}, histogram L1/2 {l1:.4f}",
+              flush=True)
+    a = np.array(rows)
+    rate, cv, wm = [(a[:, k].mean() - a[:, k + 4].mean()) / a[:, k + 4].mean() for k in range(3)]
+    l1 = a[:, 3].mean()
+    print(f"means over {len(a)} seeds: rate {rate:+.2%}, CV {cv:+.2%}, mean weight {wm:+.3%}, "
           f"histogram L1/2 {l1:.4f}", flush=True)
     t_rate, t_cv, t_mean, t_hist = COBA_TOL[fmt]
-    assert abs(rg - ro) <= t_rate * ro
-    assert abs(cg - co) <= t_cv * co
-    assert wo.std() > 0.01 * wo.mean(), "weights did not move"
-    assert abs(wg.mean() - wo.mean()) <= t_mean * wo.mean()
+    assert abs(rate) <= t_rate
+    assert abs(cv) <= t_cv
+    assert abs(wm) <= t_mean
     assert l1 <= t_hist
