@@ -454,6 +454,11 @@ struct PropArgs {
   int N, L, G, B, cap, tidx;
 };
 
+// A row is shared by kRowWaves wavefronts (wavefront c of a row takes its synapses
+// c*64 + lane, stepping by kRowWaves*64), so that a few long rows still keep enough
+// loads in flight: each lane's iterations are chains of dependent loads.
+constexpr int kRowWaves = 8;
+
 __global__ __launch_bounds__(256) void k_propagate(PropArgs a) {
   const int lane = (int)__lane_id();
   const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) / kWave);
@@ -465,11 +470,12 @@ __global__ __launch_bounds__(256) void k_propagate(PropArgs a) {
     int n = a.counts[g];
     if (n > a.cap) n = a.cap;
     const uint32_t* list = a.lists + (size_t)g * a.cap;
-    for (int j = wave; j < n; j += nwaves) {
+    for (int w = wave; w < n * kRowWaves; w += nwaves) {
+      const int j = w / kRowWaves, c = w - j * kRowWaves;
       const uint32_t u = list[j];
       const int64_t s0 = a.rowptr[u], s1 = a.rowptr[u + 1];
-      if (lane == 0) ev += (unsigned long long)(s1 - s0);
-      for (int64_t s = s0 + lane; s < s1; s += kWave) {
+      if (lane == 0 && c == 0) ev += (unsigned long long)(s1 - s0);
+      for (int64_t s = s0 + c * kWave + lane; s < s1; s += kRowWaves * kWave) {
         const int post = a.tgt[s];
         const int d = a.delay[s];
         const float sc = a.scale ? a.scale[s] : a.scale0;
@@ -610,8 +616,9 @@ struct WdEventArgs {
   const int32_t* tgt;      // CSR position -> postsynaptic neuron
   const int64_t* pre_ptr;  // [N + Nin + 1]: rule indices of presynaptic neuron u are
   const int32_t* pre_idx;  //   pre_idx[pre_ptr[u] .. pre_ptr[u+1]) (nullptr: the range itself)
-  const int64_t* post_ptr; // [N + 1]
-  const int32_t* post_idx;
+  const int64_t* post_ptr; // [N + 1]: CSR positions and presynaptic ids of the rule's
+  const uint32_t* post_pos; //   synapses onto neuron v at [post_ptr[v], post_ptr[v+1])
+  const int32_t* post_pre;
   uint32_t* act;           // listed neurons: presynaptic ids from 0, postsynaptic from N + Nin
   int* cnt;                // [2][2]: list lengths, double-buffered by `par`
   int par;
@@ -669,14 +676,15 @@ __global__ __launch_bounds__(256) void k_wd_events(WdEventArgs a) {
   const int npre = min(a.cnt[2 * a.par], U), npost = min(a.cnt[2 * a.par + 1], a.N);
   // the other buffer is next group's: clear it (nothing reads it in this group)
   if (blockIdx.x == 0 && threadIdx.x < 2) a.cnt[2 * (a.par ^ 1) + threadIdx.x] = 0;
-  for (int k = wave; k < npre + npost; k += nwaves) {
+  for (int w = wave; w < (npre + npost) * kRowWaves; w += nwaves) {
+    const int k = w / kRowWaves, c = w - k * kRowWaves;
     if (k < npre) {
       const int u = (int)a.act[k];
       const float l = u < a.N ? a.last_n[u] : a.last_in[u - a.N];
       const int pg = wd_pre_step(l, a.t, a.dt, G);
       const float xpre = a.xa[u];
       const int64_t e1 = a.pre_ptr[u + 1];
-      for (int64_t e = a.pre_ptr[u] + lane; e < e1; e += kWave) {
+      for (int64_t e = a.pre_ptr[u] + c * kWave + lane; e < e1; e += kRowWaves * kWave) {
         const int64_t j = a.pre_idx ? (int64_t)a.pre_idx[e] : e;
         const int64_t s = a.pos[j];
         const int post = a.tgt[s];
@@ -689,13 +697,12 @@ __global__ __launch_bounds__(256) void k_wd_events(WdEventArgs a) {
       const int qg = wd_post_step(a.last_n[v], a.t, a.dt, G);
       const float xpost = a.xb[v];
       const int64_t e1 = a.post_ptr[v + 1];
-      for (int64_t e = a.post_ptr[v] + lane; e < e1; e += kWave) {
-        const int64_t j = a.post_idx[e];
-        const int u = a.pre[j];
+      for (int64_t e = a.post_ptr[v] + c * kWave + lane; e < e1; e += kRowWaves * kWave) {
+        const int u = a.post_pre[e];
         const float l = u < a.N ? a.last_n[u] : a.last_in[u - a.N];
         const int pg = wd_pre_step(l, a.t, a.dt, G);
         if (pg >= 0) continue;              // updated from the presynaptic side
-        const int64_t s = a.pos[j];
+        const int64_t s = (int64_t)a.post_pos[e];
         const float w = wload(a.wts, s);
         wupdate(a.wts, s, w, wd_weight(w, a.xa[u], xpost, pg, qg, a.tab, G), 0.0f, __builtin_inff());
       }

@@ -119,7 +119,8 @@ struct RuleHost {
   int64_t* d_ev_pre_ptr = nullptr;
   int32_t* d_ev_pre_idx = nullptr;   // nullptr: rule order already groups by presynaptic neuron
   int64_t* d_ev_post_ptr = nullptr;
-  int32_t* d_ev_post_idx = nullptr;
+  uint32_t* d_ev_post_pos = nullptr;  // CSR position of each synapse, grouped by postsynaptic neuron
+  int32_t* d_ev_post_pre = nullptr;   // its presynaptic unit
   float* d_xn_a = nullptr;           // stored per-neuron traces (FP32 or binary16)
   float* d_xn_b = nullptr;
   uint16_t* d_xn_a16 = nullptr;
@@ -313,7 +314,8 @@ void free_device(spk_network* n) {
   for (auto& r : n->rules) {
     dfree(r.d_pos); dfree(r.d_pre); dfree(r.d_ta); dfree(r.d_tb); dfree(r.d_tab); dfree(r.d_part);
     dfree(r.d_ta16); dfree(r.d_tb16);
-    dfree(r.d_ev_pre_ptr); dfree(r.d_ev_pre_idx); dfree(r.d_ev_post_ptr); dfree(r.d_ev_post_idx);
+    dfree(r.d_ev_pre_ptr); dfree(r.d_ev_pre_idx); dfree(r.d_ev_post_ptr); dfree(r.d_ev_post_pos);
+    dfree(r.d_ev_post_pre);
     dfree(r.d_xn_a); dfree(r.d_xn_b); dfree(r.d_xn_a16); dfree(r.d_xn_b16);
     dfree(r.d_xcur_a); dfree(r.d_xcur_b); dfree(r.d_act); dfree(r.d_act_cnt);
   }
@@ -503,14 +505,14 @@ int launch_wd_events(spk_network* n, RuleHost& r, float t) {
   a.last_n = n->d_last_n; a.last_in = n->d_last_in; a.tab = r.d_tab;
   a.pos = r.d_pos; a.pre = r.d_pre; a.tgt = n->d_tgt;
   a.pre_ptr = r.d_ev_pre_ptr; a.pre_idx = r.d_ev_pre_idx;
-  a.post_ptr = r.d_ev_post_ptr; a.post_idx = r.d_ev_post_idx;
+  a.post_ptr = r.d_ev_post_ptr; a.post_pos = r.d_ev_post_pos; a.post_pre = r.d_ev_post_pre;
   a.act = r.d_act; a.cnt = r.d_act_cnt; a.par = r.par;
   a.N = n->N; a.Nin = n->Nin; a.G = n->G; a.nearest = r.wd.nearest_spike_only;
   a.t = t; a.dt = n->dt;
   hipLaunchKernelGGL(spk::k_wd_neurons, dim3(grid_for((int64_t)n->nu() + n->N, 256)), dim3(256), 0,
                      n->stream, a);
   SPK_HIP(hipGetLastError());
-  hipLaunchKernelGGL(spk::k_wd_events, dim3(512), dim3(256), 0, n->stream, a);
+  hipLaunchKernelGGL(spk::k_wd_events, dim3(2048), dim3(256), 0, n->stream, a);
   SPK_HIP(hipGetLastError());
   r.par ^= 1;
   r.syn_tr = false;
@@ -1380,7 +1382,7 @@ int spk_prepare(spk_network* n, int G) {
     if ((rc = upload(r.d_part, r.part.data(), (size_t)n->P + 1, st))) return rc;
     // the weight-dependent rule's spike-driven form: its rule indices grouped by
     // presynaptic neuron (already so in CSR order) and by postsynaptic neuron
-    r.ev_ok = r.kind == 0 && P > 0 && P < (size_t)INT32_MAX;
+    r.ev_ok = r.kind == 0 && P > 0 && P < (size_t)INT32_MAX && S < (int64_t)UINT32_MAX;
     if (r.ev_ok) {
       const int U = n->nu();
       std::vector<int64_t> pp((size_t)U + 1, 0), qp((size_t)n->N + 1, 0);
@@ -1392,17 +1394,24 @@ int spk_prepare(spk_network* n, int G) {
       }
       for (int u = 0; u < U; ++u) pp[u + 1] += pp[u];
       for (int v = 0; v < n->N; ++v) qp[v + 1] += qp[v];
-      std::vector<int32_t> qi(P);
+      std::vector<uint32_t> qs(P);
+      std::vector<int32_t> qu(P);
       {
         std::vector<int64_t> fill(qp.begin(), qp.end() - 1);
-        for (size_t j = 0; j < P; ++j) qi[fill[n->h_post[r.syn[r.order[j]]]]++] = (int32_t)j;
+        for (size_t j = 0; j < P; ++j) {
+          const int64_t k = fill[n->h_post[r.syn[r.order[j]]]]++;
+          qs[k] = (uint32_t)pos[j];
+          qu[k] = pre[j];
+        }
       }
       if ((rc = dalloc(&r.d_ev_pre_ptr, (size_t)U + 1))) return rc;
       if ((rc = dalloc(&r.d_ev_post_ptr, (size_t)n->N + 1))) return rc;
-      if ((rc = dalloc(&r.d_ev_post_idx, P))) return rc;
+      if ((rc = dalloc(&r.d_ev_post_pos, P))) return rc;
+      if ((rc = dalloc(&r.d_ev_post_pre, P))) return rc;
       if ((rc = upload(r.d_ev_pre_ptr, pp.data(), pp.size(), st))) return rc;
       if ((rc = upload(r.d_ev_post_ptr, qp.data(), qp.size(), st))) return rc;
-      if ((rc = upload(r.d_ev_post_idx, qi.data(), P, st))) return rc;
+      if ((rc = upload(r.d_ev_post_pos, qs.data(), P, st))) return rc;
+      if ((rc = upload(r.d_ev_post_pre, qu.data(), P, st))) return rc;
       if (!grouped) {
         std::vector<int32_t> pi(P);
         std::vector<int64_t> fill(pp.begin(), pp.end() - 1);
