@@ -11,7 +11,8 @@ the reference's truncated group count).  Synaptic events = recurrent deliveries
 
 Second quantity of the metric ("sweep" in the JSON line): 256 Brunel networks of uneven
 size, 1 s each, in one batched persistent launch versus one network per launch, with the
-batched FP32 rates checked against the FP64 oracle on a sample of the points.
+batched FP32 rates checked against the FP64 oracle on a sample of the points.  Also
+reported: the surrogate MLP kernels ("mlp") and 2 s of the large plastic network ("large").
 
 Prints one JSON line with the metric, the roofline of the dominant kernel and the CPU
 baseline (the FP32 oracle on the host cores, on a bounded sample of the same network).
@@ -172,6 +173,49 @@ def mlp_leg(log, members=8, rows=1 << 20, batch=4096, steps=200):
     }
 
 
+def large_leg(seconds, log):
+    """Large plastic network at full size: 5e5 LIF neurons (80 % excitatory), 1000 random
+    outgoing synapses each (5e8 synapses, delays 1..20 steps so G = 1, FP16 weights),
+    2.5e4 Poisson inputs at 10 Hz with 1000 targets each, nearest-spike weight-dependent
+    STDP on the 3.2e8 E->E synapses; dt = 0.1 ms.  Per-phase kernels (the ring slice of a
+    partition does not fit in LDS), weight-dependent rule driven by spikes.  Host-built
+    connectivity (seed 42).  Times are HIP events on the network's stream, after 20
+    warm-up steps (the first runs the rule's per-synapse form once)."""
+    from spike_amd import abi, builders
+    from spike_amd.netdesc import WEIGHTS_FP16
+    t0 = time.perf_counter()
+    d = builders.large_plastic(monitors=())
+    d.weight_format = WEIGHTS_FP16
+    build_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    net = abi.Network(d)
+    prepare_s = time.perf_counter() - t0
+    n, n_in, S, P = d.n_neurons, d.n_inputs, d.n_synapses, len(d.rules[0].synapses)
+    warm = 20
+    t = float(d.group_time(0.0, warm))
+    groups = d.groups_for(seconds)
+    del d
+    log(f"large: built in {build_s:.1f} s, prepared in {prepare_s:.1f} s")
+    net.run(0.0, warm)
+    tm = net.run(t, groups)
+    net.close()
+    steps = groups   # G = 1
+    ms = tm.total_ms
+    log(f"large: {seconds} s simulated in {ms:.1f} ms ({ms / steps * 1e3:.1f} us per step), "
+        f"{tm.spikes_neurons} spikes, {tm.synaptic_events} deliveries")
+    return {
+        "workload": "large plastic net: 5e5 LIF (80/20), 1000 random out-synapses each, delays 1-20 steps, "
+                    "FP16 weights, 2.5e4 Poisson inputs at 10 Hz, nearest-spike WD STDP on E->E, dt 0.1 ms",
+        "n_neurons": n, "n_inputs": n_in, "n_synapses": S, "plastic_synapses": P,
+        "simulated_s": seconds, "steps": steps, "ms": round(ms, 2),
+        "us_per_step": round(ms / steps * 1e3, 2),
+        "syn_events_per_s": round(tm.synaptic_events / (ms / 1e3), 1),
+        "rate_hz": round(tm.spikes_neurons / n / (steps * 1e-4), 3),
+        "spikes": tm.spikes_neurons, "input_spikes": tm.spikes_inputs, "deliveries": tm.synaptic_events,
+        "build_s": round(build_s, 1), "prepare_s": round(prepare_s, 1), "path": "per-phase kernels",
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -189,6 +233,8 @@ def main():
     ap.add_argument("--sweep-cpu-sample", type=int, default=8,
                     help="sweep points re-run on the FP64 oracle (rate-error gate, CPU baseline)")
     ap.add_argument("--no-mlp", action="store_true", help="skip the surrogate MLP kernels")
+    ap.add_argument("--large-seconds", type=float, default=2.0,
+                    help="simulated seconds of the large plastic network (0: skip it)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run a plain device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -300,6 +346,9 @@ def main():
                                  0 if args.no_cpu else args.sweep_cpu_sample, threads, log)
     if not args.no_mlp:
         out["mlp"] = mlp_leg(lambda m: print(m, file=sys.stderr, flush=True))
+    if args.large_seconds > 0:
+        net.close()
+        out["large"] = large_leg(args.large_seconds, lambda m: print(m, file=sys.stderr, flush=True))
     print(json.dumps(out))
 
 
