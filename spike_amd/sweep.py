@@ -217,13 +217,15 @@ def plan_from_work(points, events_per_group, target_events, neurons_per_wg_max, 
 
 
 def autotune(points: Sequence[SweepPoint], probe_seconds: float = 0.02,
-             targets=(2e4, 5e4, 1e5, 2e5), neurons_per_wg_max: int = 384, log=None) -> Dict:
+             targets=(2e4, 5e4, 1e5, 2e5, 5e5), neurons_per_wg_max=(384, 1024), log=None) -> Dict:
     """The project's tiling autotuner.  (1) A probe run of the whole sweep with the
     size-only plan measures every network's deliveries per group and its time per group;
     (2) a least-squares fit of time = fixed + per_event * events/workgroup gives the cost
-    model; (3) every candidate events-per-workgroup target is turned into a plan
-    (plan_from_work) and timed on a probe run; the fastest plan is returned."""
-    base = default_plan(points, neurons_per_wg_max)
+    model; (3) every candidate (events-per-workgroup target, most neurons per workgroup)
+    is turned into a plan (plan_from_work) and timed on a probe run; the fastest plan is
+    returned.  Fewer, larger partitions pay the per-group fixed cost fewer times; a
+    partition whose ring slice would not fit in LDS is split further when built."""
+    base = default_plan(points, neurons_per_wg_max[0])
     r0 = run_batched(points, probe_seconds, base, profiling=True)
     groups = np.array([p.groups(probe_seconds) for p in points], dtype=np.float64)
     ev = r0.extra["events"] / np.maximum(groups, 1.0)
@@ -237,8 +239,8 @@ def autotune(points: Sequence[SweepPoint], probe_seconds: float = 0.02,
     best = dict(plan=base, ms=r0.ms)
     if log:
         log(f"autotune probe {base['name']}: {r0.ms:.2f} ms; fit {fixed:.2f} us + {per_ev * 1e3:.3f} ns/event")
-    for tgt in targets:
-        plan = plan_from_work(points, ev, tgt, neurons_per_wg_max, wg, fixed, per_ev)
+    for npw, tgt in [(npw, tgt) for npw in neurons_per_wg_max for tgt in targets]:
+        plan = plan_from_work(points, ev, tgt, npw, wg, fixed, per_ev, name=f"work/{tgt:.0f}/{npw}")
         try:
             r = run_batched(points, probe_seconds, plan)
         except abi.SpikeError as e:
