@@ -94,6 +94,7 @@ def sweep_leg(n_points, seconds, cpu_sample, threads, log):
         "speedup_vs_plain": round(rp.ms / rb.ms, 3),
         "batched_ms": round(rb.ms, 3), "plain_ms": round(rp.ms, 3),
         "plan": best["plan"]["name"], "workgroups": rb.config["workgroups"],
+        "threads_per_workgroup": rb.config["threads"],
         "autotune_s": round(tune_s, 2), "cost_fit_us": best["fit"],
         "batched_vs_plain_rate_max_rel_diff": float(np.max(np.abs(rb.rate - rp.rate) / np.maximum(rp.rate, 1e-12))),
         "mean_rate_hz": round(float(rb.rate.mean()), 3),

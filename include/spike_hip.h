@@ -97,6 +97,7 @@ typedef struct {
   int record_bytes;          /* bytes the persistent kernel reads per delivery: 8 for the
                                 full records {target|delay|label, weight}; 2 for the compact
                                 form (16-bit target; weight and delay per source type) */
+  int lds_bytes;             /* LDS per workgroup of the persistent kernel (0: not used) */
 } spk_info;
 
 /* Per-run timing of the last spk_run call, from HIP events on the network's stream. */
@@ -347,6 +348,10 @@ int spk_batch_set_cost_model(spk_batch* batch, double fixed_us, double per_neuro
  * run; replaces the model above for scheduling.  NULL returns to the model. */
 int spk_batch_set_costs(spk_batch* batch, const double* cost_per_group_us);
 int spk_batch_workgroups(const spk_batch* batch);
+/* Threads per workgroup of the batch's launches: 512 (two workgroups per compute unit)
+ * when every partition has at most 512 neurons and two workgroups' LDS fits a compute
+ * unit, else 1024. */
+int spk_batch_threads(const spk_batch* batch);
 /* Network k runs ngroups[k] groups from starts[k]; returns when all have finished. */
 int spk_batch_run(spk_batch* batch, const float* starts, const int64_t* ngroups, int flags);
 /* Kernel time of the last run (HIP events) and the scheduler's estimate. */

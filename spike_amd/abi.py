@@ -48,7 +48,7 @@ class Info(C.Structure):
                 ("ring_slots", C.c_int), ("history_words", C.c_int), ("frac_bits", C.c_int),
                 ("min_delay", C.c_int), ("max_delay", C.c_int), ("n_synapses", C.c_int64),
                 ("n_rules", C.c_int), ("n_monitors", C.c_int), ("partitions", C.c_int),
-                ("fused", C.c_int), ("record_bytes", C.c_int)]
+                ("fused", C.c_int), ("record_bytes", C.c_int), ("lds_bytes", C.c_int)]
 
 
 class Timing(C.Structure):
@@ -125,6 +125,7 @@ SIGNATURES = {
     "spk_batch_destroy": (C.c_int, [P]),
     "spk_batch_set_cost_model": (C.c_int, [P, C.c_double, C.c_double, C.c_double]),
     "spk_batch_workgroups": (C.c_int, [P]),
+    "spk_batch_threads": (C.c_int, [P]),
     "spk_batch_set_costs": (C.c_int, [P, C.POINTER(C.c_double)]),
     "spk_batch_run": (C.c_int, [P, C.POINTER(C.c_float), C.POINTER(C.c_int64), C.c_int]),
     "spk_batch_last_ms": (C.c_int, [P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
@@ -448,6 +449,10 @@ class Batch:
     @property
     def workgroups(self) -> int:
         return self.lib.spk_batch_workgroups(self.h)
+
+    @property
+    def threads(self) -> int:
+        return self.lib.spk_batch_threads(self.h)
 
     def set_cost_model(self, fixed_us: float, per_neuron_us: float, per_synapse_us: float):
         check(self.lib.spk_batch_set_cost_model(self.h, fixed_us, per_neuron_us, per_synapse_us))

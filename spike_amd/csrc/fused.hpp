@@ -27,6 +27,7 @@
 namespace spk {
 
 constexpr int kFusedBlock = 1024;      // threads per workgroup
+constexpr int kBatchSmallBlock = 512;  // threads per workgroup of the two-per-CU batch kernel
 constexpr int kMaxPartitions = 1024;    // workgroups of one network
 constexpr uint32_t kIdMask = 0x00FFFFFFu;
 constexpr int kBarGroups = 8;          // level-1 barrier counters (blockIdx % 8)
@@ -257,6 +258,7 @@ __device__ __forceinline__ void group_time(const FusedArgs& a, int s, float& t, 
 // `scdf` the count thresholds in LDS and `sguide` their guide table: sguide[j] is the count
 // of u = j * 2^24, a lower bound of the count of every u in [j * 2^24, (j + 1) * 2^24)
 // (Chen & Asau 1974), so the upward scan from it takes about one step.
+template <int BS>
 __device__ __forceinline__ void draw_group(const FusedArgs& a, int s, uint8_t* ext,
                                            unsigned long long* imask, int nlo, int nown, int ilo,
                                            int iown, unsigned long long& st_ext, uint32_t* su,
@@ -274,7 +276,7 @@ __device__ __forceinline__ void draw_group(const FusedArgs& a, int s, uint8_t* e
     // (1) one Philox call per (neuron, block) into su[i*G + g]; (2) every (neuron, step)
     // counts its events in parallel: the number of thresholds C[k] <= u, scanned upwards
     // from the guide entry of u until the first C[k] > u (C is non-decreasing)
-    for (int k = tid; k < nown * nbk; k += kFusedBlock) {
+    for (int k = tid; k < nown * nbk; k += BS) {
       const int i = k / nbk;
       const int64_t blk = b0 + (k - i * nbk);
       const u32x4 r = draw_block(a.seed, blk, (uint32_t)(nlo + i), kStreamExternal);
@@ -285,7 +287,7 @@ __device__ __forceinline__ void draw_group(const FusedArgs& a, int s, uint8_t* e
       }
     }
     __syncthreads();
-    for (int k = tid; k < nown * G; k += kFusedBlock) {
+    for (int k = tid; k < nown * G; k += BS) {
       const uint32_t u = su[k];
       int cnt = sguide[u >> 24];
       while (cnt < a.ext_k && u >= scdf[cnt]) ++cnt;
@@ -294,9 +296,9 @@ __device__ __forceinline__ void draw_group(const FusedArgs& a, int s, uint8_t* e
     }
   }
   if (iown > 0) {
-    for (int k = tid; k < iown; k += kFusedBlock) imask[k] = 0ull;
+    for (int k = tid; k < iown; k += BS) imask[k] = 0ull;
     __syncthreads();
-    for (int k = tid; k < iown * nbk; k += kFusedBlock) {
+    for (int k = tid; k < iown * nbk; k += BS) {
       const int i = k / nbk;
       const int64_t blk = b0 + (k - i * nbk);
       const u32x4 r = draw_block(a.seed, blk, (uint32_t)(ilo + i), kStreamInput);
@@ -467,6 +469,7 @@ __device__ __forceinline__ void deliver_quad_compact(const FusedArgs& a, const u
 }
 
 // Plasticity of the synapses onto partition p (one group).
+template <int BS>
 __device__ __forceinline__ void fused_plasticity(const FusedRule* rules, int n_rules, int p, int tidx,
                                               float t) {
   for (int r = 0; r < n_rules; ++r) {
@@ -475,7 +478,7 @@ __device__ __forceinline__ void fused_plasticity(const FusedRule* rules, int n_r
     ra.t = t;
     const int kind = rules[r].kind;
     const int64_t j0 = rules[r].part[p], j1 = rules[r].part[p + 1];
-    for (int64_t j = j0 + threadIdx.x; j < j1; j += kFusedBlock) {
+    for (int64_t j = j0 + threadIdx.x; j < j1; j += BS) {
       if (kind == 0) stdp_wd_one(ra, j);
       else if (kind == 1) stdp_vogels_one(ra, j);
       else stdp_evans_one(ra, j);
@@ -522,7 +525,7 @@ __device__ __forceinline__ int partition_of_block(int b, int P) {
 
 // The group loop of one network for partition p, run by the workgroup that is member vb of
 // the network's P workgroups.
-template <int KIND, bool L1, bool PLASTIC>
+template <int KIND, bool L1, bool PLASTIC, int BS>
 __device__ __forceinline__ void fused_body(const FusedArgs& a, const int p, const int vb, unsigned char* smem) {
   const int tid = threadIdx.x;
   const int lane = (int)__lane_id();
@@ -566,15 +569,15 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const int p, cons
     for (int j = 0; j < BL; ++j) {
       const int s = t0 + (j - r0 + BL) % BL;
       const int64_t* src = a.ring + (size_t)(s % B) * NL + (size_t)nlo * L;
-      for (int k = tid; k < nownL; k += kFusedBlock) sring[(size_t)j * nownL + k] = src[k];
+      for (int k = tid; k < nownL; k += BS) sring[(size_t)j * nownL + k] = src[k];
     }
   }
-  for (int k = tid; k < W * nown; k += kFusedBlock) {
+  for (int k = tid; k < W * nown; k += BS) {
     const int wd = k / nown, i = k - wd * nown;
     shist[k] = a.hist_n[(size_t)wd * a.N + nlo + i];
   }
   if (a.st.on) {
-    for (int i = tid; i < nown; i += kFusedBlock) {
+    for (int i = tid; i < nown; i += BS) {
       sst_cnt[i] = a.st.count[nlo + i];
       sst_nisi[i] = a.st.nisi[nlo + i];
       sst_prev[i] = a.st.prev[nlo + i];
@@ -583,7 +586,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const int p, cons
     }
     if (tid < 64) sst_g[tid] = 0u;
   }
-  // per-lane neuron state (lane i owns neuron nlo+i; nown <= kFusedBlock)
+  // per-lane neuron state (lane i owns neuron nlo+i; nown <= BS)
   const bool own = tid < nown;
   const int gi = nlo + (own ? tid : 0);
   float V = 0.f, last = 0.f, cv0 = 0.f, omc = 0.f, cib = 0.f, thr = 0.f, vr = 0.f, rp = 0.f;
@@ -611,7 +614,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const int p, cons
   const int bits = W * 32;
   const int nown_waves = ((nown + kWave - 1) / kWave) * kWave;
   if (tid == 0) scounter[0] = 0;
-  for (int j = tid; j < a.ext_k; j += kFusedBlock) scdf[j] = a.ext_cdf[j];
+  for (int j = tid; j < a.ext_k; j += BS) scdf[j] = a.ext_cdf[j];
   __syncthreads();
   if (tid < 256) {
     const uint32_t u0 = (uint32_t)tid << 24;
@@ -620,7 +623,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const int p, cons
     sguide[tid] = (uint8_t)c;
   }
   __syncthreads();
-  draw_group(a, 0, sext, simask, nlo, nown, ilo, iown, st_ext, (uint32_t*)sinp, scdf, sguide);
+  draw_group<BS>(a, 0, sext, simask, nlo, nown, ilo, iown, st_ext, (uint32_t*)sinp, scdf, sguide);
   __syncthreads();
   PhaseClock clk;
   clk.start(a, s_prof, vb);
@@ -637,7 +640,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const int p, cons
     const int b0 = tidx % bits;
     const int slot0 = tidx % BL;
     // -- A. Poisson inputs: append this group's spikes, update histories ------------
-    for (int i = tid; i < iown; i += kFusedBlock) {
+    for (int i = tid; i < iown; i += BS) {
       const unsigned long long m = imask[i];
       const int c = __popcll(m);
       int pos = c ? atomicAdd(scounter, c) : 0;
@@ -663,7 +666,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const int p, cons
     // The ring slots of the group's steps are complete (every delay is >= G): read and
     // clear them in parallel.  Voltage synapses keep the label sum, conductance synapses
     // the per-label values.
-    for (int k = tid; k < nown * G; k += kFusedBlock) {
+    for (int k = tid; k < nown * G; k += BS) {
       const int i = k / G, g = k - i * G;
       int slot = slot0 + g;
       if (slot >= BL) slot -= BL;
@@ -816,11 +819,11 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const int p, cons
     // more spikes than the exchange list holds: stop the launch with error 2
     if (tid == 0 && scounter[0] > a.xcap) atomicExch(a.error, 2);
     uint32_t* xl = a.xlist + ((size_t)par * a.P + p) * a.xstride;
-    for (int k = tid; k < nx; k += kFusedBlock) store_coherent(xl + 1 + k, sxbuf[k]);
+    for (int k = tid; k < nx; k += BS) store_coherent(xl + 1 + k, sxbuf[k]);
     if (tid == 0) store_coherent(xl, (uint32_t)nx);
     if (PLASTIC) {
       if (own) a.last_n[gi] = last;
-      for (int k = tid; k < W * nown; k += kFusedBlock) {
+      for (int k = tid; k < W * nown; k += BS) {
         const int wd = k / nown, i = k - wd * nown;
         a.hist_n[(size_t)wd * a.N + nlo + i] = shist[k];
       }
@@ -831,7 +834,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const int p, cons
     for (int mi = 0; mi < kMaxFusedMons; ++mi) {
       if (mi >= a.n_mons) continue;
       const FusedMon m = a.mon[mi];
-      for (int k0 = 0; k0 < nx; k0 += kFusedBlock) {
+      for (int k0 = 0; k0 < nx; k0 += BS) {
         const int k = k0 + tid;
         const uint32_t e = k < nx ? sxbuf[k] : 0u;
         const uint32_t id = e & kIdMask;
@@ -858,7 +861,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const int p, cons
       scounter[0] = 0;
     }
     if (s + 1 < a.ngroups)
-      draw_group(a, s + 1, sext + (size_t)(par ^ 1) * a.nown_max * G,
+      draw_group<BS>(a, s + 1, sext + (size_t)(par ^ 1) * a.nown_max * G,
                  simask + (size_t)(par ^ 1) * a.iown_max, nlo, nown, ilo, iown, st_ext,
                  (uint32_t*)sinp, scdf, sguide);
     clk.lap(a, 3);
@@ -868,7 +871,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const int p, cons
     clk.lap(a, 4);
     // -- E. plasticity of the synapses onto this partition ---------------------------
     if (PLASTIC) {
-      fused_plasticity(a.rules, a.n_rules, p, tidx, t);
+      fused_plasticity<BS>(a.rules, a.n_rules, p, tidx, t);
       __syncthreads();
     }
     clk.lap(a, 5);
@@ -876,7 +879,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const int p, cons
     {
       const uint32_t* xbase = a.xlist + (size_t)par * a.P * a.xstride;
       // (1) heads of all source blocks: count and first entries, one coalesced trip
-      for (int k = tid; k < a.P * kHeadWords; k += kFusedBlock) {
+      for (int k = tid; k < a.P * kHeadWords; k += BS) {
         const int src = k / kHeadWords, wd = k - src * kHeadWords;
         shead[k] = (wd <= a.xcap) ? load_coherent(xbase + (size_t)src * a.xstride + wd) : 0u;
       }
@@ -914,7 +917,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const int p, cons
         //     their start with the next one); the first kHeadWords-1 entries of a source
         //     come from its head in LDS
         const int nst = min(kEntCap, total - r0);
-        for (int e = tid; e < nst; e += kFusedBlock) {
+        for (int e = tid; e < nst; e += BS) {
           const int gidx = r0 + e;
           int lo = 0, hi = a.P;
           while (hi - lo > 1) {
@@ -931,7 +934,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const int p, cons
         // (4) quads of lanes over the spikes, kSpikesPerQuad at a time
         const int nr = min(kEntCap, total - r0);
         const int quad = tid / kQuad, q = tid % kQuad;
-        const int nquads = kFusedBlock / kQuad;
+        const int nquads = BS / kQuad;
         for (int j0 = quad * kSpikesPerQuad; j0 < nr; j0 += nquads * kSpikesPerQuad) {
           const int n = min(kSpikesPerQuad, nr - j0);
           if (a.syn16 != nullptr)
@@ -966,15 +969,15 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const int p, cons
       int64_t* dst = a.ring + (size_t)k * NL + (size_t)nlo * L;
       const bool live = s - t1 < BL;
       const int64_t* src = sring + (size_t)(s % BL) * nownL;
-      for (int j = tid; j < nownL; j += kFusedBlock) dst[j] = live ? src[j] : 0;
+      for (int j = tid; j < nownL; j += BS) dst[j] = live ? src[j] : 0;
     }
   }
-  for (int k = tid; k < W * nown; k += kFusedBlock) {
+  for (int k = tid; k < W * nown; k += BS) {
     const int wd = k / nown, i = k - wd * nown;
     a.hist_n[(size_t)wd * a.N + nlo + i] = shist[k];
   }
   if (a.st.on) {
-    for (int i = tid; i < nown; i += kFusedBlock) {
+    for (int i = tid; i < nown; i += BS) {
       a.st.count[nlo + i] = sst_cnt[i];
       a.st.nisi[nlo + i] = sst_nisi[i];
       a.st.prev[nlo + i] = sst_prev[i];
@@ -995,25 +998,27 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const int p, cons
 template <int KIND, bool L1, bool PLASTIC>
 __global__ __launch_bounds__(kFusedBlock) void k_fused(FusedArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
-  fused_body<KIND, L1, PLASTIC>(a, partition_of_block(blockIdx.x, gridDim.x), blockIdx.x, smem);
+  fused_body<KIND, L1, PLASTIC, kFusedBlock>(a, partition_of_block(blockIdx.x, gridDim.x), blockIdx.x, smem);
 }
 
 // Many independent networks in one cooperative launch.  Workgroup b runs the tasks
 // tasks[task_off[b] .. task_off[b+1]) in order, each the whole group loop of one partition
 // of one network (x = network, y = partition).  Every workgroup's list is in increasing
 // network order, so a workgroup waiting at a network's barrier waits only for workgroups
-// that are finishing earlier networks: every wait ends.
-template <int KIND, bool L1, bool PLASTIC>
-__global__ __launch_bounds__(kFusedBlock) void k_fused_batch(const FusedArgs* __restrict__ nets,
-                                                             const int2* __restrict__ tasks,
-                                                             const int* __restrict__ task_off) {
+// that are finishing earlier networks: every wait ends.  BS threads per workgroup: with
+// kBatchSmallBlock, two workgroups share a compute unit, so one network's partition runs
+// while the other's waits at its exchange (partitions of at most BS neurons).
+template <int KIND, bool L1, bool PLASTIC, int BS>
+__global__ __launch_bounds__(BS, 4) void k_fused_batch(const FusedArgs* __restrict__ nets,
+                                                    const int2* __restrict__ tasks,
+                                                    const int* __restrict__ task_off) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int j0 = __builtin_amdgcn_readfirstlane(task_off[blockIdx.x]);
   const int j1 = __builtin_amdgcn_readfirstlane(task_off[blockIdx.x + 1]);
   for (int j = j0; j < j1; ++j) {
     const int k = __builtin_amdgcn_readfirstlane(tasks[j].x);
     const int p = __builtin_amdgcn_readfirstlane(tasks[j].y);
-    fused_body<KIND, L1, PLASTIC>(nets[k], p, p, smem);
+    fused_body<KIND, L1, PLASTIC, BS>(nets[k], p, p, smem);
     __syncthreads();
   }
 }

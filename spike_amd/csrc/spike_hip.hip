@@ -1490,6 +1490,7 @@ int spk_get_info(const spk_network* n, spk_info* info) {
   info->partitions = n->P;
   info->fused = n->fused_ok ? 1 : 0;
   info->record_bytes = (n->compact && n->rules.empty()) ? 2 : 8;
+  info->lds_bytes = n->fused_ok ? (int)n->fused_lds : 0;
   return SPK_OK;
 }
 
@@ -2073,6 +2074,7 @@ struct spk_batch {
   int kind = 0;
   bool l1 = true;
   int nb = 0;                 // workgroups of the launch
+  int bs = spk::kFusedBlock;  // threads per workgroup
   size_t lds = 0;
   hipStream_t stream = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -2088,10 +2090,15 @@ struct spk_batch {
 
 namespace {
 
-const void* batch_kernel(const spk_batch* b) {
+template <int BS>
+const void* batch_kernel_bs(const spk_batch* b) {
   if (b->kind == SPK_SYN_VOLTAGE)
-    return b->l1 ? (const void*)spk::k_fused_batch<0, true, false> : (const void*)spk::k_fused_batch<0, false, false>;
-  return b->l1 ? (const void*)spk::k_fused_batch<1, true, false> : (const void*)spk::k_fused_batch<1, false, false>;
+    return b->l1 ? (const void*)spk::k_fused_batch<0, true, false, BS> : (const void*)spk::k_fused_batch<0, false, false, BS>;
+  return b->l1 ? (const void*)spk::k_fused_batch<1, true, false, BS> : (const void*)spk::k_fused_batch<1, false, false, BS>;
+}
+
+const void* batch_kernel(const spk_batch* b, int bs) {
+  return bs == spk::kBatchSmallBlock ? batch_kernel_bs<spk::kBatchSmallBlock>(b) : batch_kernel_bs<spk::kFusedBlock>(b);
 }
 
 }  // namespace
@@ -2112,10 +2119,25 @@ int spk_batch_create(spk_network* const* nets, int n_nets, int workgroups, spk_b
     b->lds = std::max(b->lds, n->fused_lds);
     b->nets.push_back(n);
   }
-  const void* fn = batch_kernel(b);
+  const void* fn = batch_kernel(b, spk::kFusedBlock);
   if (b->lds > 65536) SPK_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)b->lds));
   int per_cu = 0;
   SPK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, spk::kFusedBlock, b->lds));
+  // smaller workgroups when two of them fit a compute unit (every partition at most
+  // kBatchSmallBlock neurons, LDS and registers of two): one partition runs while the
+  // other waits at its network's exchange
+  int pmax_own = 0;
+  for (auto* n : b->nets) pmax_own = std::max(pmax_own, n->nown_max);
+  if (pmax_own <= spk::kBatchSmallBlock) {
+    const void* fs = batch_kernel(b, spk::kBatchSmallBlock);
+    if (b->lds > 65536) SPK_HIP(hipFuncSetAttribute(fs, hipFuncAttributeMaxDynamicSharedMemorySize, (int)b->lds));
+    int per_cu_s = 0;
+    SPK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_s, fs, spk::kBatchSmallBlock, b->lds));
+    if (per_cu_s >= 2 * per_cu) {
+      b->bs = spk::kBatchSmallBlock;
+      per_cu = per_cu_s;
+    }
+  }
   const int cap = per_cu * nets[0]->cu_count;
   b->nb = workgroups > 0 ? std::min(workgroups, cap) : cap;
   int pmax = 0;
@@ -2164,6 +2186,8 @@ int spk_batch_set_costs(spk_batch* b, const double* cost_per_group_us) {
 }
 
 int spk_batch_workgroups(const spk_batch* b) { return b ? b->nb : 0; }
+
+int spk_batch_threads(const spk_batch* b) { return b ? b->bs : 0; }
 
 int spk_batch_run(spk_batch* b, const float* starts, const int64_t* ngroups, int flags) {
   if (!b || !starts || !ngroups) return fail(SPK_ERR_ARG, "null argument");
@@ -2230,13 +2254,13 @@ int spk_batch_run(spk_batch* b, const float* starts, const int64_t* ngroups, int
   SPK_HIP(hipMemcpyAsync(b->d_tasks, tasks.data(), tasks.size() * sizeof(int2), hipMemcpyHostToDevice, b->stream));
   SPK_HIP(hipMemcpyAsync(b->d_off, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice, b->stream));
   SPK_HIP(hipMemcpyAsync(b->d_args, args.data(), args.size() * sizeof(spk::FusedArgs), hipMemcpyHostToDevice, b->stream));
-  const void* fn = batch_kernel(b);
+  const void* fn = batch_kernel(b, b->bs);
   const spk::FusedArgs* pa = b->d_args;
   const int2* pt = b->d_tasks;
   const int* po = b->d_off;
   void* kargs[] = {(void*)&pa, (void*)&pt, (void*)&po};
   SPK_HIP(hipEventRecord(b->e0, b->stream));
-  SPK_HIP(hipLaunchCooperativeKernel(fn, dim3(b->nb), dim3(spk::kFusedBlock), kargs, (unsigned int)b->lds, b->stream));
+  SPK_HIP(hipLaunchCooperativeKernel(fn, dim3(b->nb), dim3(b->bs), kargs, (unsigned int)b->lds, b->stream));
   SPK_HIP(hipEventRecord(b->e1, b->stream));
   SPK_HIP(hipEventSynchronize(b->e1));
   float ms = 0.0f;

@@ -95,17 +95,27 @@ def brunel_params(p: SweepPoint) -> abi.BrunelParams:
         timestep=k["dt"], seed=p.seed)
 
 
-def build(p: SweepPoint, partitions: int = 0) -> abi.Network:
+# LDS of one workgroup when two share a compute unit (160 KB per CU on MI355X, less a
+# margin for the kernel's static LDS)
+LDS_HALF_CU = 80 * 1024 - 1024
+
+
+def build(p: SweepPoint, partitions: int = 0, lds_max: int = 0) -> abi.Network:
     """The point's network on the device; with partitions > 0 the count is raised until the
-    per-workgroup input ring fits in LDS."""
+    per-workgroup input ring fits in LDS (and in lds_max bytes when given)."""
     P = partitions
     while True:
         try:
-            return abi.generated_brunel(brunel_params(p), P)
+            net = abi.generated_brunel(brunel_params(p), P)
         except abi.SpikeError as e:
             if "LDS" not in str(e) or P <= 0 or P >= p.n:
                 raise
             P = min(p.n, int(P * 1.25) + 1)
+            continue
+        if lds_max <= 0 or P <= 0 or P >= p.n or net.info().lds_bytes <= lds_max:
+            return net
+        net.close()
+        P = min(p.n, int(P * 1.25) + 1)
 
 
 @dataclasses.dataclass
@@ -148,10 +158,10 @@ def run_plain(points: Sequence[SweepPoint], seconds: float, bin_steps: int = 10)
                        dict(mode="plain"))
 
 
-def _build_all(points, partitions, seconds, bin_steps, profiling=False):
+def _build_all(points, partitions, seconds, bin_steps, profiling=False, lds_max=0):
     nets = []
     for p, P in zip(points, partitions):
-        n = build(p, int(P))
+        n = build(p, int(P), lds_max)
         n.stats_begin(bin_steps, p.groups(seconds) * p.delay_steps)
         if profiling:
             n.set_profiling(True)
@@ -174,7 +184,7 @@ def run_batched(points: Sequence[SweepPoint], seconds: float, plan: Optional[Dic
     plan = plan or default_plan(points)
     t0 = time.perf_counter()
     order = list(plan["order"])
-    nets = _build_all(points, plan["partitions"], seconds, bin_steps, profiling)
+    nets = _build_all(points, plan["partitions"], seconds, bin_steps, profiling, plan.get("lds_max", 0))
     batch = abi.Batch([nets[k] for k in order], workgroups)
     if plan.get("cost") is not None:
         batch.set_costs([plan["cost"][k] for k in order])
@@ -187,13 +197,13 @@ def run_batched(points: Sequence[SweepPoint], seconds: float, plan: Optional[Dic
         dur = np.array([(lambda pr: pr["s11"] - pr["s10"])(n.phase_profile()) for n in nets])
         extra = dict(events=ev, duration_us=dur)
     parts = [n.info().partitions for n in nets]
-    wg = batch.workgroups
+    wg, threads = batch.workgroups, batch.threads
     batch.close()
     for n in nets:
         n.close()
     r = SweepResult(ms, time.perf_counter() - t0, rate, cv, sync,
-                    dict(mode="batched", plan=plan.get("name", "custom"), workgroups=wg, estimated_ms=est,
-                         partitions_total=int(sum(parts))))
+                    dict(mode="batched", plan=plan.get("name", "custom"), workgroups=wg, threads=threads,
+                         estimated_ms=est, partitions_total=int(sum(parts))))
     r.extra = dict(extra, partitions=parts)
     return r
 
@@ -213,18 +223,21 @@ def plan_from_work(points, events_per_group, target_events, neurons_per_wg_max, 
         cost.append(fixed_us + per_event_us * events_per_group[k] / P)
     total = [cost[k] * points[k].groups(1.0) * parts[k] for k in range(K)]
     order = sorted(range(K), key=lambda k: -cost[k] * points[k].groups(1.0))
-    return dict(partitions=parts, order=order, cost=cost, name=name or f"work/{target_events:.0f}")
+    return dict(partitions=parts, order=order, cost=cost, name=name or f"work/{target_events:.0f}",
+                lds_max=LDS_HALF_CU if neurons_per_wg_max <= 512 else 0)
 
 
 def autotune(points: Sequence[SweepPoint], probe_seconds: float = 0.02,
-             targets=(2e4, 5e4, 1e5, 2e5, 5e5), neurons_per_wg_max=(384, 1024), log=None) -> Dict:
+             targets=(2e4, 5e4, 1e5, 2e5, 5e5), neurons_per_wg_max=(384, 1024, 256), log=None) -> Dict:
     """The project's tiling autotuner.  (1) A probe run of the whole sweep with the
     size-only plan measures every network's deliveries per group and its time per group;
     (2) a least-squares fit of time = fixed + per_event * events/workgroup gives the cost
     model; (3) every candidate (events-per-workgroup target, most neurons per workgroup)
     is turned into a plan (plan_from_work) and timed on a probe run; the fastest plan is
     returned.  Fewer, larger partitions pay the per-group fixed cost fewer times; a
-    partition whose ring slice would not fit in LDS is split further when built."""
+    partition whose ring slice would not fit in LDS is split further when built.  Plans of
+    at most 512 neurons per workgroup also keep every partition's LDS within half a compute
+    unit's, so that the batch runs two 512-thread workgroups per compute unit."""
     base = default_plan(points, neurons_per_wg_max[0])
     r0 = run_batched(points, probe_seconds, base, profiling=True)
     groups = np.array([p.groups(probe_seconds) for p in points], dtype=np.float64)
@@ -248,7 +261,8 @@ def autotune(points: Sequence[SweepPoint], probe_seconds: float = 0.02,
             continue
         trials.append(dict(plan=plan["name"], ms=r.ms, estimated_ms=r.config["estimated_ms"]))
         if log:
-            log(f"autotune {plan['name']}: {r.ms:.2f} ms (scheduler estimate {r.config['estimated_ms']:.2f})")
+            log(f"autotune {plan['name']}: {r.ms:.2f} ms (scheduler estimate {r.config['estimated_ms']:.2f}, "
+                f"{r.config['workgroups']} x {r.config['threads']} threads)")
         if r.ms < best["ms"]:
             best = dict(plan=plan, ms=r.ms)
     best["trials"] = trials
