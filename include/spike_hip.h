@@ -144,6 +144,19 @@ int spk_set_lif(spk_network* net, const float* v0, const float* vreset,
 /* Poisson inputs.  Replaces Backend::PoissonInputSpikingNeurons::prepare
  * (CUDA PoissonInputSpikingNeurons.cu:36-47): rates[s*n_inputs + i] in Hz. */
 int spk_set_poisson_rates(spk_network* net, const float* rates, int n_stimuli);
+/* Generator inputs instead of Poisson inputs: the input population replays scheduled
+ * spikes.  Replaces Backend::GeneratorInputSpikingNeurons::prepare / reset_state (CUDA
+ * GeneratorInputSpikingNeurons.cu:14-40) and its stimuli (GeneratorInputSpikingNeurons.cpp:
+ * 28-77): stimulus s holds counts[s] spikes, (ids, times) concatenated over the stimuli,
+ * ids in [0, n_inputs), times in seconds.  In each group starting at t, entry e fires
+ * input ids[e] at sub-step g when |(t - onset + g*dt) - times[e]| < 0.5*dt
+ * (GeneratorInputSpikingNeurons.cu:106-123); an input spikes at most once per step, and
+ * the activated synapses are those of ids[e] (the reference kernel passes the entry index
+ * instead).  Before spk_prepare; the network then runs the per-phase kernels. */
+int spk_set_generator_stimuli(spk_network* net, const int32_t* counts, int n_stimuli,
+                              const int32_t* ids, const float* times);
+/* GeneratorInputSpikingNeurons::stimulus_onset_adjustment (default 0). */
+int spk_set_generator_onset(spk_network* net, float onset);
 /* InputSpikingNeurons::select_stimulus (Spike/Neurons/InputSpikingNeurons.cpp:12-19). */
 int spk_select_stimulus(spk_network* net, int stimulus);
 /* Philox key of all random streams (replaces RandomStateManager, CUDA

@@ -32,6 +32,8 @@ SIG = {
     "or_set_lif": (C.c_int, [P, fp, fp, fp, fp, fp, C.c_float, C.c_float]),
     "or_set_poisson_rates": (C.c_int, [P, fp, C.c_int]),
     "or_select_stimulus": (C.c_int, [P, C.c_int]),
+    "or_set_generator_stimuli": (C.c_int, [P, ip, C.c_int, ip, fp]),
+    "or_set_generator_onset": (C.c_int, [P, C.c_float]),
     "or_set_seed": (C.c_int, [P, C.c_uint64]),
     "or_set_external_drive": (C.c_int, [P, C.c_float, C.c_float]),
     "or_set_weight_format": (C.c_int, [P, C.c_int]),
@@ -138,7 +140,14 @@ class OracleNetwork:
             arrs = [f32(x) for x in (d.v0, d.vreset, d.threshold, d.decay_c, d.resist)]
             check(lib.or_set_lif(h, *[_p(a, C.c_float) for a in arrs], float(d.refractory),
                                  float(d.background)))
-        if d.n_inputs:
+        if d.n_inputs and getattr(d, "stimuli", None) is not None:
+            counts = i32([len(ids) for ids, _ in d.stimuli])
+            ids = i32(np.concatenate([np.asarray(i) for i, _ in d.stimuli]) if len(d.stimuli) else [])
+            times = f32(np.concatenate([np.asarray(t) for _, t in d.stimuli]) if len(d.stimuli) else [])
+            check(lib.or_set_generator_stimuli(h, _p(counts, C.c_int32), len(counts), _p(ids, C.c_int32),
+                                               _p(times, C.c_float)))
+            check(lib.or_set_generator_onset(h, float(d.onset)))
+        elif d.n_inputs:
             rates = f32(d.rates if d.rates is not None else np.zeros((1, d.n_inputs))).reshape(-1, d.n_inputs)
             check(lib.or_set_poisson_rates(h, _p(rates, C.c_float), rates.shape[0]))
         check(lib.or_set_seed(h, int(d.seed)))

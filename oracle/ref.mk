@@ -7,6 +7,7 @@
 #                        CMakeLists.txt:55 selects its CUDA backend the same way)
 #   gen_connectivity     tests/golden/gen_connectivity.cpp on the Dummy front-end
 #   Brunel10K_hip        Examples/Brunel10K.cpp unchanged, on the HIP backend
+#   SimpleExample_hip    Examples/SimpleExample.cpp unchanged, on the HIP backend
 #   frontend_net_hip     tests/frontend/frontend_net.cpp on the HIP backend
 # The CUDA backend cannot be built here (no nvcc); Examples/VogelsAbbottNet.cpp includes
 # Spike/Backend/CUDA/Helpers/ErrorCheck.hpp, which needs cuda.h, so it is not built.
@@ -30,7 +31,7 @@ BACKEND_SRC := $(ROOT)/spike_amd/backend/HIPBackend.cpp
 
 all: $(OUT)/libspike_front.so $(OUT)/context_dummy.o $(OUT)/context_hip.o $(OUT)/gen_connectivity
 
-hip: all $(BACKEND_LIB) $(OUT)/Brunel10K_hip $(OUT)/frontend_net_hip
+hip: all $(BACKEND_LIB) $(OUT)/Brunel10K_hip $(OUT)/SimpleExample_hip $(OUT)/frontend_net_hip
 
 # The HIP backend (spike_amd/backend, this project's code) compiled against the Spike
 # headers; it depends on the front-end library so that the factory maps it registers into
@@ -60,6 +61,11 @@ $(OUT)/gen_connectivity: $(ROOT)/tests/golden/gen_connectivity.cpp $(OUT)/libspi
 	  -Wl,-rpath,'$$ORIGIN' -o $@
 
 $(OUT)/Brunel10K_hip: $(REF)/Examples/Brunel10K.cpp $(OUT)/libspike_front.so $(OUT)/context_hip.o $(BACKEND_LIB)
+	$(CXX) $(CXXFLAGS) -I$(REF)/Examples $< $(OUT)/context_hip.o -L$(OUT) -lspike_front \
+	  -L$(HIP_LIB) -Wl,--no-as-needed -lspike_hip_backend -Wl,--as-needed \
+	  -Wl,-rpath,'$$ORIGIN' -Wl,-rpath,'$$ORIGIN/../../spike_amd/_lib' -o $@
+
+$(OUT)/SimpleExample_hip: $(REF)/Examples/SimpleExample.cpp $(OUT)/libspike_front.so $(OUT)/context_hip.o $(BACKEND_LIB)
 	$(CXX) $(CXXFLAGS) -I$(REF)/Examples $< $(OUT)/context_hip.o -L$(OUT) -lspike_front \
 	  -L$(HIP_LIB) -Wl,--no-as-needed -lspike_hip_backend -Wl,--as-needed \
 	  -Wl,-rpath,'$$ORIGIN' -Wl,-rpath,'$$ORIGIN/../../spike_amd/_lib' -o $@

@@ -225,6 +225,12 @@ struct Net {
   bool lif_set = false;
   // Poisson
   std::vector<float> rates; int n_stim = 0, stim = 0;
+  // generator inputs: per stimulus the entries sorted by time (stable), offsets [n_stim+1]
+  bool gen = false;
+  float gen_onset = 0.0f;
+  std::vector<int64_t> gen_off;
+  std::vector<int32_t> gen_ids;
+  std::vector<float> gen_times;
   uint64_t seed = 42;
   int64_t arm_step = -1;  // step at which inputs arm (no spike); -1: next processed step
   bool pending_arm = true;
@@ -476,8 +482,33 @@ struct Net {
     for (int g = 0; g < G; ++g) cnt_n += (int64_t)spikes[g].size();
   }
 
+  // GeneratorInputSpikingNeurons.cu:71-128: entry e fires input ids[e] at sub-step g when
+  // |(t - onset + g*dt) - times[e]| < 0.5*dt (float difference, compared in double); an
+  // input spikes at most once per step, entries taken in time order
+  void update_generator(float t) {
+    if (spikes.size() != (size_t)G) spikes.assign(G, {});
+    const int tidx = (int)std::roundf(t / dt);
+    for (int i = 0; i < Nin; ++i)
+      for (int g = 0; g < G; ++g) set_bit(hist_in, Nin, W, i, (int64_t)tidx + g, false);
+    const float base = t - gen_onset;
+    for (int64_t e = gen_off[stim]; e < gen_off[stim + 1]; ++e) {
+      const int id = gen_ids[e];
+      for (int g = 0; g < G; ++g) {
+        const float d = (base + (float)g * dt) - gen_times[e];
+        if (!((double)std::fabs(d) < 0.5 * (double)dt)) continue;
+        const int64_t step = (int64_t)tidx + g;
+        if (get_bit(hist_in, Nin, W, id, step)) continue;
+        set_bit(hist_in, Nin, W, id, step, true);
+        last_in[id] = t + (float)g * dt;
+        spikes[g].push_back(N + id);
+        cnt_in++;
+      }
+    }
+  }
+
   // PoissonInputSpikingNeurons.cu:73-130 in its Bernoulli form
   void update_inputs(float t) {
+    if (gen) { update_generator(t); return; }
     if (spikes.size() != (size_t)G) spikes.assign(G, {});
     int tidx = (int)std::roundf(t / dt);
     if (pending_arm) { arm_step = tidx; pending_arm = false; }
@@ -732,9 +763,37 @@ int or_set_lif(void* p, const float* v0, const float* vr, const float* th, const
 
 int or_set_poisson_rates(void* p, const float* rates, int n_stim) {
   return dispatch((Handle*)p, [&](auto& n) {
-    n.rates.assign(rates, rates + (size_t)n_stim * n.Nin); n.n_stim = n_stim;
+    n.rates.assign(rates, rates + (size_t)n_stim * n.Nin); n.n_stim = n_stim; n.gen = false;
     return 0;
   });
+}
+
+int or_set_generator_stimuli(void* p, const int32_t* counts, int n_stim, const int32_t* ids,
+                             const float* times) {
+  return dispatch((Handle*)p, [&](auto& n) {
+    if (n_stim < 1) return fail("bad generator stimuli");
+    n.gen_off.assign(1, 0); n.gen_ids.clear(); n.gen_times.clear();
+    int64_t o = 0;
+    for (int k = 0; k < n_stim; ++k) {
+      std::vector<int64_t> idx(counts[k]);
+      for (int j = 0; j < counts[k]; ++j) idx[j] = o + j;
+      std::stable_sort(idx.begin(), idx.end(), [&](int64_t x, int64_t y) { return times[x] < times[y]; });
+      for (int64_t e : idx) {
+        if (ids[e] < 0 || ids[e] >= n.Nin) return fail("stimulus neuron id out of range");
+        n.gen_ids.push_back(ids[e]);
+        n.gen_times.push_back(times[e]);
+      }
+      o += counts[k];
+      n.gen_off.push_back(o);
+    }
+    n.gen = true; n.n_stim = n_stim; n.stim = 0;
+    n.rates.assign((size_t)n_stim * n.Nin, 0.0f);
+    return 0;
+  });
+}
+
+int or_set_generator_onset(void* p, float onset) {
+  return dispatch((Handle*)p, [&](auto& n) { n.gen_onset = onset; return 0; });
 }
 
 int or_select_stimulus(void* p, int s) {

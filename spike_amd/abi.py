@@ -92,6 +92,8 @@ SIGNATURES = {
     "spk_set_lif": (C.c_int, [P, fp, fp, fp, fp, fp, C.c_float, C.c_float]),
     "spk_set_poisson_rates": (C.c_int, [P, fp, C.c_int]),
     "spk_select_stimulus": (C.c_int, [P, C.c_int]),
+    "spk_set_generator_stimuli": (C.c_int, [P, ip, C.c_int, ip, fp]),
+    "spk_set_generator_onset": (C.c_int, [P, C.c_float]),
     "spk_set_seed": (C.c_int, [P, C.c_uint64]),
     "spk_set_external_drive": (C.c_int, [P, C.c_float, C.c_float]),
     "spk_set_labels": (C.c_int, [P, fp, fp]),
@@ -220,7 +222,15 @@ class Network:
             arrs = [f32(x) for x in (d.v0, d.vreset, d.threshold, d.decay_c, d.resist)]
             check(lib.spk_set_lif(h, *[_ptr(a, C.c_float) for a in arrs],
                                   float(d.refractory), float(d.background)))
-        if d.n_inputs:
+        if d.n_inputs and d.stimuli is not None:
+            counts = i32([len(ids) for ids, _ in d.stimuli])
+            ids = i32(np.concatenate([np.asarray(i) for i, _ in d.stimuli]) if len(d.stimuli) else [])
+            times = f32(np.concatenate([np.asarray(t) for _, t in d.stimuli]) if len(d.stimuli) else [])
+            self._keep += [counts, ids, times]
+            check(lib.spk_set_generator_stimuli(h, _ptr(counts, C.c_int32), len(counts), _ptr(ids, C.c_int32),
+                                                _ptr(times, C.c_float)))
+            check(lib.spk_set_generator_onset(h, float(d.onset)))
+        elif d.n_inputs:
             rates = f32(d.rates if d.rates is not None else np.zeros((1, d.n_inputs)))
             rates = rates.reshape(-1, d.n_inputs)
             check(lib.spk_set_poisson_rates(h, _ptr(rates, C.c_float), rates.shape[0]))

@@ -103,8 +103,9 @@ void ModelNetwork::build() {
   if (!lif) die("HIP backend: the neuron population must be LIFSpikingNeurons");
   ::SpikingNeurons* in = m->input_spiking_neurons;
   auto* poisson = dynamic_cast<::PoissonInputSpikingNeurons*>(in);
-  if (in && in->total_number_of_neurons > 0 && !poisson)
-    die("HIP backend: input neurons must be PoissonInputSpikingNeurons");
+  auto* generator = dynamic_cast<::GeneratorInputSpikingNeurons*>(in);
+  if (in && in->total_number_of_neurons > 0 && !poisson && !generator)
+    die("HIP backend: input neurons must be PoissonInputSpikingNeurons or GeneratorInputSpikingNeurons");
   ::SpikingSynapses* syn = m->spiking_synapses;
   auto* cond = dynamic_cast<::ConductanceSpikingSynapses*>(syn);
   auto* volt = dynamic_cast<::VoltageSpikingSynapses*>(syn);
@@ -129,7 +130,29 @@ void ModelNetwork::build() {
                     lif->thresholds_for_action_potential_spikes, c.data(), rp.data(),
                     lif->refractory_period_in_seconds, lif->background_current),
         "spk_set_lif");
-  if (Nin > 0) check(spk_set_poisson_rates(net, poisson->rates, 1), "spk_set_poisson_rates");
+  if (Nin > 0 && generator) {
+    // every stimulus of the front-end (GeneratorInputSpikingNeurons.cpp:28-77), concatenated
+    const int ns = generator->total_number_of_input_stimuli;
+    std::vector<int32_t> counts, ids;
+    std::vector<float> times;
+    for (int s = 0; s < ns; ++s) {
+      const int k = generator->number_of_spikes_in_stimuli[s];
+      counts.push_back(k);
+      ids.insert(ids.end(), generator->neuron_id_matrix_for_stimuli[s], generator->neuron_id_matrix_for_stimuli[s] + k);
+      times.insert(times.end(), generator->spike_times_matrix_for_stimuli[s],
+                   generator->spike_times_matrix_for_stimuli[s] + k);
+    }
+    if (ns < 1) {   // no stimulus added: inputs that never spike
+      counts.push_back(0);
+    }
+    check(spk_set_generator_stimuli(net, counts.data(), (int)counts.size(), ids.data(), times.data()),
+          "spk_set_generator_stimuli");
+    check(spk_set_generator_onset(net, generator->stimulus_onset_adjustment), "spk_set_generator_onset");
+    if (generator->current_stimulus_index >= 0 && generator->current_stimulus_index < (int)counts.size())
+      check(spk_select_stimulus(net, generator->current_stimulus_index), "spk_select_stimulus");
+  } else if (Nin > 0) {
+    check(spk_set_poisson_rates(net, poisson->rates, 1), "spk_set_poisson_rates");
+  }
   // Philox key: the reference seeds its generator with 1
   // (Spike/Backend/CUDA/Helpers/RandomStateManager.hpp:24); SPIKE_HIP_SEED overrides it.
   uint64_t seed = 1;
@@ -228,6 +251,14 @@ void PoissonInputSpikingNeurons::reset_state() {
   if (mn->built()) check(spk_reset_inputs(mn->net), "spk_reset_inputs");
 }
 
+void GeneratorInputSpikingNeurons::reset_state() {
+  // GeneratorInputSpikingNeurons::select_stimulus resets the inputs through this call
+  // (GeneratorInputSpikingNeurons.cu:24-40 copies the selected stimulus to the device)
+  auto* g = dynamic_cast<::GeneratorInputSpikingNeurons*>(front());
+  ModelNetwork* mn = model_network(model_of(front()));
+  if (mn->built() && g) check(spk_select_stimulus(mn->net, g->current_stimulus_index), "spk_select_stimulus");
+}
+
 void Synapses::prepare() {}
 
 void Synapses::reset_state() {
@@ -303,13 +334,6 @@ void SpikingActivityMonitor::collect_spikes_for_timestep(float, float) {
 // message instead of silently falling back to the Dummy backend.
 #define SPK_HIP_UNSUPPORTED_PREPARE(NAME) \
   void prepare() override { die("HIP backend: " NAME " is not provided by this backend"); }
-
-class GeneratorInputSpikingNeurons : public virtual ::Backend::HIP::InputSpikingNeurons,
-                                     public virtual ::Backend::GeneratorInputSpikingNeurons {
- public:
-  SPIKE_MAKE_BACKEND_CONSTRUCTOR(GeneratorInputSpikingNeurons);
-  SPK_HIP_UNSUPPORTED_PREPARE("GeneratorInputSpikingNeurons")
-};
 
 class ImagePoissonInputSpikingNeurons : public virtual ::Backend::HIP::PoissonInputSpikingNeurons,
                                         public virtual ::Backend::ImagePoissonInputSpikingNeurons {

@@ -69,6 +69,17 @@ class PoissonInputSpikingNeurons : public virtual ::Backend::HIP::InputSpikingNe
   void reset_state() override;
 };
 
+// Scheduled input spikes (Spike/Backend/CUDA/Neurons/GeneratorInputSpikingNeurons.cu): the
+// front-end's stimuli are handed to the library when the network is built; selecting a
+// stimulus resets the inputs and replays that stimulus.
+class GeneratorInputSpikingNeurons : public virtual ::Backend::HIP::InputSpikingNeurons,
+                                     public virtual ::Backend::GeneratorInputSpikingNeurons {
+ public:
+  SPIKE_MAKE_BACKEND_CONSTRUCTOR(GeneratorInputSpikingNeurons);
+  GeneratorInputSpikingNeurons() = default;
+  void reset_state() override;
+};
+
 class Synapses : public virtual ::Backend::Synapses {
  public:
   ~Synapses() override = default;

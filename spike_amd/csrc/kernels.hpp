@@ -436,6 +436,64 @@ __global__ __launch_bounds__(256) void k_poisson(PoissonArgs a) {
 }
 
 // ---------------------------------------------------------------------------------
+// Generator inputs (GeneratorInputSpikingNeurons.cu:71-128): scheduled spikes of the
+// selected stimulus.  The host passes the stimulus entries [lo, hi) (sorted by time) that
+// can fall in this group; entry e fires input ids[e] at sub-step g when
+// |(t - onset + g*dt) - times[e]| < 0.5*dt, the reference's test (its float difference
+// compared in double with 0.5*dt).  One lane per input: it clears its G history bits,
+// then walks the entries in time order, so its last spike time is its latest one; an
+// input spikes at most once per step (a second entry for the same step finds the history
+// bit set), like every other unit.
+
+struct GeneratorArgs {
+  float* last;
+  uint32_t* hist;
+  const int32_t* ids;
+  const float* times;
+  uint32_t* lists;
+  int* counts;
+  unsigned long long* spike_stat;
+  int64_t lo, hi;
+  int N, Nin, G, W, cap, tidx;
+  float t, onset, dt;
+};
+
+__global__ __launch_bounds__(256) void k_generator(GeneratorArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool active = i < a.Nin;
+  if (active) {
+    for (int g = 0; g < a.G; ++g) {
+      const int b = hist_bitpos((int64_t)a.tidx + g, a.W);
+      a.hist[(size_t)(b >> 5) * a.Nin + i] &= ~(1u << (b & 31));
+    }
+  }
+  const float base = a.t - a.onset;
+  float last = active ? a.last[i] : 0.0f;
+  bool fired = false;
+  for (int64_t e = a.lo; e < a.hi; ++e) {
+    const int id = a.ids[e];
+    const float te = a.times[e];
+    for (int g = 0; g < a.G; ++g) {
+      const float d = (base + (float)g * a.dt) - te;
+      bool spk = active && id == i && (double)fabsf(d) < 0.5 * (double)a.dt;
+      if (spk) {
+        const int b = hist_bitpos((int64_t)a.tidx + g, a.W);
+        uint32_t* wp = a.hist + (size_t)(b >> 5) * a.Nin + i;
+        spk = !((*wp >> (b & 31)) & 1u);
+        *wp |= (1u << (b & 31));
+      }
+      if (spk) {
+        last = a.t + (float)g * a.dt;
+        fired = true;
+      }
+      wave_append(spk, (uint32_t)(a.N + i), a.lists + (size_t)g * a.cap, a.counts + g, a.cap,
+                  a.spike_stat);
+    }
+  }
+  if (active && fired) a.last[i] = last;
+}
+
+// ---------------------------------------------------------------------------------
 // Delivery (SpikingSynapses.cu:168-202): one wavefront per spike, CSR rows sorted by
 // delay, int64 fixed-point atomics into the ring.
 

@@ -162,6 +162,15 @@ struct spk_network {
   bool lif_set = false;
   std::vector<float> rates;
   int n_stim = 0, stim = 0;
+  // generator inputs (scheduled spikes) instead of Poisson inputs: per stimulus the
+  // entries sorted by time, concatenated; offsets [n_stim + 1]
+  bool gen = false;
+  float gen_onset = 0.0f;
+  std::vector<int64_t> gen_off;
+  std::vector<int32_t> gen_ids;
+  std::vector<float> gen_times;
+  int32_t* d_gen_ids = nullptr;
+  float* d_gen_times = nullptr;
   uint64_t seed = 42;
   float ext_rate = 0.0f, ext_w = 0.0f;
   std::vector<float> decay_g, erev;
@@ -307,7 +316,7 @@ void free_device(spk_network* n) {
   dfree(n->d_V); dfree(n->d_v0); dfree(n->d_vreset); dfree(n->d_thr); dfree(n->d_c);
   dfree(n->d_rp); dfree(n->d_last_n); dfree(n->d_last_in); dfree(n->d_hist_n);
   dfree(n->d_hist_in); dfree(n->d_ring); dfree(n->d_g); dfree(n->d_decay); dfree(n->d_erev);
-  dfree(n->d_in_thr); dfree(n->d_ext_cdf); dfree(n->d_rowptr); dfree(n->d_tgt); dfree(n->d_syn); dfree(n->d_syn16);
+  dfree(n->d_in_thr); dfree(n->d_ext_cdf); dfree(n->d_gen_ids); dfree(n->d_gen_times); dfree(n->d_rowptr); dfree(n->d_tgt); dfree(n->d_syn); dfree(n->d_syn16);
   dfree(n->d_w16); dfree(n->d_c16);
   dfree(n->d_scale); dfree(n->d_delay); dfree(n->d_label); dfree(n->d_lists);
   dfree(n->d_counts); dfree(n->d_slot_time); dfree(n->d_slot_tidx); dfree(n->d_stats);
@@ -446,8 +455,30 @@ int launch_neurons(spk_network* n, float t, int slot) {
   return SPK_OK;
 }
 
+// Scheduled spikes of the selected stimulus: the entries that can fall in the group
+// starting at t (a conservative time window; the kernel applies the exact test).
+int launch_generator(spk_network* n, float t, int slot) {
+  spk::GeneratorArgs a{};
+  const float base = t - n->gen_onset;
+  const auto b0 = n->gen_times.begin() + n->gen_off[n->stim];
+  const auto b1 = n->gen_times.begin() + n->gen_off[n->stim + 1];
+  const double lo_t = (double)base - 2.0 * (double)n->dt;
+  const double hi_t = (double)base + (double)(n->G + 1) * (double)n->dt;
+  a.lo = std::lower_bound(b0, b1, lo_t, [](float x, double v) { return (double)x < v; }) - n->gen_times.begin();
+  a.hi = std::upper_bound(b0, b1, hi_t, [](double v, float x) { return v < (double)x; }) - n->gen_times.begin();
+  a.last = n->d_last_in; a.hist = n->d_hist_in; a.ids = n->d_gen_ids; a.times = n->d_gen_times;
+  a.lists = slot_lists(n, slot); a.counts = slot_counts(n, slot);
+  a.spike_stat = n->d_stats + 3;
+  a.N = n->N; a.Nin = n->Nin; a.G = n->G; a.W = n->W; a.cap = n->cap; a.tidx = (int)roundf(t / n->dt);
+  a.t = t; a.onset = n->gen_onset; a.dt = n->dt;
+  hipLaunchKernelGGL(spk::k_generator, dim3(grid_for(n->Nin, 256)), dim3(256), 0, n->stream, a);
+  SPK_HIP(hipGetLastError());
+  return SPK_OK;
+}
+
 int launch_inputs(spk_network* n, float t, int slot) {
   if (n->Nin == 0) return SPK_OK;
+  if (n->gen) return launch_generator(n, t, slot);
   const int tidx = (int)roundf(t / n->dt);
   if (n->pending_arm) {
     n->arm_step = tidx;
@@ -947,8 +978,54 @@ int spk_set_poisson_rates(spk_network* n, const float* rates, int n_stimuli) {
   if (n_stimuli < 1 || (n->Nin > 0 && !rates)) return fail(SPK_ERR_ARG, "bad Poisson rates");
   n->rates.assign(rates, rates + (size_t)n_stimuli * n->Nin);
   n->n_stim = n_stimuli;
+  n->gen = false;
   for (float r : n->rates)
     if (!(r >= 0.0f) || std::isinf(r)) return fail(SPK_ERR_ARG, "Poisson rates must be finite and >= 0");
+  return SPK_OK;
+}
+
+int spk_set_generator_stimuli(spk_network* n, const int32_t* counts, int n_stimuli, const int32_t* ids,
+                              const float* times) {
+  if (int rc = check_net(n)) return rc;
+  if (n->prepared) return fail(SPK_ERR_STATE, "network already prepared");
+  if (n->Nin < 1) return fail(SPK_ERR_ARG, "generator stimuli need an input population");
+  if (n_stimuli < 1 || !counts) return fail(SPK_ERR_ARG, "bad generator stimuli");
+  int64_t total = 0;
+  for (int k = 0; k < n_stimuli; ++k) {
+    if (counts[k] < 0) return fail(SPK_ERR_ARG, "negative spike count in a stimulus");
+    total += counts[k];
+  }
+  if (total > 0 && (!ids || !times)) return fail(SPK_ERR_ARG, "null stimulus arrays");
+  for (int64_t e = 0; e < total; ++e) {
+    if (ids[e] < 0 || ids[e] >= n->Nin) return fail(SPK_ERR_ARG, "stimulus neuron id out of range");
+    if (!std::isfinite(times[e])) return fail(SPK_ERR_ARG, "stimulus spike times must be finite");
+  }
+  n->gen_off.assign(1, 0);
+  n->gen_ids.clear();
+  n->gen_times.clear();
+  int64_t o = 0;
+  for (int k = 0; k < n_stimuli; ++k) {
+    std::vector<int64_t> idx(counts[k]);
+    for (int j = 0; j < counts[k]; ++j) idx[j] = o + j;
+    std::stable_sort(idx.begin(), idx.end(), [&](int64_t x, int64_t y) { return times[x] < times[y]; });
+    for (int64_t e : idx) {
+      n->gen_ids.push_back(ids[e]);
+      n->gen_times.push_back(times[e]);
+    }
+    o += counts[k];
+    n->gen_off.push_back(o);
+  }
+  n->gen = true;
+  n->n_stim = n_stimuli;
+  n->stim = 0;
+  n->rates.assign((size_t)n_stimuli * n->Nin, 0.0f);
+  return SPK_OK;
+}
+
+int spk_set_generator_onset(spk_network* n, float onset) {
+  if (int rc = check_net(n)) return rc;
+  if (!std::isfinite(onset)) return fail(SPK_ERR_ARG, "onset must be finite");
+  n->gen_onset = onset;
   return SPK_OK;
 }
 
@@ -1172,6 +1249,7 @@ int spk_prepare(spk_network* n, int G) {
   if (S >= ((int64_t)1 << 32)) no_fused("2^32 or more synapses");
   if (n->P > spk::kMaxPartitions) no_fused("too many partitions");
   if ((int)n->mons.size() > spk::kMaxFusedMons) no_fused("more than 4 spike monitors");
+  if (n->gen) no_fused("generator inputs");
   n->BL = std::max(n->dmax, G);
   n->fused_lds = spk::fused_lds_bytes(n->BL, n->L, n->W, G, n->nown_max, n->iown_max, n->xcap, n->P);
   {
@@ -1284,6 +1362,10 @@ int spk_prepare(spk_network* n, int G) {
   ALLOC(n->d_counts, (size_t)n->slots * G);
   ALLOC(n->d_slot_time, n->slots); ALLOC(n->d_slot_tidx, n->slots);
   ALLOC(n->d_stats, 4);
+  if (n->gen) {
+    ALLOC(n->d_gen_ids, std::max<size_t>(1, n->gen_ids.size()));
+    ALLOC(n->d_gen_times, std::max<size_t>(1, n->gen_times.size()));
+  }
   if (n->fused_ok) {
     ALLOC(n->d_nlo, (size_t)n->P + 1); ALLOC(n->d_ilo, (size_t)n->P + 1);
     ALLOC(n->d_seg, seg.size());
@@ -1299,6 +1381,10 @@ int spk_prepare(spk_network* n, int G) {
   if ((rc = upload(n->d_rp, n->rp.data(), N, st))) return rc;
   if ((rc = upload(n->d_decay, n->decay_g.data(), L, st))) return rc;
   if ((rc = upload(n->d_erev, n->erev.data(), L, st))) return rc;
+  if (n->gen) {
+    if ((rc = upload(n->d_gen_ids, n->gen_ids.data(), n->gen_ids.size(), st))) return rc;
+    if ((rc = upload(n->d_gen_times, n->gen_times.data(), n->gen_times.size(), st))) return rc;
+  }
   {
     std::vector<uint32_t> thr(n->rates.size());
     for (size_t i = 0; i < thr.size(); ++i) thr[i] = bernoulli_threshold(n->rates[i], n->dt);

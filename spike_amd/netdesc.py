@@ -58,6 +58,9 @@ class NetDesc:
     background: float = 0.0
     # Poisson inputs: rates[n_stimuli, n_inputs] in Hz
     rates: Optional[np.ndarray] = None
+    # or generator inputs: per stimulus (ids int32, times float32 in s); onset adjustment
+    stimuli: Optional[List] = None
+    onset: float = 0.0
     seed: int = 42
     ext_rate: float = 0.0
     ext_weight: float = 0.0

@@ -14,7 +14,10 @@
 //   frontend_net_hip <case> <out_dir> <simtime_s> [plastic]
 //   cases: brunel (voltage synapses, Poisson inputs, optional WD STDP),
 //          coba (conductance synapses, two labels, Vogels + Evans STDP),
-//          plain (Poisson -> LIF, RANDOM conductance synapses, delays 1-10 steps)
+//          plain (Poisson -> LIF, RANDOM conductance synapses, delays 1-10 steps),
+//          generator (Examples/SimpleExample.cpp's layout: generator inputs -> conductance
+//          LIF E/I; stimulus 0 for simtime, then stimulus 1 selected for another simtime;
+//          gen_{counts,ids,times}.bin hold the stimuli)
 #include "Spike/Spike.hpp"
 #include "UtilityFunctions.hpp"
 
@@ -66,7 +69,11 @@ int main(int argc, char** argv) {
   const float dt = 0.0001f;
   model->SetTimestep(dt);
   LIFSpikingNeurons* neurons = new LIFSpikingNeurons();
-  PoissonInputSpikingNeurons* inputs = new PoissonInputSpikingNeurons();
+  PoissonInputSpikingNeurons* poisson = nullptr;
+  GeneratorInputSpikingNeurons* generator = nullptr;
+  InputSpikingNeurons* inputs;
+  if (which == "generator") inputs = generator = new GeneratorInputSpikingNeurons();
+  else inputs = poisson = new PoissonInputSpikingNeurons();
   model->spiking_neurons = neurons;
   model->input_spiking_neurons = inputs;
   std::vector<STDPPlasticity*> rules;
@@ -187,6 +194,59 @@ int main(int argc, char** argv) {
       xi->decay_term_tau_g = 0.001f;
       model->AddSynapseGroup(in_id, e_id, xi);
     }
+  } else if (which == "generator") {
+    // Examples/SimpleExample.cpp:33-150 with more inputs and neurons, RANDOM E -> I
+    ConductanceSpikingSynapses* syn = new ConductanceSpikingSynapses(42);
+    model->spiking_synapses = syn;
+    generator_input_spiking_neuron_parameters_struct* gp = new generator_input_spiking_neuron_parameters_struct();
+    gp->group_shape[0] = 1;
+    gp->group_shape[1] = 20;
+    lif_spiking_neuron_parameters_struct* E = new lif_spiking_neuron_parameters_struct();
+    lif_spiking_neuron_parameters_struct* I = new lif_spiking_neuron_parameters_struct();
+    lif(E, 500e-12f, 25e-9f, -0.074f, -0.074f, -0.053f, 2e-3f, 100);
+    lif(I, 214e-12f, 18e-9f, -0.082f, -0.082f, -0.053f, 2e-3f, 100);
+    const int in_id = model->AddInputNeuronGroup(gp);
+    const int e_id = model->AddNeuronGroup(E);
+    const int i_id = model->AddNeuronGroup(I);
+    conductance_spiking_synapse_parameters_struct* ie = new conductance_spiking_synapse_parameters_struct();
+    ie->weight_range[0] = 0.5f;
+    ie->weight_range[1] = 10.0f;
+    ie->weight_scaling_constant = 25e-9f;
+    ie->delay_range[0] = ie->delay_range[1] = 8 * dt;
+    ie->connectivity_type = CONNECTIVITY_TYPE_ALL_TO_ALL;
+    conductance_spiking_synapse_parameters_struct* ei = new conductance_spiking_synapse_parameters_struct();
+    ei->weight_range[0] = ei->weight_range[1] = 10.0f;
+    ei->weight_scaling_constant = 18e-9f;
+    ei->delay_range[0] = 5 * dt;
+    ei->delay_range[1] = 3e-3f;
+    ei->connectivity_type = CONNECTIVITY_TYPE_RANDOM;
+    ei->random_connectivity_probability = 0.05f;
+    conductance_spiking_synapse_parameters_struct* iex = new conductance_spiking_synapse_parameters_struct();
+    iex->weight_range[0] = -5.0f;
+    iex->weight_range[1] = -2.5f;
+    iex->weight_scaling_constant = 25e-9f;
+    iex->delay_range[0] = 5 * dt;
+    iex->delay_range[1] = 3e-3f;
+    iex->connectivity_type = CONNECTIVITY_TYPE_ALL_TO_ALL;
+    model->AddSynapseGroup(in_id, e_id, ie);
+    model->AddSynapseGroup(e_id, i_id, ei);
+    model->AddSynapseGroup(i_id, e_id, iex);
+    // two stimuli of scheduled spikes over [0, simtime) and [simtime, 2 simtime); an entry
+    // repeated within one step in each
+    for (int st = 0; st < 2; ++st) {
+      std::vector<int> ids;
+      std::vector<float> times;
+      unsigned int x = 12345u + 77u * (unsigned)st;
+      for (int k = 0; k < 120; ++k) {
+        x = x * 1664525u + 1013904223u;
+        ids.push_back((int)((x >> 8) % 20u));
+        x = x * 1664525u + 1013904223u;
+        times.push_back(simtime * ((float)st + (float)((x >> 8) & 0xFFFF) / 65536.0f));
+      }
+      ids.push_back(ids[0]);
+      times.push_back(times[0] + 1e-6f);
+      generator->add_stimulus(ids, times);
+    }
   } else {
     fprintf(stderr, "unknown case %s\n", which.c_str());
     return 2;
@@ -212,7 +272,24 @@ int main(int argc, char** argv) {
   put("lif_thr", neurons->thresholds_for_action_potential_spikes, N);
   put("lif_tau", neurons->membrane_time_constants_tau_m, N);
   put("lif_R", neurons->membrane_resistances_R, N);
-  put("rates", inputs->rates, inputs->total_number_of_neurons);
+  if (poisson) {
+    put("rates", poisson->rates, inputs->total_number_of_neurons);
+  } else {
+    std::vector<float> zero(inputs->total_number_of_neurons, 0.0f);
+    put("rates", zero.data(), zero.size());
+    std::vector<int> counts, ids;
+    std::vector<float> times;
+    for (int st = 0; st < generator->total_number_of_input_stimuli; ++st) {
+      const int k = generator->number_of_spikes_in_stimuli[st];
+      counts.push_back(k);
+      ids.insert(ids.end(), generator->neuron_id_matrix_for_stimuli[st], generator->neuron_id_matrix_for_stimuli[st] + k);
+      times.insert(times.end(), generator->spike_times_matrix_for_stimuli[st],
+                   generator->spike_times_matrix_for_stimuli[st] + k);
+    }
+    put("gen_counts", counts.data(), counts.size());
+    put("gen_ids", ids.data(), ids.size());
+    put("gen_times", times.data(), times.size());
+  }
   std::vector<float> decay;
   std::vector<float> rev;
   if (auto* cs = dynamic_cast<ConductanceSpikingSynapses*>(s)) {
@@ -227,6 +304,10 @@ int main(int argc, char** argv) {
     put("plastic_" + std::to_string(r), rules[r]->plastic_synapses.data(), rules[r]->plastic_synapses.size());
 
   model->run(simtime, plastic);
+  if (generator) {
+    generator->select_stimulus(1);
+    model->run(simtime, plastic);
+  }
 
   s->backend()->copy_weights_to_host();
   put("weights_after", s->synaptic_efficacies_or_weights, S);
@@ -241,7 +322,8 @@ int main(int argc, char** argv) {
     << neurons->refractory_period_in_seconds << ", \"background_current\": " << neurons->background_current
     << ", \"num_syn_labels\": " << s->num_syn_labels << ", \"conductance\": "
     << (dynamic_cast<ConductanceSpikingSynapses*>(s) ? 1 : 0) << ", \"simtime\": " << simtime
-    << ", \"plastic\": " << (plastic ? 1 : 0) << ", \"rule_kinds\": [";
+    << ", \"plastic\": " << (plastic ? 1 : 0) << ", \"generator\": " << (generator ? 1 : 0)
+    << ", \"rule_kinds\": [";
   for (size_t r = 0; r < rule_kinds.size(); ++r) j << (r ? ", " : "") << "\"" << rule_kinds[r] << "\"";
   j << "], \"rule_params\": [";
   for (size_t r = 0; r < rules.size(); ++r) {

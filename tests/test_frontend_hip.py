@@ -65,6 +65,11 @@ def desc_from_dump(meta, arr):
         scale=arr("syn_scale", F32), delay=arr("syn_delay", np.int32), label=arr("syn_label", np.int32),
         rules=rules, monitors=[netdesc.POP_NEURONS, netdesc.POP_INPUTS],
         grouping=meta["timestep_grouping"], name="frontend_" + meta["case"])
+    if meta.get("generator"):
+        counts = arr("gen_counts", np.int32)
+        ids, times = arr("gen_ids", np.int32), arr("gen_times", F32)
+        off = np.concatenate([[0], np.cumsum(counts)])
+        d.stimuli = [(ids[off[k]:off[k + 1]], times[off[k]:off[k + 1]]) for k in range(len(counts))]
     return d
 
 
@@ -72,6 +77,11 @@ def oracle_run(d, simtime, plastic):
     import oracle_py
     o = oracle_py.OracleNetwork(d, precision=32, threads=4)
     o.run(0.0, d.groups_for(simtime), plasticity=plastic)
+    if d.stimuli is not None:
+        # the program selects stimulus 1 and runs again from the current time
+        # (SpikingModel::run continues at start + simtime, SpikingModel.cpp:214-236)
+        assert o.lib.or_select_stimulus(o.h, 1) == 0
+        o.run(float(F32(0.0) + F32(simtime)), d.groups_for(simtime), plasticity=plastic)
     n_ids, n_t = o.take_spikes(0)
     i_ids, i_t = o.take_spikes(1)
     w = o.weights().astype(np.float32)   # the FP32 oracle's weights, returned widened
@@ -84,7 +94,8 @@ def sort_raster(ids, times):
     return ids[order], times[order]
 
 
-CASES = [("brunel", 0.1, False), ("brunel", 0.1, True), ("coba", 0.1, True), ("plain", 0.05, False)]
+CASES = [("brunel", 0.1, False), ("brunel", 0.1, True), ("coba", 0.1, True), ("plain", 0.05, False),
+         ("generator", 0.1, False)]
 
 
 @pytest.mark.gpu
@@ -122,3 +133,24 @@ def test_brunel_example_unchanged_runs():
     n = int(lines[-1].split(":")[1])
     # 10,000 neurons over 0.2 s: a silent or runaway network is a failure
     assert 0 < n < 10000 * 0.2 * 500
+
+
+@pytest.mark.gpu
+def test_simple_example_unchanged_runs(tmp_path):
+    """Examples/SimpleExample.cpp, compiled unchanged against the HIP backend: generator
+    inputs -> conductance LIF, 2 x 50 s.  Its input monitor records exactly the five
+    scheduled spikes of the first stimulus (the second stimulus's times, 5-10 s, lie
+    before the second run's start at 50 s, so it never fires)."""
+    prog = os.path.join(ROOT, "oracle", "_ref", "SimpleExample_hip")
+    if not os.path.exists(prog):
+        pytest.skip("SimpleExample_hip not built")
+    print("SimpleExample_hip", flush=True)
+    r = subprocess.run([prog], capture_output=True, text=True, timeout=600, cwd=str(tmp_path))
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    ids = np.loadtxt(tmp_path / "SpikeIDs.txt", dtype=np.int64, ndmin=1)
+    times = np.loadtxt(tmp_path / "SpikeTimes.txt", dtype=np.float64, ndmin=1)
+    order = np.argsort(times, kind="stable")
+    assert ids[order].tolist() == [0, 3, 1, 6, 7]
+    assert np.allclose(times[order], [0.1, 0.2, 0.3, 0.5, 0.9], atol=1e-6)
+    for f in ("PostsynapticIDs.txt", "SynapticWeights.txt"):
+        assert (tmp_path / f).stat().st_size > 0

@@ -242,6 +242,43 @@ def test_wd_spike_driven_form_csr_order():
     _compare(gpu, ora, d, "wd csr order")
 
 
+@pytest.mark.parametrize("kind", [0, SYN_CONDUCTANCE])
+def test_generator_inputs(kind):
+    """Scheduled input spikes (three stimuli, random ids and times, an entry repeated
+    within one step, entries after the run, an onset adjustment) on the per-phase path,
+    with a stimulus switch: bit-exact against the oracle."""
+    d = builders.random_net(seed=81, n_neurons=120, n_inputs=40, kind=kind, p=0.2, whi=4e-3, dmin=2,
+                            dmax=6, monitors=(0, 1))
+    if kind == SYN_CONDUCTANCE:
+        d.weight = (d.weight * 400).astype(np.float32)
+    rng = np.random.default_rng(81)
+    stim = []
+    for k in range(3):
+        n = 400 + 50 * k
+        ids = rng.integers(0, d.n_inputs, n).astype(np.int32)
+        times = rng.uniform(0.0, 0.07, n).astype(np.float32)
+        ids[:3] = ids[3]
+        times[:3] = times[3] + np.float32(1e-6) * np.arange(3, dtype=np.float32)
+        stim.append((ids, times))
+    d.stimuli = stim
+    d.onset = 5e-4
+    print(f"[parity] generator inputs kind={kind}", flush=True)
+    gpu = abi.Network(d)
+    ora = oracle_py.OracleNetwork(d, precision=32)
+    assert gpu.info().fused == 0 and "generator" in gpu.fused_status()
+    ng = 150
+    gpu.run(0.0, ng)
+    ora.run(0.0, ng)
+    _compare(gpu, ora, d, "generator stimulus 0")
+    assert gpu.counters().spikes_inputs > 0
+    assert abi.check(gpu.lib.spk_select_stimulus(gpu.h, 2)) is None
+    assert ora.lib.or_select_stimulus(ora.h, 2) == 0
+    t = float(d.group_time(0.0, ng))
+    gpu.run(t, ng)
+    ora.run(t, ng)
+    _compare(gpu, ora, d, "generator stimulus 2")
+
+
 def test_time_only_advances_and_rules_list_synapses_once():
     d = _plastic_net(RULE_WD, 70)
     gpu = abi.Network(d)
