@@ -304,6 +304,13 @@ int spk_monitor_reset(spk_network* net, int monitor_id);
 /* Copy device records (SpikingActivityMonitor.cu:41-50) and clear the device store. */
 int spk_monitor_drain(spk_network* net, int monitor_id, int32_t* ids, float* times,
                       int64_t cap, int64_t* n);
+/* Rate monitor (RateActivityMonitor.cpp:14-40, CUDA RateActivityMonitor.cu:37-78): the
+ * number of spikes of every neuron (or input) of a population, counted where the spikes
+ * are made.  Added before spk_prepare; counts since prepare or the monitor's last reset
+ * (copy_spike_count_to_host / reset_state). */
+int spk_add_rate_monitor(spk_network* net, int population, int* monitor_id);
+int spk_rate_monitor_counts(spk_network* net, int monitor_id, int32_t* counts);
+int spk_rate_monitor_reset(spk_network* net, int monitor_id);
 /* Host-side records accumulated by spk_run since the last take; ids are local to the
  * population, sorted by (time, id).  Pass ids=NULL to query the count only. */
 int spk_monitor_take(spk_network* net, int monitor_id, int32_t* ids, float* times,

@@ -94,6 +94,9 @@ SIGNATURES = {
     "spk_select_stimulus": (C.c_int, [P, C.c_int]),
     "spk_set_generator_stimuli": (C.c_int, [P, ip, C.c_int, ip, fp]),
     "spk_set_generator_onset": (C.c_int, [P, C.c_float]),
+    "spk_add_rate_monitor": (C.c_int, [P, C.c_int, C.POINTER(C.c_int)]),
+    "spk_rate_monitor_counts": (C.c_int, [P, C.c_int, ip]),
+    "spk_rate_monitor_reset": (C.c_int, [P, C.c_int]),
     "spk_set_seed": (C.c_int, [P, C.c_uint64]),
     "spk_set_external_drive": (C.c_int, [P, C.c_float, C.c_float]),
     "spk_set_labels": (C.c_int, [P, fp, fp]),
@@ -271,7 +274,22 @@ class Network:
             mid = C.c_int()
             check(lib.spk_add_monitor(h, int(pop), int(cap), C.byref(mid)))
             self.monitor_ids.append(mid.value)
+        self.rate_ids = []
+        for pop in getattr(d, "rate_monitors", []):
+            mid = C.c_int()
+            check(lib.spk_add_rate_monitor(h, int(pop), C.byref(mid)))
+            self.rate_ids.append((mid.value, int(pop)))
         check(lib.spk_prepare(h, d.timestep_grouping()))
+
+    def rate_counts(self, k: int = 0) -> np.ndarray:
+        """Spike counts per unit of rate monitor k since prepare or its last reset."""
+        mid, pop = self.rate_ids[k]
+        out = np.zeros(self.desc.n_neurons if pop == 0 else self.desc.n_inputs, np.int32)
+        check(self.lib.spk_rate_monitor_counts(self.h, mid, _ptr(out, C.c_int32)))
+        return out
+
+    def rate_reset(self, k: int = 0):
+        check(self.lib.spk_rate_monitor_reset(self.h, self.rate_ids[k][0]))
 
     def close(self):
         if getattr(self, "h", None):

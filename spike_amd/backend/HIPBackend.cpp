@@ -219,15 +219,24 @@ void ModelNetwork::build() {
     }
   }
   for (::ActivityMonitor* am : m->monitors_vec) {
-    auto* sm = dynamic_cast<::SpikingActivityMonitor*>(am);
-    if (!sm) die("HIP backend: only SpikingActivityMonitor is provided");
+    // SpikingActivityMonitor declares its own `neurons` (SpikingActivityMonitor.hpp:45),
+    // hiding ActivityMonitor's, which RateActivityMonitor uses
+    auto* sm_ = dynamic_cast<::SpikingActivityMonitor*>(am);
+    ::SpikingNeurons* watched = sm_ ? sm_->neurons : am->neurons;
     int pop;
-    if (sm->neurons == m->spiking_neurons) pop = SPK_POP_NEURONS;
-    else if (sm->neurons == m->input_spiking_neurons) pop = SPK_POP_INPUTS;
-    else die("HIP backend: a spike monitor must watch the model's neurons or inputs");
+    if (watched == m->spiking_neurons) pop = SPK_POP_NEURONS;
+    else if (watched == m->input_spiking_neurons) pop = SPK_POP_INPUTS;
+    else die("HIP backend: a monitor must watch the model's neurons or inputs");
     int id = -1;
-    check(spk_add_monitor(net, pop, std::max(1, sm->size_of_device_spike_store), &id), "spk_add_monitor");
-    monitor_id[sm] = id;
+    if (auto* sm = sm_) {
+      check(spk_add_monitor(net, pop, std::max(1, sm->size_of_device_spike_store), &id), "spk_add_monitor");
+      monitor_id[sm] = id;
+    } else if (auto* rm = dynamic_cast<::RateActivityMonitor*>(am)) {
+      check(spk_add_rate_monitor(net, pop, &id), "spk_add_rate_monitor");
+      monitor_id[rm] = id;
+    } else {
+      die("HIP backend: unsupported activity monitor");
+    }
   }
   grouping = m->timestep_grouping;
   check(spk_prepare(net, grouping), "spk_prepare");
@@ -303,6 +312,22 @@ void SpikingActivityMonitor::reset_state() {
   if (mn->built()) check(spk_monitor_reset(mn->net, mn->monitor(front())), "spk_monitor_reset");
 }
 
+void RateActivityMonitor::reset_state() {
+  ModelNetwork* mn = model_network(front()->model);
+  if (mn->built()) check(spk_rate_monitor_reset(mn->net, mn->monitor(front())), "spk_rate_monitor_reset");
+}
+
+void RateActivityMonitor::copy_spike_count_to_host() {
+  ModelNetwork* mn = model_network(front()->model);
+  if (mn->built())
+    check(spk_rate_monitor_counts(mn->net, mn->monitor(front()), front()->per_neuron_spike_counts),
+          "spk_rate_monitor_counts");
+}
+
+void RateActivityMonitor::add_spikes_to_per_neuron_spike_count(float, float) {
+  // the counts are kept by the group step where the spikes are made
+}
+
 void SpikingActivityMonitor::copy_spikecount_to_front() {
   ModelNetwork* mn = model_network(front()->model);
   int64_t c = 0;
@@ -372,15 +397,6 @@ class WeightNormSTDPPlasticity : public virtual ::Backend::HIP::STDPPlasticity,
   SPIKE_MAKE_BACKEND_CONSTRUCTOR(WeightNormSTDPPlasticity);
   SPK_HIP_UNSUPPORTED_PREPARE("WeightNormSTDPPlasticity")
   void weight_normalization() override {}
-};
-
-class RateActivityMonitor : public virtual ::Backend::HIP::ActivityMonitor,
-                            public virtual ::Backend::RateActivityMonitor {
- public:
-  SPIKE_MAKE_BACKEND_CONSTRUCTOR(RateActivityMonitor);
-  SPK_HIP_UNSUPPORTED_PREPARE("RateActivityMonitor")
-  void copy_spike_count_to_host() override {}
-  void add_spikes_to_per_neuron_spike_count(float, float) override {}
 };
 
 #undef SPK_HIP_UNSUPPORTED_PREPARE

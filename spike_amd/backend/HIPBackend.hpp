@@ -176,6 +176,21 @@ class SpikingActivityMonitor : public virtual ::Backend::HIP::ActivityMonitor,
   ::SpikingActivityMonitor* front() const { return static_cast<::SpikingActivityMonitor*>(_frontend); }
 };
 
+// Per-neuron spike counts (Spike/Backend/CUDA/ActivityMonitor/RateActivityMonitor.cu): the
+// library counts spikes where they are made; the front-end's per_neuron_spike_counts are
+// filled at final_update.
+class RateActivityMonitor : public virtual ::Backend::HIP::ActivityMonitor,
+                            public virtual ::Backend::RateActivityMonitor {
+ public:
+  SPIKE_MAKE_BACKEND_CONSTRUCTOR(RateActivityMonitor);
+  void prepare() override {}
+  void reset_state() override;
+  void copy_spike_count_to_host() override;
+  void add_spikes_to_per_neuron_spike_count(float current_time_in_seconds, float timestep) override;
+ private:
+  ::RateActivityMonitor* front() const { return static_cast<::RateActivityMonitor*>(_frontend); }
+};
+
 class RandomStateManager : public virtual ::Backend::RandomStateManager {
  public:
   SPIKE_MAKE_BACKEND_CONSTRUCTOR(RandomStateManager);

@@ -59,6 +59,9 @@ struct FusedArgs {
   // partitions: neurons [nlo[p], nlo[p+1]), inputs [ilo[p], ilo[p+1])
   const int* nlo;
   const int* ilo;
+  // spike counts per neuron / input of a rate monitor, or nullptr
+  uint32_t* rate_n;
+  uint32_t* rate_in;
   // LIF state and constants
   float* V;
   float* last_n;
@@ -661,6 +664,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const int p, cons
       }
       if (m) a.last_in[ilo + i] = t + (float)(63 - __clzll((long long)m)) * a.dt;
       st_in += (unsigned long long)c;
+      if (a.rate_in != nullptr && c) a.rate_in[ilo + i] += (uint32_t)c;
     }
     // -- B1. this group's synaptic input per (neuron, sub-step[, label]) as float ------
     // The ring slots of the group's steps are complete (every delay is >= G): read and
@@ -775,6 +779,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const int p, cons
             if (pos >= bits) pos -= bits;
           }
         }
+        if (a.rate_n != nullptr && smask) a.rate_n[gi] += (uint32_t)__popcll(smask);
         if (a.st.on && smask) {
           // window statistics of this neuron's spikes, in time order
           unsigned int c0 = sst_cnt[tid], c1 = sst_nisi[tid];

@@ -257,6 +257,7 @@ struct LifArgs {
   int64_t q_ext;              // fixed-point value of one external event
   uint64_t seed;
   StatsArgs st;
+  uint32_t* rate;             // [N] spike counts of a rate monitor, or nullptr
 };
 
 template <int KIND>
@@ -286,6 +287,7 @@ __global__ __launch_bounds__(256) void k_lif(LifArgs a) {
   int64_t ext_blk = -1;
   u32x4 ext_r{0u, 0u, 0u, 0u};
   unsigned int st_cnt = 0u, st_nisi = 0u;
+  uint32_t nspk = 0u;
   float st_prev = 0.0f;
   double st_s1 = 0.0, st_s2 = 0.0;
   if (a.st.on && active) {
@@ -359,9 +361,11 @@ __global__ __launch_bounds__(256) void k_lif(LifArgs a) {
       if (step >= a.st.t0 && step < a.st.t0 + a.st.nsteps) atomicAdd(a.st.hist + (step - a.st.t0) / a.st.bin, 1u);
     }
     wave_append(spk, (uint32_t)i, a.lists + (size_t)g * a.cap, a.counts + g, a.cap, a.spike_stat);
+    nspk += spk ? 1u : 0u;
   }
   if (active) {
     if (cur_w >= 0) a.hist[(size_t)cur_w * a.N + i] = word;
+    if (a.rate != nullptr && nspk) a.rate[i] += nspk;
     a.V[i] = V;
     a.last[i] = last;
     if (KIND == 1) {
@@ -394,6 +398,7 @@ struct PoissonArgs {
   int64_t arm_step;
   float t, dt;
   uint64_t seed;
+  uint32_t* rate;          // [Nin] spike counts of a rate monitor, or nullptr
 };
 
 __global__ __launch_bounds__(256) void k_poisson(PoissonArgs a) {
@@ -406,6 +411,7 @@ __global__ __launch_bounds__(256) void k_poisson(PoissonArgs a) {
   u32x4 in_r{0u, 0u, 0u, 0u};
   int cur_w = -1;
   uint32_t word = 0;
+  uint32_t nspk = 0u;
   for (int g = 0; g < a.G; ++g) {
     const int64_t step = (int64_t)a.tidx + g;
     const int b = hist_bitpos(step, a.W);
@@ -425,6 +431,7 @@ __global__ __launch_bounds__(256) void k_poisson(PoissonArgs a) {
     if (spk) {
       word |= (1u << (b & 31));
       last = a.t + (float)g * a.dt;
+      ++nspk;
     }
     wave_append(spk, (uint32_t)(a.N + i), a.lists + (size_t)g * a.cap, a.counts + g, a.cap,
                 a.spike_stat);
@@ -432,6 +439,7 @@ __global__ __launch_bounds__(256) void k_poisson(PoissonArgs a) {
   if (active) {
     if (cur_w >= 0) a.hist[(size_t)cur_w * a.Nin + i] = word;
     a.last[i] = last;
+    if (a.rate != nullptr && nspk) a.rate[i] += nspk;
   }
 }
 
@@ -456,6 +464,7 @@ struct GeneratorArgs {
   int64_t lo, hi;
   int N, Nin, G, W, cap, tidx;
   float t, onset, dt;
+  uint32_t* rate;          // [Nin] spike counts of a rate monitor, or nullptr
 };
 
 __global__ __launch_bounds__(256) void k_generator(GeneratorArgs a) {
@@ -470,6 +479,7 @@ __global__ __launch_bounds__(256) void k_generator(GeneratorArgs a) {
   const float base = a.t - a.onset;
   float last = active ? a.last[i] : 0.0f;
   bool fired = false;
+  uint32_t nspk = 0u;
   for (int64_t e = a.lo; e < a.hi; ++e) {
     const int id = a.ids[e];
     const float te = a.times[e];
@@ -485,12 +495,14 @@ __global__ __launch_bounds__(256) void k_generator(GeneratorArgs a) {
       if (spk) {
         last = a.t + (float)g * a.dt;
         fired = true;
+        ++nspk;
       }
       wave_append(spk, (uint32_t)(a.N + i), a.lists + (size_t)g * a.cap, a.counts + g, a.cap,
                   a.spike_stat);
     }
   }
   if (active && fired) a.last[i] = last;
+  if (active && a.rate != nullptr && nspk) a.rate[i] += nspk;
 }
 
 // ---------------------------------------------------------------------------------

@@ -260,6 +260,11 @@ struct spk_network {
   // (SpikingModel::run continues from the current time), so last-spike times never lie
   // ahead of a group, which the plasticity tables rely on
   float t_hi = -INFINITY;
+  // rate monitors: one spike counter per neuron / input (shared by the monitors of a
+  // population; each monitor keeps the counts at its last reset as a baseline)
+  uint32_t* d_rate[2] = {nullptr, nullptr};
+  std::vector<int> rate_pop;
+  std::vector<std::vector<uint32_t>> rate_base;
   int cur_slot = 0;
   // timing
   hipEvent_t ev_a = nullptr, ev_b = nullptr;
@@ -316,7 +321,8 @@ void free_device(spk_network* n) {
   dfree(n->d_V); dfree(n->d_v0); dfree(n->d_vreset); dfree(n->d_thr); dfree(n->d_c);
   dfree(n->d_rp); dfree(n->d_last_n); dfree(n->d_last_in); dfree(n->d_hist_n);
   dfree(n->d_hist_in); dfree(n->d_ring); dfree(n->d_g); dfree(n->d_decay); dfree(n->d_erev);
-  dfree(n->d_in_thr); dfree(n->d_ext_cdf); dfree(n->d_gen_ids); dfree(n->d_gen_times); dfree(n->d_rowptr); dfree(n->d_tgt); dfree(n->d_syn); dfree(n->d_syn16);
+  dfree(n->d_in_thr); dfree(n->d_ext_cdf); dfree(n->d_gen_ids); dfree(n->d_gen_times);
+  dfree(n->d_rate[0]); dfree(n->d_rate[1]); dfree(n->d_rowptr); dfree(n->d_tgt); dfree(n->d_syn); dfree(n->d_syn16);
   dfree(n->d_w16); dfree(n->d_c16);
   dfree(n->d_scale); dfree(n->d_delay); dfree(n->d_label); dfree(n->d_lists);
   dfree(n->d_counts); dfree(n->d_slot_time); dfree(n->d_slot_tidx); dfree(n->d_stats);
@@ -445,6 +451,7 @@ int launch_neurons(spk_network* n, float t, int slot) {
   a.inv_scale = ldexpf(1.0f, -n->frac);
   a.q_ext = n->q_ext; a.seed = n->seed;
   a.st = stats_args(n, t);
+  a.rate = n->d_rate[SPK_POP_NEURONS];
   stats_advance(n, a.tidx, 1);
   const int grid = grid_for(n->N, 256);
   if (n->kind == SPK_SYN_VOLTAGE)
@@ -471,6 +478,7 @@ int launch_generator(spk_network* n, float t, int slot) {
   a.spike_stat = n->d_stats + 3;
   a.N = n->N; a.Nin = n->Nin; a.G = n->G; a.W = n->W; a.cap = n->cap; a.tidx = (int)roundf(t / n->dt);
   a.t = t; a.onset = n->gen_onset; a.dt = n->dt;
+  a.rate = n->d_rate[SPK_POP_INPUTS];
   hipLaunchKernelGGL(spk::k_generator, dim3(grid_for(n->Nin, 256)), dim3(256), 0, n->stream, a);
   SPK_HIP(hipGetLastError());
   return SPK_OK;
@@ -491,6 +499,7 @@ int launch_inputs(spk_network* n, float t, int slot) {
   a.spike_stat = n->d_stats + 3;
   a.N = n->N; a.Nin = n->Nin; a.G = n->G; a.W = n->W; a.cap = n->cap; a.tidx = tidx;
   a.arm_step = n->arm_step; a.t = t; a.dt = n->dt; a.seed = n->seed;
+  a.rate = n->d_rate[SPK_POP_INPUTS];
   hipLaunchKernelGGL(spk::k_poisson, dim3(grid_for(n->Nin, 256)), dim3(256), 0, n->stream, a);
   SPK_HIP(hipGetLastError());
   return SPK_OK;
@@ -688,6 +697,7 @@ int fill_fused_args(spk_network* n, bool plastic, spk::FusedArgs& a) {
     if (int rc = refresh_rule_tables(n, r)) return rc;
   a = spk::FusedArgs{};
   a.nlo = n->d_nlo; a.ilo = n->d_ilo;
+  a.rate_n = n->d_rate[SPK_POP_NEURONS]; a.rate_in = n->d_rate[SPK_POP_INPUTS];
   a.V = n->d_V; a.last_n = n->d_last_n; a.hist_n = n->d_hist_n; a.gcond = n->d_g; a.ring = n->d_ring;
   a.v0 = n->d_v0; a.vreset = n->d_vreset; a.thr = n->d_thr; a.c = n->d_c; a.rp = n->d_rp;
   a.decay_g = n->d_decay; a.erev = n->d_erev; a.ext_cdf = n->d_ext_cdf;
@@ -1154,6 +1164,17 @@ int spk_add_monitor(spk_network* n, int population, int64_t capacity, int* monit
   return SPK_OK;
 }
 
+int spk_add_rate_monitor(spk_network* n, int population, int* monitor_id) {
+  if (int rc = check_net(n)) return rc;
+  if (n->prepared) return fail(SPK_ERR_STATE, "network already prepared");
+  if (population != SPK_POP_NEURONS && population != SPK_POP_INPUTS)
+    return fail(SPK_ERR_ARG, "unknown population");
+  n->rate_pop.push_back(population);
+  n->rate_base.emplace_back();
+  if (monitor_id) *monitor_id = (int)n->rate_pop.size() - 1;
+  return SPK_OK;
+}
+
 int spk_prepare(spk_network* n, int G) {
   if (int rc = check_net(n)) return rc;
   if (n->prepared) return fail(SPK_ERR_STATE, "network already prepared");
@@ -1362,6 +1383,12 @@ int spk_prepare(spk_network* n, int G) {
   ALLOC(n->d_counts, (size_t)n->slots * G);
   ALLOC(n->d_slot_time, n->slots); ALLOC(n->d_slot_tidx, n->slots);
   ALLOC(n->d_stats, 4);
+  for (size_t r = 0; r < n->rate_pop.size(); ++r) {
+    const int pop = n->rate_pop[r];
+    const size_t cnt = std::max<size_t>(1, pop == SPK_POP_NEURONS ? N : Nin);
+    if (!n->d_rate[pop]) ALLOC(n->d_rate[pop], cnt);
+    n->rate_base[r].assign(cnt, 0u);
+  }
   if (n->gen) {
     ALLOC(n->d_gen_ids, std::max<size_t>(1, n->gen_ids.size()));
     ALLOC(n->d_gen_times, std::max<size_t>(1, n->gen_times.size()));
@@ -1381,6 +1408,9 @@ int spk_prepare(spk_network* n, int G) {
   if ((rc = upload(n->d_rp, n->rp.data(), N, st))) return rc;
   if ((rc = upload(n->d_decay, n->decay_g.data(), L, st))) return rc;
   if ((rc = upload(n->d_erev, n->erev.data(), L, st))) return rc;
+  for (int pop = 0; pop < 2; ++pop)
+    if (n->d_rate[pop])
+      SPK_HIP(hipMemsetAsync(n->d_rate[pop], 0, sizeof(uint32_t) * std::max<size_t>(1, pop == SPK_POP_NEURONS ? N : Nin), st));
   if (n->gen) {
     if ((rc = upload(n->d_gen_ids, n->gen_ids.data(), n->gen_ids.size(), st))) return rc;
     if ((rc = upload(n->d_gen_times, n->gen_times.data(), n->gen_times.size(), st))) return rc;
@@ -1992,6 +2022,25 @@ int spk_get_traces(spk_network* n, int rid, float* a, float* b) {
     b[r.order[j]] = tb[j];
   }
   return SPK_OK;
+}
+
+int spk_rate_monitor_counts(spk_network* n, int mid, int32_t* counts) {
+  if (int rc = check_prepared(n)) return rc;
+  if (mid < 0 || mid >= (int)n->rate_pop.size()) return fail(SPK_ERR_ARG, "rate monitor id out of range");
+  const int pop = n->rate_pop[mid];
+  const size_t cnt = (size_t)(pop == SPK_POP_NEURONS ? n->N : n->Nin);
+  std::vector<uint32_t> cur(std::max<size_t>(1, cnt));
+  if (int rc = download(n, cur.data(), n->d_rate[pop], cnt)) return rc;
+  for (size_t i = 0; i < cnt; ++i) counts[i] = (int32_t)(cur[i] - n->rate_base[mid][i]);
+  return SPK_OK;
+}
+
+int spk_rate_monitor_reset(spk_network* n, int mid) {
+  if (int rc = check_prepared(n)) return rc;
+  if (mid < 0 || mid >= (int)n->rate_pop.size()) return fail(SPK_ERR_ARG, "rate monitor id out of range");
+  const int pop = n->rate_pop[mid];
+  const size_t cnt = (size_t)(pop == SPK_POP_NEURONS ? n->N : n->Nin);
+  return download(n, n->rate_base[mid].data(), n->d_rate[pop], cnt);
 }
 
 // ---------------------------------------------------------------------------------------

@@ -76,6 +76,7 @@ class NetDesc:
     label: Optional[np.ndarray] = None
     rules: List[Rule] = dataclasses.field(default_factory=list)
     monitors: List[int] = dataclasses.field(default_factory=list)
+    rate_monitors: List[int] = dataclasses.field(default_factory=list)   # populations
     grouping: Optional[int] = None   # default: minimum delay (SpikingModel.cpp:98)
     name: str = "net"
     weight_format: int = 0        # 0 FP32, 1 BF16, 2 FP16 storage of weights and traces

@@ -9,6 +9,7 @@
 //   lif_{v0,vreset,thr,tau,R}.bin, rates.bin, decay_g.bin (expf(-dt/tau_g) per label),
 //   plastic_<r>.bin, weights_after.bin (weights copied back after the run),
 //   <pop>_SpikeIDs.bin / <pop>_SpikeTimes.bin (SpikingActivityMonitor::save_spikes_as_binary),
+//   rate_counts.bin (a RateActivityMonitor's per_neuron_spike_counts of the neurons),
 //   meta.json.
 //
 //   frontend_net_hip <case> <out_dir> <simtime_s> [plastic]
@@ -254,8 +255,10 @@ int main(int argc, char** argv) {
 
   SpikingActivityMonitor* mon_n = new SpikingActivityMonitor(neurons);
   SpikingActivityMonitor* mon_i = new SpikingActivityMonitor(inputs);
+  RateActivityMonitor* rate_n = new RateActivityMonitor(neurons);
   model->AddActivityMonitor(mon_n);
   model->AddActivityMonitor(mon_i);
+  model->AddActivityMonitor(rate_n);
 
   model->finalise_model();
   SpikingSynapses* s = model->spiking_synapses;
@@ -311,6 +314,7 @@ int main(int argc, char** argv) {
 
   s->backend()->copy_weights_to_host();
   put("weights_after", s->synaptic_efficacies_or_weights, S);
+  put("rate_counts", rate_n->per_neuron_spike_counts, N);
   mon_n->save_spikes_as_binary(g_out, "neurons_");
   mon_i->save_spikes_as_binary(g_out, "inputs_");
 

@@ -114,6 +114,8 @@ def test_frontend_run_matches_oracle(tmp_path, case, simtime, plastic):
     np.testing.assert_array_equal(hi_t.view(np.uint32), oi_t.view(np.uint32))
     np.testing.assert_array_equal(hn_ids, on_ids)
     np.testing.assert_array_equal(hn_t.view(np.uint32), on_t.view(np.uint32))
+    # the RateActivityMonitor on the neurons counted every spike of the run
+    np.testing.assert_array_equal(arr("rate_counts", np.int32), np.bincount(on_ids, minlength=meta["n_neurons"]))
     w = arr("weights_after", F32)
     np.testing.assert_array_equal(w.view(np.uint32), ow.view(np.uint32))
     if plastic:

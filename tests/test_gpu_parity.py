@@ -279,6 +279,44 @@ def test_generator_inputs(kind):
     _compare(gpu, ora, d, "generator stimulus 2")
 
 
+@pytest.mark.parametrize("path", ["fused", "launches", "generator"])
+def test_rate_monitor_counts(path):
+    """RateActivityMonitor: per-unit spike counts kept by the kernels that make the
+    spikes equal the bin counts of the spike monitors' rasters (which equal the oracle's),
+    for the neurons and the inputs, on the persistent and per-phase paths and with
+    generator inputs; a reset restarts one monitor's counts only."""
+    d = builders.random_net(seed=91, input_rate=400.0, whi=4e-3, dmin=2, dmax=7, monitors=(0, 1))
+    if path == "generator":
+        rng = np.random.default_rng(91)
+        d.stimuli = [(rng.integers(0, d.n_inputs, 300).astype(np.int32),
+                      rng.uniform(0.0, 0.05, 300).astype(np.float32))]
+    d.rate_monitors = [0, 1, 0]
+    print(f"[parity] rate monitors path={path}", flush=True)
+    gpu = abi.Network(d)
+    ora = oracle_py.OracleNetwork(d, precision=32)
+    launches = path != "fused"
+    assert gpu.info().fused == (0 if path == "generator" else 1)
+    gpu.run(0.0, 80, force_launches=launches)
+    ora.run(0.0, 80)
+    oi, _ = ora.take_spikes(0)
+    oin, _ = ora.take_spikes(1)
+    gi, _ = gpu.take_spikes(0)
+    gin, _ = gpu.take_spikes(1)
+    assert np.array_equal(np.sort(gi), np.sort(oi)) and np.array_equal(np.sort(gin), np.sort(oin))
+    assert len(oi) > 0 and len(oin) > 0
+    cn = np.bincount(oi, minlength=d.n_neurons)
+    assert np.array_equal(gpu.rate_counts(0), cn)
+    assert np.array_equal(gpu.rate_counts(1), np.bincount(oin, minlength=d.n_inputs))
+    gpu.rate_reset(2)
+    t = float(d.group_time(0.0, 80))
+    gpu.run(t, 60, force_launches=launches)
+    ora.run(t, 60)
+    oi2, _ = ora.take_spikes(0)
+    c2 = np.bincount(oi2, minlength=d.n_neurons)
+    assert np.array_equal(gpu.rate_counts(0), cn + c2)
+    assert np.array_equal(gpu.rate_counts(2), c2)
+
+
 def test_time_only_advances_and_rules_list_synapses_once():
     d = _plastic_net(RULE_WD, 70)
     gpu = abi.Network(d)
