@@ -44,7 +44,7 @@ enum {
 };
 
 /* synapse models (Spike/Synapses/{Voltage,Conductance}SpikingSynapses.hpp) */
-enum { SPK_SYN_VOLTAGE = 0, SPK_SYN_CONDUCTANCE = 1 };
+enum { SPK_SYN_VOLTAGE = 0, SPK_SYN_CONDUCTANCE = 1, SPK_SYN_CURRENT = 2 };
 
 /* populations */
 enum { SPK_POP_NEURONS = 0, SPK_POP_INPUTS = 1 };
@@ -178,7 +178,12 @@ int spk_set_external_drive(spk_network* net, float rate_hz, float weight);
 int spk_set_weight_format(spk_network* net, int format);
 
 /* Conductance synapse labels.  Replaces ConductanceSpikingSynapses::prepare
- * (CUDA ConductanceSpikingSynapses.cu:19-87): decay_g[l] = expf(-dt/tau_g[l]). */
+ * (CUDA ConductanceSpikingSynapses.cu:19-87): decay_g[l] = expf(-dt/tau_g[l]).
+ * Current synapses (SPK_SYN_CURRENT, CurrentSpikingSynapses.cu:13-83) use the same call:
+ * decay_g[l] = expf(-dt/tau[l]) of the current trace, reversal ignored; the injected
+ * voltage is R' * sum_l I_l, with I_l = decay_g[l] * I_l + input (the reference kernel
+ * never allocates the trace array, never copies tau to it and clears a different ring
+ * index than it reads, InlineDeviceFunctions.hpp:52-84: none of that is reproduced). */
 int spk_set_labels(spk_network* net, const float* decay_g, const float* reversal);
 
 /* Synapses.  Replaces Backend::SpikingSynapses::prepare / Synapses::prepare

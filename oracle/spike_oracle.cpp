@@ -446,7 +446,8 @@ struct Net {
             }
             dV = total;
           } else {
-            // InlineDeviceFunctions.hpp:22-51
+            // conductance: InlineDeviceFunctions.hpp:22-51; current (kind 2): :52-84, the
+            // trace itself instead of g (E - V), the slot it reads cleared
             R total = (R)0;
             for (int l = 0; l < L; ++l) {
               Acc a = ring[slot * NL + (size_t)i * L + l];
@@ -455,7 +456,8 @@ struct Net {
               R in;
               if constexpr (kFixed) in = (float)(double)a * inv; else in = (R)a;
               R gl = (R)decay_g[l] * gcond[(size_t)i * L + l] + in;
-              total += gl * ((R)Erev[l] - Vi);
+              if (kind == 1) total += gl * ((R)Erev[l] - Vi);
+              else total += gl;
               gcond[(size_t)i * L + l] = gl;
             }
             dV = total * (R)Rp[i];

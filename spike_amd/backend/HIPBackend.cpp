@@ -109,13 +109,14 @@ void ModelNetwork::build() {
   ::SpikingSynapses* syn = m->spiking_synapses;
   auto* cond = dynamic_cast<::ConductanceSpikingSynapses*>(syn);
   auto* volt = dynamic_cast<::VoltageSpikingSynapses*>(syn);
-  if (syn->total_number_of_synapses > 0 && !cond && !volt)
-    die("HIP backend: synapses must be VoltageSpikingSynapses or ConductanceSpikingSynapses");
+  auto* curr = dynamic_cast<::CurrentSpikingSynapses*>(syn);
+  if (syn->total_number_of_synapses > 0 && !cond && !volt && !curr)
+    die("HIP backend: synapses must be VoltageSpikingSynapses, ConductanceSpikingSynapses or CurrentSpikingSynapses");
 
   const int N = lif->total_number_of_neurons;
   const int Nin = in ? in->total_number_of_neurons : 0;
-  const int kind = cond ? SPK_SYN_CONDUCTANCE : SPK_SYN_VOLTAGE;
-  const int L = cond ? std::max(1, cond->num_syn_labels) : 1;
+  const int kind = cond ? SPK_SYN_CONDUCTANCE : (curr ? SPK_SYN_CURRENT : SPK_SYN_VOLTAGE);
+  const int L = cond ? std::max(1, cond->num_syn_labels) : (curr ? std::max(1, curr->num_syn_labels) : 1);
   const float dt = m->timestep;
   check(spk_create(N, Nin, kind, L, dt, &net), "spk_create");
 
@@ -174,6 +175,13 @@ void ModelNetwork::build() {
       dg[l] = expf(-dt / cond->decay_terms_tau_g[l]);
       ev[l] = cond->reversal_potentials_Vhat[l];
     }
+    check(spk_set_labels(net, dg.data(), ev.data()), "spk_set_labels");
+  }
+  if (curr) {
+    // the current traces' decay per label, expf(-dt/tau) as the reference kernel computes
+    // it (InlineDeviceFunctions.hpp:66-68), evaluated here on the host once
+    std::vector<float> dg(L, 1.0f), ev(L, 0.0f);
+    for (int l = 0; l < L && l < (int)curr->decay_terms_tau.size(); ++l) dg[l] = expf(-dt / curr->decay_terms_tau[l]);
     check(spk_set_labels(net, dg.data(), ev.data()), "spk_set_labels");
   }
   const int S = syn->total_number_of_synapses;
@@ -374,13 +382,6 @@ class PatternedPoissonInputSpikingNeurons : public virtual ::Backend::HIP::Poiss
   SPIKE_MAKE_BACKEND_CONSTRUCTOR(PatternedPoissonInputSpikingNeurons);
   SPK_HIP_UNSUPPORTED_PREPARE("PatternedPoissonInputSpikingNeurons")
   void copy_rates_to_device() override {}
-};
-
-class CurrentSpikingSynapses : public virtual ::Backend::HIP::SpikingSynapses,
-                               public virtual ::Backend::CurrentSpikingSynapses {
- public:
-  SPIKE_MAKE_BACKEND_CONSTRUCTOR(CurrentSpikingSynapses);
-  SPK_HIP_UNSUPPORTED_PREPARE("CurrentSpikingSynapses")
 };
 
 class CustomSTDPPlasticity : public virtual ::Backend::HIP::STDPPlasticity,

@@ -119,6 +119,12 @@ class ConductanceSpikingSynapses : public virtual ::Backend::HIP::SpikingSynapse
   SPIKE_MAKE_BACKEND_CONSTRUCTOR(ConductanceSpikingSynapses);
 };
 
+class CurrentSpikingSynapses : public virtual ::Backend::HIP::SpikingSynapses,
+                               public virtual ::Backend::CurrentSpikingSynapses {
+ public:
+  SPIKE_MAKE_BACKEND_CONSTRUCTOR(CurrentSpikingSynapses);
+};
+
 class Plasticity : public virtual ::Backend::Plasticity {
  public:
   ~Plasticity() override = default;

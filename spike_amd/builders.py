@@ -8,7 +8,7 @@ from __future__ import annotations
 
 import numpy as np
 
-from .netdesc import (F32, NetDesc, Rule, RULE_EVANS, RULE_VOGELS, RULE_WD, SYN_CONDUCTANCE,
+from .netdesc import (F32, NetDesc, Rule, RULE_EVANS, RULE_VOGELS, RULE_WD, SYN_CONDUCTANCE, SYN_CURRENT,
                       SYN_VOLTAGE, conductance_decay, lif_constants)
 
 
@@ -90,6 +90,11 @@ def random_net(n_neurons=64, n_inputs=32, p=0.2, kind=SYN_VOLTAGE, n_labels=1, s
         taus = [5e-3, 10e-3, 3e-3, 7e-3, 2e-3, 4e-3, 6e-3, 8e-3][:n_labels]
         d.decay_g = np.array([conductance_decay(dt, t) for t in taus], F32)
         d.reversal = np.array([0.0, -80e-3, 0.0, -70e-3, 0.0, -75e-3, 0.0, -90e-3][:n_labels], F32)
+    elif kind == SYN_CURRENT:
+        # current traces: decay expf(-dt/tau) per label; reversal potentials unused
+        taus = [2e-3, 5e-3, 1e-3, 3e-3, 4e-3, 6e-3, 8e-3, 10e-3][:n_labels]
+        d.decay_g = np.array([conductance_decay(dt, t) for t in taus], F32)
+        d.reversal = np.zeros(n_labels, F32)
     if n_inputs:
         d.rates = rng.uniform(0.5 * input_rate, 1.5 * input_rate, (n_stimuli, n_inputs)).astype(F32)
     d.ext_rate, d.ext_weight = float(ext_rate), float(ext_w)

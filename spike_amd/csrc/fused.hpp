@@ -606,7 +606,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const int p, cons
     thr = a.thr[gi];
     vr = a.vreset[gi];
     rp = a.rp[gi];
-    if (KIND == 1)
+    if (KIND != 0)
       for (int l = 0; l < L; ++l) gl[l] = a.gcond[(size_t)gi * L + l];
   }
   unsigned long long st_del = 0, st_ext = 0, st_spk = 0, st_in = 0;
@@ -746,7 +746,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const int p, cons
                 if (l < L) {
                   const float in = sinp[((size_t)tid * G + g) * L + l];
                   gl[l] = a.decay_g[l] * gl[l] + in;
-                  total += gl[l] * (a.erev[l] - V);
+                  total += (KIND == 1) ? gl[l] * (a.erev[l] - V) : gl[l];
                 }
               }
               const float dV = total * rp;
@@ -960,7 +960,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const int p, cons
   if (own) {
     a.V[gi] = V;
     a.last_n[gi] = last;
-    if (KIND == 1)
+    if (KIND != 0)
       for (int l = 0; l < L; ++l) a.gcond[(size_t)gi * L + l] = gl[l];
   }
   // write the whole global ring slice back in its B-slot layout (zeros past the LDS window)

@@ -276,7 +276,7 @@ __global__ __launch_bounds__(256) void k_lif(LifArgs a) {
   const float rp = a.rp[ii];
   const size_t NL = (size_t)a.N * a.L;
   float gl[kMaxLabels];
-  if (KIND == 1) {
+  if (KIND != 0) {
 #pragma unroll
     for (int l = 0; l < kMaxLabels; ++l) gl[l] = (l < a.L) ? a.gcond[(size_t)ii * a.L + l] : 0.0f;
   }
@@ -339,7 +339,9 @@ __global__ __launch_bounds__(256) void k_lif(LifArgs a) {
           if (active) rs[l] = 0;
           const float in = fixed_to_float(q) * a.inv_scale;
           gl[l] = a.decay_g[l] * gl[l] + in;
-          total += gl[l] * (a.erev[l] - V);
+          // conductance: g (E - V) (InlineDeviceFunctions.hpp:22-51); current: the
+          // current trace itself (:52-84)
+          total += (KIND == 1) ? gl[l] * (a.erev[l] - V) : gl[l];
         }
       }
       dV = total * rp;
@@ -368,7 +370,7 @@ __global__ __launch_bounds__(256) void k_lif(LifArgs a) {
     if (a.rate != nullptr && nspk) a.rate[i] += nspk;
     a.V[i] = V;
     a.last[i] = last;
-    if (KIND == 1) {
+    if (KIND != 0) {
       for (int l = 0; l < a.L; ++l) a.gcond[(size_t)i * a.L + l] = gl[l];
     }
     if (a.st.on) {

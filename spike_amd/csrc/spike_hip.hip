@@ -456,8 +456,10 @@ int launch_neurons(spk_network* n, float t, int slot) {
   const int grid = grid_for(n->N, 256);
   if (n->kind == SPK_SYN_VOLTAGE)
     hipLaunchKernelGGL(spk::k_lif<0>, dim3(grid), dim3(256), 0, n->stream, a);
-  else
+  else if (n->kind == SPK_SYN_CONDUCTANCE)
     hipLaunchKernelGGL(spk::k_lif<1>, dim3(grid), dim3(256), 0, n->stream, a);
+  else
+    hipLaunchKernelGGL(spk::k_lif<2>, dim3(grid), dim3(256), 0, n->stream, a);
   SPK_HIP(hipGetLastError());
   return SPK_OK;
 }
@@ -686,7 +688,8 @@ const void* fused_kernel(const spk_network* n, bool plastic) {
   (plastic ? (l1 ? (const void*)spk::k_fused<K, true, true> : (const void*)spk::k_fused<K, false, true>) \
            : (l1 ? (const void*)spk::k_fused<K, true, false> : (const void*)spk::k_fused<K, false, false>))
   if (n->kind == SPK_SYN_VOLTAGE) return SPK_FUSED_PICK(0);
-  return SPK_FUSED_PICK(1);
+  if (n->kind == SPK_SYN_CONDUCTANCE) return SPK_FUSED_PICK(1);
+  return SPK_FUSED_PICK(2);
 #undef SPK_FUSED_PICK
 }
 
@@ -907,7 +910,7 @@ int spk_create(int n_neurons, int n_inputs, int synapse_kind, int n_labels, floa
   *out = nullptr;
   if (n_neurons < 0 || n_inputs < 0) return fail(SPK_ERR_ARG, "negative population size");
   if ((int64_t)n_neurons + n_inputs >= (int64_t)1 << 30) return fail(SPK_ERR_ARG, "too many neurons");
-  if (synapse_kind != SPK_SYN_VOLTAGE && synapse_kind != SPK_SYN_CONDUCTANCE)
+  if (synapse_kind != SPK_SYN_VOLTAGE && synapse_kind != SPK_SYN_CONDUCTANCE && synapse_kind != SPK_SYN_CURRENT)
     return fail(SPK_ERR_ARG, "unknown synapse kind");
   if (n_labels < 1 || n_labels > spk::kMaxLabels)
     return fail(SPK_ERR_ARG, "n_labels must be in [1, 8]");
@@ -2229,7 +2232,9 @@ template <int BS>
 const void* batch_kernel_bs(const spk_batch* b) {
   if (b->kind == SPK_SYN_VOLTAGE)
     return b->l1 ? (const void*)spk::k_fused_batch<0, true, false, BS> : (const void*)spk::k_fused_batch<0, false, false, BS>;
-  return b->l1 ? (const void*)spk::k_fused_batch<1, true, false, BS> : (const void*)spk::k_fused_batch<1, false, false, BS>;
+  if (b->kind == SPK_SYN_CONDUCTANCE)
+    return b->l1 ? (const void*)spk::k_fused_batch<1, true, false, BS> : (const void*)spk::k_fused_batch<1, false, false, BS>;
+  return b->l1 ? (const void*)spk::k_fused_batch<2, true, false, BS> : (const void*)spk::k_fused_batch<2, false, false, BS>;
 }
 
 const void* batch_kernel(const spk_batch* b, int bs) {

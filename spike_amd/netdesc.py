@@ -26,6 +26,7 @@ F32 = np.float32
 WEIGHTS_FP32, WEIGHTS_BF16, WEIGHTS_FP16 = 0, 1, 2
 SYN_VOLTAGE = 0
 SYN_CONDUCTANCE = 1
+SYN_CURRENT = 2
 POP_NEURONS = 0
 POP_INPUTS = 1
 

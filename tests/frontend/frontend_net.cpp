@@ -16,6 +16,7 @@
 //   cases: brunel (voltage synapses, Poisson inputs, optional WD STDP),
 //          coba (conductance synapses, two labels, Vogels + Evans STDP),
 //          plain (Poisson -> LIF, RANDOM conductance synapses, delays 1-10 steps),
+//          current (Poisson -> LIF E/I, CurrentSpikingSynapses with two decay constants),
 //          generator (Examples/SimpleExample.cpp's layout: generator inputs -> conductance
 //          LIF E/I; stimulus 0 for simtime, then stimulus 1 selected for another simtime;
 //          gen_{counts,ids,times}.bin hold the stimuli)
@@ -195,6 +196,39 @@ int main(int argc, char** argv) {
       xi->decay_term_tau_g = 0.001f;
       model->AddSynapseGroup(in_id, e_id, xi);
     }
+  } else if (which == "current") {
+    // exponential current synapses: input -> E and E -> I with tau 2 ms, I -> E with 5 ms
+    CurrentSpikingSynapses* syn = new CurrentSpikingSynapses(42);
+    model->spiking_synapses = syn;
+    lif_spiking_neuron_parameters_struct* E = new lif_spiking_neuron_parameters_struct();
+    lif_spiking_neuron_parameters_struct* I = new lif_spiking_neuron_parameters_struct();
+    lif(E, 200e-12f, 10e-9f, -0.070f, -0.070f, -0.050f, 2e-3f, 150);
+    lif(I, 200e-12f, 10e-9f, -0.070f, -0.070f, -0.050f, 2e-3f, 50);
+    poisson_input_spiking_neuron_parameters_struct* ip = new poisson_input_spiking_neuron_parameters_struct();
+    ip->group_shape[0] = 1;
+    ip->group_shape[1] = 100;
+    ip->rate = 60.0f;
+    const int in_id = model->AddInputNeuronGroup(ip);
+    const int e_id = model->AddNeuronGroup(E);
+    const int i_id = model->AddNeuronGroup(I);
+    current_spiking_synapse_parameters_struct* xe = new current_spiking_synapse_parameters_struct();
+    current_spiking_synapse_parameters_struct* ie = new current_spiking_synapse_parameters_struct();
+    for (auto* c : {xe, ie}) {
+      c->delay_range[0] = 0.0002f;
+      c->delay_range[1] = 0.0012f;
+      c->connectivity_type = CONNECTIVITY_TYPE_RANDOM;
+      c->random_connectivity_probability = 0.1f;
+      c->weight_scaling_constant = 1e-10f;
+    }
+    xe->weight_range[0] = 1.0f;
+    xe->weight_range[1] = 4.0f;
+    xe->decay_term_tau = 0.002f;
+    ie->weight_range[0] = -6.0f;
+    ie->weight_range[1] = -2.0f;
+    ie->decay_term_tau = 0.005f;
+    model->AddSynapseGroup(in_id, e_id, xe);
+    model->AddSynapseGroup(e_id, i_id, xe);
+    model->AddSynapseGroup(i_id, e_id, ie);
   } else if (which == "generator") {
     // Examples/SimpleExample.cpp:33-150 with more inputs and neurons, RANDOM E -> I
     ConductanceSpikingSynapses* syn = new ConductanceSpikingSynapses(42);
@@ -301,6 +335,12 @@ int main(int argc, char** argv) {
       rev.push_back(cs->reversal_potentials_Vhat[l]);
     }
   }
+  if (auto* cu = dynamic_cast<CurrentSpikingSynapses*>(s)) {
+    for (size_t l = 0; l < cu->decay_terms_tau.size(); ++l) {
+      decay.push_back(expf(-dt / cu->decay_terms_tau[l]));
+      rev.push_back(0.0f);
+    }
+  }
   put("decay_g", decay.data(), decay.size());
   put("reversal", rev.data(), rev.size());
   for (size_t r = 0; r < rules.size(); ++r)
@@ -325,7 +365,8 @@ int main(int argc, char** argv) {
     << ", \"timestep_grouping\": " << model->timestep_grouping << ", \"refractory\": "
     << neurons->refractory_period_in_seconds << ", \"background_current\": " << neurons->background_current
     << ", \"num_syn_labels\": " << s->num_syn_labels << ", \"conductance\": "
-    << (dynamic_cast<ConductanceSpikingSynapses*>(s) ? 1 : 0) << ", \"simtime\": " << simtime
+    << (dynamic_cast<ConductanceSpikingSynapses*>(s) ? 1 : 0) << ", \"current\": "
+    << (dynamic_cast<CurrentSpikingSynapses*>(s) ? 1 : 0) << ", \"simtime\": " << simtime
     << ", \"plastic\": " << (plastic ? 1 : 0) << ", \"generator\": " << (generator ? 1 : 0)
     << ", \"rule_kinds\": [";
   for (size_t r = 0; r < rule_kinds.size(); ++r) j << (r ? ", " : "") << "\"" << rule_kinds[r] << "\"";

@@ -49,18 +49,20 @@ def desc_from_dump(meta, arr):
     rp = (c * R).astype(F32)
     n_in = meta["n_inputs"]
     cond = meta["conductance"] == 1
+    curr = meta.get("current", 0) == 1
     kinds = {"wd": netdesc.RULE_WD, "vogels": netdesc.RULE_VOGELS, "evans": netdesc.RULE_EVANS}
     rules = []
     for r, (k, p) in enumerate(zip(meta["rule_kinds"], meta["rule_params"])):
         rules.append(netdesc.Rule(kinds[k], arr("plastic_%d" % r, np.int32).astype(np.int64), p))
     d = netdesc.NetDesc(
         n_neurons=meta["n_neurons"], n_inputs=n_in,
-        kind=netdesc.SYN_CONDUCTANCE if cond else netdesc.SYN_VOLTAGE,
-        n_labels=max(1, meta["num_syn_labels"]) if cond else 1, dt=float(dt),
+        kind=netdesc.SYN_CONDUCTANCE if cond else (netdesc.SYN_CURRENT if curr else netdesc.SYN_VOLTAGE),
+        n_labels=max(1, meta["num_syn_labels"]) if (cond or curr) else 1, dt=float(dt),
         v0=arr("lif_v0", F32), vreset=arr("lif_vreset", F32), threshold=arr("lif_thr", F32),
         decay_c=c, resist=rp, refractory=meta["refractory"], background=meta["background_current"],
         rates=arr("rates", F32).reshape(1, n_in), seed=1,
-        decay_g=arr("decay_g", F32) if cond else None, reversal=arr("reversal", F32) if cond else None,
+        decay_g=arr("decay_g", F32) if (cond or curr) else None,
+        reversal=arr("reversal", F32) if (cond or curr) else None,
         pre=arr("syn_pre", np.int32), post=arr("syn_post", np.int32), weight=arr("syn_w", F32),
         scale=arr("syn_scale", F32), delay=arr("syn_delay", np.int32), label=arr("syn_label", np.int32),
         rules=rules, monitors=[netdesc.POP_NEURONS, netdesc.POP_INPUTS],
@@ -95,7 +97,7 @@ def sort_raster(ids, times):
 
 
 CASES = [("brunel", 0.1, False), ("brunel", 0.1, True), ("coba", 0.1, True), ("plain", 0.05, False),
-         ("generator", 0.1, False)]
+         ("generator", 0.1, False), ("current", 0.1, False)]
 
 
 @pytest.mark.gpu

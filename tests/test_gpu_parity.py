@@ -28,7 +28,7 @@ def _compare(gpu, ora, d, tag=""):
         assert np.array_equal(gpu.history(1), ora.history(1)), f"{tag} input history differs"
     assert np.array_equal(gpu.history(0), ora.history(0)), f"{tag} history differs"
     assert np.array_equal(gpu.ring(), ora.ring()), f"{tag} input ring differs"
-    if d.kind == SYN_CONDUCTANCE:
+    if d.kind != 0:   # conductances, or the current traces of current synapses
         assert np.array_equal(gpu.conductances().astype(np.float64), ora.conductances()), f"{tag} g"
     if d.n_synapses:
         assert np.array_equal(gpu.weights().astype(np.float64), ora.weights()), f"{tag} weights"
@@ -315,6 +315,20 @@ def test_rate_monitor_counts(path):
     c2 = np.bincount(oi2, minlength=d.n_neurons)
     assert np.array_equal(gpu.rate_counts(0), cn + c2)
     assert np.array_equal(gpu.rate_counts(2), c2)
+
+
+@pytest.mark.parametrize("labels", [1, 3])
+@pytest.mark.parametrize("fused", [True, False])
+def test_current_synapses(labels, fused):
+    """Exponential current synapses (CurrentSpikingSynapses with the reference kernel's
+    defects fixed): per label the current trace decays and takes the slot's input, and
+    the neuron receives R' times their sum.  Persistent and per-phase paths."""
+    d = builders.random_net(seed=95 + labels, kind=2, n_labels=labels, input_rate=500.0, whi=6e-10,
+                            scale=1.0, dmin=2, dmax=6)
+    gpu, ora = _run_both(d, 150, partitions=5, launches=not fused)
+    assert gpu.info().synapse_kind == 2
+    assert ora.counters()["spikes_neurons"] > 0
+    _compare(gpu, ora, d, f"current L={labels}")
 
 
 def test_time_only_advances_and_rules_list_synapses_once():

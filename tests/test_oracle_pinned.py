@@ -217,3 +217,37 @@ def test_frac_bits_rule():
     assert lib.or_choose_frac_bits(1.0) == 52
     assert lib.or_choose_frac_bits(0.1) == 52 + 3
     assert lib.or_choose_frac_bits(0.0) == 32
+
+
+def test_current_synapse_trace_and_injection():
+    """One generator input spike (weight w, delay 2) into a current synapse of one LIF
+    neuron: the trace is w at the arrival step and decays by d = expf(-dt/tau) per step
+    after it; each step the membrane takes c*v0 + (1-c)*V + R'*I, evaluated left to right
+    in float32 -- recomputed here step by step, and the trace also against the closed
+    form w d^k."""
+    from spike_amd.netdesc import SYN_CURRENT, conductance_decay
+    d = NetDesc(n_neurons=1, n_inputs=1, kind=SYN_CURRENT, n_labels=1)
+    d.v0 = np.full(1, F32(-70e-3), F32)
+    d.vreset = np.full(1, F32(-70e-3), F32)
+    d.threshold = np.full(1, F32(1.0), F32)
+    d.decay_c, d.resist = lif_constants(1, 200e-12, 10e-9, d.dt)
+    d.pre, d.post = np.array([-1], np.int32), np.array([0], np.int32)
+    w = F32(2.0 ** -30)                        # exact in the fixed-point ring
+    d.weight, d.delay = np.array([w], F32), np.array([2], np.int32)
+    d.decay_g = np.array([conductance_decay(d.dt, 3e-3)], F32)
+    d.reversal = np.zeros(1, F32)
+    d.stimuli = [(np.array([0], np.int32), np.array([0.0010], F32))]   # fires at step 10
+    d.grouping = 1
+    o = oracle_py.OracleNetwork(d, precision=32)
+    dec, c, rp, v0 = d.decay_g[0], d.decay_c[0], d.resist[0], d.v0[0]
+    I, V = F32(0.0), v0
+    for step in range(30):
+        o.run(float(F32(step) * F32(1e-4)), 1)
+        inp = w if step == 12 else F32(0.0)
+        I = F32(dec * I + inp)
+        V = F32(F32(F32(c * v0) + F32(F32(F32(1.0) - c) * V)) + F32(0.0)) + F32(I * rp)
+        assert o.conductances()[0] == I, step
+        assert o.membrane()[0] == np.float64(V), step
+        if step >= 12:
+            assert abs(float(I) - float(w) * float(dec) ** (step - 12)) <= 1e-6 * float(w)
+    assert I > 0 and V > v0
