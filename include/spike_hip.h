@@ -82,6 +82,14 @@ typedef struct {
         learning_rate_rho;
 } spk_evans_stdp_params;
 
+/* Custom STDP parameters.
+ * Replaces custom_stdp_plasticity_parameters_struct
+ * (Spike/Plasticity/CustomSTDPPlasticity.hpp:31-40). */
+typedef struct {
+  float a_minus, a_plus, a_star, tau_minus, tau_plus, w_max, learning_rate;
+  int nearest_spike_only;
+} spk_custom_stdp_params;
+
 /* Sizes fixed by spk_prepare. */
 typedef struct {
   int n_neurons, n_inputs, n_labels, synapse_kind;
@@ -204,6 +212,13 @@ int spk_add_vogels_stdp(spk_network* net, const int64_t* synapses, int64_t n,
                         const spk_vogels_stdp_params* p, int* rule_id);
 int spk_add_evans_stdp(spk_network* net, const int64_t* synapses, int64_t n,
                        const spk_evans_stdp_params* p, int* rule_id);
+/* CustomSTDPPlasticity (CustomSTDPPlasticity.cu:13-159): per sub-step, traces decayed by
+ * expf(-dt/tau_minus) (post) and expf(-dt/tau_plus) (pre), raised by a_minus / a_plus
+ * (or set to them, nearest_spike_only) at the delayed pre and the post spike; a pre spike
+ * depresses by lr*(w/w_max)*x_post + lr*a_star, a post spike potentiates by lr*x_pre;
+ * weights clamped at 0. */
+int spk_add_custom_stdp(spk_network* net, const int64_t* synapses, int64_t n,
+                        const spk_custom_stdp_params* p, int* rule_id);
 
 /* Spike monitors.  Replaces Backend::SpikingActivityMonitor::prepare
  * (CUDA SpikingActivityMonitor.cu:22-33): a bounded device store of `capacity`

@@ -44,6 +44,7 @@ SIG = {
     "or_add_wd_stdp": (C.c_int, [P, lp, C.c_int64, fp, C.c_int]),
     "or_add_vogels_stdp": (C.c_int, [P, lp, C.c_int64, fp]),
     "or_add_evans_stdp": (C.c_int, [P, lp, C.c_int64, fp]),
+    "or_add_custom_stdp": (C.c_int, [P, lp, C.c_int64, fp, C.c_int]),
     "or_add_monitor": (C.c_int, [P, C.c_int]),
     "or_prepare": (C.c_int, [P, C.c_int]),
     "or_set_timestep_grouping": (C.c_int, [P, C.c_int]),
@@ -177,6 +178,11 @@ class OracleNetwork:
             elif r.kind == 1:
                 par = f32([pr["tau_istdp"], pr["learningrate"], pr["targetrate"], pr.get("momentumrate", 0.0)])
                 check(lib.or_add_vogels_stdp(h, _p(syn, C.c_int64), len(syn), _p(par, C.c_float)))
+            elif r.kind == 3:
+                par = f32([pr["a_minus"], pr["a_plus"], pr["a_star"], pr["tau_minus"], pr["tau_plus"],
+                           pr["w_max"], pr["learning_rate"]])
+                check(lib.or_add_custom_stdp(h, _p(syn, C.c_int64), len(syn), _p(par, C.c_float),
+                                             int(pr.get("nearest_spike_only", 0))))
             else:
                 par = f32([pr["decay_term_tau_D"], pr["model_parameter_alpha_D"],
                            pr["synaptic_neurotransmitter_concentration_alpha_C"],

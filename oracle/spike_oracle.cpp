@@ -17,6 +17,7 @@
 //   WD STDP ............. Spike/Backend/CUDA/Plasticity/WeightDependentSTDPPlasticity.cu:67-163
 //   Vogels iSTDP ........ Spike/Backend/CUDA/Plasticity/InhibitorySTDPPlasticity.cu:238-321
 //   Evans STDP .......... Spike/Backend/CUDA/Plasticity/EvansSTDPPlasticity.cu:59-143
+//   Custom STDP ......... Spike/Backend/CUDA/Plasticity/CustomSTDPPlasticity.cu:67-155
 //   nearest-spike traces  Spike/Backend/CUDA/Plasticity/CustomSTDPPlasticity.cu:117-128
 //   raster .............. Spike/Backend/CUDA/ActivityMonitor/SpikingActivityMonitor.cu:69-98
 //   group loop .......... Spike/Models/SpikingModel.cpp:266-305
@@ -201,11 +202,12 @@ static float o_round16(float x, int fmt) {
 }
 
 struct Rule {
-  int kind;  // 0 WD, 1 Vogels, 2 Evans
+  int kind;  // 0 WD, 1 Vogels, 2 Evans, 3 Custom
   std::vector<int64_t> syn;  // caller's synapse indices
   float wd[8]; int nearest;
   float vg[4];
   float ev[5];
+  float cu[7];  // a_minus, a_plus, a_star, tau_minus, tau_plus, w_max, learning_rate
   std::vector<double> ta, tb;  // traces per plastic synapse
 };
 
@@ -333,7 +335,8 @@ struct Net {
     std::vector<double> cap(S, 0.0);
     for (int64_t s = 0; s < S; ++s) cap[s] = std::fabs((double)sw[s]);
     for (auto& r : rules) for (int64_t s : r.syn) {
-      double c0 = r.kind == 0 ? std::fabs((double)r.wd[6]) : (r.kind == 2 ? 1.0 : 0.0);
+      double c0 = r.kind == 0 ? std::fabs((double)r.wd[6])
+                : (r.kind == 2 ? 1.0 : (r.kind == 3 ? std::fabs((double)r.cu[5]) : 0.0));
       cap[s] = std::max(cap[s], c0);
     }
     std::vector<double> ent((size_t)N * L, 0.0);
@@ -646,6 +649,37 @@ struct Net {
         wstore(s, w0, w, 0.0f, INFINITY);
         r.ta[j] = tstore(xpre); r.tb[j] = tstore(xpost);
       }
+    } else if (r.kind == 3) {
+      // CustomSTDPPlasticity.cu:67-155: per sub-step, traces decayed (post by tau_minus,
+      // pre by tau_plus) then set or raised by the delayed pre / the post spike; a pre spike
+      // depresses by lr*(w/w_max)*x_post + lr*a_star, a post spike potentiates by lr*x_pre;
+      // weights clamped at 0 only.  Decays computed in float on the host.
+      const float am = r.cu[0], ap = r.cu[1], ast = r.cu[2], wmax = r.cu[5], lr = r.cu[6];
+      const float post_decay = std::exp(-dt / r.cu[3]);
+      const float pre_decay = std::exp(-dt / r.cu[4]);
+      for (int64_t j = 0; j < P; ++j) {
+        int64_t s = r.syn[j];
+        R xpre = (R)r.ta[j], xpost = (R)r.tb[j];
+        const R w0 = sw[s];
+        R w = w0;
+        for (int g = 0; g < G; ++g) {
+          xpost *= (R)post_decay;
+          xpre *= (R)pre_decay;
+          int64_t step = (int64_t)tidx + g;
+          bool pre = pre_bit(spre[s], step - sdelay[s]);
+          bool post = get_bit(hist_n, N, W, spost[s], step);
+          if (pre) { xpre += (R)ap; if (r.nearest) xpre = (R)ap; }
+          if (post) { xpost += (R)am; if (r.nearest) xpost = (R)am; }
+          R upd = (R)0;
+          const R wo = w;
+          if (pre) upd -= (((R)lr * (wo / (R)wmax)) * xpost + (R)lr * (R)ast);
+          if (post) upd += (R)lr * xpre;
+          w = wo + upd;
+          if (w < (R)0) w = (R)0;
+        }
+        wstore(s, w0, w, 0.0f, INFINITY);
+        r.ta[j] = tstore(xpre); r.tb[j] = tstore(xpost);
+      }
     } else {
       // EvansSTDPPlasticity.cu:59-143
       const float dC = 1.0f - (dt / r.ev[3]);
@@ -862,6 +896,7 @@ static int add_rule(void* p, int kind, const int64_t* syn, int64_t n, const floa
     if (kind == 0) std::memcpy(r.wd, par, sizeof(float) * npar);
     if (kind == 1) std::memcpy(r.vg, par, sizeof(float) * npar);
     if (kind == 2) std::memcpy(r.ev, par, sizeof(float) * npar);
+    if (kind == 3) std::memcpy(r.cu, par, sizeof(float) * npar);
     r.nearest = nearest;
     net.rules.push_back(r);
     return 0;
@@ -879,6 +914,10 @@ int or_add_vogels_stdp(void* p, const int64_t* syn, int64_t n, const float* par)
 // par: tau_D, alpha_D, alpha_C, tau_C, rho
 int or_add_evans_stdp(void* p, const int64_t* syn, int64_t n, const float* par) {
   return add_rule(p, 2, syn, n, par, 5, 0);
+}
+// par: a_minus, a_plus, a_star, tau_minus, tau_plus, w_max, learning_rate
+int or_add_custom_stdp(void* p, const int64_t* syn, int64_t n, const float* par, int nearest) {
+  return add_rule(p, 3, syn, n, par, 7, nearest);
 }
 
 int or_add_monitor(void* p, int pop) {

@@ -12,7 +12,7 @@ from typing import Optional
 
 import numpy as np
 
-from .netdesc import NetDesc, RULE_WD, RULE_VOGELS, RULE_EVANS
+from .netdesc import NetDesc, RULE_WD, RULE_VOGELS, RULE_EVANS, RULE_CUSTOM
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "_lib", "libspike_hip.so")
@@ -40,6 +40,12 @@ class EvansParams(C.Structure):
     _fields_ = [("decay_term_tau_D", C.c_float), ("model_parameter_alpha_D", C.c_float),
                 ("synaptic_neurotransmitter_concentration_alpha_C", C.c_float),
                 ("decay_term_tau_C", C.c_float), ("learning_rate_rho", C.c_float)]
+
+
+class CustomParams(C.Structure):
+    _fields_ = [("a_minus", C.c_float), ("a_plus", C.c_float), ("a_star", C.c_float),
+                ("tau_minus", C.c_float), ("tau_plus", C.c_float), ("w_max", C.c_float),
+                ("learning_rate", C.c_float), ("nearest_spike_only", C.c_int)]
 
 
 class Info(C.Structure):
@@ -104,6 +110,7 @@ SIGNATURES = {
     "spk_add_wd_stdp": (C.c_int, [P, lp, C.c_int64, C.POINTER(WDParams), C.POINTER(C.c_int)]),
     "spk_add_vogels_stdp": (C.c_int, [P, lp, C.c_int64, C.POINTER(VogelsParams), C.POINTER(C.c_int)]),
     "spk_add_evans_stdp": (C.c_int, [P, lp, C.c_int64, C.POINTER(EvansParams), C.POINTER(C.c_int)]),
+    "spk_add_custom_stdp": (C.c_int, [P, lp, C.c_int64, C.POINTER(CustomParams), C.POINTER(C.c_int)]),
     "spk_add_monitor": (C.c_int, [P, C.c_int, C.c_int64, C.POINTER(C.c_int)]),
     "spk_prepare": (C.c_int, [P, C.c_int]),
     "spk_set_timestep_grouping": (C.c_int, [P, C.c_int]),
@@ -266,6 +273,9 @@ class Network:
             elif r.kind == RULE_EVANS:
                 p = EvansParams(**r.params)
                 check(lib.spk_add_evans_stdp(h, _ptr(syn, C.c_int64), len(syn), C.byref(p), C.byref(rid)))
+            elif r.kind == RULE_CUSTOM:
+                p = CustomParams(**r.params)
+                check(lib.spk_add_custom_stdp(h, _ptr(syn, C.c_int64), len(syn), C.byref(p), C.byref(rid)))
             self.rule_ids.append(rid.value)
         self.monitor_ids = []
         for pop in d.monitors:

@@ -222,6 +222,17 @@ void ModelNetwork::build() {
       p.decay_term_tau_C = ev->stdp_params->decay_term_tau_C;
       p.learning_rate_rho = ev->stdp_params->learning_rate_rho;
       check(spk_add_evans_stdp(net, idx.data(), (int64_t)idx.size(), &p, &rid), "spk_add_evans_stdp");
+    } else if (auto* cs = dynamic_cast<::CustomSTDPPlasticity*>(rule)) {
+      spk_custom_stdp_params p{};
+      p.a_minus = cs->stdp_params->a_minus;
+      p.a_plus = cs->stdp_params->a_plus;
+      p.a_star = cs->stdp_params->a_star;
+      p.tau_minus = cs->stdp_params->tau_minus;
+      p.tau_plus = cs->stdp_params->tau_plus;
+      p.w_max = cs->stdp_params->w_max;
+      p.learning_rate = cs->stdp_params->learning_rate;
+      p.nearest_spike_only = cs->stdp_params->nearest_spike_only ? 1 : 0;
+      check(spk_add_custom_stdp(net, idx.data(), (int64_t)idx.size(), &p, &rid), "spk_add_custom_stdp");
     } else {
       die("HIP backend: unsupported plasticity rule");
     }
@@ -315,6 +326,8 @@ void WeightDependentSTDPPlasticity::apply_stdp_to_synapse_weights(float t, float
 void InhibitorySTDPPlasticity::apply_stdp_to_synapse_weights(float t, float) { mark_applied(t); }
 void EvansSTDPPlasticity::update_synaptic_efficacies_or_weights(float t, float) { mark_applied(t); }
 
+void CustomSTDPPlasticity::apply_stdp_to_synapse_weights(float t, float) { mark_applied(t); }
+
 void SpikingActivityMonitor::reset_state() {
   ModelNetwork* mn = model_network(front()->model);
   if (mn->built()) check(spk_monitor_reset(mn->net, mn->monitor(front())), "spk_monitor_reset");
@@ -382,14 +395,6 @@ class PatternedPoissonInputSpikingNeurons : public virtual ::Backend::HIP::Poiss
   SPIKE_MAKE_BACKEND_CONSTRUCTOR(PatternedPoissonInputSpikingNeurons);
   SPK_HIP_UNSUPPORTED_PREPARE("PatternedPoissonInputSpikingNeurons")
   void copy_rates_to_device() override {}
-};
-
-class CustomSTDPPlasticity : public virtual ::Backend::HIP::STDPPlasticity,
-                             public virtual ::Backend::CustomSTDPPlasticity {
- public:
-  SPIKE_MAKE_BACKEND_CONSTRUCTOR(CustomSTDPPlasticity);
-  SPK_HIP_UNSUPPORTED_PREPARE("CustomSTDPPlasticity")
-  void apply_stdp_to_synapse_weights(float, float) override {}
 };
 
 class WeightNormSTDPPlasticity : public virtual ::Backend::HIP::STDPPlasticity,

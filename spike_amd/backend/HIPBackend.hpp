@@ -162,6 +162,13 @@ class EvansSTDPPlasticity : public virtual ::Backend::HIP::STDPPlasticity,
   void update_synaptic_efficacies_or_weights(float current_time_in_seconds, float timestep) override;
 };
 
+class CustomSTDPPlasticity : public virtual ::Backend::HIP::STDPPlasticity,
+                             public virtual ::Backend::CustomSTDPPlasticity {
+ public:
+  SPIKE_MAKE_BACKEND_CONSTRUCTOR(CustomSTDPPlasticity);
+  void apply_stdp_to_synapse_weights(float current_time_in_seconds, float timestep) override;
+};
+
 class ActivityMonitor : public virtual ::Backend::ActivityMonitor {
  public:
   ~ActivityMonitor() override = default;

@@ -202,9 +202,15 @@ def evans_params(tau_D=0.005, alpha_D=0.5, alpha_C=0.5, tau_C=0.004, rho=0.1):
                 decay_term_tau_C=tau_C, learning_rate_rho=rho)
 
 
+def custom_params(a_minus=1.0, a_plus=1.0, a_star=0.0, tau_minus=0.02, tau_plus=0.02, w_max=1.0,
+                  learning_rate=1e-3, nearest=0):
+    return dict(a_minus=a_minus, a_plus=a_plus, a_star=a_star, tau_minus=tau_minus, tau_plus=tau_plus,
+                w_max=w_max, learning_rate=learning_rate, nearest_spike_only=nearest)
+
+
 def vogels_params(tau=0.02, lr=4e-4, rate=10.0):
     return dict(tau_istdp=tau, learningrate=lr, targetrate=rate, momentumrate=0.0)
 
 
-__all__ = ["brunel", "random_net", "plain_small", "coba_vogels", "large_plastic", "wd_params", "evans_params",
+__all__ = ["brunel", "random_net", "plain_small", "coba_vogels", "large_plastic", "wd_params", "evans_params", "custom_params",
            "vogels_params", "Rule", "RULE_WD", "RULE_VOGELS", "RULE_EVANS"]

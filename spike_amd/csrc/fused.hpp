@@ -484,7 +484,8 @@ __device__ __forceinline__ void fused_plasticity(const FusedRule* rules, int n_r
     for (int64_t j = j0 + threadIdx.x; j < j1; j += BS) {
       if (kind == 0) stdp_wd_one(ra, j);
       else if (kind == 1) stdp_vogels_one(ra, j);
-      else stdp_evans_one(ra, j);
+      else if (kind == 2) stdp_evans_one(ra, j);
+      else stdp_custom_one(ra, j);
     }
   }
 }

@@ -886,6 +886,44 @@ __global__ __launch_bounds__(256) void k_stdp_evans(StdpArgs a) {
   if (j < a.P) stdp_evans_one(a, j);
 }
 
+// CustomSTDPPlasticity.cu:67-155. tab = {post decay expf(-dt/tau_minus), pre decay
+// expf(-dt/tau_plus), a_plus, a_minus, a_star, w_max, learning_rate}; nearest: a spike
+// sets its trace to the amplitude instead of adding it.  Weights clamped at 0 only.
+__device__ __forceinline__ void stdp_custom_one(const StdpArgs& a, int64_t j) {
+  const int64_t s = a.pos[j];
+  const int u = a.pre[j];
+  const int post = a.tgt[s];
+  const int d = a.delay[s];
+  const float dpost = a.tab[0], dpre = a.tab[1], ap = a.tab[2], am = a.tab[3], ast = a.tab[4];
+  const float wmax = a.tab[5], lr = a.tab[6];
+  float xpre = tload(a.tr, 0, j), xpost = tload(a.tr, 1, j);
+  const float w0 = wload(a.wts, s);
+  float w = w0;
+  for (int g = 0; g < a.G; ++g) {
+    xpost *= dpost;
+    xpre *= dpre;
+    const int64_t step = (int64_t)a.tidx + g;
+    const bool pr = pre_bit(a, u, step - d);
+    const bool po = hist_get(a.hist_n, a.N, a.W, post, step);
+    if (pr) xpre = a.nearest ? ap : xpre + ap;
+    if (po) xpost = a.nearest ? am : xpost + am;
+    float upd = 0.0f;
+    const float wo = w;
+    if (pr) upd -= ((lr * (wo / wmax)) * xpost + lr * ast);
+    if (po) upd += lr * xpre;
+    w = wo + upd;
+    if (w < 0.0f) w = 0.0f;
+  }
+  wupdate(a.wts, s, w0, w, 0.0f, __builtin_inff());
+  tstore(a.tr, 0, j, xpre);
+  tstore(a.tr, 1, j, xpost);
+}
+
+__global__ __launch_bounds__(256) void k_stdp_custom(StdpArgs a) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < a.P) stdp_custom_one(a, j);
+}
+
 // ---------------------------------------------------------------------------------
 // Monitor collection (SpikingActivityMonitor.cu:69-98): copy the spikes of one
 // population from list slots [slot0, slot0+nslots) into a bounded store.

@@ -95,11 +95,13 @@ int choose_frac_bits(double max_entry) {
 }
 
 struct RuleHost {
-  int kind = 0;  // 0 WD, 1 Vogels, 2 Evans
+  int kind = 0;  // 0 WD, 1 Vogels, 2 Evans, 3 Custom
   std::vector<int64_t> syn;
   spk_wd_stdp_params wd{};
   spk_vogels_stdp_params vg{};
   spk_evans_stdp_params ev{};
+  spk_custom_stdp_params cu{};
+  int nearest = 0;   // nearest-spike traces (WD, Custom)
   // device
   int64_t* d_pos = nullptr;
   int32_t* d_pre = nullptr;
@@ -370,6 +372,9 @@ int refresh_rule_tables(spk_network* n, RuleHost& r) {
     for (int k = 0; k <= G; ++k) tab.push_back(Ep[k]);
     for (int k = 0; k <= G; ++k) tab.push_back(1.0f / Em[k]);
     for (int k = 0; k <= G; ++k) tab.push_back(1.0f / Ep[k]);
+  } else if (r.kind == 3) {
+    tab = {expf(-dt / r.cu.tau_minus), expf(-dt / r.cu.tau_plus), r.cu.a_plus, r.cu.a_minus,
+           r.cu.a_star, r.cu.w_max, r.cu.learning_rate};
   } else if (r.kind == 1) {
     tab = {expf(-dt / r.vg.tau_istdp), r.vg.learningrate};
     r.dterm = (double)r.vg.learningrate * (2.0 * (double)r.vg.targetrate * (double)r.vg.tau_istdp);
@@ -549,7 +554,7 @@ int launch_wd_events(spk_network* n, RuleHost& r, float t) {
   a.pre_ptr = r.d_ev_pre_ptr; a.pre_idx = r.d_ev_pre_idx;
   a.post_ptr = r.d_ev_post_ptr; a.post_pos = r.d_ev_post_pos; a.post_pre = r.d_ev_post_pre;
   a.act = r.d_act; a.cnt = r.d_act_cnt; a.par = r.par;
-  a.N = n->N; a.Nin = n->Nin; a.G = n->G; a.nearest = r.wd.nearest_spike_only;
+  a.N = n->N; a.Nin = n->Nin; a.G = n->G; a.nearest = r.nearest;
   a.t = t; a.dt = n->dt;
   hipLaunchKernelGGL(spk::k_wd_neurons, dim3(grid_for((int64_t)n->nu() + n->N, 256)), dim3(256), 0,
                      n->stream, a);
@@ -575,11 +580,12 @@ int launch_rule(spk_network* n, int rid, float t) {
   a.tr = traces_of(n, r); a.last_n = n->d_last_n; a.last_in = n->d_last_in;
   a.hist_n = n->d_hist_n; a.hist_in = n->d_hist_in; a.tab = r.d_tab; a.dterm = r.dterm;
   a.P = (int64_t)r.syn.size(); a.N = n->N; a.Nin = n->Nin; a.G = n->G; a.W = n->W;
-  a.tidx = (int)roundf(t / n->dt); a.nearest = r.wd.nearest_spike_only; a.t = t; a.dt = n->dt;
+  a.tidx = (int)roundf(t / n->dt); a.nearest = r.nearest; a.t = t; a.dt = n->dt;
   const int grid = grid_for(a.P, 256);
   if (r.kind == 0) hipLaunchKernelGGL(spk::k_stdp_wd, dim3(grid), dim3(256), 0, n->stream, a);
   else if (r.kind == 1) hipLaunchKernelGGL(spk::k_stdp_vogels, dim3(grid), dim3(256), 0, n->stream, a);
-  else hipLaunchKernelGGL(spk::k_stdp_evans, dim3(grid), dim3(256), 0, n->stream, a);
+  else if (r.kind == 2) hipLaunchKernelGGL(spk::k_stdp_evans, dim3(grid), dim3(256), 0, n->stream, a);
+  else hipLaunchKernelGGL(spk::k_stdp_custom, dim3(grid), dim3(256), 0, n->stream, a);
   SPK_HIP(hipGetLastError());
   return SPK_OK;
 }
@@ -730,7 +736,7 @@ int fill_fused_args(spk_network* n, bool plastic, spk::FusedArgs& a) {
       ra.tr = traces_of(n, rh); ra.last_n = n->d_last_n; ra.last_in = n->d_last_in;
       ra.hist_n = n->d_hist_n; ra.hist_in = n->d_hist_in; ra.tab = rh.d_tab; ra.dterm = rh.dterm;
       ra.P = (int64_t)rh.syn.size(); ra.N = n->N; ra.Nin = n->Nin; ra.G = G; ra.W = n->W;
-      ra.nearest = rh.wd.nearest_spike_only; ra.dt = n->dt;
+      ra.nearest = rh.nearest; ra.dt = n->dt;
       fr[r].kind = rh.kind;
       fr[r].part = rh.d_part;
     }
@@ -1134,6 +1140,7 @@ int spk_add_wd_stdp(spk_network* n, const int64_t* syn, int64_t cnt, const spk_w
   if (!p) return fail(SPK_ERR_ARG, "null parameters");
   RuleHost r;
   r.wd = *p;
+  r.nearest = p->nearest_spike_only;
   return add_rule(n, 0, syn, cnt, std::move(r), rule_id);
 }
 
@@ -1151,6 +1158,16 @@ int spk_add_evans_stdp(spk_network* n, const int64_t* syn, int64_t cnt,
   RuleHost r;
   r.ev = *p;
   return add_rule(n, 2, syn, cnt, std::move(r), rule_id);
+}
+
+int spk_add_custom_stdp(spk_network* n, const int64_t* syn, int64_t cnt, const spk_custom_stdp_params* p,
+                        int* rule_id) {
+  if (!p) return fail(SPK_ERR_ARG, "null parameters");
+  if (!(p->w_max != 0.0f)) return fail(SPK_ERR_ARG, "custom STDP needs a nonzero w_max");
+  RuleHost r;
+  r.cu = *p;
+  r.nearest = p->nearest_spike_only;
+  return add_rule(n, 3, syn, cnt, std::move(r), rule_id);
 }
 
 int spk_add_monitor(spk_network* n, int population, int64_t capacity, int* monitor_id) {
@@ -1207,7 +1224,8 @@ int spk_prepare(spk_network* n, int G) {
   for (int64_t s = 0; s < S; ++s) cap[s] = std::fabs((double)n->h_w[s]);
   for (auto& r : n->rules)
     for (int64_t s : r.syn) {
-      double c0 = r.kind == 0 ? std::fabs((double)r.wd.w_max) : (r.kind == 2 ? 1.0 : 0.0);
+      double c0 = r.kind == 0 ? std::fabs((double)r.wd.w_max)
+                : (r.kind == 2 ? 1.0 : (r.kind == 3 ? std::fabs((double)r.cu.w_max) : 0.0));
       cap[s] = std::max(cap[s], c0);
     }
   std::vector<double> ent((size_t)n->N * n->L, 0.0);

@@ -33,6 +33,7 @@ POP_INPUTS = 1
 RULE_WD = 0
 RULE_VOGELS = 1
 RULE_EVANS = 2
+RULE_CUSTOM = 3
 
 
 @dataclasses.dataclass

@@ -14,7 +14,7 @@
 //
 //   frontend_net_hip <case> <out_dir> <simtime_s> [plastic]
 //   cases: brunel (voltage synapses, Poisson inputs, optional WD STDP),
-//          coba (conductance synapses, two labels, Vogels + Evans STDP),
+//          coba (conductance synapses, two labels, Vogels + Evans + custom STDP),
 //          plain (Poisson -> LIF, RANDOM conductance synapses, delays 1-10 steps),
 //          current (Poisson -> LIF E/I, CurrentSpikingSynapses with two decay constants),
 //          generator (Examples/SimpleExample.cpp's layout: generator inputs -> conductance
@@ -154,13 +154,21 @@ int main(int argc, char** argv) {
       evans_stdp_plasticity_parameters_struct* ep = new evans_stdp_plasticity_parameters_struct;
       EvansSTDPPlasticity* ev = new EvansSTDPPlasticity(
           (SpikingSynapses*)syn, (SpikingNeurons*)neurons, (SpikingNeurons*)inputs, (stdp_plasticity_parameters_struct*)ep);
+      custom_stdp_plasticity_parameters_struct* cp = new custom_stdp_plasticity_parameters_struct;
+      cp->a_minus = 0.8f; cp->a_plus = 1.0f; cp->a_star = 0.05f; cp->tau_minus = 0.0337f;
+      cp->tau_plus = 0.0168f; cp->w_max = 1.0f; cp->learning_rate = 1e-3f; cp->nearest_spike_only = false;
+      CustomSTDPPlasticity* cu = new CustomSTDPPlasticity(
+          (SpikingSynapses*)syn, (SpikingNeurons*)neurons, (SpikingNeurons*)inputs, (stdp_plasticity_parameters_struct*)cp);
       if (plastic) {
         model->AddPlasticityRule(vog);
         model->AddPlasticityRule(ev);
+        model->AddPlasticityRule(cu);
         rules.push_back(vog);
         rules.push_back(ev);
+        rules.push_back(cu);
         rule_kinds.push_back("vogels");
         rule_kinds.push_back("evans");
+        rule_kinds.push_back("custom");
       }
       conductance_spiking_synapse_parameters_struct* ex = new conductance_spiking_synapse_parameters_struct();
       conductance_spiking_synapse_parameters_struct* ih = new conductance_spiking_synapse_parameters_struct();
@@ -182,7 +190,9 @@ int main(int argc, char** argv) {
       ih->decay_term_tau_g = 0.010f;
       if (plastic) xi->plasticity_vec.push_back(ev);
       model->AddSynapseGroup(in_id, e_id, xi);
+      if (plastic) ex->plasticity_vec.push_back(cu);
       model->AddSynapseGroup(e_id, e_id, ex);
+      ex->plasticity_vec.clear();
       model->AddSynapseGroup(e_id, i_id, ex);
       if (plastic) ih->plasticity_vec.push_back(vog);
       model->AddSynapseGroup(i_id, e_id, ih);
@@ -382,6 +392,12 @@ int main(int argc, char** argv) {
       auto* q = vg->stdp_params;
       j << "\"tau_istdp\": " << q->tau_istdp << ", \"learningrate\": " << q->learningrate
         << ", \"targetrate\": " << q->targetrate << ", \"momentumrate\": " << q->momentumrate;
+    } else if (auto* cs = dynamic_cast<CustomSTDPPlasticity*>(rules[r])) {
+      auto* q = cs->stdp_params;
+      j << "\"a_minus\": " << q->a_minus << ", \"a_plus\": " << q->a_plus << ", \"a_star\": " << q->a_star
+        << ", \"tau_minus\": " << q->tau_minus << ", \"tau_plus\": " << q->tau_plus << ", \"w_max\": "
+        << q->w_max << ", \"learning_rate\": " << q->learning_rate << ", \"nearest_spike_only\": "
+        << (q->nearest_spike_only ? 1 : 0);
     } else if (auto* ev = dynamic_cast<EvansSTDPPlasticity*>(rules[r])) {
       auto* q = ev->stdp_params;
       j << "\"decay_term_tau_D\": " << q->decay_term_tau_D << ", \"model_parameter_alpha_D\": "

@@ -50,7 +50,8 @@ def desc_from_dump(meta, arr):
     n_in = meta["n_inputs"]
     cond = meta["conductance"] == 1
     curr = meta.get("current", 0) == 1
-    kinds = {"wd": netdesc.RULE_WD, "vogels": netdesc.RULE_VOGELS, "evans": netdesc.RULE_EVANS}
+    kinds = {"wd": netdesc.RULE_WD, "vogels": netdesc.RULE_VOGELS, "evans": netdesc.RULE_EVANS,
+             "custom": netdesc.RULE_CUSTOM}
     rules = []
     for r, (k, p) in enumerate(zip(meta["rule_kinds"], meta["rule_params"])):
         rules.append(netdesc.Rule(kinds[k], arr("plastic_%d" % r, np.int32).astype(np.int64), p))

@@ -8,7 +8,7 @@ import pytest
 
 import oracle_py
 from spike_amd import abi, builders
-from spike_amd.netdesc import (SYN_CONDUCTANCE, Rule, RULE_WD, RULE_VOGELS, RULE_EVANS, WEIGHTS_BF16,
+from spike_amd.netdesc import (SYN_CONDUCTANCE, Rule, RULE_WD, RULE_VOGELS, RULE_EVANS, RULE_CUSTOM, WEIGHTS_BF16,
                                WEIGHTS_FP16, WEIGHTS_FP32)
 
 pytestmark = pytest.mark.gpu
@@ -170,6 +170,8 @@ def _plastic_net(kind, seed):
     syn = np.sort(rng.choice(S, size=S // 2, replace=False)).astype(np.int64)
     if kind == RULE_WD:
         p = builders.wd_params(lam=1e-2, alpha=2.02, w_max=4e-3)
+    elif kind == RULE_CUSTOM:
+        p = builders.custom_params(a_minus=0.8, a_plus=1.0, a_star=0.05, w_max=4e-3, learning_rate=2e-5)
     elif kind == RULE_VOGELS:
         p = builders.vogels_params(lr=1e-5, rate=10.0)
     else:
@@ -180,7 +182,7 @@ def _plastic_net(kind, seed):
     return d
 
 
-@pytest.mark.parametrize("kind", [RULE_WD, RULE_VOGELS, RULE_EVANS])
+@pytest.mark.parametrize("kind", [RULE_WD, RULE_VOGELS, RULE_EVANS, RULE_CUSTOM])
 @pytest.mark.parametrize("fused", [True, False])
 def test_plasticity(kind, fused):
     d = _plastic_net(kind, 30 + kind)
@@ -189,7 +191,7 @@ def test_plasticity(kind, fused):
 
 
 @pytest.mark.parametrize("fmt", [WEIGHTS_BF16, WEIGHTS_FP16])
-@pytest.mark.parametrize("kind", [RULE_WD, RULE_VOGELS, RULE_EVANS])
+@pytest.mark.parametrize("kind", [RULE_WD, RULE_VOGELS, RULE_EVANS, RULE_CUSTOM])
 @pytest.mark.parametrize("fused", [True, False])
 def test_plasticity_16bit_weights(kind, fused, fmt):
     """16-bit weights (Kahan-compensated updates) and binary16 traces: bit-exact against the
@@ -356,6 +358,14 @@ def test_conductance_16bit_weights(fmt):
     d.weight_format = fmt
     gpu, ora = _run_both(d, 250)
     _compare(gpu, ora, d, f"conductance, weights format {fmt}")
+
+
+def test_custom_nearest_spike():
+    d = _plastic_net(RULE_CUSTOM, 44)
+    d.rules[0].params["nearest_spike_only"] = 1
+    gpu, ora = _run_both(d, 200)
+    _compare(gpu, ora, d, "custom nearest")
+    assert not np.array_equal(gpu.weights(), d.weight), "plasticity changed no weight"
 
 
 def test_wd_nearest_spike():

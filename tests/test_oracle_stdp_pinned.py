@@ -14,7 +14,11 @@ in float64 from the spike steps alone, with the textbook forms of each rule:
 * Vogels inhibitory STDP (InhibitorySTDPPlasticity.cu:109-144): symmetric window with
   unit traces, the target term eta*2*rho0*tau at every (delayed) presynaptic arrival;
 * Evans STDP (EvansSTDPPlasticity.cu:96-131): saturating traces with geometric decay
-  (1-dt/tau)^n across gaps, w clamped to [0, 1].
+  (1-dt/tau)^n across gaps, w clamped to [0, 1];
+* custom STDP (CustomSTDPPlasticity.cu:106-148): exponential traces summed over all
+  earlier (delayed) presynaptic arrivals / postsynaptic spikes, or the nearest one; at an
+  arrival w -= lr*(w/w_max)*x_post + lr*a_star, at a postsynaptic spike w += lr*x_pre
+  (including that step's own arrival), clamp at 0.
 
 A dropped term, a swapped A+/A- or tau, a wrong sign or a wrong window index changes the
 weight change by O(1); the tolerance only absorbs the oracle's float32 decay constants.
@@ -32,7 +36,7 @@ sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
 import oracle_py  # noqa: E402
 from spike_amd import netdesc  # noqa: E402
-from spike_amd.netdesc import NetDesc, Rule, RULE_WD, RULE_VOGELS, RULE_EVANS  # noqa: E402
+from spike_amd.netdesc import NetDesc, Rule, RULE_WD, RULE_VOGELS, RULE_EVANS, RULE_CUSTOM  # noqa: E402
 
 F32 = np.float32
 DT = 1e-4
@@ -123,6 +127,27 @@ def evans_expected(pre, post, w0, p, delay, nsteps):
     return w
 
 
+def custom_expected(pre, post, w0, p, delay, nsteps, nearest):
+    am, ap, ast = p["a_minus"], p["a_plus"], p["a_star"]
+    tm, tp, wmax, lr = p["tau_minus"], p["tau_plus"], p["w_max"], p["learning_rate"]
+    arr = [s + delay for s in pre if s + delay < nsteps]
+    w = w0
+    for t in sorted(set(arr) | set(s for s in post if s < nsteps)):
+        a_le = [a for a in arr if a <= t]
+        q_le = [q for q in post if q <= t]
+        if nearest:
+            a_le, q_le = a_le[-1:], q_le[-1:]
+        xpre = sum(ap * math.exp(-(t - a) * DT / tp) for a in a_le)
+        xpost = sum(am * math.exp(-(t - q) * DT / tm) for q in q_le)
+        dw = 0.0
+        if t in arr:
+            dw -= lr * (w / wmax) * xpost + lr * ast
+        if t in post:
+            dw += lr * xpre
+        w = max(0.0, w + dw)
+    return w
+
+
 def close(got, exp, w0, rtol=3e-5):
     """Weight changes agree to rtol (and the change is not trivially zero)."""
     dg, de = got - w0, exp - w0
@@ -198,3 +223,15 @@ def test_fp32_oracle_follows_fp64_on_pair_protocol():
     _, _, w64 = run_oracle(d, 1000, precision=64)
     _, _, w32 = run_oracle(d, 1000, precision=32)
     assert abs(w32 - w64) <= 1e-5 * abs(w64 - 0.4)
+
+
+@pytest.mark.parametrize("delay,nearest,a_star", [(1, 0, 0.0), (4, 0, 2e-3), (2, 1, 0.0)])
+def test_custom_protocol(delay, nearest, a_star):
+    p = dict(a_minus=0.8, a_plus=1.0, a_star=a_star, tau_minus=0.0337, tau_plus=0.0168, w_max=1.0,
+             learning_rate=0.01, nearest_spike_only=nearest)
+    w0 = 0.4
+    d = pair_net(Rule(RULE_CUSTOM, np.array([0], np.int64), p), w0, delay=delay)
+    ng = 3000 // delay
+    pre, post, w = run_oracle(d, ng)
+    assert len(pre) > 20 and len(post) > 20
+    close(w, custom_expected(pre, post, w0, p, delay, ng * delay, nearest), w0)
