@@ -219,6 +219,18 @@ int spk_add_evans_stdp(spk_network* net, const int64_t* synapses, int64_t n,
  * weights clamped at 0. */
 int spk_add_custom_stdp(spk_network* net, const int64_t* synapses, int64_t n,
                         const spk_custom_stdp_params* p, int* rule_id);
+/* WeightNormSTDPPlasticity (WeightNormSTDPPlasticity.cpp:31-75, CUDA
+ * WeightNormSTDPPlasticity.cu:68-172): each application rescales the rule's synapses onto
+ * every neuron v with in_set[v] by sqrtf(sum_sq[v] + u[v]) / sqrtf(sum_sq[v]), where
+ * u[v] = sum over v's synapses, in the caller's order, of d*d + 2*w0*d with d = w - w0
+ * (w0: the initial weight), when that divisor is not 1.  sum_sq and in_set ([n_neurons])
+ * are the front-end's (it computes them, and may rescale the weights, before the backend
+ * is prepared).  The initial weights are the weights at spk_prepare, replaced by
+ * spk_weightnorm_set_initial (caller's order) as the reference's reset_state does.  A
+ * network with this rule runs the per-phase kernels. */
+int spk_add_weightnorm(spk_network* net, const int64_t* synapses, int64_t n, const float* sum_sq,
+                       const uint8_t* in_set, int* rule_id);
+int spk_weightnorm_set_initial(spk_network* net, int rule_id, const float* initial_weights);
 
 /* Spike monitors.  Replaces Backend::SpikingActivityMonitor::prepare
  * (CUDA SpikingActivityMonitor.cu:22-33): a bounded device store of `capacity`
@@ -348,7 +360,8 @@ int spk_get_conductances(spk_network* net, float* g);
 int spk_get_ring(spk_network* net, int64_t* ring);
 /* Spike history bits, words[w*n_pop + i] for the population. */
 int spk_get_history(spk_network* net, int population, uint32_t* words);
-/* Plasticity traces of a rule in its synapse order (two arrays of n). */
+/* Plasticity traces of a rule in its synapse order (two arrays of n); for a weight
+ * normalization rule, its initial weights and zeros. */
 int spk_get_traces(spk_network* net, int rule_id, float* trace_a, float* trace_b);
 
 /* ---------------------------------------------------------------------------- */

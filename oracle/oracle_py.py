@@ -45,6 +45,8 @@ SIG = {
     "or_add_vogels_stdp": (C.c_int, [P, lp, C.c_int64, fp]),
     "or_add_evans_stdp": (C.c_int, [P, lp, C.c_int64, fp]),
     "or_add_custom_stdp": (C.c_int, [P, lp, C.c_int64, fp, C.c_int]),
+    "or_add_weightnorm": (C.c_int, [P, lp, C.c_int64, fp, C.POINTER(C.c_uint8)]),
+    "or_weightnorm_set_initial": (C.c_int, [P, C.c_int, fp]),
     "or_add_monitor": (C.c_int, [P, C.c_int]),
     "or_prepare": (C.c_int, [P, C.c_int]),
     "or_set_timestep_grouping": (C.c_int, [P, C.c_int]),
@@ -178,6 +180,11 @@ class OracleNetwork:
             elif r.kind == 1:
                 par = f32([pr["tau_istdp"], pr["learningrate"], pr["targetrate"], pr.get("momentumrate", 0.0)])
                 check(lib.or_add_vogels_stdp(h, _p(syn, C.c_int64), len(syn), _p(par, C.c_float)))
+            elif r.kind == 4:
+                sq = f32(pr["sum_sq"])
+                ins = np.ascontiguousarray(pr["in_set"], dtype=np.uint8)
+                check(lib.or_add_weightnorm(h, _p(syn, C.c_int64), len(syn), _p(sq, C.c_float),
+                                            ins.ctypes.data_as(C.POINTER(C.c_uint8))))
             elif r.kind == 3:
                 par = f32([pr["a_minus"], pr["a_plus"], pr["a_star"], pr["tau_minus"], pr["tau_plus"],
                            pr["w_max"], pr["learning_rate"]])

@@ -208,6 +208,8 @@ struct Rule {
   float vg[4];
   float ev[5];
   float cu[7];  // a_minus, a_plus, a_star, tau_minus, tau_plus, w_max, learning_rate
+  std::vector<float> wn_sumsq; std::vector<uint8_t> wn_inset;  // weight normalization
+  std::vector<double> wn_w0;                                    // its initial weights
   std::vector<double> ta, tb;  // traces per plastic synapse
 };
 
@@ -337,6 +339,7 @@ struct Net {
     for (auto& r : rules) for (int64_t s : r.syn) {
       double c0 = r.kind == 0 ? std::fabs((double)r.wd[6])
                 : (r.kind == 2 ? 1.0 : (r.kind == 3 ? std::fabs((double)r.cu[5]) : 0.0));
+      if (r.kind == 4) c0 = std::sqrt(std::fabs((double)r.wn_sumsq[spost[s]]));
       cap[s] = std::max(cap[s], c0);
     }
     std::vector<double> ent((size_t)N * L, 0.0);
@@ -374,6 +377,12 @@ struct Net {
     std::vector<std::vector<int64_t>> fill(T);
     for (int t = 0; t < T; ++t) fill[t].assign(rp[t].begin(), rp[t].end() - 1);
     for (int64_t s = 0; s < S; ++s) { int t = owner[spost[s]]; rs[t][fill[t][uid(spre[s])]++] = s; }
+    // weight normalization: the initial weights are the weights as stored now
+    for (auto& r : rules)
+      if (r.kind == 4) {
+        r.wn_w0.resize(r.syn.size());
+        for (size_t j = 0; j < r.syn.size(); ++j) r.wn_w0[j] = (double)sw[r.syn[j]];
+      }
     prepared = true;
     return reset_state();
   }
@@ -649,6 +658,27 @@ struct Net {
         wstore(s, w0, w, 0.0f, INFINITY);
         r.ta[j] = tstore(xpre); r.tb[j] = tstore(xpost);
       }
+    } else if (r.kind == 4) {
+      // WeightNormSTDPPlasticity.cu:68-172: per neuron of the set, u = sum of
+      // d*d + 2*w0*d (d = w - w0 != 0) over its synapses in the caller's order, then
+      // w /= sqrtf(sum_sq + u) / sqrtf(sum_sq) unless that is 1
+      std::vector<R> u(N, (R)0);
+      for (int64_t j = 0; j < P; ++j) {
+        const int64_t s = r.syn[j];
+        const R w0 = (R)r.wn_w0[j];
+        const R d = sw[s] - w0;
+        if (d != (R)0) u[spost[s]] += d * d + ((R)2 * w0) * d;
+      }
+      std::vector<R> dv(N, (R)1);
+      for (int v = 0; v < N; ++v)
+        if (r.wn_inset[v]) dv[v] = std::sqrt((R)r.wn_sumsq[v] + u[v]) / std::sqrt((R)r.wn_sumsq[v]);
+      for (int64_t j = 0; j < P; ++j) {
+        const int64_t s = r.syn[j];
+        const int v = spost[s];
+        if (!r.wn_inset[v] || dv[v] == (R)1) continue;
+        const R w = sw[s];
+        wstore(s, w, w / dv[v], -INFINITY, INFINITY);
+      }
     } else if (r.kind == 3) {
       // CustomSTDPPlasticity.cu:67-155: per sub-step, traces decayed (post by tau_minus,
       // pre by tau_plus) then set or raised by the delayed pre / the post spike; a pre spike
@@ -920,6 +950,27 @@ int or_add_custom_stdp(void* p, const int64_t* syn, int64_t n, const float* par,
   return add_rule(p, 3, syn, n, par, 7, nearest);
 }
 
+int or_add_weightnorm(void* p, const int64_t* syn, int64_t n, const float* sum_sq, const uint8_t* in_set) {
+  const float none[1] = {0.0f};
+  int rc = add_rule(p, 4, syn, n, none, 0, 0);
+  if (rc) return rc;
+  return dispatch((Handle*)p, [&](auto& net) {
+    Rule& r = net.rules.back();
+    r.wn_sumsq.assign(sum_sq, sum_sq + net.N);
+    r.wn_inset.assign(in_set, in_set + net.N);
+    return 0;
+  });
+}
+
+int or_weightnorm_set_initial(void* p, int rid, const float* w0) {
+  return dispatch((Handle*)p, [&](auto& net) {
+    if (rid < 0 || rid >= (int)net.rules.size() || net.rules[rid].kind != 4) return fail("not a weight normalization rule");
+    Rule& r = net.rules[rid];
+    for (size_t j = 0; j < r.syn.size(); ++j) r.wn_w0[j] = (double)w0[j];
+    return 0;
+  });
+}
+
 int or_add_monitor(void* p, int pop) {
   return dispatch((Handle*)p, [&](auto& n) {
     Monitor m; m.pop = pop; n.mons.push_back(m); return (int)n.mons.size() - 1;
@@ -1059,6 +1110,11 @@ int or_get_history(void* p, int pop, uint32_t* out) {
 int or_get_traces(void* p, int r, double* a, double* b) {
   return dispatch((Handle*)p, [&](auto& n) {
     if (r < 0 || r >= (int)n.rules.size()) return fail("bad rule");
+    if (n.rules[r].kind == 4) {   // weight normalization: its initial weights, and zeros
+      std::copy(n.rules[r].wn_w0.begin(), n.rules[r].wn_w0.end(), a);
+      std::fill(b, b + n.rules[r].syn.size(), 0.0);
+      return 0;
+    }
     std::copy(n.rules[r].ta.begin(), n.rules[r].ta.end(), a);
     std::copy(n.rules[r].tb.begin(), n.rules[r].tb.end(), b);
     return 0;

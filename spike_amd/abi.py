@@ -12,7 +12,7 @@ from typing import Optional
 
 import numpy as np
 
-from .netdesc import NetDesc, RULE_WD, RULE_VOGELS, RULE_EVANS, RULE_CUSTOM
+from .netdesc import NetDesc, RULE_WD, RULE_VOGELS, RULE_EVANS, RULE_CUSTOM, RULE_WEIGHTNORM
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "_lib", "libspike_hip.so")
@@ -111,6 +111,8 @@ SIGNATURES = {
     "spk_add_vogels_stdp": (C.c_int, [P, lp, C.c_int64, C.POINTER(VogelsParams), C.POINTER(C.c_int)]),
     "spk_add_evans_stdp": (C.c_int, [P, lp, C.c_int64, C.POINTER(EvansParams), C.POINTER(C.c_int)]),
     "spk_add_custom_stdp": (C.c_int, [P, lp, C.c_int64, C.POINTER(CustomParams), C.POINTER(C.c_int)]),
+    "spk_add_weightnorm": (C.c_int, [P, lp, C.c_int64, fp, C.POINTER(C.c_uint8), C.POINTER(C.c_int)]),
+    "spk_weightnorm_set_initial": (C.c_int, [P, C.c_int, fp]),
     "spk_add_monitor": (C.c_int, [P, C.c_int, C.c_int64, C.POINTER(C.c_int)]),
     "spk_prepare": (C.c_int, [P, C.c_int]),
     "spk_set_timestep_grouping": (C.c_int, [P, C.c_int]),
@@ -273,6 +275,12 @@ class Network:
             elif r.kind == RULE_EVANS:
                 p = EvansParams(**r.params)
                 check(lib.spk_add_evans_stdp(h, _ptr(syn, C.c_int64), len(syn), C.byref(p), C.byref(rid)))
+            elif r.kind == RULE_WEIGHTNORM:
+                sq = f32(r.params["sum_sq"])
+                ins = np.ascontiguousarray(r.params["in_set"], dtype=np.uint8)
+                self._keep += [sq, ins]
+                check(lib.spk_add_weightnorm(h, _ptr(syn, C.c_int64), len(syn), _ptr(sq, C.c_float),
+                                             ins.ctypes.data_as(C.POINTER(C.c_uint8)), C.byref(rid)))
             elif r.kind == RULE_CUSTOM:
                 p = CustomParams(**r.params)
                 check(lib.spk_add_custom_stdp(h, _ptr(syn, C.c_int64), len(syn), C.byref(p), C.byref(rid)))

@@ -62,6 +62,7 @@ class ModelNetwork {
   int grouping = 0;
   bool plastic_pending = false;
   std::map<const void*, int> monitor_id;
+  std::map<const void*, int> rule_id;
 
  private:
   void build();
@@ -222,6 +223,19 @@ void ModelNetwork::build() {
       p.decay_term_tau_C = ev->stdp_params->decay_term_tau_C;
       p.learning_rate_rho = ev->stdp_params->learning_rate_rho;
       check(spk_add_evans_stdp(net, idx.data(), (int64_t)idx.size(), &p, &rid), "spk_add_evans_stdp");
+    } else if (auto* wn = dynamic_cast<::WeightNormSTDPPlasticity*>(rule)) {
+      // the front-end computed the squared afferent norms (and rescaled the weights for a
+      // target) in its prepare_backend_early (WeightNormSTDPPlasticity.cpp:31-75)
+      const int N = lif->total_number_of_neurons;
+      std::vector<float> sq(N, 0.0f);
+      std::vector<uint8_t> in(N, 0);
+      if (wn->sum_squared_afferent_values && wn->neuron_in_plasticity_set)
+        for (int v = 0; v < N; ++v) {
+          sq[v] = wn->sum_squared_afferent_values[v];
+          in[v] = wn->neuron_in_plasticity_set[v] ? 1 : 0;
+        }
+      check(spk_add_weightnorm(net, idx.data(), (int64_t)idx.size(), sq.data(), in.data(), &rid),
+            "spk_add_weightnorm");
     } else if (auto* cs = dynamic_cast<::CustomSTDPPlasticity*>(rule)) {
       spk_custom_stdp_params p{};
       p.a_minus = cs->stdp_params->a_minus;
@@ -236,6 +250,7 @@ void ModelNetwork::build() {
     } else {
       die("HIP backend: unsupported plasticity rule");
     }
+    rule_id[rule] = rid;
   }
   for (::ActivityMonitor* am : m->monitors_vec) {
     // SpikingActivityMonitor declares its own `neurons` (SpikingActivityMonitor.hpp:45),
@@ -328,6 +343,18 @@ void EvansSTDPPlasticity::update_synaptic_efficacies_or_weights(float t, float) 
 
 void CustomSTDPPlasticity::apply_stdp_to_synapse_weights(float t, float) { mark_applied(t); }
 
+void WeightNormSTDPPlasticity::weight_normalization() { mark_applied(0.0f); }
+
+void WeightNormSTDPPlasticity::reset_state() {
+  // WeightNormSTDPPlasticity.cu:16-26: the front-end's initial weights become the rule's
+  auto* f = dynamic_cast<::WeightNormSTDPPlasticity*>(front());
+  ModelNetwork* mn = model_network(front()->model);
+  if (!f || !mn->built() || !f->initial_weights) return;
+  auto it = mn->rule_id.find(front());
+  if (it == mn->rule_id.end()) return;
+  check(spk_weightnorm_set_initial(mn->net, it->second, f->initial_weights), "spk_weightnorm_set_initial");
+}
+
 void SpikingActivityMonitor::reset_state() {
   ModelNetwork* mn = model_network(front()->model);
   if (mn->built()) check(spk_monitor_reset(mn->net, mn->monitor(front())), "spk_monitor_reset");
@@ -395,14 +422,6 @@ class PatternedPoissonInputSpikingNeurons : public virtual ::Backend::HIP::Poiss
   SPIKE_MAKE_BACKEND_CONSTRUCTOR(PatternedPoissonInputSpikingNeurons);
   SPK_HIP_UNSUPPORTED_PREPARE("PatternedPoissonInputSpikingNeurons")
   void copy_rates_to_device() override {}
-};
-
-class WeightNormSTDPPlasticity : public virtual ::Backend::HIP::STDPPlasticity,
-                                 public virtual ::Backend::WeightNormSTDPPlasticity {
- public:
-  SPIKE_MAKE_BACKEND_CONSTRUCTOR(WeightNormSTDPPlasticity);
-  SPK_HIP_UNSUPPORTED_PREPARE("WeightNormSTDPPlasticity")
-  void weight_normalization() override {}
 };
 
 #undef SPK_HIP_UNSUPPORTED_PREPARE

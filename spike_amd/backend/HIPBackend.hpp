@@ -169,6 +169,14 @@ class CustomSTDPPlasticity : public virtual ::Backend::HIP::STDPPlasticity,
   void apply_stdp_to_synapse_weights(float current_time_in_seconds, float timestep) override;
 };
 
+class WeightNormSTDPPlasticity : public virtual ::Backend::HIP::STDPPlasticity,
+                                 public virtual ::Backend::WeightNormSTDPPlasticity {
+ public:
+  SPIKE_MAKE_BACKEND_CONSTRUCTOR(WeightNormSTDPPlasticity);
+  void weight_normalization() override;
+  void reset_state() override;
+};
+
 class ActivityMonitor : public virtual ::Backend::ActivityMonitor {
  public:
   ~ActivityMonitor() override = default;

@@ -924,6 +924,42 @@ __global__ __launch_bounds__(256) void k_stdp_custom(StdpArgs a) {
   if (j < a.P) stdp_custom_one(a, j);
 }
 
+// WeightNormSTDPPlasticity.cu:68-172.  One lane per postsynaptic neuron of the rule: the
+// change of its afferent squared norm, u = sum of d*d + 2*w0*d over its synapses with
+// d = w - w0 != 0 (the reference's float atomicAdd, here summed in the caller's order of
+// the synapses), then w /= sqrtf(sum_sq + u) / sqrtf(sum_sq) when that is not 1.
+struct WeightNormArgs {
+  Weights wts;
+  const int64_t* pos;       // rule index -> CSR position
+  const float* w0;          // rule index -> initial weight
+  const float* sum_sq;      // [N]
+  const uint8_t* in_set;    // [N]
+  const int64_t* ptr;       // [N + 1] rule indices of neuron v: idx[ptr[v] .. ptr[v+1]),
+  const int32_t* idx;       //   in the caller's order
+  int N;
+};
+
+__global__ __launch_bounds__(256) void k_weightnorm(WeightNormArgs a) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= a.N || !a.in_set[v]) return;
+  const int64_t e0 = a.ptr[v], e1 = a.ptr[v + 1];
+  float u = 0.0f;
+  for (int64_t e = e0; e < e1; ++e) {
+    const int64_t j = a.idx[e];
+    const float w0 = a.w0[j];
+    const float d = wload(a.wts, a.pos[j]) - w0;
+    if (d != 0.0f) u += d * d + (2.0f * w0) * d;
+  }
+  const float sq = a.sum_sq[v];
+  const float div = sqrtf(sq + u) / sqrtf(sq);
+  if (div == 1.0f) return;
+  for (int64_t e = e0; e < e1; ++e) {
+    const int64_t s = a.pos[a.idx[e]];
+    const float w = wload(a.wts, s);
+    wupdate(a.wts, s, w, w / div, -__builtin_inff(), __builtin_inff());
+  }
+}
+
 // ---------------------------------------------------------------------------------
 // Monitor collection (SpikingActivityMonitor.cu:69-98): copy the spikes of one
 // population from list slots [slot0, slot0+nslots) into a bounded store.

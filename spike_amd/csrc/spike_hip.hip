@@ -102,6 +102,14 @@ struct RuleHost {
   spk_evans_stdp_params ev{};
   spk_custom_stdp_params cu{};
   int nearest = 0;   // nearest-spike traces (WD, Custom)
+  // weight normalization (kind 4): the front-end's squared norms and neuron set, the
+  // initial weights (in the per-synapse trace array a), rule indices per neuron
+  std::vector<float> wn_sumsq;
+  std::vector<uint8_t> wn_inset;
+  float* d_wn_sumsq = nullptr;
+  uint8_t* d_wn_inset = nullptr;
+  int64_t* d_wn_ptr = nullptr;
+  int32_t* d_wn_idx = nullptr;
   // device
   int64_t* d_pos = nullptr;
   int32_t* d_pre = nullptr;
@@ -335,6 +343,7 @@ void free_device(spk_network* n) {
     dfree(r.d_ev_post_pre);
     dfree(r.d_xn_a); dfree(r.d_xn_b); dfree(r.d_xn_a16); dfree(r.d_xn_b16);
     dfree(r.d_xcur_a); dfree(r.d_xcur_b); dfree(r.d_act); dfree(r.d_act_cnt);
+    dfree(r.d_wn_sumsq); dfree(r.d_wn_inset); dfree(r.d_wn_ptr); dfree(r.d_wn_idx);
   }
   dfree(n->d_nlo); dfree(n->d_ilo); dfree(n->d_seg); dfree(n->d_xlist);
   dfree(n->d_bar); dfree(n->d_err); dfree(n->d_prof); dfree(n->d_rules);
@@ -378,6 +387,8 @@ int refresh_rule_tables(spk_network* n, RuleHost& r) {
   } else if (r.kind == 1) {
     tab = {expf(-dt / r.vg.tau_istdp), r.vg.learningrate};
     r.dterm = (double)r.vg.learningrate * (2.0 * (double)r.vg.targetrate * (double)r.vg.tau_istdp);
+  } else if (r.kind == 4) {
+    tab = {0.0f};   // weight normalization keeps no per-rule constants
   } else {
     tab = {1.0f - (dt / r.ev.decay_term_tau_C), 1.0f - (dt / r.ev.decay_term_tau_D),
            dt * r.ev.synaptic_neurotransmitter_concentration_alpha_C,
@@ -572,6 +583,14 @@ int launch_rule(spk_network* n, int rid, float t) {
   int rc = refresh_rule_tables(n, r);
   if (rc) return rc;
   if (r.kind == 0 && r.ev_ok && !r.dense_next) return launch_wd_events(n, r, t);
+  if (r.kind == 4) {
+    spk::WeightNormArgs w{};
+    w.wts = weights_of(n); w.pos = r.d_pos; w.w0 = r.d_ta; w.sum_sq = r.d_wn_sumsq;
+    w.in_set = r.d_wn_inset; w.ptr = r.d_wn_ptr; w.idx = r.d_wn_idx; w.N = n->N;
+    hipLaunchKernelGGL(spk::k_weightnorm, dim3(grid_for(n->N, 256)), dim3(256), 0, n->stream, w);
+    SPK_HIP(hipGetLastError());
+    return SPK_OK;
+  }
   if ((rc = traces_per_synapse(n, r))) return rc;
   r.neu_tr = false;
   r.dense_next = false;
@@ -1170,6 +1189,30 @@ int spk_add_custom_stdp(spk_network* n, const int64_t* syn, int64_t cnt, const s
   return add_rule(n, 3, syn, cnt, std::move(r), rule_id);
 }
 
+int spk_add_weightnorm(spk_network* n, const int64_t* syn, int64_t cnt, const float* sum_sq,
+                       const uint8_t* in_set, int* rule_id) {
+  if (int rc = check_net(n)) return rc;
+  if (!sum_sq || !in_set) return fail(SPK_ERR_ARG, "null weight normalization arrays");
+  RuleHost r;
+  r.wn_sumsq.assign(sum_sq, sum_sq + n->N);
+  r.wn_inset.assign(in_set, in_set + n->N);
+  return add_rule(n, 4, syn, cnt, std::move(r), rule_id);
+}
+
+int spk_weightnorm_set_initial(spk_network* n, int rid, const float* w0) {
+  if (int rc = check_prepared(n)) return rc;
+  if (rid < 0 || rid >= (int)n->rules.size() || n->rules[rid].kind != 4)
+    return fail(SPK_ERR_ARG, "not a weight normalization rule");
+  RuleHost& r = n->rules[rid];
+  const size_t P = r.syn.size();
+  if (P && !w0) return fail(SPK_ERR_ARG, "null initial weights");
+  std::vector<float> dev(P);
+  for (size_t j = 0; j < P; ++j) dev[j] = w0[r.order[j]];
+  if (int rc = upload(r.d_ta, dev.data(), P, n->stream)) return rc;
+  SPK_HIP(hipStreamSynchronize(n->stream));
+  return SPK_OK;
+}
+
 int spk_add_monitor(spk_network* n, int population, int64_t capacity, int* monitor_id) {
   if (int rc = check_net(n)) return rc;
   if (n->prepared) return fail(SPK_ERR_STATE, "network already prepared");
@@ -1226,6 +1269,8 @@ int spk_prepare(spk_network* n, int G) {
     for (int64_t s : r.syn) {
       double c0 = r.kind == 0 ? std::fabs((double)r.wd.w_max)
                 : (r.kind == 2 ? 1.0 : (r.kind == 3 ? std::fabs((double)r.cu.w_max) : 0.0));
+      // normalization keeps a neuron's afferent norm: no weight exceeds sqrt(sum_sq)
+      if (r.kind == 4) c0 = std::sqrt(std::fabs((double)r.wn_sumsq[n->h_post[s]]));
       cap[s] = std::max(cap[s], c0);
     }
   std::vector<double> ent((size_t)n->N * n->L, 0.0);
@@ -1292,6 +1337,8 @@ int spk_prepare(spk_network* n, int G) {
   if (n->P > spk::kMaxPartitions) no_fused("too many partitions");
   if ((int)n->mons.size() > spk::kMaxFusedMons) no_fused("more than 4 spike monitors");
   if (n->gen) no_fused("generator inputs");
+  for (auto& r : n->rules)
+    if (r.kind == 4) no_fused("weight normalization rule");
   n->BL = std::max(n->dmax, G);
   n->fused_lds = spk::fused_lds_bytes(n->BL, n->L, n->W, G, n->nown_max, n->iown_max, n->xcap, n->P);
   {
@@ -1569,6 +1616,33 @@ int spk_prepare(spk_network* n, int G) {
       if ((rc = dalloc(&r.d_act_cnt, 4))) return rc;
       SPK_HIP(hipStreamSynchronize(st));   // the host index arrays end with this scope
     }
+    if (r.kind == 4) {
+      // initial weights (as stored) and the rule indices of each neuron in caller order
+      std::vector<int64_t> inv(P);
+      for (size_t j = 0; j < P; ++j) inv[r.order[j]] = (int64_t)j;
+      std::vector<float> w0(P);
+      std::vector<int64_t> qp((size_t)n->N + 1, 0);
+      for (size_t j = 0; j < P; ++j) {
+        const int64_t sidx = r.syn[r.order[j]];
+        const float w = n->h_w[sidx];
+        w0[j] = n->wfmt ? spk::dec16(spk::enc16(w, n->wfmt), n->wfmt) : w;
+        qp[(size_t)n->h_post[sidx] + 1]++;
+      }
+      for (int v = 0; v < n->N; ++v) qp[v + 1] += qp[v];
+      std::vector<int32_t> qi(std::max<size_t>(1, P));
+      std::vector<int64_t> fill(qp.begin(), qp.end() - 1);
+      for (size_t c = 0; c < P; ++c) qi[fill[n->h_post[r.syn[c]]]++] = (int32_t)inv[c];
+      if ((rc = dalloc(&r.d_wn_sumsq, (size_t)std::max(1, n->N)))) return rc;
+      if ((rc = dalloc(&r.d_wn_inset, (size_t)std::max(1, n->N)))) return rc;
+      if ((rc = dalloc(&r.d_wn_ptr, (size_t)n->N + 1))) return rc;
+      if ((rc = dalloc(&r.d_wn_idx, std::max<size_t>(1, P)))) return rc;
+      if ((rc = upload(r.d_wn_sumsq, r.wn_sumsq.data(), (size_t)n->N, st))) return rc;
+      if ((rc = upload(r.d_wn_inset, r.wn_inset.data(), (size_t)n->N, st))) return rc;
+      if ((rc = upload(r.d_wn_ptr, qp.data(), qp.size(), st))) return rc;
+      if ((rc = upload(r.d_wn_idx, qi.data(), P, st))) return rc;
+      if ((rc = upload(r.d_ta, w0.data(), P, st))) return rc;
+      SPK_HIP(hipStreamSynchronize(st));
+    }
   }
   // monitors: device store of the requested capacity
   for (auto& m : n->mons) {
@@ -1658,7 +1732,7 @@ int spk_reset_state(spk_network* n) {
   }
   for (auto& r : n->rules) {
     const size_t P = r.syn.size();
-    if (P) {
+    if (P && r.kind != 4) {   // (weight normalization keeps its initial weights there)
       SPK_HIP(hipMemsetAsync(r.d_ta, 0, sizeof(float) * P, st));
       SPK_HIP(hipMemsetAsync(r.d_tb, 0, sizeof(float) * P, st));
       if (r.d_ta16) {
@@ -2026,7 +2100,9 @@ int spk_get_traces(spk_network* n, int rid, float* a, float* b) {
   const size_t P = r.syn.size();
   if (int rc = traces_per_synapse(n, r)) return rc;
   std::vector<float> ta(P), tb(P);
-  if (r.d_ta16) {
+  if (r.kind == 4) {   // weight normalization: its initial weights, and zeros
+    if (int rc = download(n, ta.data(), r.d_ta, P)) return rc;
+  } else if (r.d_ta16) {
     std::vector<uint16_t> a16(P), b16(P);
     if (int rc = download(n, a16.data(), r.d_ta16, P)) return rc;
     if (int rc = download(n, b16.data(), r.d_tb16, P)) return rc;

@@ -34,6 +34,7 @@ RULE_WD = 0
 RULE_VOGELS = 1
 RULE_EVANS = 2
 RULE_CUSTOM = 3
+RULE_WEIGHTNORM = 4   # params: sum_sq [n_neurons] float32, in_set [n_neurons] bool
 
 
 @dataclasses.dataclass

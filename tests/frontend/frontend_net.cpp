@@ -13,7 +13,7 @@
 //   meta.json.
 //
 //   frontend_net_hip <case> <out_dir> <simtime_s> [plastic]
-//   cases: brunel (voltage synapses, Poisson inputs, optional WD STDP),
+//   cases: brunel (voltage synapses, Poisson inputs, optional WD STDP + weight normalization),
 //          coba (conductance synapses, two labels, Vogels + Evans + custom STDP),
 //          plain (Poisson -> LIF, RANDOM conductance synapses, delays 1-10 steps),
 //          current (Poisson -> LIF E/I, CurrentSpikingSynapses with two decay constants),
@@ -90,10 +90,16 @@ int main(int argc, char** argv) {
     wd->lambda = 1.0e-2f; wd->alpha = 2.02; wd->w_max = 0.3e-3f;
     WeightDependentSTDPPlasticity* rule = new WeightDependentSTDPPlasticity(
         (SpikingSynapses*)syn, (SpikingNeurons*)neurons, (SpikingNeurons*)inputs, (stdp_plasticity_parameters_struct*)wd);
+    weightnorm_stdp_plasticity_parameters_struct* np = new weightnorm_stdp_plasticity_parameters_struct;
+    WeightNormSTDPPlasticity* norm = new WeightNormSTDPPlasticity(
+        (SpikingSynapses*)syn, (SpikingNeurons*)neurons, (SpikingNeurons*)inputs, (plasticity_parameters_struct*)np);
     if (plastic) {
       model->AddPlasticityRule(rule);
+      model->AddPlasticityRule(norm);
       rules.push_back(rule);
+      rules.push_back(norm);
       rule_kinds.push_back("wd");
+      rule_kinds.push_back("weightnorm");
     }
     lif_spiking_neuron_parameters_struct* E = new lif_spiking_neuron_parameters_struct();
     lif_spiking_neuron_parameters_struct* I = new lif_spiking_neuron_parameters_struct();
@@ -120,7 +126,10 @@ int main(int argc, char** argv) {
     connect_with_sparsity(in_id, e_id, ip, E, IP, 0.1f, model);
     connect_with_sparsity(in_id, i_id, ip, I, IP, 0.1f, model);
     connect_with_sparsity(e_id, i_id, E, I, EX, 0.1f, model);
-    if (plastic) EX->plasticity_vec.push_back(rule);
+    if (plastic) {
+      EX->plasticity_vec.push_back(rule);
+      EX->plasticity_vec.push_back(norm);
+    }
     connect_with_sparsity(e_id, e_id, E, E, EX, 0.1f, model);
     connect_with_sparsity(i_id, e_id, I, E, IN, 0.1f, model);
     connect_with_sparsity(i_id, i_id, I, I, IN, 0.1f, model);
@@ -353,8 +362,15 @@ int main(int argc, char** argv) {
   }
   put("decay_g", decay.data(), decay.size());
   put("reversal", rev.data(), rev.size());
-  for (size_t r = 0; r < rules.size(); ++r)
+  for (size_t r = 0; r < rules.size(); ++r) {
     put("plastic_" + std::to_string(r), rules[r]->plastic_synapses.data(), rules[r]->plastic_synapses.size());
+    if (auto* wn = dynamic_cast<WeightNormSTDPPlasticity*>(rules[r])) {
+      std::vector<uint8_t> in(N);
+      for (int v = 0; v < N; ++v) in[v] = wn->neuron_in_plasticity_set[v] ? 1 : 0;
+      put("wn_sumsq_" + std::to_string(r), wn->sum_squared_afferent_values, N);
+      put("wn_inset_" + std::to_string(r), in.data(), in.size());
+    }
+  }
 
   model->run(simtime, plastic);
   if (generator) {

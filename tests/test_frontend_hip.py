@@ -51,9 +51,11 @@ def desc_from_dump(meta, arr):
     cond = meta["conductance"] == 1
     curr = meta.get("current", 0) == 1
     kinds = {"wd": netdesc.RULE_WD, "vogels": netdesc.RULE_VOGELS, "evans": netdesc.RULE_EVANS,
-             "custom": netdesc.RULE_CUSTOM}
+             "custom": netdesc.RULE_CUSTOM, "weightnorm": netdesc.RULE_WEIGHTNORM}
     rules = []
     for r, (k, p) in enumerate(zip(meta["rule_kinds"], meta["rule_params"])):
+        if k == "weightnorm":
+            p = dict(sum_sq=arr("wn_sumsq_%d" % r, F32), in_set=arr("wn_inset_%d" % r, np.uint8).astype(bool))
         rules.append(netdesc.Rule(kinds[k], arr("plastic_%d" % r, np.int32).astype(np.int64), p))
     d = netdesc.NetDesc(
         n_neurons=meta["n_neurons"], n_inputs=n_in,

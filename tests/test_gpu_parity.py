@@ -360,6 +360,26 @@ def test_conductance_16bit_weights(fmt):
     _compare(gpu, ora, d, f"conductance, weights format {fmt}")
 
 
+@pytest.mark.parametrize("fmt", [WEIGHTS_FP32, WEIGHTS_FP16])
+def test_weight_normalization_after_stdp(fmt):
+    """WeightNormSTDPPlasticity after a WD rule on the same synapses (per-phase path):
+    every array bit-exact against the oracle, FP32 and binary16 weights."""
+    from spike_amd.netdesc import RULE_WEIGHTNORM
+    d = _plastic_net(RULE_WD, 47)
+    d.weight_format = fmt
+    syn = d.rules[0].synapses
+    w0 = d.weight[syn].astype(np.float64)
+    sum_sq = np.zeros(d.n_neurons, np.float64)
+    np.add.at(sum_sq, d.post[syn], w0 ** 2)
+    in_set = np.zeros(d.n_neurons, bool)
+    in_set[d.post[syn]] = True
+    d.rules.append(Rule(RULE_WEIGHTNORM, syn, dict(sum_sq=sum_sq.astype(np.float32), in_set=in_set)))
+    gpu, ora = _run_both(d, 200, expect_fused=False)
+    assert "normalization" in gpu.fused_status()
+    _compare(gpu, ora, d, f"weight normalization fmt {fmt}")
+    assert not np.array_equal(gpu.weights()[syn], d.weight[syn])
+
+
 def test_custom_nearest_spike():
     d = _plastic_net(RULE_CUSTOM, 44)
     d.rules[0].params["nearest_spike_only"] = 1

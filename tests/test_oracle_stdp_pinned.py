@@ -235,3 +235,42 @@ def test_custom_protocol(delay, nearest, a_star):
     pre, post, w = run_oracle(d, ng)
     assert len(pre) > 20 and len(post) > 20
     close(w, custom_expected(pre, post, w0, p, delay, ng * delay, nearest), w0)
+
+
+def test_weight_normalization_keeps_afferent_norms():
+    """WeightNormSTDPPlasticity after a WD rule on the same synapses: once the rule has
+    moved the weights, each application rescales every neuron's afferent weights so that
+    their squared norm is back at sum_sq (the divisor sqrtf(sum_sq + u)/sqrtf(sum_sq), with
+    u the change of the squared norm since the initial weights).  Checked against that
+    invariant in float64; without a learning rule the weights are left bit for bit."""
+    from spike_amd import builders
+    from spike_amd.netdesc import RULE_WEIGHTNORM
+    d = builders.random_net(seed=5, n_neurons=40, n_inputs=30, p=0.3, input_rate=600.0, whi=4e-3,
+                            dmin=1, dmax=4)
+    S = d.n_synapses
+    syn = np.nonzero(d.pre >= 0)[0].astype(np.int64)      # recurrent synapses
+    w0 = d.weight[syn].astype(np.float64)
+    sum_sq = np.zeros(d.n_neurons, np.float64)
+    np.add.at(sum_sq, d.post[syn], w0 ** 2)
+    in_set = np.zeros(d.n_neurons, bool)
+    in_set[d.post[syn]] = True
+    wn = dict(sum_sq=sum_sq.astype(np.float32), in_set=in_set)
+    plain = builders.random_net(seed=5, n_neurons=40, n_inputs=30, p=0.3, input_rate=600.0, whi=4e-3,
+                                dmin=1, dmax=4)
+    plain.rules = [Rule(RULE_WEIGHTNORM, syn, wn)]
+    o = oracle_py.OracleNetwork(plain, precision=64)
+    o.run(0.0, 200)
+    assert np.array_equal(o.weights(), plain.weight.astype(np.float64))
+    wd = dict(a_plus=1.0, a_minus=1.0, tau_plus=0.02, tau_minus=0.02, **{"lambda": 0.05}, alpha=1.0,
+              w_max=8e-3, nearest_spike_only=0)
+    d.rules = [Rule(RULE_WD, syn, wd), Rule(RULE_WEIGHTNORM, syn, wn)]
+    o = oracle_py.OracleNetwork(d, precision=64)
+    o.run(0.0, 400)
+    w = o.weights()[syn]
+    assert np.abs(w - w0).max() > 1e-2 * w0.max(), "the learning rule moved no weight"
+    got = np.zeros(d.n_neurons)
+    np.add.at(got, d.post[syn], w ** 2)
+    assert np.allclose(got[in_set], sum_sq[in_set], rtol=1e-6)
+    # the rule reports its initial weights in place of a trace
+    a, _ = o.traces(1)
+    assert np.allclose(a, np.float32(w0))
