@@ -8,6 +8,8 @@
 //                                           the order the backend receives them)
 //   lif_{v0,vreset,thr,tau,R}.bin, rates.bin, decay_g.bin (expf(-dt/tau_g) per label),
 //   plastic_<r>.bin, weights_after.bin (weights copied back after the run),
+//   conn_*.bin (Synapses::save_connectivity_as_binary after the run), weights_reloaded.bin
+//   (those weights loaded back with load_weights_from_binary and copied from the device),
 //   <pop>_SpikeIDs.bin / <pop>_SpikeTimes.bin (SpikingActivityMonitor::save_spikes_as_binary),
 //   rate_counts.bin (a RateActivityMonitor's per_neuron_spike_counts of the neurons),
 //   meta.json.
@@ -380,6 +382,14 @@ int main(int argc, char** argv) {
 
   s->backend()->copy_weights_to_host();
   put("weights_after", s->synaptic_efficacies_or_weights, S);
+  // the connectivity files (weights copied from the device) and a reload of their weights
+  // through Synapses::load_weights_from_binary (copy_to_backend), read back from the device
+  s->save_connectivity_as_binary(g_out, "conn_");
+  for (int k = 0; k < S; ++k) s->synaptic_efficacies_or_weights[k] = 0.0f;
+  s->load_weights_from_binary(g_out + "/conn_SynapticWeights.bin");
+  for (int k = 0; k < S; ++k) s->synaptic_efficacies_or_weights[k] = -1.0f;
+  s->backend()->copy_weights_to_host();
+  put("weights_reloaded", s->synaptic_efficacies_or_weights, S);
   put("rate_counts", rate_n->per_neuron_spike_counts, N);
   mon_n->save_spikes_as_binary(g_out, "neurons_");
   mon_i->save_spikes_as_binary(g_out, "inputs_");

@@ -123,6 +123,17 @@ def test_frontend_run_matches_oracle(tmp_path, case, simtime, plastic):
     np.testing.assert_array_equal(arr("rate_counts", np.int32), np.bincount(on_ids, minlength=meta["n_neurons"]))
     w = arr("weights_after", F32)
     np.testing.assert_array_equal(w.view(np.uint32), ow.view(np.uint32))
+    # Spike's connectivity files, read with spike_amd.io: the same synapses (creation
+    # order) with the device's weights; loading them back leaves the device weights as they were
+    from spike_amd import io
+    conn = io.read_connectivity(str(tmp_path / case), "conn_")
+    key = lambda p, q, dl, ww: np.lexsort((ww.view(np.uint32), dl, q, p))
+    a = key(conn["pre"], conn["post"], conn["delay"], conn["weight"])
+    b = key(arr("syn_pre", np.int32), arr("syn_post", np.int32), arr("syn_delay", np.int32), w)
+    assert np.array_equal(conn["pre"][a], arr("syn_pre", np.int32)[b])
+    assert np.array_equal(conn["delay"][a], arr("syn_delay", np.int32)[b])
+    assert np.array_equal(conn["weight"][a].view(np.uint32), w[b].view(np.uint32))
+    np.testing.assert_array_equal(arr("weights_reloaded", F32).view(np.uint32), w.view(np.uint32))
     if plastic:
         assert not np.array_equal(w, arr("syn_w", F32)), "plasticity changed no weight"
 
