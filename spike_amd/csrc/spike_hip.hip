@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/spike_hip.h"
@@ -94,6 +95,20 @@ int choose_frac_bits(double max_entry) {
   return std::max(0, std::min(120, k));
 }
 
+// Host arrays of synapse size: resize leaves elements uninitialised, so that the first
+// touch of their pages happens in the threaded passes that fill them.
+template <typename T>
+struct NoInit : std::allocator<T> {
+  template <typename U>
+  struct rebind { using other = NoInit<U>; };
+  template <typename U>
+  void construct(U* p) noexcept { ::new (static_cast<void*>(p)) U; }
+  template <typename U, typename... A>
+  void construct(U* p, A&&... a) { ::new (static_cast<void*>(p)) U(std::forward<A>(a)...); }
+};
+template <typename T>
+using hvec = std::vector<T, NoInit<T>>;
+
 struct RuleHost {
   int kind = 0;  // 0 WD, 1 Vogels, 2 Evans, 3 Custom
   std::vector<int64_t> syn;
@@ -120,7 +135,7 @@ struct RuleHost {
   float* d_tab = nullptr;
   int tab_G = -1;
   double dterm = 0.0;
-  std::vector<int64_t> order;     // device order j -> index into syn (grouped by post partition)
+  hvec<int64_t> order;            // device order j -> index into syn (grouped by post partition)
   std::vector<int64_t> part;      // [P+1] offsets of each postsynaptic partition
   int64_t* d_part = nullptr;
   // spike-driven form of the weight-dependent rule (per-phase kernels): rule indices by
@@ -184,8 +199,8 @@ struct spk_network {
   uint64_t seed = 42;
   float ext_rate = 0.0f, ext_w = 0.0f;
   std::vector<float> decay_g, erev;
-  std::vector<int32_t> h_pre, h_post, h_delay, h_label;
-  std::vector<float> h_w, h_scale;
+  hvec<int32_t> h_pre, h_post, h_delay, h_label;
+  hvec<float> h_w, h_scale;
   bool syn_set = false;
   std::vector<RuleHost> rules;
   std::vector<MonitorHost> mons;
@@ -200,7 +215,7 @@ struct spk_network {
   int64_t q_ext = 0;
   int64_t arm_step = -1;
   bool pending_arm = true;
-  std::vector<int64_t> perm;  // caller synapse index -> CSR position
+  hvec<int64_t> perm;  // caller synapse index -> CSR position
   // device state
   float *d_V = nullptr, *d_v0 = nullptr, *d_vreset = nullptr, *d_thr = nullptr, *d_c = nullptr,
         *d_rp = nullptr;
@@ -362,6 +377,53 @@ int upload(T* d, const T* h, size_t n, hipStream_t s) {
 }
 
 int grid_for(int64_t n, int block) { return (int)std::max<int64_t>(1, (n + block - 1) / block); }
+
+// Host threads for the O(S) passes of preparation: f(t, lo, hi) on contiguous chunks
+// [lo, hi) of [0, n), chunk t to thread t; each pass writes disjoint outputs, so results
+// do not depend on the thread count.
+template <typename F>
+void host_chunks(int64_t n, int threads, F f, int64_t serial_below = (int64_t)1 << 20) {
+  if (threads <= 1 || n < serial_below) {
+    f(0, (int64_t)0, n);
+    return;
+  }
+  std::vector<std::thread> pool;
+  for (int t = 0; t < threads; ++t)
+    pool.emplace_back([&, t] { f(t, n * t / threads, n * (t + 1) / threads); });
+  for (auto& th : pool) th.join();
+}
+
+// Stable counting sort of the items [0, n) by key(i) in [0, K): ptr (K + 1 entries) gets
+// the bucket starts and place(i, k) is called once per item with its sorted position k.
+// Items of one chunk keep their order and the chunks follow each other, so the result is
+// that of the sequential sort whatever the thread count.
+template <typename Key, typename Place>
+void counting_sort(int64_t n, int64_t K, int threads, Key key, Place place, std::vector<int64_t>& ptr) {
+  const int T = n >= ((int64_t)1 << 20) ? std::max(1, threads) : 1;
+  std::vector<std::vector<int64_t>> cnt(T);
+  host_chunks(n, T, [&](int t, int64_t a, int64_t b) {
+    cnt[t].assign((size_t)K, 0);
+    for (int64_t i = a; i < b; ++i) cnt[t][key(i)]++;
+  });
+  ptr.assign((size_t)K + 1, 0);
+  for (int64_t k = 0; k < K; ++k) {
+    int64_t o = ptr[k];
+    for (int t = 0; t < T; ++t) {
+      const int64_t c = cnt[t][k];
+      cnt[t][k] = o;
+      o += c;
+    }
+    ptr[k + 1] = o;
+  }
+  host_chunks(n, T, [&](int t, int64_t a, int64_t b) {
+    for (int64_t i = a; i < b; ++i) place(i, cnt[t][key(i)]++);
+  });
+}
+
+int host_threads() {
+  const unsigned h = std::thread::hardware_concurrency();
+  return (int)std::max(1u, std::min(h == 0 ? 1u : h, 16u));
+}
 
 // Per-rule device constants that depend on G.
 int refresh_rule_tables(spk_network* n, RuleHost& r) {
@@ -1115,21 +1177,45 @@ int spk_set_synapses(spk_network* n, int64_t S, const int32_t* pre, const int32_
   if (n->prepared) return fail(SPK_ERR_STATE, "network already prepared");
   if (S < 0) return fail(SPK_ERR_ARG, "negative synapse count");
   if (S > 0 && (!pre || !post || !weight || !delay)) return fail(SPK_ERR_ARG, "null synapse array");
-  for (int64_t s = 0; s < S; ++s) {
-    if (post[s] < 0 || post[s] >= n->N) return fail(SPK_ERR_ARG, "postsynaptic id out of range");
-    if (pre[s] >= n->N || (pre[s] < 0 && -(int64_t)pre[s] - 1 >= n->Nin))
-      return fail(SPK_ERR_ARG, "presynaptic id out of range");
-    if (delay[s] < 1 || delay[s] > 65535) return fail(SPK_ERR_ARG, "delay must be in [1, 65535] steps");
-    if (label && (label[s] < 0 || label[s] >= n->L)) return fail(SPK_ERR_ARG, "label out of range");
+  // the first bad synapse names the error, whatever the thread count
+  static const char* const why[] = {nullptr, "postsynaptic id out of range", "presynaptic id out of range",
+                                    "delay must be in [1, 65535] steps", "label out of range"};
+  const int T = host_threads();
+  std::vector<int> bad(T, 0);   // chunks are in synapse order: the first nonzero names it
+  host_chunks(S, T, [&](int t, int64_t a, int64_t b) {
+    for (int64_t s = a; s < b; ++s) {
+      int e = 0;
+      if (post[s] < 0 || post[s] >= n->N) e = 1;
+      else if (pre[s] >= n->N || (pre[s] < 0 && -(int64_t)pre[s] - 1 >= n->Nin)) e = 2;
+      else if (delay[s] < 1 || delay[s] > 65535) e = 3;
+      else if (label && (label[s] < 0 || label[s] >= n->L)) e = 4;
+      if (e) {
+        bad[t] = e;
+        return;
+      }
+    }
+  });
+  for (int t = 0; t < T; ++t)
+    if (bad[t]) return fail(SPK_ERR_ARG, why[bad[t]]);
+  for (auto* v : {&n->h_pre, &n->h_post, &n->h_delay, &n->h_label}) {
+    v->clear();
+    v->resize(S);
   }
-  n->h_pre.assign(pre, pre + S);
-  n->h_post.assign(post, post + S);
-  n->h_w.assign(weight, weight + S);
-  n->h_delay.assign(delay, delay + S);
-  n->h_label.assign(S, 0);
-  if (label) n->h_label.assign(label, label + S);
-  n->h_scale.assign(S, 1.0f);
-  if (scale) n->h_scale.assign(scale, scale + S);
+  for (auto* v : {&n->h_w, &n->h_scale}) {
+    v->clear();
+    v->resize(S);
+  }
+  host_chunks(S, T, [&](int, int64_t a, int64_t b) {
+    const size_t m = (size_t)(b - a);
+    std::memcpy(n->h_pre.data() + a, pre + a, m * sizeof(int32_t));
+    std::memcpy(n->h_post.data() + a, post + a, m * sizeof(int32_t));
+    std::memcpy(n->h_w.data() + a, weight + a, m * sizeof(float));
+    std::memcpy(n->h_delay.data() + a, delay + a, m * sizeof(int32_t));
+    if (label) std::memcpy(n->h_label.data() + a, label + a, m * sizeof(int32_t));
+    else std::fill(n->h_label.begin() + a, n->h_label.begin() + b, 0);
+    if (scale) std::memcpy(n->h_scale.data() + a, scale + a, m * sizeof(float));
+    else std::fill(n->h_scale.begin() + a, n->h_scale.begin() + b, 1.0f);
+  });
   n->S = S;
   n->syn_set = true;
   return SPK_OK;
@@ -1251,9 +1337,24 @@ int spk_prepare(spk_network* n, int G) {
   n->G = n->Gprep = G;
   n->dmin = S ? 1 << 30 : 0;
   n->dmax = 0;
-  for (int64_t s = 0; s < S; ++s) {
-    n->dmin = std::min(n->dmin, n->h_delay[s]);
-    n->dmax = std::max(n->dmax, n->h_delay[s]);
+  const int T = host_threads();
+  {
+    std::vector<int> lo(T, 1 << 30), hi(T, 0);
+    host_chunks(S, T, [&](int t, int64_t a, int64_t b) {
+      const int* d = n->h_delay.data();
+      int l = 1 << 30, h = 0;   // locals: the per-thread slots share cache lines
+      for (int64_t s = a; s < b; ++s) {
+        l = std::min(l, d[s]);
+        h = std::max(h, d[s]);
+      }
+      lo[t] = l;
+      hi[t] = h;
+    });
+    for (int t = 0; t < T; ++t)
+      if (S) {
+        n->dmin = std::min(n->dmin, lo[t]);
+        n->dmax = std::max(n->dmax, hi[t]);
+      }
   }
   n->B = n->dmax + 2 * G + 1;           // SpikingSynapses.cu:48
   n->W = (n->dmax + 2 * G) / 32 + 1;    // >= dmax + 2G bits of history (SpikingNeurons.cu:33)
@@ -1263,16 +1364,21 @@ int spk_prepare(spk_network* n, int G) {
   for (int64_t s = 0; s < S; ++s)
     if (n->h_scale[s] != n->scale0) { n->uniform_scale = false; break; }
   // fixed-point fraction bits
-  std::vector<double> cap(S);
-  for (int64_t s = 0; s < S; ++s) cap[s] = std::fabs((double)n->h_w[s]);
-  for (auto& r : n->rules)
-    for (int64_t s : r.syn) {
-      double c0 = r.kind == 0 ? std::fabs((double)r.wd.w_max)
-                : (r.kind == 2 ? 1.0 : (r.kind == 3 ? std::fabs((double)r.cu.w_max) : 0.0));
-      // normalization keeps a neuron's afferent norm: no weight exceeds sqrt(sum_sq)
-      if (r.kind == 4) c0 = std::sqrt(std::fabs((double)r.wn_sumsq[n->h_post[s]]));
-      cap[s] = std::max(cap[s], c0);
-    }
+  hvec<double> cap(S);
+  host_chunks(S, T, [&](int, int64_t a, int64_t b) {
+    for (int64_t s = a; s < b; ++s) cap[s] = std::fabs((double)n->h_w[s]);
+  });
+  for (auto& r : n->rules)   // a rule lists a synapse at most once
+    host_chunks((int64_t)r.syn.size(), T, [&](int, int64_t a, int64_t b) {
+      for (int64_t j = a; j < b; ++j) {
+        const int64_t s = r.syn[j];
+        double c0 = r.kind == 0 ? std::fabs((double)r.wd.w_max)
+                  : (r.kind == 2 ? 1.0 : (r.kind == 3 ? std::fabs((double)r.cu.w_max) : 0.0));
+        // normalization keeps a neuron's afferent norm: no weight exceeds sqrt(sum_sq)
+        if (r.kind == 4) c0 = std::sqrt(std::fabs((double)r.wn_sumsq[n->h_post[s]]));
+        cap[s] = std::max(cap[s], c0);
+      }
+    });
   std::vector<double> ent((size_t)n->N * n->L, 0.0);
   for (int64_t s = 0; s < S; ++s)
     ent[(size_t)n->h_post[s] * n->L + n->h_label[s]] += cap[s] * std::fabs((double)n->h_scale[s]);
@@ -1316,13 +1422,9 @@ int spk_prepare(spk_network* n, int G) {
   const int U = n->nu();
   std::vector<int64_t> rowptr(U + 1, 0);
   auto uid = [&](int32_t pre) { return pre >= 0 ? pre : n->N + (-pre - 1); };
-  for (int64_t s = 0; s < S; ++s) rowptr[uid(n->h_pre[s]) + 1]++;
-  for (int u = 0; u < U; ++u) rowptr[u + 1] += rowptr[u];
-  std::vector<int64_t> order(S);
-  {
-    std::vector<int64_t> fill(rowptr.begin(), rowptr.end() - 1);
-    for (int64_t s = 0; s < S; ++s) order[fill[uid(n->h_pre[s])]++] = s;
-  }
+  hvec<int64_t> order(S);
+  counting_sort(S, U, T, [&](int64_t s) { return uid(n->h_pre[s]); },
+                [&](int64_t s, int64_t k) { order[k] = s; }, rowptr);
   // ---- eligibility of the persistent kernel ----
   n->fused_ok = true;
   n->fused_why.clear();
@@ -1349,44 +1451,54 @@ int spk_prepare(spk_network* n, int G) {
   // the (partition, delay) order inside a row serves only the persistent kernel's
   // segments; the per-phase kernels take a row in any order (the ring sums are exact)
   if (n->fused_ok) {
-    for (int u = 0; u < U; ++u) {
-      auto b = order.begin() + rowptr[u], e = order.begin() + rowptr[u + 1];
-      std::stable_sort(b, e, [&](int64_t x, int64_t y) {
-        const int ox = owner[n->h_post[x]], oy = owner[n->h_post[y]];
-        if (ox != oy) return ox < oy;
-        return n->h_delay[x] < n->h_delay[y];
-      });
+    host_chunks(U, T, [&](int, int64_t u0, int64_t u1) {
+      for (int64_t u = u0; u < u1; ++u) {
+        auto b = order.begin() + rowptr[u], e = order.begin() + rowptr[u + 1];
+        std::stable_sort(b, e, [&](int64_t x, int64_t y) {
+          const int ox = owner[n->h_post[x]], oy = owner[n->h_post[y]];
+          if (ox != oy) return ox < oy;
+          return n->h_delay[x] < n->h_delay[y];
+        });
+      }
+    }, S < ((int64_t)1 << 20) ? INT64_MAX : 0);
+  }
+  n->perm.clear();
+  n->perm.resize(S);   // the order is a permutation: every entry is written below
+  hvec<int32_t> tgt(S);
+  hvec<float> w(S), sc(S);
+  hvec<uint16_t> dl(S);
+  hvec<uint8_t> lb(S);
+  host_chunks(S, T, [&](int, int64_t a, int64_t b) {
+    for (int64_t k = a; k < b; ++k) {
+      const int64_t s = order[k];
+      n->perm[s] = k;
+      tgt[k] = n->h_post[s];
+      w[k] = n->h_w[s];
+      sc[k] = n->h_scale[s];
+      dl[k] = (uint16_t)n->h_delay[s];
+      lb[k] = (uint8_t)n->h_label[s];
     }
-  }
-  n->perm.assign(S, 0);
-  std::vector<int32_t> tgt(S);
-  std::vector<float> w(S), sc(S);
-  std::vector<uint16_t> dl(S);
-  std::vector<uint8_t> lb(S);
-  for (int64_t k = 0; k < S; ++k) {
-    const int64_t s = order[k];
-    n->perm[s] = k;
-    tgt[k] = n->h_post[s];
-    w[k] = n->h_w[s];
-    sc[k] = n->h_scale[s];
-    dl[k] = (uint16_t)n->h_delay[s];
-    lb[k] = (uint8_t)n->h_label[s];
-  }
-  std::vector<uint32_t> seg, packed;
+  });
+  std::vector<uint32_t> seg;
+  hvec<uint32_t> packed;
   if (n->fused_ok) {
     seg.assign((size_t)U * (n->P + 1), 0);
-    for (int u = 0; u < U; ++u) {
-      int64_t k = rowptr[u];
-      for (int q = 0; q <= n->P; ++q) {
-        while (k < rowptr[u + 1] && owner[tgt[k]] < q) ++k;
-        seg[(size_t)u * (n->P + 1) + q] = (uint32_t)k;
+    host_chunks(U, T, [&](int, int64_t a, int64_t b) {
+      for (int64_t u = a; u < b; ++u) {
+        int64_t k = rowptr[u];
+        for (int q = 0; q <= n->P; ++q) {
+          while (k < rowptr[u + 1] && owner[tgt[k]] < q) ++k;
+          seg[(size_t)u * (n->P + 1) + q] = (uint32_t)k;
+        }
       }
-    }
+    }, S < ((int64_t)1 << 20) ? INT64_MAX : 0);
     packed.resize(S);
-    for (int64_t k = 0; k < S; ++k) {
-      const uint32_t loc = (uint32_t)(tgt[k] - n->h_nlo[owner[tgt[k]]]);
-      packed[k] = loc | ((uint32_t)dl[k] << 20) | ((uint32_t)lb[k] << 28);
-    }
+    host_chunks(S, T, [&](int, int64_t a, int64_t b) {
+      for (int64_t k = a; k < b; ++k) {
+        const uint32_t loc = (uint32_t)(tgt[k] - n->h_nlo[owner[tgt[k]]]);
+        packed[k] = loc | ((uint32_t)dl[k] << 20) | ((uint32_t)lb[k] << 28);
+      }
+    });
     // compact form: one label, uniform scale, no plasticity, every row with one weight and
     // one delay, one delay for all rows, and the row weights w_lo below a split source and
     // w_hi from it on (an excitatory/inhibitory network numbered E first)
@@ -1492,23 +1604,30 @@ int spk_prepare(spk_network* n, int G) {
   if ((rc = upload(n->d_rowptr, rowptr.data(), (size_t)U + 1, st))) return rc;
   if ((rc = upload(n->d_tgt, tgt.data(), S, st))) return rc;
   {
-    std::vector<uint2> rec(S);
-    for (int64_t k = 0; k < S; ++k) {
-      rec[k].x = packed.empty() ? 0u : packed[k];
-      std::memcpy(&rec[k].y, &w[k], sizeof(float));
-    }
+    hvec<uint2> rec(S);
+    host_chunks(S, T, [&](int, int64_t a, int64_t b) {
+      for (int64_t k = a; k < b; ++k) {
+        rec[k].x = packed.empty() ? 0u : packed[k];
+        std::memcpy(&rec[k].y, &w[k], sizeof(float));
+      }
+    });
     if ((rc = upload(n->d_syn, rec.data(), S, st))) return rc;
   }
   if (n->wfmt != SPK_WEIGHTS_FP32) {
     // 16-bit weights of the records' weights, compensations zero
-    std::vector<uint16_t> w16(S), c16(S, 0);
-    for (int64_t k = 0; k < S; ++k) w16[k] = spk::enc16(w[k], n->wfmt);
+    hvec<uint16_t> w16(S);
+    std::vector<uint16_t> c16(S, 0);
+    host_chunks(S, T, [&](int, int64_t a, int64_t b) {
+      for (int64_t k = a; k < b; ++k) w16[k] = spk::enc16(w[k], n->wfmt);
+    });
     if ((rc = dalloc(&n->d_w16, S)) || (rc = dalloc(&n->d_c16, S))) return rc;
     if ((rc = upload(n->d_w16, w16.data(), S, st)) || (rc = upload(n->d_c16, c16.data(), S, st))) return rc;
   }
   if (n->compact) {
-    std::vector<uint16_t> t16(S);
-    for (int64_t k = 0; k < S; ++k) t16[k] = (uint16_t)(packed[k] & 0xFFFFFu);
+    hvec<uint16_t> t16(S);
+    host_chunks(S, T, [&](int, int64_t a, int64_t b) {
+      for (int64_t k = a; k < b; ++k) t16[k] = (uint16_t)(packed[k] & 0xFFFFFu);
+    });
     if ((rc = dalloc(&n->d_syn16, S))) return rc;
     if ((rc = upload(n->d_syn16, t16.data(), S, st))) return rc;
   }
@@ -1532,10 +1651,8 @@ int spk_prepare(spk_network* n, int G) {
     r.part.assign(n->P + 1, 0);
     r.order.resize(P);
     if (n->fused_ok) {
-      for (size_t j = 0; j < P; ++j) r.part[owner[n->h_post[r.syn[j]]] + 1]++;
-      for (int q = 0; q < n->P; ++q) r.part[q + 1] += r.part[q];
-      std::vector<int64_t> fill(r.part.begin(), r.part.end() - 1);
-      for (size_t j = 0; j < P; ++j) r.order[fill[owner[n->h_post[r.syn[j]]]]++] = (int64_t)j;
+      counting_sort((int64_t)P, n->P, T, [&](int64_t j) { return owner[n->h_post[r.syn[j]]]; },
+                    [&](int64_t j, int64_t k) { r.order[k] = j; }, r.part);
     } else {
       for (size_t j = 0; j < P; ++j) r.order[j] = (int64_t)j;
       bool sorted = true;
@@ -1554,13 +1671,15 @@ int spk_prepare(spk_network* n, int G) {
       if ((rc = dalloc(&r.d_tb16, P))) return rc;
     }
     if ((rc = dalloc(&r.d_part, (size_t)n->P + 1))) return rc;
-    std::vector<int64_t> pos(P);
-    std::vector<int32_t> pre(P);
-    for (size_t j = 0; j < P; ++j) {
-      const int64_t sidx = r.syn[r.order[j]];
-      pos[j] = n->perm[sidx];
-      pre[j] = uid(n->h_pre[sidx]);
-    }
+    hvec<int64_t> pos(P);
+    hvec<int32_t> pre(P);
+    host_chunks((int64_t)P, T, [&](int, int64_t a, int64_t b) {
+      for (int64_t j = a; j < b; ++j) {
+        const int64_t sidx = r.syn[r.order[j]];
+        pos[j] = n->perm[sidx];
+        pre[j] = uid(n->h_pre[sidx]);
+      }
+    });
     if ((rc = upload(r.d_pos, pos.data(), P, st))) return rc;
     if ((rc = upload(r.d_pre, pre.data(), P, st))) return rc;
     if ((rc = upload(r.d_part, r.part.data(), (size_t)n->P + 1, st))) return rc;
@@ -1569,25 +1688,19 @@ int spk_prepare(spk_network* n, int G) {
     r.ev_ok = r.kind == 0 && P > 0 && P < (size_t)INT32_MAX && S < (int64_t)UINT32_MAX;
     if (r.ev_ok) {
       const int U = n->nu();
-      std::vector<int64_t> pp((size_t)U + 1, 0), qp((size_t)n->N + 1, 0);
+      std::vector<int64_t> pp, qp;
       bool grouped = true;
-      for (size_t j = 0; j < P; ++j) {
-        pp[(size_t)pre[j] + 1]++;
-        qp[(size_t)n->h_post[r.syn[r.order[j]]] + 1]++;
-        if (j && pre[j] < pre[j - 1]) grouped = false;
-      }
-      for (int u = 0; u < U; ++u) pp[u + 1] += pp[u];
-      for (int v = 0; v < n->N; ++v) qp[v + 1] += qp[v];
-      std::vector<uint32_t> qs(P);
-      std::vector<int32_t> qu(P);
-      {
-        std::vector<int64_t> fill(qp.begin(), qp.end() - 1);
-        for (size_t j = 0; j < P; ++j) {
-          const int64_t k = fill[n->h_post[r.syn[r.order[j]]]]++;
-          qs[k] = (uint32_t)pos[j];
-          qu[k] = pre[j];
-        }
-      }
+      for (size_t j = 1; j < P && grouped; ++j) grouped = pre[j] >= pre[j - 1];
+      hvec<uint32_t> qs(P);
+      hvec<int32_t> qu(P);
+      hvec<int32_t> pi(grouped ? 0 : P);
+      counting_sort((int64_t)P, U, T, [&](int64_t j) { return pre[j]; },
+                    [&](int64_t j, int64_t k) { if (!grouped) pi[k] = (int32_t)j; }, pp);
+      counting_sort((int64_t)P, n->N, T, [&](int64_t j) { return n->h_post[r.syn[r.order[j]]]; },
+                    [&](int64_t j, int64_t k) {
+                      qs[k] = (uint32_t)pos[j];
+                      qu[k] = pre[j];
+                    }, qp);
       if ((rc = dalloc(&r.d_ev_pre_ptr, (size_t)U + 1))) return rc;
       if ((rc = dalloc(&r.d_ev_post_ptr, (size_t)n->N + 1))) return rc;
       if ((rc = dalloc(&r.d_ev_post_pos, P))) return rc;
@@ -1597,9 +1710,6 @@ int spk_prepare(spk_network* n, int G) {
       if ((rc = upload(r.d_ev_post_pos, qs.data(), P, st))) return rc;
       if ((rc = upload(r.d_ev_post_pre, qu.data(), P, st))) return rc;
       if (!grouped) {
-        std::vector<int32_t> pi(P);
-        std::vector<int64_t> fill(pp.begin(), pp.end() - 1);
-        for (size_t j = 0; j < P; ++j) pi[fill[pre[j]]++] = (int32_t)j;
         if ((rc = dalloc(&r.d_ev_pre_idx, P))) return rc;
         if ((rc = upload(r.d_ev_pre_idx, pi.data(), P, st))) return rc;
       }
