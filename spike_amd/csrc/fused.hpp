@@ -92,6 +92,10 @@ struct FusedArgs {
   const uint16_t* syn16;
   float w_lo, w_hi;
   int src_split, src_delay;
+  // compact form with the partition's delivery table in LDS (copied from seg and syn16 when
+  // the group loop starts): 16-bit local starts of the lseg_U source units' segments (and
+  // the end), at most lrec targets per partition; 0: off
+  int lseg_U, lrec;
   // plasticity (rules grouped by postsynaptic partition), descriptors in device memory
   const FusedRule* rules;
   int n_rules;
@@ -151,7 +155,7 @@ struct PhaseClock {
 // spike masks take 64-bit LDS atomics, which need 8-byte aligned addresses).
 struct FusedLayout {
   size_t ring, hist, ext, imask, inp, xbuf, head, scan, ent, counter, st_cnt, st_nisi, st_prev, st_s1,
-      st_s2, st_g, cdf, guide, total;
+      st_s2, st_g, cdf, guide, lseg, lrec, total;
 };
 
 constexpr int kHeadWords = 16;         // words of a source block fetched in the first trip
@@ -162,8 +166,10 @@ constexpr int kSpikesPerQuad = 4;      // spikes a quad handles with its loads i
 __host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
 
 // `BL` is the number of ring slots kept in LDS (see k_fused: max(largest delay, G)).
+// `segU`, `lrec`: source units and 2-byte targets of the LDS-resident delivery table (0: the
+// table is read from global memory).
 __host__ __device__ inline FusedLayout fused_layout(int BL, int L, int W, int G, int nown, int iown,
-                                                    int xcap, int P) {
+                                                    int xcap, int P, int segU = 0, int lrec = 0) {
   FusedLayout f;
   size_t o = 0;
   f.ring = o;    o = align16(o + (size_t)BL * nown * L * 8);
@@ -185,13 +191,15 @@ __host__ __device__ inline FusedLayout fused_layout(int BL, int L, int W, int G,
   f.st_g = o;    o = align16(o + 64 * 4);
   f.cdf = o;     o = align16(o + (size_t)kMaxExtTable * 4);   // external count thresholds
   f.guide = o;   o = align16(o + 256);                          // guide table of the thresholds
+  f.lseg = o;    o = align16(o + (segU ? (size_t)(segU + 1) * 2 : 0));   // local starts
+  f.lrec = o;    o = align16(o + (size_t)lrec * 2);             // this partition's targets
   f.total = o;
   return f;
 }
 
 __host__ __device__ inline size_t fused_lds_bytes(int BL, int L, int W, int G, int nown, int iown,
-                                                  int xcap, int P) {
-  return fused_layout(BL, L, W, G, nown, iown, xcap, P).total;
+                                                  int xcap, int P, int segU = 0, int lrec = 0) {
+  return fused_layout(BL, L, W, G, nown, iown, xcap, P, segU, lrec).total;
 }
 
 __host__ __device__ inline int fused_xstride(int xcap) { return ((1 + xcap + 7) / 8) * 8; }
@@ -471,6 +479,25 @@ __device__ __forceinline__ void deliver_quad_compact(const FusedArgs& a, const u
   }
 }
 
+// Compact delivery from the LDS-resident table: entry -> local start and end (the next
+// unit's start) -> targets, all in LDS; the quad's lanes take every kQuad-th target.
+__device__ __forceinline__ void deliver_quad_lds(const FusedArgs& a, const uint32_t* ents, int n, int q,
+                                                 int slot0, int B, int nown, int64_t* sring,
+                                                 unsigned long long& st_del, long long q_lo, long long q_hi,
+                                                 const uint16_t* slseg, const uint16_t* slrec) {
+  for (int r = 0; r < n; ++r) {
+    const uint32_t e = ents[r];
+    const uint32_t id = e & kIdMask;
+    const uint32_t s0 = slseg[id], s1 = slseg[id + 1];
+    const unsigned long long qv = (unsigned long long)((int)id < a.src_split ? q_lo : q_hi);
+    int sl = slot0 + (int)(e >> 24) + a.src_delay;
+    while (sl >= B) sl -= B;
+    if (q == 0) st_del += (unsigned long long)(s1 - s0);
+    unsigned long long* row = (unsigned long long*)(sring + (size_t)sl * nown);
+    for (uint32_t k = s0 + (uint32_t)q; k < s1; k += kQuad) atomicAdd(&row[slrec[k]], qv);
+  }
+}
+
 // Plasticity of the synapses onto partition p (one group).
 template <int BS>
 __device__ __forceinline__ void fused_plasticity(const FusedRule* rules, int n_rules, int p, int tidx,
@@ -537,7 +564,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const int p, cons
   const int ilo = a.ilo[p], ihi = a.ilo[p + 1];
   const int nown = nhi - nlo, iown = ihi - ilo;
   const int L = L1 ? 1 : a.L, G = a.G, B = a.B, BL = a.BL, W = a.W;
-  const FusedLayout f = fused_layout(BL, L, W, G, a.nown_max, a.iown_max, a.xcap, a.P);
+  const FusedLayout f = fused_layout(BL, L, W, G, a.nown_max, a.iown_max, a.xcap, a.P, a.lseg_U, a.lrec);
   int64_t* sring = (int64_t*)(smem + f.ring);
   uint32_t* shist = (uint32_t*)(smem + f.hist);
   uint8_t* sext = (uint8_t*)(smem + f.ext);
@@ -556,7 +583,10 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const int p, cons
   unsigned int* sst_g = (unsigned int*)(smem + f.st_g);
   uint32_t* scdf = (uint32_t*)(smem + f.cdf);
   uint8_t* sguide = (uint8_t*)(smem + f.guide);
+  uint16_t* slseg = (uint16_t*)(smem + f.lseg);
+  uint16_t* slrec = (uint16_t*)(smem + f.lrec);
   __shared__ int s_ok;
+  __shared__ int s_wsum[BS / kWave];
   __shared__ unsigned long long s_prof[kProfPhases];
 
   const int NL = a.N * L;
@@ -589,6 +619,40 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const int p, cons
       sst_s2[i] = a.st.s2[nlo + i];
     }
     if (tid < 64) sst_g[tid] = 0u;
+  }
+  // LDS-resident delivery table: this partition's segment of every source row, concatenated
+  // in source order (so start[u + 1] ends segment u).  Thread t takes a run of consecutive
+  // sources: lengths, a workgroup scan of the run sums, then the segments' targets copied
+  // to their local starts.  The host allows it only when a partition has < 2^16 targets.
+  if (a.lseg_U > 0) {
+    const int U = a.lseg_U;
+    const int per = (U + BS - 1) / BS;
+    const int u0 = min(U, tid * per), u1 = min(U, u0 + per);
+    int run = 0;
+    for (int u = u0; u < u1; ++u) {
+      const size_t sb = (size_t)u * (a.P + 1) + p;
+      const uint32_t len = a.seg[sb + 1] - a.seg[sb];
+      slseg[u] = (uint16_t)len;
+      run += (int)len;
+    }
+    int incl = run;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+      const int y = __shfl_up(incl, off);
+      if (lane >= off) incl += y;
+    }
+    if (lane == kWave - 1) s_wsum[tid / kWave] = incl;
+    __syncthreads();
+    int off = incl - run;
+    for (int k = 0; k < tid / kWave; ++k) off += s_wsum[k];
+    for (int u = u0; u < u1; ++u) {
+      const uint32_t len = slseg[u];
+      const uint32_t g0 = a.seg[(size_t)u * (a.P + 1) + p];
+      for (uint32_t k = 0; k < len; ++k) slrec[off + (int)k] = a.syn16[g0 + k];
+      slseg[u] = (uint16_t)off;
+      off += (int)len;
+    }
+    if (u0 < u1 && u1 == U) slseg[U] = (uint16_t)off;
   }
   // per-lane neuron state (lane i owns neuron nlo+i; nown <= BS)
   const bool own = tid < nown;
@@ -943,7 +1007,9 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const int p, cons
         const int nquads = BS / kQuad;
         for (int j0 = quad * kSpikesPerQuad; j0 < nr; j0 += nquads * kSpikesPerQuad) {
           const int n = min(kSpikesPerQuad, nr - j0);
-          if (a.syn16 != nullptr)
+          if (a.lseg_U > 0)
+            deliver_quad_lds(a, sent + j0, n, q, slot0, BL, nownL, sring, st_del, q_lo, q_hi, slseg, slrec);
+          else if (a.syn16 != nullptr)
             deliver_quad_compact(a, sent + j0, n, q, p, slot0, BL, nownL, sring, st_del, q_lo, q_hi);
           else if (a.scale == nullptr)
             deliver_quad<false>(a, sent + j0, n, q, p, slot0, BL, nownL, L, sring, st_del);

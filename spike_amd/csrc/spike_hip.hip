@@ -254,7 +254,9 @@ struct spk_network {
   int req_partitions = 0;
   bool fused_ok = false;
   std::string fused_why;
-  size_t fused_lds = 0;
+  size_t fused_lds = 0;        // dynamic LDS of the persistent kernel
+  size_t fused_lds_base = 0;   // the same without the LDS-resident delivery table
+  int lseg_U = 0, lrec = 0;    // that table: source units, targets per partition (0: off)
   int cu_count = 0;
   std::vector<int> h_nlo, h_ilo;
   int *d_nlo = nullptr, *d_ilo = nullptr;
@@ -780,6 +782,34 @@ const void* fused_kernel(const spk_network* n, bool plastic) {
 #undef SPK_FUSED_PICK
 }
 
+// The compact form's delivery table of each partition -- a 16-bit local start per source
+// unit and the partition's 2-byte targets -- stays in LDS for the whole launch when the
+// largest partition's table fits beside the rest (fewer than 2^16 targets).  `seg` is the
+// [U][P+1] segment table.
+int plan_lds_table(spk_network* n, const uint32_t* seg, int U) {
+  n->fused_lds_base = n->fused_lds;
+  n->lseg_U = 0;
+  n->lrec = 0;
+  if (!n->fused_ok || !n->compact || !n->rules.empty() || U <= 0 || U >= (1 << 24)) return SPK_OK;
+  const int P = n->P;
+  std::vector<int64_t> cnt(P, 0);
+  for (int u = 0; u < U; ++u) {
+    const uint32_t* r = seg + (size_t)u * (P + 1);
+    for (int q = 0; q < P; ++q) cnt[q] += r[q + 1] - r[q];
+  }
+  const int64_t mx = *std::max_element(cnt.begin(), cnt.end());
+  if (mx > 65535) return SPK_OK;
+  const size_t full = spk::fused_lds_bytes(n->BL, n->L, n->W, n->G, n->nown_max, n->iown_max, n->xcap, P,
+                                           U, (int)mx);
+  hipDeviceProp_t prop;
+  SPK_HIP(hipGetDeviceProperties(&prop, n->device));
+  if (full + 2048 > (size_t)prop.sharedMemPerBlock) return SPK_OK;   // room for static LDS
+  n->lseg_U = U;
+  n->lrec = (int)mx;
+  n->fused_lds = full;
+  return SPK_OK;
+}
+
 // Kernel arguments of the persistent kernel (all but the run position).
 int fill_fused_args(spk_network* n, bool plastic, spk::FusedArgs& a) {
   const int G = n->G;
@@ -799,6 +829,8 @@ int fill_fused_args(spk_network* n, bool plastic, spk::FusedArgs& a) {
   if (n->compact && n->rules.empty()) {
     a.syn16 = n->d_syn16;
     a.w_lo = n->cw_lo; a.w_hi = n->cw_hi; a.src_split = n->csplit; a.src_delay = n->cdelay;
+    a.lseg_U = n->lseg_U;
+    a.lrec = n->lrec;
   }
   a.n_rules = plastic ? (int)n->rules.size() : 0;
   for (int r = 0; r < a.n_rules; ++r) {
@@ -1541,6 +1573,7 @@ int spk_prepare(spk_network* n, int G) {
       }
     }
   }
+  if (int rc = plan_lds_table(n, seg.data(), U)) return rc;
   // list capacity: every neuron of both populations may spike once per sub-step
   n->cap = std::max(1, U);
   n->slots = 1;
@@ -1811,7 +1844,7 @@ int spk_get_info(const spk_network* n, spk_info* info) {
   info->partitions = n->P;
   info->fused = n->fused_ok ? 1 : 0;
   info->record_bytes = (n->compact && n->rules.empty()) ? 2 : 8;
-  info->lds_bytes = n->fused_ok ? (int)n->fused_lds : 0;
+  info->lds_bytes = n->fused_ok ? (int)n->fused_lds_base : 0;
   return SPK_OK;
 }
 
@@ -2390,6 +2423,10 @@ int spk_create_brunel(const spk_brunel_params* p, int partitions, spk_network** 
     cleanup();
     return bail(rc);
   }
+  if ((rc = plan_lds_table(n, seg.data(), N))) {
+    cleanup();
+    return bail(rc);
+  }
   if (S > 0) {
     hipLaunchKernelGGL(spk::k_gen_scatter, dim3(grid), dim3(256), 0, st, ga, S);
     SPK_HIP(hipGetLastError());
@@ -2418,6 +2455,7 @@ struct spk_batch {
   int nb = 0;                 // workgroups of the launch
   int bs = spk::kFusedBlock;  // threads per workgroup
   size_t lds = 0;
+  bool lds_tables = true;     // networks keep their delivery tables in LDS where planned
   hipStream_t stream = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   spk::FusedArgs* d_args = nullptr;
@@ -2463,6 +2501,8 @@ int spk_batch_create(spk_network* const* nets, int n_nets, int workgroups, spk_b
     b->lds = std::max(b->lds, n->fused_lds);
     b->nets.push_back(n);
   }
+  size_t lds_base = 0;
+  for (auto* n : b->nets) lds_base = std::max(lds_base, n->fused_lds_base);
   const void* fn = batch_kernel(b, spk::kFusedBlock);
   if (b->lds > 65536) SPK_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)b->lds));
   int per_cu = 0;
@@ -2472,13 +2512,16 @@ int spk_batch_create(spk_network* const* nets, int n_nets, int workgroups, spk_b
   // other waits at its network's exchange
   int pmax_own = 0;
   for (auto* n : b->nets) pmax_own = std::max(pmax_own, n->nown_max);
+  // (without the LDS delivery tables, which would take the second workgroup's room)
   if (pmax_own <= spk::kBatchSmallBlock) {
     const void* fs = batch_kernel(b, spk::kBatchSmallBlock);
-    if (b->lds > 65536) SPK_HIP(hipFuncSetAttribute(fs, hipFuncAttributeMaxDynamicSharedMemorySize, (int)b->lds));
+    if (lds_base > 65536) SPK_HIP(hipFuncSetAttribute(fs, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_base));
     int per_cu_s = 0;
-    SPK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_s, fs, spk::kBatchSmallBlock, b->lds));
+    SPK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_s, fs, spk::kBatchSmallBlock, lds_base));
     if (per_cu_s >= 2 * per_cu) {
       b->bs = spk::kBatchSmallBlock;
+      b->lds = lds_base;
+      b->lds_tables = false;
       per_cu = per_cu_s;
     }
   }
@@ -2586,6 +2629,7 @@ int spk_batch_run(spk_batch* b, const float* starts, const int64_t* ngroups, int
   for (int k = 0; k < K; ++k) {
     spk_network* n = b->nets[k];
     if (int rc = fill_fused_args(n, false, args[k])) return rc;
+    if (!b->lds_tables) args[k].lseg_U = args[k].lrec = 0;
     if (int rc = stage_fused(n, args[k], starts[k], 0, (int)ngroups[k], b->stream)) return rc;
     if (n->profiling) {
       if (!n->d_prof) {
