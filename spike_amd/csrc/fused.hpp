@@ -981,23 +981,39 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const int p, cons
       }
       __syncthreads();
       const int total = sscan[a.P];
+      int lps = 1;   // lanes per source: the largest power of two (up to 64) with lps * P <= BS
+      while (lps < kWave && lps * 2 * a.P <= BS) lps *= 2;
+      // lane groups when the sources are many (short lists, mostly in the heads)
+      const bool by_source = lps <= 16;
       for (int r0 = 0; r0 < total; r0 += kEntCap) {
-        // (3) stage entries [r0, r0 + kEntCap) of the concatenated lists, one entry per
-        //     thread: its source is the last with sscan[src] <= index (empty sources share
-        //     their start with the next one); the first kHeadWords-1 entries of a source
-        //     come from its head in LDS
-        const int nst = min(kEntCap, total - r0);
-        for (int e = tid; e < nst; e += BS) {
-          const int gidx = r0 + e;
-          int lo = 0, hi = a.P;
-          while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (sscan[mid] <= gidx) lo = mid;
-            else hi = mid;
+        // (3) stage entries [r0, r0 + kEntCap) of the concatenated lists; the first
+        //     kHeadWords-1 entries of a source come from its head in LDS, the rest from its
+        //     block.  Either a group of lps lanes per source, or one entry per thread with
+        //     its source found by binary search (the last with sscan[src] <= index; empty
+        //     sources share their start with the next one), which spreads long lists over
+        //     the whole workgroup.
+        if (by_source) {
+          for (int src = tid / lps; src < a.P; src += BS / lps) {
+            const int b0 = sscan[src], b1 = sscan[src + 1];
+            const int j0 = max(0, r0 - b0), j1 = min(b1 - b0, r0 + kEntCap - b0);
+            for (int j = j0 + (tid & (lps - 1)); j < j1; j += lps)
+              sent[b0 + j - r0] = (j + 1 < kHeadWords) ? shead[src * kHeadWords + 1 + j]
+                                                       : load_coherent(xbase + (size_t)src * a.xstride + 1 + j);
           }
-          const int j = gidx - sscan[lo];
-          sent[e] = (j + 1 < kHeadWords) ? shead[lo * kHeadWords + 1 + j]
-                                         : load_coherent(xbase + (size_t)lo * a.xstride + 1 + j);
+        } else {
+          const int nst = min(kEntCap, total - r0);
+          for (int e = tid; e < nst; e += BS) {
+            const int gidx = r0 + e;
+            int lo = 0, hi = a.P;
+            while (hi - lo > 1) {
+              const int mid = (lo + hi) >> 1;
+              if (sscan[mid] <= gidx) lo = mid;
+              else hi = mid;
+            }
+            const int j = gidx - sscan[lo];
+            sent[e] = (j + 1 < kHeadWords) ? shead[lo * kHeadWords + 1 + j]
+                                           : load_coherent(xbase + (size_t)lo * a.xstride + 1 + j);
+          }
         }
         __syncthreads();
         clk.lap(a, 7);
