@@ -479,6 +479,20 @@ __device__ __forceinline__ void deliver_quad_compact(const FusedArgs& a, const u
   }
 }
 
+// Inclusive prefix sum over the 64 lanes of a wavefront by DPP: shifts by 1, 2, 4, 8 inside
+// each row of 16 lanes (lanes past the row start read 0), then row 0's total added to row 1
+// and row 2's to row 3 (row_bcast:15), then the total of rows 0-1 added to rows 2 and 3
+// (row_bcast:31).  Lanes outside a step's row mask add 0.
+__device__ __forceinline__ int wave_inclusive_sum(int v) {
+  v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, true);   // row_shr:1
+  v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, true);   // row_shr:2
+  v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, true);   // row_shr:4
+  v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, true);   // row_shr:8
+  v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);  // row_bcast:15, rows 1, 3
+  v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);  // row_bcast:31, rows 2, 3
+  return v;
+}
+
 // Compact delivery from the LDS-resident table: entry -> local start and end (the next
 // unit's start) -> targets, all in LDS; the quad's lanes take every kQuad-th target.
 __device__ __forceinline__ void deliver_quad_lds(const FusedArgs& a, const uint32_t* ents, int n, int q,
@@ -955,29 +969,18 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const int p, cons
       }
       __syncthreads();
       clk.lap(a, 6);
-      // (2) wavefront 0: exclusive scan of the counts (each lane a run of sources)
+      // (2) wavefront 0: exclusive scan of the counts, kWave consecutive sources at a time
       const int cps = (a.P + kWave - 1) / kWave;
       if (tid < kWave) {
-        int run = 0;
+        int base = 0;
         for (int k = 0; k < cps; ++k) {
-          const int src = tid * cps + k;
-          if (src < a.P) run += min((int)shead[src * kHeadWords], a.xcap);
+          const int src = k * kWave + lane;
+          const int c = src < a.P ? min((int)shead[src * kHeadWords], a.xcap) : 0;
+          const int incl = wave_inclusive_sum(c);
+          if (src < a.P) sscan[src] = base + incl - c;
+          base += __builtin_amdgcn_readlane(incl, kWave - 1);
         }
-        int incl = run;
-#pragma unroll
-        for (int off = 1; off < kWave; off <<= 1) {
-          const int y = __shfl_up(incl, off);
-          if (lane >= off) incl += y;
-        }
-        int off = incl - run;
-        for (int k = 0; k < cps; ++k) {
-          const int src = tid * cps + k;
-          if (src < a.P) {
-            sscan[src] = off;
-            off += min((int)shead[src * kHeadWords], a.xcap);
-          }
-        }
-        if (tid == kWave - 1) sscan[a.P] = incl;
+        if (tid == 0) sscan[a.P] = base;
       }
       __syncthreads();
       const int total = sscan[a.P];
@@ -995,7 +998,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const int p, cons
         if (by_source) {
           for (int src = tid / lps; src < a.P; src += BS / lps) {
             const int b0 = sscan[src], b1 = sscan[src + 1];
-            const int j0 = max(0, r0 - b0), j1 = min(b1 - b0, r0 + kEntCap - b0);
+            const int j0 = max(0, r0 - b0), j1 = min(min(b1 - b0, a.xcap), r0 + kEntCap - b0);
             for (int j = j0 + (tid & (lps - 1)); j < j1; j += lps)
               sent[b0 + j - r0] = (j + 1 < kHeadWords) ? shead[src * kHeadWords + 1 + j]
                                                        : load_coherent(xbase + (size_t)src * a.xstride + 1 + j);
@@ -1012,7 +1015,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const int p, cons
             }
             const int j = gidx - sscan[lo];
             sent[e] = (j + 1 < kHeadWords) ? shead[lo * kHeadWords + 1 + j]
-                                           : load_coherent(xbase + (size_t)lo * a.xstride + 1 + j);
+                    : (j < a.xcap ? load_coherent(xbase + (size_t)lo * a.xstride + 1 + j) : 0u);
           }
         }
         __syncthreads();
