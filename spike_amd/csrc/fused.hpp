@@ -158,7 +158,7 @@ struct FusedLayout {
       st_s2, st_g, cdf, guide, lseg, lrec, total;
 };
 
-constexpr int kHeadWords = 16;         // words of a source block fetched in the first trip
+constexpr int kHeadWords = 12;         // words of a source block fetched in the first trip
 constexpr int kEntCap = 4096;          // spike entries staged in LDS per delivery round
 constexpr int kQuad = 4;               // lanes cooperating on one spike's row segment
 constexpr int kSpikesPerQuad = 4;      // spikes a quad handles with its loads in flight together

@@ -782,6 +782,22 @@ const void* fused_kernel(const spk_network* n, bool plastic) {
 #undef SPK_FUSED_PICK
 }
 
+// Dynamic LDS a persistent workgroup may take: the per-workgroup limit less the static LDS
+// of the persistent kernel's two forms, and a margin for the batched form's (the same body
+// with its own scan scratch).  Without it a partition sized just under the limit would
+// leave the grid no co-resident workgroup.
+size_t fused_lds_limit(const spk_network* n) {
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, n->device) != hipSuccess) return 0;
+  size_t st = 0;
+  for (bool plastic : {false, true}) {
+    hipFuncAttributes fa;
+    if (hipFuncGetAttributes(&fa, fused_kernel(n, plastic)) == hipSuccess) st = std::max(st, fa.sharedSizeBytes);
+  }
+  const size_t lim = (size_t)prop.sharedMemPerBlock;
+  return lim > st + 512 ? lim - st - 512 : 0;
+}
+
 // The compact form's delivery table of each partition -- a 16-bit local start per source
 // unit and the partition's 2-byte targets -- stays in LDS for the whole launch when the
 // largest partition's table fits beside the rest (fewer than 2^16 targets).  `seg` is the
@@ -801,9 +817,13 @@ int plan_lds_table(spk_network* n, const uint32_t* seg, int U) {
   if (mx > 65535) return SPK_OK;
   const size_t full = spk::fused_lds_bytes(n->BL, n->L, n->W, n->G, n->nown_max, n->iown_max, n->xcap, P,
                                            U, (int)mx);
-  hipDeviceProp_t prop;
-  SPK_HIP(hipGetDeviceProperties(&prop, n->device));
-  if (full + 2048 > (size_t)prop.sharedMemPerBlock) return SPK_OK;   // room for static LDS
+  if (full > fused_lds_limit(n)) return SPK_OK;
+  // and the grid stays co-resident with the larger LDS (both forms of the kernel)
+  for (bool plastic : {false, true}) {
+    int per_cu = 0;
+    SPK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fused_kernel(n, plastic), spk::kFusedBlock, full));
+    if ((int64_t)per_cu * n->cu_count < n->P) return SPK_OK;
+  }
   n->lseg_U = U;
   n->lrec = (int)mx;
   n->fused_lds = full;
@@ -1475,11 +1495,7 @@ int spk_prepare(spk_network* n, int G) {
     if (r.kind == 4) no_fused("weight normalization rule");
   n->BL = std::max(n->dmax, G);
   n->fused_lds = spk::fused_lds_bytes(n->BL, n->L, n->W, G, n->nown_max, n->iown_max, n->xcap, n->P);
-  {
-    hipDeviceProp_t prop;
-    SPK_HIP(hipGetDeviceProperties(&prop, n->device));
-    if (n->fused_lds > (size_t)prop.sharedMemPerBlock) no_fused("input ring slice does not fit in LDS");
-  }
+  if (n->fused_lds > fused_lds_limit(n)) no_fused("input ring slice does not fit in LDS");
   // the (partition, delay) order inside a row serves only the persistent kernel's
   // segments; the per-phase kernels take a row in any order (the ring sums are exact)
   if (n->fused_ok) {
@@ -2346,7 +2362,7 @@ int spk_create_brunel(const spk_brunel_params* p, int partitions, spk_network** 
   n->fused_lds = spk::fused_lds_bytes(n->BL, 1, n->W, G, n->nown_max, 0, n->xcap, P);
   if (n->nown_max > spk::kFusedBlock) return bail(fail(SPK_ERR_ARG, "more neurons per partition than threads per workgroup"));
   if (P > spk::kMaxPartitions) return bail(fail(SPK_ERR_ARG, "too many partitions"));
-  if (n->fused_lds > (size_t)prop.sharedMemPerBlock)
+  if (n->fused_lds > fused_lds_limit(n))
     return bail(fail(SPK_ERR_ARG, "input ring slice does not fit in LDS: use more partitions"));
   n->fused_ok = true;
   n->fused_why.clear();
