@@ -105,7 +105,9 @@ typedef struct {
   int record_bytes;          /* bytes the persistent kernel reads per delivery: 8 for the
                                 full records {target|delay|label, weight}; 2 for the compact
                                 form (16-bit target; weight and delay per source type) */
-  int lds_bytes;             /* LDS per workgroup of the persistent kernel (0: not used) */
+  int lds_bytes;             /* LDS per workgroup the persistent kernel needs (0: not used);
+                                the optional delivery table it keeps in LDS when that fits
+                                is not counted */
 } spk_info;
 
 /* Per-run timing of the last spk_run call, from HIP events on the network's stream. */
