@@ -96,3 +96,28 @@ def test_batched_rates_within_2pc_of_fp64():
         rate64, _ = oracle_rate(p, seconds, precision=64, threads=8)
         print(f"point {k}: N={p.n} rate FP32 batch {r.rate[k]:.3f} FP64 {rate64:.3f}", flush=True)
         assert abs(r.rate[k] - rate64) <= 0.02 * rate64
+
+
+def test_every_partition_count_builds_co_resident_or_reports_lds():
+    """Partition counts around the LDS limit: generation either fails with the LDS message
+    (so the sweep builder splits further) or gives a network whose persistent grid is
+    co-resident -- it runs alone and in a batch.  A partition sized just under the
+    per-workgroup limit, with the kernel's static LDS on top, must not get through."""
+    p = sweep.SweepPoint(n=12000, g=5.0, nu_ratio=2.0, J=0.1e-3, delay_steps=20, seed=41)
+    built, refused = 0, 0
+    for P in range(12, 40):   # 12: at most 1000 neurons per partition
+        print(f"[batch] N=12000 delay 20 P={P}", flush=True)
+        try:
+            net = abi.generated_brunel(sweep.brunel_params(p), P)
+        except abi.SpikeError as e:
+            assert "LDS" in str(e), str(e)
+            refused += 1
+            continue
+        built += 1
+        assert net.info().fused == 1
+        net.run(0.0, 2, plasticity=False)
+        b = abi.Batch([net], 0)
+        b.run(np.zeros(1), [2])
+        b.close()
+        net.close()
+    assert built > 0 and refused > 0
