@@ -32,7 +32,7 @@ pytestmark = pytest.mark.gpu
 from test_gpu_parity import _compare  # noqa: E402
 
 
-def _bit_exact_first_steps(d, steps=1000, tag=""):
+def _bit_exact_first_steps(d, steps=1000, tag="", force_launches=False):
     G = d.timestep_grouping()
     ng = -(-steps // G)
     print(f"[configs] {tag}: N={d.n_neurons} Nin={d.n_inputs} S={d.n_synapses} G={G} groups={ng}", flush=True)
@@ -41,7 +41,7 @@ def _bit_exact_first_steps(d, steps=1000, tag=""):
     gi, oi = gpu.info(), ora.info()
     assert (gi.timestep_grouping, gi.ring_slots, gi.history_words, gi.frac_bits) == \
         (oi["G"], oi["B"], oi["W"], oi["frac"])
-    gpu.run(0.0, ng)
+    gpu.run(0.0, ng, force_launches=force_launches)
     ora.run(0.0, ng)
     c, oc = gpu.counters(), ora.counters()
     assert c.spikes_neurons == oc["spikes_neurons"], tag
@@ -70,6 +70,19 @@ def test_coba_vogels_first_1000_steps():
 def test_large_plastic_scaled_first_1000_steps():
     d = builders.large_plastic(n=5000, out_degree=100, monitors=(0, 1))
     _bit_exact_first_steps(d, tag="large plastic / 100")
+
+
+@pytest.mark.parametrize("launches", [False, True])
+def test_large_plastic_threaded_preparation(launches):
+    """Above 2^20 synapses the host preparation runs on threads (validation, counting sorts,
+    gathers, the rule's index sorts); the device layout and so every result must be the
+    sequential one: bit-exact against the oracle with FP16 weights, on the persistent
+    kernel and on the per-phase kernels (whose rule is driven by the spike-indexed arrays)."""
+    d = builders.large_plastic(n=10000, out_degree=170, monitors=(0, 1))
+    d.weight_format = 2
+    assert len(d.rules[0].synapses) > (1 << 20)
+    _bit_exact_first_steps(d, steps=300, tag=f"large plastic / 170, threaded preparation, launches={launches}",
+                           force_launches=launches)
 
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "coba_vogels_fp64_2s.json")
