@@ -228,13 +228,16 @@ def plan_from_work(points, events_per_group, target_events, neurons_per_wg_max, 
 
 
 def autotune(points: Sequence[SweepPoint], probe_seconds: float = 0.02,
-             targets=(2e4, 5e4, 1e5, 2e5, 5e5), neurons_per_wg_max=(384, 1024, 256), log=None) -> Dict:
+             targets=(2e4, 5e4, 1e5, 2e5, 5e5), neurons_per_wg_max=(384, 1024, 256), log=None,
+             refine_seconds: float = 0.1, refine_top: int = 3) -> Dict:
     """The project's tiling autotuner.  (1) A probe run of the whole sweep with the
     size-only plan measures every network's deliveries per group and its time per group;
     (2) a least-squares fit of time = fixed + per_event * events/workgroup gives the cost
     model; (3) every candidate (events-per-workgroup target, most neurons per workgroup)
-    is turned into a plan (plan_from_work) and timed on a probe run; the fastest plan is
-    returned.  Fewer, larger partitions pay the per-group fixed cost fewer times; a
+    is turned into a plan (plan_from_work) and timed on a probe run; (4) the refine_top
+    fastest are timed again on refine_seconds (a short probe ranks plans within a few
+    per cent of each other unreliably: their schedules diverge later in the run) and the
+    fastest of those is returned.  Fewer, larger partitions pay the per-group fixed cost fewer times; a
     partition whose ring slice would not fit in LDS is split further when built.  Plans of
     at most 512 neurons per workgroup also keep every partition's LDS within half a compute
     unit's, so that the batch runs two 512-thread workgroups per compute unit."""
@@ -250,6 +253,7 @@ def autotune(points: Sequence[SweepPoint], probe_seconds: float = 0.02,
     wg = r0.config["workgroups"]
     trials = [dict(plan=base["name"], ms=r0.ms)]
     best = dict(plan=base, ms=r0.ms)
+    timed = [(r0.ms, base)]
     if log:
         log(f"autotune probe {base['name']}: {r0.ms:.2f} ms; fit {fixed:.2f} us + {per_ev * 1e3:.3f} ns/event")
     for npw, tgt in [(npw, tgt) for npw in neurons_per_wg_max for tgt in targets]:
@@ -260,11 +264,22 @@ def autotune(points: Sequence[SweepPoint], probe_seconds: float = 0.02,
             trials.append(dict(plan=plan["name"], error=str(e)))
             continue
         trials.append(dict(plan=plan["name"], ms=r.ms, estimated_ms=r.config["estimated_ms"]))
+        timed.append((r.ms, plan))
         if log:
             log(f"autotune {plan['name']}: {r.ms:.2f} ms (scheduler estimate {r.config['estimated_ms']:.2f}, "
                 f"{r.config['workgroups']} x {r.config['threads']} threads)")
         if r.ms < best["ms"]:
             best = dict(plan=plan, ms=r.ms)
+    if refine_top > 1 and refine_seconds > probe_seconds:
+        finalists = [pl for _, pl in sorted(timed, key=lambda x: x[0])[:refine_top]]
+        best = None
+        for plan in finalists:
+            r = run_batched(points, refine_seconds, plan)
+            trials.append(dict(plan=plan["name"], refine_ms=r.ms))
+            if log:
+                log(f"autotune refine {plan['name']}: {r.ms:.2f} ms over {refine_seconds} s")
+            if best is None or r.ms < best["ms"]:
+                best = dict(plan=plan, ms=r.ms)
     best["trials"] = trials
     best["fit"] = dict(fixed_us=fixed, per_event_us=per_ev)
     return best
