@@ -15,7 +15,7 @@
  *   - Synapse arrays are passed in the caller's order; the library keeps its own
  *     delay-sorted CSR internally and returns weights in the caller's order.
  *   - Times are float seconds exactly as the reference passes them to its backends
- *     (t = start + (s*G)*dt, Spike/Models/SpikingModel.cpp:294-296); the timestep index
+ *     (t = start + (s*G)*dt, Spike/Models/SpikingModel.cpp:225-226); the timestep index
  *     is roundf(t/dt) as in Spike/Backend/CUDA/Neurons/LIFSpikingNeurons.cu:84.
  *   - Synaptic input is accumulated into the per-delay input ring as exact 64-bit
  *     fixed-point sums (2^-frac_bits units), so results do not depend on the order in
@@ -205,7 +205,7 @@ int spk_set_synapses(spk_network* net, int64_t n, const int32_t* pre, const int3
 
 /* Plasticity rules over synapse indices (caller's order).  Replace
  * Backend::STDPPlasticity::prepare (CUDA STDPPlasticity.cu:13-41) plus the rule's
- * prepare (WeightDependentSTDPPlasticity.cu:28-43, InhibitorySTDPPlasticity.cu:196-214,
+ * prepare (WeightDependentSTDPPlasticity.cu:28-43, InhibitorySTDPPlasticity.cu:30-48,
  * EvansSTDPPlasticity.cu:30-37).  rule_id receives the index used by
  * spk_update_plasticity. */
 int spk_add_wd_stdp(spk_network* net, const int64_t* synapses, int64_t n,
@@ -221,7 +221,7 @@ int spk_add_evans_stdp(spk_network* net, const int64_t* synapses, int64_t n,
  * weights clamped at 0. */
 int spk_add_custom_stdp(spk_network* net, const int64_t* synapses, int64_t n,
                         const spk_custom_stdp_params* p, int* rule_id);
-/* WeightNormSTDPPlasticity (WeightNormSTDPPlasticity.cpp:31-75, CUDA
+/* WeightNormSTDPPlasticity (WeightNormSTDPPlasticity.cpp:32-72, CUDA
  * WeightNormSTDPPlasticity.cu:68-172): each application rescales the rule's synapses onto
  * every neuron v with in_set[v] by sqrtf(sum_sq[v] + u[v]) / sqrtf(sum_sq[v]), where
  * u[v] = sum over v's synapses, in the caller's order, of d*d + 2*w0*d with d = w - w0
@@ -338,7 +338,7 @@ int spk_monitor_reset(spk_network* net, int monitor_id);
 /* Copy device records (SpikingActivityMonitor.cu:41-50) and clear the device store. */
 int spk_monitor_drain(spk_network* net, int monitor_id, int32_t* ids, float* times,
                       int64_t cap, int64_t* n);
-/* Rate monitor (RateActivityMonitor.cpp:14-40, CUDA RateActivityMonitor.cu:37-78): the
+/* Rate monitor (RateActivityMonitor.cpp:17-34, CUDA RateActivityMonitor.cu:38-74): the
  * number of spikes of every neuron (or input) of a population, counted where the spikes
  * are made.  Added before spk_prepare; counts since prepare or the monitor's last reset
  * (copy_spike_count_to_host / reset_state). */

@@ -13,7 +13,9 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "_build", "libspike_oracle.so")
+# SPIKE_ORACLE_LIB selects another build of the same source (the mutation test loads
+# oracles with planted mistakes this way)
+LIB_PATH = os.environ.get("SPIKE_ORACLE_LIB") or os.path.join(HERE, "_build", "libspike_oracle.so")
 
 _lib = None
 

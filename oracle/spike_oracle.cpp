@@ -15,12 +15,12 @@
 //   Poisson inputs ...... Spike/Backend/CUDA/Neurons/PoissonInputSpikingNeurons.cu:73-130
 //   spike history ....... Spike/Backend/CUDA/Neurons/SpikingNeurons.cu:23-94
 //   WD STDP ............. Spike/Backend/CUDA/Plasticity/WeightDependentSTDPPlasticity.cu:67-163
-//   Vogels iSTDP ........ Spike/Backend/CUDA/Plasticity/InhibitorySTDPPlasticity.cu:238-321
+//   Vogels iSTDP ........ Spike/Backend/CUDA/Plasticity/InhibitorySTDPPlasticity.cu:72-155
 //   Evans STDP .......... Spike/Backend/CUDA/Plasticity/EvansSTDPPlasticity.cu:59-143
 //   Custom STDP ......... Spike/Backend/CUDA/Plasticity/CustomSTDPPlasticity.cu:67-155
 //   nearest-spike traces  Spike/Backend/CUDA/Plasticity/CustomSTDPPlasticity.cu:117-128
 //   raster .............. Spike/Backend/CUDA/ActivityMonitor/SpikingActivityMonitor.cu:69-98
-//   group loop .......... Spike/Models/SpikingModel.cpp:266-305
+//   group loop .......... Spike/Models/SpikingModel.cpp:197-236
 //
 // Decisions shared with the product (they define the semantics both must meet):
 //   * FP32 oracle: the input ring holds exact int64 fixed-point sums with `frac_bits`
@@ -629,7 +629,7 @@ struct Net {
         r.ta[j] = tstore(xpre); r.tb[j] = tstore(xpost);
       }
     } else if (r.kind == 1) {
-      // InhibitorySTDPPlasticity.cu:238-321 (the target term is evaluated in double)
+      // InhibitorySTDPPlasticity.cu:72-155 (the target term is evaluated in double)
       const float decay = std::exp(-dt / r.vg[0]);
       const float lr = r.vg[1];
       const double cterm = (double)lr * (2.0 * (double)r.vg[2] * (double)r.vg[0]);
@@ -758,7 +758,7 @@ struct Net {
     }
   }
 
-  // SpikingModel.cpp:266-305
+  // SpikingModel.cpp:197-236
   void run(float start, int64_t ngroups, int flags) {
     for (int64_t s = 0; s < ngroups; ++s) {
       float t = start + (float)(s * G) * dt;

@@ -225,7 +225,7 @@ void ModelNetwork::build() {
       check(spk_add_evans_stdp(net, idx.data(), (int64_t)idx.size(), &p, &rid), "spk_add_evans_stdp");
     } else if (auto* wn = dynamic_cast<::WeightNormSTDPPlasticity*>(rule)) {
       // the front-end computed the squared afferent norms (and rescaled the weights for a
-      // target) in its prepare_backend_early (WeightNormSTDPPlasticity.cpp:31-75)
+      // target) in its prepare_backend_early (WeightNormSTDPPlasticity.cpp:32-72)
       const int N = lif->total_number_of_neurons;
       std::vector<float> sq(N, 0.0f);
       std::vector<uint8_t> in(N, 0);

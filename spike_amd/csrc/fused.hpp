@@ -1,5 +1,5 @@
 // fused.hpp -- persistent single-network kernel: the whole timestep-group loop of
-// SpikingModel::run (SpikingModel.cpp:266-305) in one cooperative launch.
+// SpikingModel::run (SpikingModel.cpp:197-236) in one cooperative launch.
 //
 // Owner computes.  The LIF neurons are split into P contiguous partitions, one per
 // workgroup (the Poisson inputs likewise).  A workgroup keeps its neurons' slice of the

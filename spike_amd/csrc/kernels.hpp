@@ -813,7 +813,7 @@ __device__ __forceinline__ bool pre_bit(const StdpArgs& a, int u, int64_t step) 
                    : hist_get(a.hist_in, a.Nin, a.W, u - a.N, step);
 }
 
-// InhibitorySTDPPlasticity.cu:238-321. tab = {decay, lr}; dterm = lr*(2*rate*tau) (double)
+// InhibitorySTDPPlasticity.cu:72-155. tab = {decay, lr}; dterm = lr*(2*rate*tau) (double)
 __device__ __forceinline__ void stdp_vogels_one(const StdpArgs& a, int64_t j) {
   const int64_t s = a.pos[j];
   const int u = a.pre[j];

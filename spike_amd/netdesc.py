@@ -96,13 +96,13 @@ class NetDesc:
         return int(np.min(self.delay))
 
     def groups_for(self, seconds: float) -> int:
-        """Group count of SpikingModel::run (SpikingModel.cpp:289-291): the int/int
+        """Group count of SpikingModel::run (SpikingModel.cpp:221-222): the int/int
         division truncates, so a 1 s Brunel run at G=15 covers 9990 steps."""
         steps = int(math.ceil(F32(seconds) / F32(self.dt)))
         return steps // self.timestep_grouping()
 
     def group_time(self, start: float, s: int) -> np.float32:
-        """t = start + s*G*dt in float32 (SpikingModel.cpp:295)."""
+        """t = start + s*G*dt in float32 (SpikingModel.cpp:226)."""
         return F32(F32(start) + F32(F32(s * self.timestep_grouping()) * F32(self.dt)))
 
 

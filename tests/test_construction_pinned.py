@@ -10,7 +10,6 @@ import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-sys.path.insert(0, os.path.join(ROOT, "oracle"))
 import construction as C  # noqa: E402
 
 GOLD = os.path.join(ROOT, "tests", "golden")

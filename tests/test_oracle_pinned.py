@@ -184,32 +184,271 @@ def test_poisson_input_rate_and_isi():
     assert abs(np.mean(isis == 1) - p) < 0.01
 
 
-def test_conductance_decay_closed_form():
-    """g_n = d^n g_0 after a single delivery; the conductance-driven V follows it."""
-    d = _single(kind=SYN_CONDUCTANCE, theta=1.0)
-    d.n_inputs = 1
-    d.rates = np.zeros((1, 1), F32)
-    d.decay_g = np.array([F32(math.exp(-1e-4 / 5e-3))], F32)
-    d.reversal = np.array([0.0], F32)
-    d.pre = np.array([0], np.int32)
-    d.post = np.array([0], np.int32)
-    d.weight = np.array([0.0], F32)
-    d.delay = np.array([1], np.int32)
+def test_voltage_injection_step_by_step():
+    """Delta synapses (InlineDeviceFunctions.hpp:86-111 with LIFSpikingNeurons.cu:168-175):
+    two generator inputs with delays 2 and 5 into one neuron; every step the membrane
+    takes c*v0 + (1-c)*V_old + c*I_bg + (sum of the step's ring slot), recomputed in
+    float32, and the slot is emptied once read -- a slot left full would deliver again one
+    ring turn (B steps) later.  One arrival crosses threshold; the arrivals inside the
+    following 2 ms are consumed and lost, and V stays at V_reset."""
+    d = NetDesc(n_neurons=1, n_inputs=2, kind=SYN_VOLTAGE)
+    d.v0 = np.full(1, F32(-70e-3), F32)
+    d.vreset = np.full(1, F32(-75e-3), F32)
+    d.threshold = np.full(1, F32(-60e-3), F32)
+    d.decay_c, d.resist = lif_constants(1, 200e-12, 10e-9, d.dt)
+    d.refractory = float(F32(2e-3))
+    d.pre = np.array([-1, -2], np.int32)
+    d.post = np.array([0, 0], np.int32)
+    w = [F32(2.0 ** -9), F32(2.0 ** -6)]            # ~2 mV and ~16 mV, exact in the ring
+    d.weight = np.array(w, F32)
+    d.delay = np.array([2, 5], np.int32)
+    fire = [[3, 4, 30, 31, 60], [20, 24]]
+    ids = np.concatenate([np.full(len(f), l, np.int32) for l, f in enumerate(fire)])
+    times = np.concatenate([np.array([F32(k) * F32(1e-4) for k in f], F32) for f in fire])
+    order = np.argsort(times, kind="stable")
+    d.stimuli = [(ids[order], times[order])]
+    d.grouping = 1
+    d.monitors = [0]
+    o = oracle_py.OracleNetwork(d, precision=32)
+    B = o.info()["B"]
+    arrivals = {}
+    for l, f in enumerate(fire):
+        for k in f:
+            arrivals[k + int(d.delay[l])] = arrivals.get(k + int(d.delay[l]), F32(0.0)) + w[l]
+    c, v0, vr, thr = d.decay_c[0], d.v0[0], d.vreset[0], d.threshold[0]
+    one = F32(1.0)
+    V, last, spikes = v0, F32(-1000.0), []
+    for step in range(60 + 3 * B):
+        t = F32(step) * F32(1e-4)
+        o.run(float(t), 1)
+        dv = arrivals.get(step, F32(0.0))
+        if F32(F32(t - last) - F32(d.refractory)) > F32(F32(0.5) * F32(1e-4)):
+            V = F32(F32(F32(c * v0) + F32(F32(one - c) * V)) + F32(c * F32(0.0))) + dv
+            if V >= thr:
+                spikes.append(step)
+                last, V = t, vr
+        assert o.membrane()[0] == np.float64(V), step
+    got = np.rint(o.take_spikes(0)[1].astype(np.float64) / 1e-4).astype(int).tolist()
+    assert got == spikes == [25]
+    # the arrival at step 29 (input 1 fired at 24, delay 5) fell inside the refractory
+    # window of the spike at 25 and was lost; nothing arrived again B steps later
+    assert 29 in arrivals and 25 + 20 > 29
+
+
+def _conductance_net(w, delays, fire_steps, taus=(5e-3, 10e-3), rev=(0.0, -80e-3), n_steps=None):
+    """One LIF neuron (threshold out of reach) fed by two generator inputs through two
+    conductance labels: input l fires at the steps in fire_steps[l] and reaches label l
+    after delays[l] steps."""
+    from spike_amd.netdesc import conductance_decay
+    d = NetDesc(n_neurons=1, n_inputs=2, kind=SYN_CONDUCTANCE, n_labels=2)
+    d.v0 = np.full(1, F32(-70e-3), F32)
+    d.vreset = np.full(1, F32(-70e-3), F32)
+    d.threshold = np.full(1, F32(1.0), F32)
+    d.decay_c, d.resist = lif_constants(1, 200e-12, 10e-9, d.dt)
+    d.decay_g = np.array([conductance_decay(d.dt, t) for t in taus], F32)
+    d.reversal = np.array(rev, F32)
+    d.pre = np.array([-1, -2], np.int32)
+    d.post = np.array([0, 0], np.int32)
+    d.label = np.array([0, 1], np.int32)
+    d.weight = np.array(w, F32)
+    d.delay = np.array(delays, np.int32)
+    ids = np.concatenate([np.full(len(s), l, np.int32) for l, s in enumerate(fire_steps)])
+    times = np.concatenate([np.array([F32(k) * F32(1e-4) for k in s], F32) for s in fire_steps])
+    order = np.argsort(times, kind="stable")
+    d.stimuli = [(ids[order], times[order])]
+    d.grouping = 1
+    d.monitors = [0]
+    return d
+
+
+def test_conductance_injection_step_by_step():
+    """Two conductance labels with opposite reversal potentials (0 mV and -80 mV) on one
+    neuron, recomputed step by step in float32 (InlineDeviceFunctions.hpp:22-51 with
+    LIFSpikingNeurons.cu:168-169): per label g = d*g + input, then
+    I += g*(E - V_old), and the membrane takes c*v0 + (1-c)*V_old + c*I_bg + R'*I.  The
+    conductances also follow the closed form g(n) = g(n0)*d^(n-n0) between arrivals, and
+    each label pushes V towards its own reversal potential."""
+    w = [F32(2.0 ** -30), F32(2.0 ** -28)]          # exact in the fixed-point ring
+    delays = [1, 2]
+    fire = [[5, 6, 20, 40], [8, 20, 21]]
+    arrive = [{s + delays[l] for s in fire[l]} for l in range(2)]
+    d = _conductance_net(w, delays, fire)
+    o = oracle_py.OracleNetwork(d, precision=32)
+    dec, rev = d.decay_g, d.reversal
+    c, rp, v0 = d.decay_c[0], d.resist[0], d.v0[0]
+    one = F32(1.0)
+    g = [F32(0.0), F32(0.0)]
+    V = v0
+    last_arrival = [0, 0]
+    g_at = [F32(0.0), F32(0.0)]
+    for step in range(80):
+        o.run(float(F32(step) * F32(1e-4)), 1)
+        total = F32(0.0)
+        for l in range(2):
+            inp = w[l] if step in arrive[l] else F32(0.0)
+            g[l] = F32(F32(dec[l] * g[l]) + inp)
+            total = F32(total + F32(g[l] * F32(rev[l] - V)))
+            if step in arrive[l]:
+                last_arrival[l], g_at[l] = step, g[l]
+        V_new = F32(F32(F32(c * v0) + F32(F32(one - c) * V)) + F32(c * F32(0.0))) + F32(total * rp)
+        assert np.array_equal(o.conductances(), np.array(g, np.float64)), step
+        assert o.membrane()[0] == np.float64(V_new), step
+        for l in range(2):
+            if step > last_arrival[l] and g_at[l] > 0:
+                k = step - last_arrival[l]
+                ref = float(g_at[l]) * float(dec[l]) ** k
+                assert abs(float(g[l]) - ref) <= 1e-6 * ref, (step, l)
+        V = V_new
+    # each label pushes V towards its own reversal potential: against the same network
+    # with that label's weight set to zero, the excitatory arrival at step 6 (E = 0)
+    # raises V, the inhibitory one at step 10 (E = -80 mV) lowers it
+    def trace(wv):
+        net = oracle_py.OracleNetwork(_conductance_net(wv, delays, fire), precision=64)
+        vs = []
+        for step in range(12):
+            net.run(float(F32(step) * F32(1e-4)), 1)
+            vs.append(net.membrane()[0])
+        return np.array(vs)
+    both, no_inh, no_exc = trace(w), trace([w[0], 0.0]), trace([0.0, w[1]])
+    assert both[6] == no_inh[6] and no_inh[6] > no_exc[6]
+    assert both[10] < no_inh[10] and no_exc[10] < no_exc[9]
+
+
+def test_conductance_at_rest_reversal_is_silent():
+    """A conductance whose reversal potential equals the resting potential injects
+    nothing while V sits there: with both labels at E = v0 the FP64 membrane equals the
+    run without any input, step for step (a dropped (E - V) factor would not)."""
+    fire = [[2, 3, 4, 5, 9], [3, 7]]
+    base = oracle_py.OracleNetwork(_conductance_net([0.0, 0.0], [1, 1], fire, rev=(-70e-3, -70e-3)),
+                                   precision=64)
+    shunt = oracle_py.OracleNetwork(_conductance_net([F32(2.0 ** -28), F32(2.0 ** -27)], [1, 1], fire,
+                                                     rev=(-70e-3, -70e-3)), precision=64)
+    for step in range(30):
+        base.run(float(F32(step) * F32(1e-4)), 1)
+        shunt.run(float(F32(step) * F32(1e-4)), 1)
+        assert shunt.membrane()[0] == base.membrane()[0], step
+    assert shunt.conductances().max() > 0
+
+
+def test_conductance_steady_state_is_the_weighted_reversal_mean():
+    """Constant input on both labels every step: g_l settles at w_l / (1 - d_l) and V at
+    the conductance-weighted mean of the reversal potentials,
+    V* = (c*v0 + R'*sum_l g_l E_l) / (c + R'*sum_l g_l)  (the fixed point of
+    V' = c*v0 + (1-c)*V + R'*sum_l g_l (E_l - V)), in FP64."""
+    steps = 4000
+    fire = [list(range(steps)), list(range(steps))]
+    w = [F32(2.0 ** -32), F32(2.0 ** -33)]
+    d = _conductance_net(w, [1, 1], fire)
     o = oracle_py.OracleNetwork(d, precision=64)
-    lib = o.lib
-    # inject 1 nS into the ring slot of step 0 through the weight API: use an input spike
-    o.close()
-    d.pre = np.array([-1], np.int32)
-    d.weight = np.array([1e-9], F32)
-    d.rates = np.full((1, 1), F32(1e9), F32)  # spikes every step after arming
+    o.run(0.0, steps)
+    dec = d.decay_g.astype(np.float64)
+    gs = np.array([float(w[0]), float(w[1])]) / (1.0 - dec)
+    assert np.allclose(o.conductances(), gs, rtol=1e-12, atol=0)
+    c, rp, v0 = float(d.decay_c[0]), float(d.resist[0]), float(d.v0[0])
+    E = d.reversal.astype(np.float64)
+    vstar = (c * v0 + rp * float(np.dot(gs, E))) / (c + rp * float(gs.sum()))
+    assert abs(o.membrane()[0] - vstar) < 1e-12
+    assert -80e-3 < vstar < -45e-3 and abs(vstar - v0) > 5e-3
+
+
+def _isolated_driven(n, lam=2.0, w=1e-3, seed=11):
+    """n unconnected LIF neurons (threshold out of reach) with the per-neuron external
+    Poisson drive of mean lam events per step, each event worth w volts."""
+    d = NetDesc(n_neurons=n, n_inputs=0, kind=SYN_VOLTAGE, seed=seed)
+    d.v0 = np.zeros(n, F32)
+    d.vreset = np.zeros(n, F32)
+    d.threshold = np.full(n, F32(1e3), F32)
+    d.decay_c, d.resist = lif_constants(n, 200e-12, 10e-9, d.dt)
+    d.ext_rate = float(lam / d.dt)
+    d.ext_weight = float(F32(w))
+    d.grouping = 1
+    return d
+
+
+def _drive_counts(o, d, prev):
+    """Per-neuron event count of the last step, recovered from the FP64 membrane
+    V' = c*v0 + (1-c)*V + k*w (v0 = 0)."""
+    c = float(d.decay_c[0])
+    v = o.membrane()
+    k = (v - (1.0 - c) * prev) / d.ext_weight
+    ki = np.rint(k)
+    assert np.max(np.abs(k - ki)) < 1e-6
+    return ki.astype(np.int64), v
+
+
+def test_external_drive_count_is_poisson_inverse_cdf_of_philox():
+    """The external drive of neuron i at step n is k = F^-1((u+1)/2^32) for the Poisson
+    (lambda) CDF F, u = draw n of (i, stream 2) -- output n mod 4 of the Philox call at
+    counter (n/4, 0, i, 2), key = seed (KAT-checked above) -- and k events of weight w
+    join the step's input.  Checked per neuron against scipy's Poisson quantile over two
+    steps of 4000 neurons."""
+    from scipy.stats import poisson
+    lam, seed, n = 2.0, 11, 4000
+    d = _isolated_driven(n, lam=lam, seed=seed)
     o = oracle_py.OracleNetwork(d, precision=64)
-    o.run(0.0, 2)  # input spikes at step 1 -> arrives at step 2
-    o.set_grouping(1)
-    dec = float(d.decay_g[0])
-    # after step 2: g = 1e-9 delivered at step 2
-    g = o.conductances()[0]
-    assert g == 0.0  # delivery lands at step 2, not yet consumed after steps 0..1
-    assert lib is not None
+    prev = np.zeros(n)
+    for step in (0, 1):
+        o.run(float(F32(step) * F32(1e-4)), 1)
+        k, prev = _drive_counts(o, d, prev)
+        u = np.array([oracle_py.philox([step // 4, 0, i, 2], [seed & 0xFFFFFFFF, seed >> 32])[step % 4]
+                      for i in range(n)], np.float64)
+        expect = poisson.ppf((u + 1.0) / 2.0 ** 32, lam).astype(np.int64)
+        assert np.array_equal(k, expect), step
+    assert o.counters()["external_events"] > 0
+
+
+def test_external_drive_count_histogram_is_poisson():
+    """Over 10^5 (neuron, step) samples the per-step event counts follow the Poisson(2)
+    pmf: every bin with expectation above 20 lies within 5 sigma, and the sample mean
+    and variance are both lambda within sampling error."""
+    lam, n, steps = 2.0, 20000, 5
+    d = _isolated_driven(n, lam=lam, seed=5)
+    o = oracle_py.OracleNetwork(d, precision=64)
+    prev = np.zeros(n)
+    ks = []
+    for step in range(steps):
+        o.run(float(F32(step) * F32(1e-4)), 1)
+        k, prev = _drive_counts(o, d, prev)
+        ks.append(k)
+    ks = np.concatenate(ks)
+    m = len(ks)
+    hist = np.bincount(ks, minlength=16)
+    for j in range(16):
+        e = m * math.exp(-lam) * lam ** j / math.factorial(j)
+        if e > 20:
+            assert abs(hist[j] - e) < 5 * math.sqrt(e), (j, hist[j], e)
+    assert abs(ks.mean() - lam) < 5 * math.sqrt(lam / m)
+    assert abs(ks.var() - lam) < 5 * lam * math.sqrt(2.0 / m) + 0.02
+    assert o.counters()["external_events"] == int(ks.sum())
+
+
+def test_brunel_connectivity_counts_and_rows():
+    """The Philox fixed in-degree rule (the sweep's generated networks): every neuron has
+    exactly C_E sources in [0, N_E) followed by C_I in [N_E, N); slot j of neuron i is
+    floor(u*N_E/2^32) (j < C_E) or N_E + floor(u*N_I/2^32), u = draw j of (i, stream 3),
+    recomputed for a few rows from the KAT-checked block function; and the sources of
+    each type are uniform (chi-square over all neurons)."""
+    seed, ne, ni, ce, ci = 2024, 1600, 400, 160, 40
+    pre = oracle_py.brunel_connectivity(seed, ne, ni, ce, ci)
+    n = ne + ni
+    assert pre.shape == (n, ce + ci)
+    assert np.all((pre[:, :ce] >= 0) & (pre[:, :ce] < ne))
+    assert np.all((pre[:, ce:] >= ne) & (pre[:, ce:] < n))
+    key = [seed & 0xFFFFFFFF, seed >> 32]
+    for i in (0, 1, 777, n - 1):
+        row = []
+        for j in range(ce + ci):
+            u = int(oracle_py.philox([j // 4, 0, i, 3], key)[j % 4])
+            row.append((u * ne) >> 32 if j < ce else ne + ((u * ni) >> 32))
+        assert pre[i].tolist() == row, i
+    from scipy.stats import chisquare
+    he = np.bincount(pre[:, :ce].ravel(), minlength=ne)
+    hi = np.bincount(pre[:, ce:].ravel() - ne, minlength=ni)
+    assert chisquare(he).pvalue > 1e-4 and chisquare(hi).pvalue > 1e-4
+    # rows are independent draws: different neurons do not share their source lists
+    assert len({tuple(r) for r in pre[:50]}) == 50
+    # another seed gives another network
+    assert not np.array_equal(pre, oracle_py.brunel_connectivity(seed + 1, ne, ni, ce, ci))
 
 
 def test_frac_bits_rule():

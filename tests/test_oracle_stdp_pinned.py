@@ -32,7 +32,6 @@ import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 sys.path.insert(0, ROOT)
-sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
 import oracle_py  # noqa: E402
 from spike_amd import netdesc  # noqa: E402
@@ -205,13 +204,17 @@ def test_evans_protocol(delay, w0):
     close(w, evans_expected(pre, post, w0, p, delay, ng * delay), w0)
 
 
-def test_evans_clamp_to_unit_interval():
+@pytest.mark.parametrize("alpha_c,w0", [(400.0, 0.5), (15000.0, 0.5), (400.0, 1.5)])
+def test_evans_clamp_to_unit_interval(alpha_c, w0):
+    """w is clamped to [0, 1] after every update (EvansSTDPPlasticity.cu:127-130).  With
+    dt*alpha_C > 1 the concentration trace overshoots 1, so potentiation overshoots w = 1;
+    a weight set above 1 is pulled to 1 by its first update."""
     p = dict(decay_term_tau_D=0.005, model_parameter_alpha_D=400.0,
-             synaptic_neurotransmitter_concentration_alpha_C=400.0, decay_term_tau_C=0.004,
+             synaptic_neurotransmitter_concentration_alpha_C=alpha_c, decay_term_tau_C=0.004,
              learning_rate_rho=0.1)
-    d = pair_net(Rule(RULE_EVANS, np.array([0], np.int64), p), 0.5, delay=2, thresholds=(5e-3, 6e-3), tref=1e-3)
+    d = pair_net(Rule(RULE_EVANS, np.array([0], np.int64), p), w0, delay=2, thresholds=(5e-3, 6e-3), tref=1e-3)
     pre, post, w = run_oracle(d, 1000)
-    exp = evans_expected(pre, post, 0.5, p, 2, 2000)
+    exp = evans_expected(pre, post, w0, p, 2, 2000)
     assert 0.0 <= w <= 1.0
     assert abs(w - exp) <= 1e-6
 
