@@ -88,6 +88,18 @@ int spk_mlp_train(spk_mlp* m, int steps, int batch, int mode, uint64_t seed, flo
 /* Outputs of every member for n rows: features [n][5] -> out [models][n][3]. */
 int spk_mlp_predict(spk_mlp* m, const float* features, int64_t n, float* out);
 
+/* spk_mlp_predict without any device allocation.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory -- tight screening loops that serve
+ * predictions, code that must not call hipMalloc.  spk_mlp_predict grows two device
+ * staging buffers on first use; this variant stages features and outputs through one
+ * fixed 4 MiB device array per GPU, held until the process ends, and calls no hipMalloc,
+ * hipFree or other allocating routine.  Limit: round_up(5 n, 64) + 3 n models floats must
+ * fit in 4 MiB (1,048,576 floats; e.g. n <= 36,157 rows for 8 members), else SPK_ERR_ARG.
+ * Every call on one GPU uses that GPU's one array: calls that overlap in time on the same
+ * GPU (from several threads, on any handles) race and must be serialised by the caller.
+ * Results equal spk_mlp_predict's bit for bit. */
+int spk_mlp_predict_noalloc(spk_mlp* m, const float* features, int64_t n, float* out);
+
 int spk_mlp_last_timing(const spk_mlp* m, spk_mlp_timing* t);
 
 #ifdef __cplusplus

@@ -172,6 +172,7 @@ SIGNATURES = {
     "spk_mlp_train": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_uint64, C.c_float, C.c_float, C.c_int,
                                 C.c_int, fp]),
     "spk_mlp_predict": (C.c_int, [P, fp, C.c_int64, fp]),
+    "spk_mlp_predict_noalloc": (C.c_int, [P, fp, C.c_int64, fp]),
     "spk_mlp_last_timing": (C.c_int, [P, C.c_void_p]),
 }
 

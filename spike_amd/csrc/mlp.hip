@@ -31,6 +31,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -535,6 +536,7 @@ struct spk_mlp {
   int* step = nullptr;
   float *px = nullptr, *py = nullptr;
   int64_t pcap = 0;
+  float* scratch = nullptr;  // this device's copy of the file-scope staging array
   float wout[3] = {1.0f, 1.0f, 1.0f};
   hipStream_t stream = nullptr;
   hipEvent_t ev[6] = {};
@@ -545,6 +547,39 @@ struct spk_mlp {
   int graph_total = 0;
   spk_mlp_timing timing = {};
 };
+
+// Staging memory of spk_mlp_predict_noalloc: features then outputs, in a file-scope
+// device array of kNoAllocBytes.  Every GPU the library runs on holds its own copy until
+// the process ends.  The array's name is a host-side stand-in in host code: its device
+// address (one per GPU) comes from hipGetSymbolAddress at spk_mlp_create.
+constexpr size_t kNoAllocBytes = size_t(4) << 20;
+constexpr size_t kNoAllocFloats = kNoAllocBytes / sizeof(float);
+static __device__ __align__(256) float scratch[kNoAllocFloats];
+
+namespace {
+
+int mfail(int code, const std::string& msg);
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float* g_scratch_addr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device `dev`, looked up once per device.
+int scratch_address(int dev, float** out) {
+  if (dev < 0 || dev >= kMaxDevices) return mfail(SPK_ERR_ARG, "device ordinal out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_addr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return mfail(SPK_ERR_HIP, std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+    g_scratch_addr[dev] = (float*)p;
+  }
+  *out = g_scratch_addr[dev];
+  return SPK_OK;
+}
+
+}  // namespace
 
 namespace {
 
@@ -664,7 +699,10 @@ int spk_mlp_create(int models, int max_batch, spk_mlp** out) {
   MLP_HIP(hipGetDevice(&dev));
   int rc = lds_attributes();
   if (rc) return rc;
+  float* scr = nullptr;
+  if ((rc = scratch_address(dev, &scr))) return rc;
   spk_mlp* m = new spk_mlp();
+  m->scratch = scr;
   m->E = models;
   m->max_batch = (max_batch + MT - 1) / MT * MT;
   const size_t E = models, Mx = m->max_batch;
@@ -897,6 +935,33 @@ int spk_mlp_predict(spk_mlp* m, const float* features, int64_t n, float* out) {
   MLP_HIP(hipEventRecord(m->ev[1], m->stream));
   MLP_HIP(hipGetLastError());
   MLP_HIP(hipMemcpyAsync(out, m->py, (size_t)m->E * n * DOUT * sizeof(float), hipMemcpyDeviceToHost, m->stream));
+  MLP_HIP(hipStreamSynchronize(m->stream));
+  const float ms = elapsed(m->ev[0], m->ev[1]);
+  m->timing = spk_mlp_timing{ms, ms, 0, 0, 0, 0, n};
+  return SPK_OK;
+}
+
+int spk_mlp_predict_noalloc(spk_mlp* m, const float* features, int64_t n, float* out) {
+  if (!m || !features || !out || n < 1) return mfail(SPK_ERR_ARG, "bad arguments");
+  // features at the start of the array, outputs from the next 256-byte boundary
+  const int64_t xfl = (n * DIN + 63) / 64 * 64;
+  const int64_t need = xfl + (int64_t)m->E * n * DOUT;
+  if (n > (int64_t)kNoAllocFloats || need > (int64_t)kNoAllocFloats)
+    return mfail(SPK_ERR_ARG, "spk_mlp_predict_noalloc: " + std::to_string(n) + " rows x " +
+                                  std::to_string(m->E) + " members need " + std::to_string(need * 4) +
+                                  " bytes of staging; the static limit is " + std::to_string(kNoAllocBytes));
+  float* px = m->scratch;
+  float* py = m->scratch + xfl;
+  MLP_HIP(hipMemcpyAsync(px, features, (size_t)n * DIN * sizeof(float), hipMemcpyHostToDevice, m->stream));
+  FwdArgs f{};
+  f.wb = m->wb; f.master = m->master; f.x = px; f.n = n; f.M = (int)n; f.mode = 0;
+  f.y = py;
+  const int tiles = (int)((n + MT - 1) / MT);
+  MLP_HIP(hipEventRecord(m->ev[0], m->stream));
+  hipLaunchKernelGGL(k_mlp_forward, dim3(tiles, m->E), dim3(NT), LDS_BYTES, m->stream, f);
+  MLP_HIP(hipEventRecord(m->ev[1], m->stream));
+  MLP_HIP(hipGetLastError());
+  MLP_HIP(hipMemcpyAsync(out, py, (size_t)m->E * n * DOUT * sizeof(float), hipMemcpyDeviceToHost, m->stream));
   MLP_HIP(hipStreamSynchronize(m->stream));
   const float ms = elapsed(m->ev[0], m->ev[1]);
   m->timing = spk_mlp_timing{ms, ms, 0, 0, 0, 0, n};

@@ -113,6 +113,15 @@ class Mlp:
         abi.check(self.lib.spk_mlp_predict(self.h, abi._ptr(x, C.c_float), x.shape[0], abi._ptr(out, C.c_float)))
         return out
 
+    def predict_noalloc(self, features: np.ndarray) -> np.ndarray:
+        """predict() through the allocation-free entry point (a fixed 4 MiB staging array
+        per GPU; calls overlapping on one GPU race)."""
+        x = _f32(features)
+        out = np.empty((self.models, x.shape[0], OUTPUTS), np.float32)
+        abi.check(self.lib.spk_mlp_predict_noalloc(self.h, abi._ptr(x, C.c_float), x.shape[0],
+                                                   abi._ptr(out, C.c_float)))
+        return out
+
     def timing(self) -> MlpTiming:
         t = MlpTiming()
         abi.check(self.lib.spk_mlp_last_timing(self.h, C.byref(t)))
