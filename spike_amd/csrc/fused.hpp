@@ -99,8 +99,13 @@ struct FusedArgs {
   // plasticity (rules grouped by postsynaptic partition), descriptors in device memory
   const FusedRule* rules;
   int n_rules;
-  // exchange blocks [2][P][xstride]: {count, entries (id | g << 24)...}
+  // exchange blocks [2][P][xstride]: {count, entries (id | g << 24)...}; 32-bit words
+  // behind the grid barrier (plastic launches), 64-bit tagged words otherwise (xlist64, the
+  // same memory): low half the count or entry, high half the tag of the group that wrote it
   uint32_t* xlist;
+  unsigned long long* xlist64;
+  unsigned int xtag_base;     // tag of group s of this launch: xtag_base + s + 1
+  int xquad;                  // tagged delivery by quads per source (small partitions) or staged
   // two-level grid barrier: counters bar[g * kBarStride] (g < 8), bar[8 * kBarStride]
   // (level 2) and bar[9 * kBarStride] (release generation)
   unsigned int* bar;
@@ -218,6 +223,22 @@ __device__ __forceinline__ void store_coherent(uint32_t* p, uint32_t v) {
 __device__ __forceinline__ uint32_t load_coherent(const uint32_t* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+
+// Tagged exchange words (launches without plasticity): every 64-bit word carries the tag of
+// the group that wrote it next to its payload, and a reader accepts a word only when the
+// tag is the group's own.  A word is written and read whole (naturally aligned 64-bit
+// accesses at agent scope), so no ordering between the words of a block is needed and the
+// writer never waits for its stores: there is no grid barrier.  Safety of the double
+// buffer: a workgroup writes the parity-p blocks of group s + 2 only after it has read
+// every source's block of group s + 1, which each source publishes only after it has
+// finished reading the parity-p blocks of group s.
+__device__ __forceinline__ void xstore64(unsigned long long* p, uint32_t v, uint32_t tag) {
+  __hip_atomic_store(p, ((unsigned long long)tag << 32) | v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned long long xload64(const unsigned long long* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t xtag(unsigned long long w) { return (uint32_t)(w >> 32); }
 
 // Barrier arrival (thread 0, after the workgroup's stores are complete): one increment of
 // the level-1 counter of this workgroup's member index vb (0..P-1); the last arrival of a level-1 group increments the
@@ -512,6 +533,222 @@ __device__ __forceinline__ void deliver_quad_lds(const FusedArgs& a, const uint3
   }
 }
 
+constexpr int kQuadEnt = 8;   // entries of one source a quad stages in LDS at a time
+
+// One quad's delivery of n entries staged in LDS, kSpikesPerQuad at a time, by the form of
+// the delivery table the launch uses.
+__device__ __forceinline__ void deliver_staged(const FusedArgs& a, const uint32_t* ents, int n, int q, int p,
+                                               int slot0, int BL, int nownL, int L, int64_t* sring,
+                                               unsigned long long& st_del, long long q_lo, long long q_hi,
+                                               const uint16_t* slseg, const uint16_t* slrec) {
+  for (int j0 = 0; j0 < n; j0 += kSpikesPerQuad) {
+    const int m = min(kSpikesPerQuad, n - j0);
+    if (a.lseg_U > 0)
+      deliver_quad_lds(a, ents + j0, m, q, slot0, BL, nownL, sring, st_del, q_lo, q_hi, slseg, slrec);
+    else if (a.syn16 != nullptr)
+      deliver_quad_compact(a, ents + j0, m, q, p, slot0, BL, nownL, sring, st_del, q_lo, q_hi);
+    else if (a.scale == nullptr)
+      deliver_quad<false>(a, ents + j0, m, q, p, slot0, BL, nownL, L, sring, st_del);
+    else
+      deliver_quad<true>(a, ents + j0, m, q, p, slot0, BL, nownL, L, sring, st_del);
+  }
+}
+
+// Delivery of group s without a grid barrier (tagged exchange words).  Quad k of the
+// workgroup takes sources k, k + BS/4, ...: lane q polls words q and q + 4 of the source's
+// block -- the count and the first seven entries -- until the count and every entry below
+// it carry the group's tag, the entries are staged in the quad's LDS slice and delivered;
+// longer lists follow eight entries at a time with their loads in flight together.
+// Returns false (in every lane of the wave) when a wait exceeded the spin limit or another
+// workgroup reported an error; the error flag is then set.
+template <int BS>
+__device__ __forceinline__ bool tagged_gather_deliver(const FusedArgs& a, int p, int par, uint32_t tag, int slot0,
+                                                      int BL, int nownL, int L, int64_t* sring,
+                                                      unsigned long long& st_del, long long q_lo, long long q_hi,
+                                                      const uint16_t* slseg, const uint16_t* slrec, uint32_t* sent) {
+  const int tid = threadIdx.x;
+  const int lane = (int)__lane_id();
+  const int quad = tid / kQuad, q = tid % kQuad;
+  constexpr int nquads = BS / kQuad;
+  const int qbase = lane & ~(kQuad - 1);
+  uint32_t* qent = sent + (size_t)quad * kQuadEnt;
+  const unsigned long long* xb = a.xlist64 + (size_t)par * a.P * a.xstride;
+  bool ok = true;
+  for (int src0 = 0; src0 < a.P; src0 += nquads) {
+    // uniform point of the loop: stop every lane once one has failed
+    if (__any(!ok)) {
+      ok = false;
+      break;
+    }
+    const int src = src0 + quad;
+    const bool active = src < a.P;
+    const unsigned long long* blk = xb + (size_t)(active ? src : 0) * a.xstride;
+    unsigned long long w0 = active ? xload64(blk + q) : 0ull;
+    unsigned long long w1 = active ? xload64(blk + q + kQuad) : 0ull;
+    long long spins = 0;
+    int c = 0;
+    // wave-uniform loop: every lane shuffles the count of its quad's source
+    while (true) {
+      const bool t0 = xtag(w0) == tag, t1 = xtag(w1) == tag;
+      const int cw = __shfl((int)(uint32_t)w0, qbase);
+      const bool cok = __shfl(t0 ? 1 : 0, qbase) != 0;
+      c = min(cw, a.xcap);
+      const bool need0 = active && !t0 && (q == 0 || !cok || q <= c);
+      const bool need1 = active && !t1 && (!cok || q + kQuad <= c);
+      if (!__any(need0 || need1)) break;
+      if (++spins > a.spin_limit) {
+        atomicCAS(a.error, 0, 1);
+        ok = false;
+        break;
+      }
+      if ((spins & 63) == 0 && __hip_atomic_load(a.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+        ok = false;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+      if (need0) w0 = xload64(blk + q);
+      if (need1) w1 = xload64(blk + q + kQuad);
+    }
+    if (!ok) break;
+    if (!active) continue;
+    // word q holds entry q - 1, word q + 4 entry q + 3
+    if (q >= 1 && q - 1 < c) qent[q - 1] = (uint32_t)w0;
+    if (q + 3 < c) qent[q + 3] = (uint32_t)w1;
+    __builtin_amdgcn_wave_barrier();
+    deliver_staged(a, qent, min(c, kQuadEnt - 1), q, p, slot0, BL, nownL, L, sring, st_del, q_lo, q_hi, slseg,
+                   slrec);
+    for (int j0 = kQuadEnt - 1; j0 < c; j0 += kQuadEnt) {
+      const int m = min(kQuadEnt, c - j0);
+      const unsigned long long* eb = blk + 1 + j0;
+      unsigned long long e0 = q < m ? xload64(eb + q) : 0ull;
+      unsigned long long e1 = q + kQuad < m ? xload64(eb + q + kQuad) : 0ull;
+      long long sp = 0;
+      while ((q < m && xtag(e0) != tag) || (q + kQuad < m && xtag(e1) != tag)) {
+        if (++sp > a.spin_limit) {
+          atomicCAS(a.error, 0, 1);
+          ok = false;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+        if (q < m && xtag(e0) != tag) e0 = xload64(eb + q);
+        if (q + kQuad < m && xtag(e1) != tag) e1 = xload64(eb + q + kQuad);
+      }
+      if (!ok) break;
+      __builtin_amdgcn_wave_barrier();
+      if (q < m) qent[q] = (uint32_t)e0;
+      if (q + kQuad < m) qent[q + kQuad] = (uint32_t)e1;
+      __builtin_amdgcn_wave_barrier();
+      deliver_staged(a, qent, m, q, p, slot0, BL, nownL, L, sring, st_del, q_lo, q_hi, slseg, slrec);
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+  return !__any(!ok);
+}
+
+// Delivery of group s without a grid barrier when partitions are large (long lists, few
+// sources): (1) thread t polls source t's count word until it carries the group's tag;
+// (2) wavefront 0 scans the counts; (3) the entries of the concatenated lists are staged
+// into LDS kEntCap at a time, spread over the whole workgroup (entry index -> source by
+// binary search over the scan), every thread's loads in flight together and then checked
+// against the tag; (4) quads deliver kSpikesPerQuad staged entries at a time.  `scnt` is
+// LDS for P counts, `sscan` for P + 1 offsets.  Returns false when a wait failed (the error
+// flag is then set); uniform across the workgroup.
+template <int BS>
+__device__ __forceinline__ bool tagged_gather_stage(const FusedArgs& a, int p, int par, uint32_t tag, int slot0,
+                                                    int BL, int nownL, int L, int64_t* sring,
+                                                    unsigned long long& st_del, long long q_lo, long long q_hi,
+                                                    const uint16_t* slseg, const uint16_t* slrec, uint32_t* sent,
+                                                    int* scnt, int* sscan, int* s_ok) {
+  constexpr int kPer = kEntCap / BS;   // staged entries per thread and round
+  const int tid = threadIdx.x;
+  const int lane = (int)__lane_id();
+  const unsigned long long* xb = a.xlist64 + (size_t)par * a.P * a.xstride;
+  for (int src = tid; src < a.P; src += BS) {
+    const unsigned long long* blk = xb + (size_t)src * a.xstride;
+    unsigned long long w = xload64(blk);
+    long long spins = 0;
+    bool ok = true;
+    while (xtag(w) != tag) {
+      if (++spins > a.spin_limit) {
+        atomicCAS(a.error, 0, 1);
+        ok = false;
+        break;
+      }
+      if ((spins & 63) == 0 && __hip_atomic_load(a.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+        ok = false;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+      w = xload64(blk);
+    }
+    if (!ok) *s_ok = 0;
+    scnt[src] = ok ? min((int)(uint32_t)w, a.xcap) : 0;
+  }
+  __syncthreads();
+  if (!*s_ok) return false;
+  if (tid < kWave) {
+    int base = 0;
+    for (int k = 0; k < a.P; k += kWave) {
+      const int src = k + lane;
+      const int c = src < a.P ? scnt[src] : 0;
+      const int incl = wave_inclusive_sum(c);
+      if (src < a.P) sscan[src] = base + incl - c;
+      base += __builtin_amdgcn_readlane(incl, kWave - 1);
+    }
+    if (tid == 0) sscan[a.P] = base;
+  }
+  __syncthreads();
+  const int total = sscan[a.P];
+  const int quad = tid / kQuad, q = tid % kQuad;
+  constexpr int nquads = BS / kQuad;
+  for (int r0 = 0; r0 < total; r0 += kEntCap) {
+    const int nst = min(kEntCap, total - r0);
+    uint32_t wo[kPer];            // word offsets from xb
+    unsigned long long w[kPer];
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const int e = tid + u * BS;
+      wo[u] = 0u;
+      w[u] = 0ull;
+      if (e < nst) {
+        const int gidx = r0 + e;
+        int lo = 0, hi = a.P;
+        while (hi - lo > 1) {
+          const int mid = (lo + hi) >> 1;
+          if (sscan[mid] <= gidx) lo = mid;
+          else hi = mid;
+        }
+        wo[u] = (uint32_t)lo * (uint32_t)a.xstride + 1u + (uint32_t)(gidx - sscan[lo]);
+        w[u] = xload64(xb + wo[u]);
+      }
+    }
+    bool ok = true;
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      if (tid + u * BS >= nst) continue;
+      long long spins = 0;
+      while (ok && xtag(w[u]) != tag) {
+        if (++spins > a.spin_limit) {
+          atomicCAS(a.error, 0, 1);
+          ok = false;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+        w[u] = xload64(xb + wo[u]);
+      }
+      sent[tid + u * BS] = (uint32_t)w[u];
+    }
+    if (!ok) *s_ok = 0;
+    __syncthreads();
+    if (!*s_ok) return false;
+    for (int j0 = quad * kSpikesPerQuad; j0 < nst; j0 += nquads * kSpikesPerQuad)
+      deliver_staged(a, sent + j0, min(kSpikesPerQuad, nst - j0), q, p, slot0, BL, nownL, L, sring, st_del,
+                     q_lo, q_hi, slseg, slrec);
+    __syncthreads();
+  }
+  return true;
+}
+
 // Plasticity of the synapses onto partition p (one group).
 template <int BS>
 __device__ __forceinline__ void fused_plasticity(const FusedRule* rules, int n_rules, int p, int tidx,
@@ -695,8 +932,20 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const int p, cons
   const long long q_hi = __float2ll_rn((a.w_hi * a.scale0) * a.two_k);
   const int bits = W * 32;
   const int nown_waves = ((nown + kWave - 1) / kWave) * kWave;
-  if (tid == 0) scounter[0] = 0;
+  // spike counters of the exchange list, one per group parity (the other parity's is
+  // cleared while the group runs)
+  if (tid < 2) scounter[tid] = 0;
+  if (tid == 0) s_ok = 1;
   for (int j = tid; j < a.ext_k; j += BS) scdf[j] = a.ext_cdf[j];
+  if (!PLASTIC) {
+    // this launch's exchange blocks start empty (tag 0 is never a group's tag), before any
+    // of its publications
+    for (int k = tid; k < 2 * a.xstride; k += BS) {
+      const int par2 = k / a.xstride, wd = k - par2 * a.xstride;
+      xstore64(a.xlist64 + ((size_t)par2 * a.P + p) * a.xstride + wd, 0u, 0u);
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+  }
   __syncthreads();
   if (tid < 256) {
     const uint32_t u0 = (uint32_t)tid << 24;
@@ -717,6 +966,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const int p, cons
     int tidx;
     group_time(a, s, t, tidx);
     const int par = s & 1;
+    int* scnt = scounter + par;
     const uint8_t* ext = sext + (size_t)par * a.nown_max * G;
     const unsigned long long* imask = simask + (size_t)par * a.iown_max;
     const int b0 = tidx % bits;
@@ -725,7 +975,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const int p, cons
     for (int i = tid; i < iown; i += BS) {
       const unsigned long long m = imask[i];
       const int c = __popcll(m);
-      int pos = c ? atomicAdd(scounter, c) : 0;
+      int pos = c ? atomicAdd(scnt, c) : 0;
       unsigned long long mm = m;
       while (mm) {
         const int g = __ffsll((unsigned long long)mm) - 1;
@@ -877,7 +1127,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const int p, cons
       }
       if (smask) {
         const int c = __popcll(smask);
-        int pos = atomicAdd(scounter, c);
+        int pos = atomicAdd(scnt, c);
         st_spk += (unsigned long long)c;
         unsigned long long mm = smask;
         while (mm) {
@@ -899,12 +1149,21 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const int p, cons
     }
     clk.lap(a, 1);
     // -- C. publish: {count, entries} block, and state other workgroups may read ------
-    const int nx = min(scounter[0], a.xcap);
+    const int nxr = scnt[0];
+    const int nx = min(nxr, a.xcap);
+    if (tid == 0) scounter[par ^ 1] = 0;
     // more spikes than the exchange list holds: stop the launch with error 2
-    if (tid == 0 && scounter[0] > a.xcap) atomicExch(a.error, 2);
-    uint32_t* xl = a.xlist + ((size_t)par * a.P + p) * a.xstride;
-    for (int k = tid; k < nx; k += BS) store_coherent(xl + 1 + k, sxbuf[k]);
-    if (tid == 0) store_coherent(xl, (uint32_t)nx);
+    if (tid == 0 && nxr > a.xcap) atomicExch(a.error, 2);
+    const uint32_t tag = a.xtag_base + (uint32_t)s + 1u;
+    if (!PLASTIC) {
+      unsigned long long* xl = a.xlist64 + ((size_t)par * a.P + p) * a.xstride;
+      for (int k = tid; k < nx; k += BS) xstore64(xl + 1 + k, sxbuf[k], tag);
+      if (tid == 0) xstore64(xl, (uint32_t)nx, tag);
+    } else {
+      uint32_t* xl = a.xlist + ((size_t)par * a.P + p) * a.xstride;
+      for (int k = tid; k < nx; k += BS) store_coherent(xl + 1 + k, sxbuf[k]);
+      if (tid == 0) store_coherent(xl, (uint32_t)nx);
+    }
     if (PLASTIC) {
       if (own) a.last_n[gi] = last;
       for (int k = tid; k < W * nown; k += BS) {
@@ -936,14 +1195,34 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const int p, cons
         }
       }
     }
+    if (!PLASTIC) {
+      clk.lap(a, 2);
+      // -- D. the next group's random draws while the other workgroups finish ----------
+      if (s + 1 < a.ngroups)
+        draw_group<BS>(a, s + 1, sext + (size_t)(par ^ 1) * a.nown_max * G,
+                       simask + (size_t)(par ^ 1) * a.iown_max, nlo, nown, ilo, iown, st_ext,
+                       (uint32_t*)sinp, scdf, sguide);
+      clk.lap(a, 3);
+      // -- F. every source's list of this group, polled by its tags, delivered into the
+      //    LDS ring (no grid barrier)
+      if (a.xquad) {
+        if (!tagged_gather_deliver<BS>(a, p, par, tag, slot0, BL, nownL, L, sring, st_del, q_lo, q_hi, slseg,
+                                       slrec, sent))
+          s_ok = 0;
+        __syncthreads();
+      } else {
+        tagged_gather_stage<BS>(a, p, par, tag, slot0, BL, nownL, L, sring, st_del, q_lo, q_hi, slseg, slrec,
+                                sent, (int*)shead, sscan, &s_ok);
+      }
+      if (!s_ok) return;
+      clk.lap(a, 8);
+      continue;
+    }
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     clk.lap(a, 2);
     // -- D. barrier: arrive, draw the next group while others finish, wait ----------
-    if (tid == 0) {
-      barrier_arrive(a, s, vb);
-      scounter[0] = 0;
-    }
+    if (tid == 0) barrier_arrive(a, s, vb);
     if (s + 1 < a.ngroups)
       draw_group<BS>(a, s + 1, sext + (size_t)(par ^ 1) * a.nown_max * G,
                  simask + (size_t)(par ^ 1) * a.iown_max, nlo, nown, ilo, iown, st_ext,

@@ -265,6 +265,7 @@ struct spk_network {
   unsigned int* d_bar = nullptr;
   int* d_err = nullptr;
   unsigned int bar_gen = 0;      // barrier generations completed since the counters were cleared
+  unsigned int xtag = 0;         // tag base of the next launch's tagged exchange words
   bool bar_clear = true;         // counters (and the error flag) must be cleared first
   bool rules_uploaded = false;   // device descriptors of the rules are current
   bool generated = false;        // connectivity generated on the device (persistent kernel only)
@@ -881,6 +882,10 @@ int fill_fused_args(spk_network* n, bool plastic, spk::FusedArgs& a) {
   }
   a.rules = n->d_rules;
   a.xlist = n->d_xlist; a.bar = n->d_bar; a.error = n->d_err;
+  a.xlist64 = (unsigned long long*)n->d_xlist;
+  // short lists (partitions of at most 64 neurons): each quad polls and delivers one
+  // source's list; longer ones are staged over the whole workgroup
+  a.xquad = n->nown_max + n->iown_max <= 64 ? 1 : 0;
   a.n_mons = (int)n->mons.size();
   for (int m = 0; m < a.n_mons; ++m) {
     const MonitorHost& mh = n->mons[m];
@@ -931,6 +936,12 @@ int stage_fused(spk_network* n, spk::FusedArgs& a, float start, int64_t group0, 
   a.ngroups = ng;
   a.bar_base = n->bar_gen;
   n->bar_gen += (unsigned int)ng;
+  // tags of the launch's groups: xtag + 1 .. xtag + ng, never 0 (the cleared words' tag);
+  // each launch starts from cleared blocks, so tags only need to differ within a launch
+  // and from the previous launch's
+  if ((uint64_t)n->xtag + (uint64_t)ng + 1u >= ((uint64_t)1 << 32)) n->xtag = 0;
+  a.xtag_base = n->xtag;
+  n->xtag += (unsigned int)ng;
   return SPK_OK;
 }
 
@@ -1625,7 +1636,7 @@ int spk_prepare(spk_network* n, int G) {
   if (n->fused_ok) {
     ALLOC(n->d_nlo, (size_t)n->P + 1); ALLOC(n->d_ilo, (size_t)n->P + 1);
     ALLOC(n->d_seg, seg.size());
-    ALLOC(n->d_xlist, (size_t)2 * n->P * spk::fused_xstride(n->xcap));
+    ALLOC(n->d_xlist, (size_t)2 * 2 * n->P * spk::fused_xstride(n->xcap));   // 64-bit words
     ALLOC(n->d_bar, (size_t)(spk::kBarGroups + 2) * spk::kBarStride); ALLOC(n->d_err, 1);
   }
 #undef ALLOC
@@ -1687,7 +1698,7 @@ int spk_prepare(spk_network* n, int G) {
     if ((rc = upload(n->d_nlo, n->h_nlo.data(), (size_t)n->P + 1, st))) return rc;
     if ((rc = upload(n->d_ilo, n->h_ilo.data(), (size_t)n->P + 1, st))) return rc;
     if ((rc = upload(n->d_seg, seg.data(), seg.size(), st))) return rc;
-    SPK_HIP(hipMemsetAsync(n->d_xlist, 0, sizeof(uint32_t) * 2 * n->P * spk::fused_xstride(n->xcap), st));
+    SPK_HIP(hipMemsetAsync(n->d_xlist, 0, sizeof(uint64_t) * 2 * n->P * spk::fused_xstride(n->xcap), st));
   }
   SPK_HIP(hipMemsetAsync(n->d_counts, 0, sizeof(int) * n->slots * G, st));
   SPK_HIP(hipMemsetAsync(n->d_stats, 0, sizeof(unsigned long long) * 4, st));
@@ -2382,7 +2393,7 @@ int spk_create_brunel(const spk_brunel_params* p, int partitions, spk_network** 
   ALLOC(n->d_stats, 4);
   ALLOC(n->d_nlo, (size_t)P + 1); ALLOC(n->d_ilo, (size_t)P + 1);
   ALLOC(n->d_seg, (size_t)N * (P + 1));
-  ALLOC(n->d_xlist, (size_t)2 * P * spk::fused_xstride(n->xcap));
+  ALLOC(n->d_xlist, (size_t)2 * 2 * P * spk::fused_xstride(n->xcap));   // 64-bit words
   ALLOC(n->d_bar, (size_t)(spk::kBarGroups + 2) * spk::kBarStride); ALLOC(n->d_err, 1);
   int* d_owner = nullptr;
   unsigned int *d_count = nullptr, *d_off = nullptr, *d_cursor = nullptr;
@@ -2404,7 +2415,7 @@ int spk_create_brunel(const spk_brunel_params* p, int partitions, spk_network** 
     cleanup();
     return bail(rc);
   }
-  SPK_HIP(hipMemsetAsync(n->d_xlist, 0, sizeof(uint32_t) * 2 * P * spk::fused_xstride(n->xcap), st));
+  SPK_HIP(hipMemsetAsync(n->d_xlist, 0, sizeof(uint64_t) * 2 * P * spk::fused_xstride(n->xcap), st));
   SPK_HIP(hipMemsetAsync(n->d_stats, 0, sizeof(unsigned long long) * 4, st));
   SPK_HIP(hipMemsetAsync(d_count, 0, sizeof(unsigned int) * NN * P, st));
   SPK_HIP(hipMemsetAsync(d_cursor, 0, sizeof(unsigned int) * NN * P, st));
