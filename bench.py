@@ -55,74 +55,142 @@ def measured_traffic(form, path):
     return None
 
 
-def cpu_baseline(desc, groups, threads):
+def host_cpus():
+    """Host CPU facts for the baseline: logical CPUs the OS reports
+    (std::thread::hardware_concurrency's value), the CPUs this process may run on, and the
+    physical cores lscpu counts."""
+    logical = os.cpu_count() or 1
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        allowed = logical
+    physical = None
+    try:
+        import subprocess
+        out = subprocess.run(["lscpu", "-p=Core,Socket"], capture_output=True, text=True, timeout=10).stdout
+        physical = len({ln for ln in out.splitlines() if ln and not ln.startswith("#")})
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return logical, allowed, physical
+
+
+def cpu_baseline(desc, warm_groups, sample_groups, one_groups, threads):
+    """FP32 oracle on the host: the network is first run through the same warm-up the GPU
+    gets (multithreaded), then a steady-state sample of `sample_groups` groups is timed with
+    `threads` threads and one of `one_groups` groups with one thread.  Returns (events/s,
+    seconds, events) for both."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle_py
     o = oracle_py.OracleNetwork(desc, precision=32, threads=threads)
-    t0 = time.perf_counter()
-    o.run(0.0, groups)
-    dt = time.perf_counter() - t0
-    c = o.counters()
-    ev = c["synaptic_events"] + c["external_events"]
+    t = 0.0
+    if warm_groups:
+        o.run(t, warm_groups)
+        t = float(desc.group_time(t, warm_groups))
+
+    def timed(groups):
+        nonlocal t
+        c0 = o.counters()
+        t0 = time.perf_counter()
+        o.run(t, groups)
+        dt = time.perf_counter() - t0
+        t = float(desc.group_time(t, groups))
+        c1 = o.counters()
+        ev = (c1["synaptic_events"] + c1["external_events"]) - (c0["synaptic_events"] + c0["external_events"])
+        return ev / dt, dt, ev
+
+    multi = timed(sample_groups)
+    o.set_threads(1)
+    one = timed(one_groups)
     o.close()
-    return ev / dt, dt, ev
+    return multi, one
 
 
-def sweep_leg(n_points, seconds, cpu_sample, threads, log):
+SWEEP_FIXTURE = os.path.join(ROOT, "tests", "golden", "sweep_fp64_rates.json")
+
+
+def sweep_leg(n_points, seconds, cpu_points, threads, log, tune_seed=7):
     """Batched sweep: `n_points` Brunel networks (N log-uniform 1250-12500, g U(3,8),
     nu_ext/nu_thr U(0.9,4), J U(0.05,0.2) mV, delay U{10..20} steps, eps 0.1, seeds
-    2024 + index) simulated for `seconds` each, (1) in one persistent batched launch with the
-    autotuned tiling and (2) one network per launch.  Points/s = points / device seconds
-    of the simulation launches (HIP events; network generation and the autotuner's probe
-    runs are reported separately).  The CPU leg runs the FP64 oracle on every
-    (n_points // cpu_sample)-th point: its points/s is the CPU baseline of this metric and
-    its rates gate the batched FP32 rates (<= 2 %)."""
+    2024 + index) simulated for `seconds` each, (1) in one persistent batched launch and
+    (2) one network per launch.  The batched tiling policy is tuned on a disjoint sample
+    of the same distribution (seed `tune_seed`); the benchmark points then get their plan
+    from a short probe run, whose device time counts in the batched time.  Points/s =
+    points / device seconds (HIP events; network generation is reported separately).
+
+    Rate gate (<= 2 % vs FP64): the chaotic 1 s rate of one run differs from another
+    realisation of the external drive by up to ~2 % at low rates, so the gate compares
+    means over R realisations of the drive (same connectivity): for every point of the FP64
+    fixture (every 8th point, R = 16, tests/golden/sweep_fp64_rates.json written by the FP64
+    oracle), err = |mean_R(batched FP32) - mean_R(FP64)| / mean_R(FP64), gate max err <= 2 %.
+    The CPU baseline of this metric times the FP64 oracle on `cpu_points` of those points."""
+    import json as _json
     import numpy as np
     from spike_amd import sweep
     pts = sweep.sample_points(n_points)
     t0 = time.perf_counter()
-    best = sweep.autotune(pts, log=log)
+    best = sweep.autotune(sweep.sample_points(n_points, seed=tune_seed), log=log)
     tune_s = time.perf_counter() - t0
-    rb = sweep.run_batched(pts, seconds, best["plan"])
-    log(f"sweep batched: {rb.ms:.2f} ms ({rb.config})")
+    plan, probe_ms = sweep.plan_for(pts, best["policy"])
+    rb = sweep.run_batched(pts, seconds, plan)
+    batched_ms = rb.ms + probe_ms
+    log(f"sweep batched: {rb.ms:.2f} ms + probe {probe_ms:.2f} ms ({rb.config})")
     rp = sweep.run_plain(pts, seconds)
     log(f"sweep plain: {rp.ms:.2f} ms")
+    speedup = rp.ms / batched_ms
     out = {
         "points": n_points, "seconds_per_point": seconds,
-        "batched_points_per_s": round(n_points / (rb.ms / 1e3), 2),
+        "batched_points_per_s": round(n_points / (batched_ms / 1e3), 2),
         "plain_points_per_s": round(n_points / (rp.ms / 1e3), 2),
-        "speedup_vs_plain": round(rp.ms / rb.ms, 3),
-        "batched_ms": round(rb.ms, 3), "plain_ms": round(rp.ms, 3),
-        "plan": best["plan"]["name"], "workgroups": rb.config["workgroups"],
-        "threads_per_workgroup": rb.config["threads"],
+        "speedup_vs_plain": round(speedup, 3),
+        "gate_speedup": ">= 10", "gate_speedup_ok": bool(speedup >= 10.0),
+        "batched_ms": round(batched_ms, 3), "batched_run_ms": round(rb.ms, 3), "probe_ms": round(probe_ms, 3),
+        "plain_ms": round(rp.ms, 3),
+        "plan": plan["name"], "tuned_on": f"sample_points({n_points}, seed={tune_seed})",
+        "workgroups": rb.config["workgroups"], "threads_per_workgroup": rb.config["threads"],
         "autotune_s": round(tune_s, 2), "cost_fit_us": best["fit"],
         "batched_vs_plain_rate_max_rel_diff": float(np.max(np.abs(rb.rate - rp.rate) / np.maximum(rp.rate, 1e-12))),
         "mean_rate_hz": round(float(rb.rate.mean()), 3),
     }
-    if cpu_sample > 0:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        from sweep_replay import oracle_rate
-        idx = list(range(0, n_points, max(1, n_points // cpu_sample)))[:cpu_sample]
-        errs, floor, secs = [], [], 0.0
-        for k in idx:
-            r64, el = oracle_rate(pts[k], seconds, precision=64, threads=threads)
-            secs += el
-            errs.append(abs(rb.rate[k] - r64) / r64)
-            # the same network and precision under another realisation of the external drive
-            r64b, _ = oracle_rate(pts[k], seconds, precision=64, threads=threads,
-                                  drive_seed=pts[k].seed + 7919)
-            floor.append(abs(r64b - r64) / r64)
-            log(f"sweep point {k} N={pts[k].n}: batched FP32 {rb.rate[k]:.3f} Hz, FP64 oracle {r64:.3f} Hz, "
-                f"FP64 other drive {r64b:.3f} Hz")
-        out["rate_err_vs_fp64_max"] = round(float(max(errs)), 6)
-        out["rate_err_vs_fp64_mean"] = round(float(np.mean(errs)), 6)
-        out["fp64_drive_realisation_spread_max"] = round(float(max(floor)), 6)
-        out["fp64_drive_realisation_spread_mean"] = round(float(np.mean(floor)), 6)
-        out["rate_err_points"] = idx
-        out["cpu_baseline"] = {"value": round(len(idx) / secs, 4), "unit": "sweep points/s",
-                               "cores": threads, "kind": "port",
-                               "sample": f"FP64 oracle on {len(idx)} of the {n_points} points, {seconds} s each, "
-                                         f"{secs:.2f} s"}
+    if os.path.exists(SWEEP_FIXTURE):
+        fx = _json.load(open(SWEEP_FIXTURE))
+        if fx["points_in_sweep"] == n_points and fx["seconds"] == seconds:
+            idx = [e["index"] for e in fx["points"]]
+            R = int(fx["realisations"])
+            r64 = np.array([e["rate_fp64"] for e in fx["points"]], dtype=np.float64)
+            sub = [pts[k] for k in idx]
+            r32 = np.zeros((len(idx), R))
+            r32[:, 0] = rb.rate[idx]
+            for r in range(1, R):
+                r32[:, r] = sweep.run_batched(sub, seconds, None, realisation=r).rate
+            m32, m64 = r32.mean(1), r64.mean(1)
+            err = np.abs(m32 - m64) / m64
+            se = np.sqrt(r32.var(1, ddof=1) / R + r64.var(1, ddof=1) / R) / m64
+            for j, k in enumerate(idx):
+                log(f"sweep point {k} N={pts[k].n}: batched FP32 mean {m32[j]:.3f} Hz, FP64 mean {m64[j]:.3f} Hz "
+                    f"over {R} drive realisations, err {100 * err[j]:.2f} % (noise {100 * se[j]:.2f} %)")
+            out.update({
+                "rate_gate": f"max over {len(idx)} points of |mean of {R} FP32 realisations - mean of {R} FP64 "
+                             f"realisations| / FP64 mean <= 2 %",
+                "rate_err_vs_fp64_max": round(float(err.max()), 6),
+                "rate_err_vs_fp64_mean": round(float(err.mean()), 6),
+                "rate_err_noise_max": round(float(se.max()), 6),
+                "gate_rate_ok": bool(err.max() <= 0.02),
+                "rate_err_points": idx,
+            })
+            if cpu_points > 0:
+                sys.path.insert(0, os.path.join(ROOT, "oracle"))
+                from sweep_replay import oracle_rate
+                secs = 0.0
+                # the middle point of each size quantile of the fixture's points
+                step = max(1, len(idx) // cpu_points)
+                cp = sorted(idx, key=lambda k: pts[k].n)[step // 2::step][:cpu_points]
+                for k in cp:
+                    _, el = oracle_rate(pts[k], seconds, precision=64, threads=threads)
+                    secs += el
+                out["cpu_baseline"] = {"value": round(len(cp) / secs, 4), "unit": "sweep points/s",
+                                       "cores": threads, "kind": "port",
+                                       "sample": f"FP64 oracle, {threads} threads, on points {cp} of the "
+                                                 f"{n_points}, {seconds} s each, {secs:.2f} s"}
     return out
 
 
@@ -225,14 +293,16 @@ def main():
     ap.add_argument("--form", default="config", choices=["config", "reference"])
     ap.add_argument("--path", default="fused", choices=["fused", "launches"])
     ap.add_argument("--cpu-groups", type=int, default=100,
-                    help="timestep groups of the CPU-baseline sample (15 steps each)")
+                    help="timestep groups of the CPU-baseline sample (15 steps each), after the warm-up")
+    ap.add_argument("--cpu-one-groups", type=int, default=20,
+                    help="timestep groups of the one-thread CPU-baseline sample")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sweep-points", type=int, default=256,
                     help="Brunel networks in the batched-sweep measurement (0 skips it)")
     ap.add_argument("--sweep-seconds", type=float, default=1.0)
-    ap.add_argument("--sweep-cpu-sample", type=int, default=8,
-                    help="sweep points re-run on the FP64 oracle (rate-error gate, CPU baseline)")
+    ap.add_argument("--sweep-cpu-sample", type=int, default=4,
+                    help="sweep points timed on the FP64 oracle (the sweep's CPU baseline)")
     ap.add_argument("--no-mlp", action="store_true", help="skip the surrogate MLP kernels")
     ap.add_argument("--large-seconds", type=float, default=2.0,
                     help="simulated seconds of the large plastic network (0: skip it)")
@@ -288,6 +358,7 @@ def main():
         rec += tm.synaptic_events
         spikes += tm.spikes_neurons + tm.spikes_inputs
     wall = time.perf_counter() - t0
+    net.close()
     steps_sim = groups * G * args.steps
     value = events / (tot_ms / 1e3)
     rate_hz = spikes / desc.n_neurons / (steps_sim * desc.dt)
@@ -306,11 +377,17 @@ def main():
                 "avg_launch_ms": round(avg_launch_ms, 5)}
     cpu = None
     if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        v, secs, ev = cpu_baseline(desc, args.cpu_groups, threads)
+        logical, allowed, physical = host_cpus()
+        threads = args.cpu_threads or allowed
+        (v, secs, ev), (v1, secs1, ev1) = cpu_baseline(desc, groups * args.warmup, args.cpu_groups,
+                                                        args.cpu_one_groups, threads)
         cpu = {"value": round(v, 1), "unit": "syn events/s", "cores": threads, "kind": "port",
-               "sample": f"FP32 oracle, Brunel10K {args.form} form, first {args.cpu_groups} groups "
-                         f"({args.cpu_groups * G} steps) from reset, {secs:.2f} s, {ev} events"}
+               "threads": threads, "hardware_concurrency": logical, "allowed_cpus": allowed,
+               "physical_cores": physical, "one_thread_value": round(v1, 1),
+               "sample": f"FP32 oracle, Brunel10K {args.form} form, after the GPU's warm-up ({args.warmup} "
+                         f"simulated s = {groups * args.warmup} groups): {args.cpu_groups} groups "
+                         f"({args.cpu_groups * G} steps) with {threads} threads in {secs:.2f} s, {ev} events; "
+                         f"{args.cpu_one_groups} groups with 1 thread in {secs1:.2f} s, {ev1} events"}
     out = {
         "metric": "syn events/s (Brunel10K); batched sweep points/s vs plain at \u22642% rate err",
         "value": round(value, 1),
@@ -342,13 +419,12 @@ def main():
         out["speedup_vs_cpu"] = round(value / cpu["value"], 2)
     if args.sweep_points > 0:
         log = lambda m: print(m, file=sys.stderr, flush=True)
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        threads = args.cpu_threads or host_cpus()[1]
         out["sweep"] = sweep_leg(args.sweep_points, args.sweep_seconds,
                                  0 if args.no_cpu else args.sweep_cpu_sample, threads, log)
     if not args.no_mlp:
         out["mlp"] = mlp_leg(lambda m: print(m, file=sys.stderr, flush=True))
     if args.large_seconds > 0:
-        net.close()
         out["large"] = large_leg(args.large_seconds, lambda m: print(m, file=sys.stderr, flush=True))
     print(json.dumps(out))
 

@@ -221,6 +221,9 @@ class OracleNetwork:
     def reset_state(self):
         check(self.lib.or_reset_state(self.h))
 
+    def set_threads(self, threads: int):
+        check(self.lib.or_set_threads(self.h, int(threads)))
+
     def set_grouping(self, g):
         check(self.lib.or_set_timestep_grouping(self.h, int(g)))
 

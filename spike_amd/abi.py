@@ -6,8 +6,10 @@ loudly when the shared library has not been built; there is no fallback path.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
+import weakref
 from typing import Optional
 
 import numpy as np
@@ -19,6 +21,30 @@ LIB_PATH = os.path.join(_HERE, "_lib", "libspike_hip.so")
 
 SPK_RUN_PLASTICITY = 1
 SPK_RUN_FORCE_LAUNCHES = 2
+
+
+# Handles still open when the interpreter exits are closed by an atexit hook, while the HIP
+# runtime is certainly still up (finalizers that run during interpreter teardown might
+# otherwise call into it after its own shutdown began).  Batches close before networks.
+_live = weakref.WeakValueDictionary()
+_live_seq = [0]
+
+
+def _track(obj, rank: int):
+    _live_seq[0] += 1
+    _live[(rank, _live_seq[0])] = obj
+
+
+@atexit.register
+def close_all():
+    """Close every network, batch and surrogate handle that is still open."""
+    for key in sorted(list(_live.keys())):
+        obj = _live.get(key)
+        if obj is not None:
+            try:
+                obj.close()
+            except Exception:
+                pass
 
 
 class SpikeError(RuntimeError):
@@ -217,6 +243,7 @@ class Network:
         h = P()
         check(lib.spk_create(d.n_neurons, d.n_inputs, d.kind, d.n_labels, float(d.dt), C.byref(h)))
         self.h = h
+        _track(self, 2)
         try:
             if partitions:
                 check(lib.spk_set_partitions(h, int(partitions)))
@@ -479,6 +506,7 @@ def generated_brunel(params: "BrunelParams", partitions: int = 0) -> Network:
     net.rule_ids = []
     net.monitor_ids = []
     net._keep = []
+    _track(net, 2)
     return net
 
 
@@ -492,6 +520,7 @@ class Batch:
         h = P()
         check(self.lib.spk_batch_create(arr, len(self.nets), int(workgroups), C.byref(h)))
         self.h = h
+        _track(self, 0)
 
     @property
     def workgroups(self) -> int:

@@ -34,8 +34,11 @@ void check(int rc, const char* call) {
 class ModelNetwork {
  public:
   explicit ModelNetwork(::SpikingModel* m) : model(m) {}
-  ~ModelNetwork() {
+  ~ModelNetwork() { release(); }
+  // frees the device network (at program exit, before the HIP runtime shuts down)
+  void release() {
     if (net) spk_destroy(net);
+    net = nullptr;
   }
   spk_network* handle() {
     if (!net) build();
@@ -75,10 +78,15 @@ std::map<::SpikingModel*, std::unique_ptr<ModelNetwork>>& registry() {
   return r;
 }
 
-// Work still queued when the program ends is waited for, so no kernel outlives it.
+// At program exit: work still queued is waited for, so no kernel outlives the program, and
+// the device networks are freed here.  The registry itself is a static constructed before
+// the HIP runtime's first call, so its destructor would run after the runtime's own
+// teardown; this handler, registered after that first call, runs before it.
 void synchronize_all() {
   for (auto& kv : registry())
     if (kv.second && kv.second->built()) spk_synchronize(kv.second->net);
+  for (auto& kv : registry())
+    if (kv.second) kv.second->release();
 }
 
 ModelNetwork* model_network(::SpikingModel* m) {
@@ -86,7 +94,6 @@ ModelNetwork* model_network(::SpikingModel* m) {
   auto& r = registry();
   auto it = r.find(m);
   if (it != r.end()) return it->second.get();
-  if (r.empty()) atexit(synchronize_all);
   auto* mn = new ModelNetwork(m);
   r[m].reset(mn);
   return mn;
@@ -120,6 +127,12 @@ void ModelNetwork::build() {
   const int L = cond ? std::max(1, cond->num_syn_labels) : (curr ? std::max(1, curr->num_syn_labels) : 1);
   const float dt = m->timestep;
   check(spk_create(N, Nin, kind, L, dt, &net), "spk_create");
+  // registered once the HIP runtime is up, so that it runs before the runtime's teardown
+  static bool exit_hook = false;
+  if (!exit_hook) {
+    atexit(synchronize_all);
+    exit_hook = true;
+  }
 
   // LIF constants as the reference backend derives them on the host
   // (Spike/Backend/CUDA/Neurons/LIFSpikingNeurons.cu:26-46): c = dt/tau, R' = c*R.

@@ -96,6 +96,12 @@ struct FusedArgs {
   // the group loop starts): 16-bit local starts of the lseg_U source units' segments (and
   // the end), at most lrec targets per partition; 0: off
   int lseg_U, lrec;
+  // compact form with a count ring: each LDS ring word holds the step's count of
+  // deliveries from sources below src_split (low 16 bits) and from the others (high 16
+  // bits); the reading neuron rebuilds the exact fixed-point sum from the counts.  The
+  // global int64 ring then only carries the inputs pending when the launch starts (read
+  // for steps below the launch's first step + BL) and receives the pending ones at its end.
+  int cring;
   // plasticity (rules grouped by postsynaptic partition), descriptors in device memory
   const FusedRule* rules;
   int n_rules;
@@ -106,6 +112,7 @@ struct FusedArgs {
   unsigned long long* xlist64;
   unsigned int xtag_base;     // tag of group s of this launch: xtag_base + s + 1
   int xquad;                  // tagged delivery by quads per source (small partitions) or staged
+  int dahead;                 // groups whose random draws are made in one pass (LDS slots)
   // two-level grid barrier: counters bar[g * kBarStride] (g < 8), bar[8 * kBarStride]
   // (level 2) and bar[9 * kBarStride] (release generation)
   unsigned int* bar;
@@ -174,13 +181,14 @@ __host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~(size_t
 // `segU`, `lrec`: source units and 2-byte targets of the LDS-resident delivery table (0: the
 // table is read from global memory).
 __host__ __device__ inline FusedLayout fused_layout(int BL, int L, int W, int G, int nown, int iown,
-                                                    int xcap, int P, int segU = 0, int lrec = 0) {
+                                                    int xcap, int P, int segU = 0, int lrec = 0, int cring = 0,
+                                                    int dahead = 1) {
   FusedLayout f;
   size_t o = 0;
-  f.ring = o;    o = align16(o + (size_t)BL * nown * L * 8);
+  f.ring = o;    o = align16(o + (size_t)BL * nown * L * (cring ? 4 : 8));
   f.hist = o;    o = align16(o + (size_t)W * nown * 4);
-  f.ext = o;     o = align16(o + (size_t)2 * nown * G);       // double buffer
-  f.imask = o;   o = align16(o + (size_t)2 * iown * 8);       // double buffer
+  f.ext = o;     o = align16(o + (size_t)dahead * nown * G);   // slots of the drawn groups
+  f.imask = o;   o = align16(o + (size_t)dahead * iown * 8);
   f.inp = o;     o = align16(o + (size_t)nown * G * L * 4);   // this group's inputs as float
   f.xbuf = o;    o = align16(o + (size_t)xcap * 4);
   f.head = o;    o = align16(o + (size_t)P * kHeadWords * 4);
@@ -203,8 +211,9 @@ __host__ __device__ inline FusedLayout fused_layout(int BL, int L, int W, int G,
 }
 
 __host__ __device__ inline size_t fused_lds_bytes(int BL, int L, int W, int G, int nown, int iown,
-                                                  int xcap, int P, int segU = 0, int lrec = 0) {
-  return fused_layout(BL, L, W, G, nown, iown, xcap, P, segU, lrec).total;
+                                                  int xcap, int P, int segU = 0, int lrec = 0, int cring = 0,
+                                                  int dahead = 1) {
+  return fused_layout(BL, L, W, G, nown, iown, xcap, P, segU, lrec, cring, dahead).total;
 }
 
 __host__ __device__ inline int fused_xstride(int xcap) { return ((1 + xcap + 7) / 8) * 8; }
@@ -283,56 +292,64 @@ __device__ __forceinline__ void group_time(const FusedArgs& a, int s, float& t, 
   tidx = (int)roundf(t / a.dt);
 }
 
-// Random draws of group s: external Poisson counts per (neuron, sub-step) and the spike
-// masks of the Poisson inputs (Bernoulli per sub-step).  Independent of network state,
-// so they are computed while the workgroup waits at the previous group's barrier.
-// `su` is LDS scratch of nown*G words (the group's float inputs, unused at this point),
-// `scdf` the count thresholds in LDS and `sguide` their guide table: sguide[j] is the count
-// of u = j * 2^24, a lower bound of the count of every u in [j * 2^24, (j + 1) * 2^24)
-// (Chen & Asau 1974), so the upward scan from it takes about one step.
+// Random draws of groups s0 .. s0 + cnt - 1: external Poisson counts per (neuron, sub-step)
+// and the spike masks of the Poisson inputs (Bernoulli per sub-step), into LDS slot
+// s % dahead of each group.  Independent of network state, so they are made ahead, once
+// every dahead groups, while the workgroup waits for the other workgroups' spikes.  One work
+// item per (group, unit, 4-step Philox block) uses all four outputs of its call; a count is
+// the number of thresholds C[k] <= u, scanned upwards from the guide entry of u (`sguide[j]`
+// is the count of u = j * 2^24, a lower bound for every u in [j * 2^24, (j + 1) * 2^24),
+// Chen & Asau 1974), so the scan takes about one step.  The input masks are ORed into slots
+// that phase A cleared when it read them.
 template <int BS>
-__device__ __forceinline__ void draw_group(const FusedArgs& a, int s, uint8_t* ext,
-                                           unsigned long long* imask, int nlo, int nown, int ilo,
-                                           int iown, unsigned long long& st_ext, uint32_t* su,
-                                           const uint32_t* scdf, const uint8_t* sguide) {
-  float t;
-  int tidx;
-  group_time(a, s, t, tidx);
+__device__ __forceinline__ void draw_groups(const FusedArgs& a, int s0, int cnt, uint8_t* sext,
+                                            unsigned long long* simask, int nlo, int nown, int ilo,
+                                            int iown, unsigned long long& st_ext, const uint32_t* scdf,
+                                            const uint8_t* sguide) {
   const int G = a.G;
   const int tid = threadIdx.x;
-  // the group's steps [tidx, tidx + G) lie in the 4-step Philox blocks b0 .. b0 + nbk - 1;
-  // one work item per (unit, block) uses all four outputs of its call
-  const int64_t b0 = (int64_t)tidx >> 2;
-  const int nbk = (int)((((int64_t)tidx + G - 1) >> 2) - b0) + 1;
+  const int nbm = (G + 3) / 4 + 1;   // 4-step blocks a group's G steps can touch
+  const int per = nown * nbm;
   if (a.ext_k > 0) {
-    // (1) one Philox call per (neuron, block) into su[i*G + g]; (2) every (neuron, step)
-    // counts its events in parallel: the number of thresholds C[k] <= u, scanned upwards
-    // from the guide entry of u until the first C[k] > u (C is non-decreasing)
-    for (int k = tid; k < nown * nbk; k += BS) {
-      const int i = k / nbk;
-      const int64_t blk = b0 + (k - i * nbk);
+    for (int k = tid; k < cnt * per; k += BS) {
+      const int j = k / per;
+      const int r0 = k - j * per;
+      const int i = r0 / nbm, bb = r0 - i * nbm;
+      float t;
+      int tidx;
+      group_time(a, s0 + j, t, tidx);
+      const int64_t b0 = (int64_t)tidx >> 2;
+      const int nbk = (int)((((int64_t)tidx + G - 1) >> 2) - b0) + 1;
+      if (bb >= nbk) continue;
+      const int64_t blk = b0 + bb;
       const u32x4 r = draw_block(a.seed, blk, (uint32_t)(nlo + i), kStreamExternal);
+      uint8_t* ext = sext + ((size_t)((s0 + j) % a.dahead) * a.nown_max + i) * G;
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
         const int g = (int)(blk * 4 + c - (int64_t)tidx);
-        if (g >= 0 && g < G) su[i * G + g] = pick(r, c);
+        if (g >= 0 && g < G) {
+          const uint32_t u = pick(r, c);
+          int n = sguide[u >> 24];
+          while (n < a.ext_k && u >= scdf[n]) ++n;
+          ext[g] = (uint8_t)n;
+          st_ext += (unsigned long long)n;
+        }
       }
-    }
-    __syncthreads();
-    for (int k = tid; k < nown * G; k += BS) {
-      const uint32_t u = su[k];
-      int cnt = sguide[u >> 24];
-      while (cnt < a.ext_k && u >= scdf[cnt]) ++cnt;
-      ext[k] = (uint8_t)cnt;
-      st_ext += (unsigned long long)cnt;
     }
   }
   if (iown > 0) {
-    for (int k = tid; k < iown; k += BS) imask[k] = 0ull;
-    __syncthreads();
-    for (int k = tid; k < iown * nbk; k += BS) {
-      const int i = k / nbk;
-      const int64_t blk = b0 + (k - i * nbk);
+    const int peri = iown * nbm;
+    for (int k = tid; k < cnt * peri; k += BS) {
+      const int j = k / peri;
+      const int r0 = k - j * peri;
+      const int i = r0 / nbm, bb = r0 - i * nbm;
+      float t;
+      int tidx;
+      group_time(a, s0 + j, t, tidx);
+      const int64_t b0 = (int64_t)tidx >> 2;
+      const int nbk = (int)((((int64_t)tidx + G - 1) >> 2) - b0) + 1;
+      if (bb >= nbk) continue;
+      const int64_t blk = b0 + bb;
       const u32x4 r = draw_block(a.seed, blk, (uint32_t)(ilo + i), kStreamInput);
       const uint32_t thr = a.in_thr[ilo + i];
       unsigned long long bits = 0ull;
@@ -342,7 +359,7 @@ __device__ __forceinline__ void draw_group(const FusedArgs& a, int s, uint8_t* e
         const int g = (int)(step - (int64_t)tidx);
         if (g >= 0 && g < G && pick(r, c) < thr && step != a.arm_step) bits |= 1ull << g;
       }
-      if (bits) atomicOr(&imask[i], bits);
+      if (bits) atomicOr(&simask[(size_t)((s0 + j) % a.dahead) * a.iown_max + i], bits);
     }
   }
 }
@@ -440,11 +457,12 @@ __device__ __forceinline__ void deliver_quad(const FusedArgs& a, const uint32_t*
 // Compact form of deliver_quad: 2-byte records, the fixed-point weight and the slot are
 // those of the spike's source (q_lo below src_split, q_hi from it on), computed once per
 // spike with the same expression as the full form.
+template <bool CR>
 __device__ __forceinline__ void deliver_quad_compact(const FusedArgs& a, const uint32_t* ents, int n, int q,
                                                      int p, int slot0, int B, int nown, int64_t* sring,
                                                      unsigned long long& st_del, long long q_lo, long long q_hi) {
   uint32_t s0[kSpikesPerQuad], s1[kSpikesPerQuad];
-  long long qv[kSpikesPerQuad];
+  long long qv[kSpikesPerQuad];   // count ring: the increment (1 or 1 << 16)
   int slot[kSpikesPerQuad];
 #pragma unroll
   for (int r = 0; r < kSpikesPerQuad; ++r) {
@@ -458,7 +476,7 @@ __device__ __forceinline__ void deliver_quad_compact(const FusedArgs& a, const u
       const size_t sb = (size_t)id * (a.P + 1) + p;
       s0[r] = a.seg[sb];
       s1[r] = a.seg[sb + 1];
-      qv[r] = (int)id < a.src_split ? q_lo : q_hi;
+      qv[r] = CR ? ((int)id < a.src_split ? 1 : 0x10000) : ((int)id < a.src_split ? q_lo : q_hi);
       int sl = slot0 + (int)(e >> 24) + a.src_delay;
       while (sl >= B) sl -= B;
       slot[r] = sl;
@@ -479,10 +497,14 @@ __device__ __forceinline__ void deliver_quad_compact(const FusedArgs& a, const u
 #pragma unroll
   for (int r = 0; r < kSpikesPerQuad; ++r) {
     int64_t* row = sring + (size_t)slot[r] * nown;
+    uint32_t* crow = (uint32_t*)sring + (size_t)slot[r] * nown;
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       const uint32_t k = s0[r] + (uint32_t)(q + h * kQuad);
-      if (k < s1[r]) atomicAdd((unsigned long long*)&row[t0[r][h]], (unsigned long long)qv[r]);
+      if (k < s1[r]) {
+        if (CR) atomicAdd(&crow[t0[r][h]], (uint32_t)qv[r]);
+        else atomicAdd((unsigned long long*)&row[t0[r][h]], (unsigned long long)qv[r]);
+      }
     }
     for (uint32_t k0 = s0[r] + (uint32_t)(q + 2 * kQuad); k0 < s1[r]; k0 += 4 * kQuad) {
       uint16_t tt[4];
@@ -494,7 +516,10 @@ __device__ __forceinline__ void deliver_quad_compact(const FusedArgs& a, const u
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         const uint32_t k = k0 + (uint32_t)(u * kQuad);
-        if (k < s1[r]) atomicAdd((unsigned long long*)&row[tt[u]], (unsigned long long)qv[r]);
+        if (k < s1[r]) {
+          if (CR) atomicAdd(&crow[tt[u]], (uint32_t)qv[r]);
+          else atomicAdd((unsigned long long*)&row[tt[u]], (unsigned long long)qv[r]);
+        }
       }
     }
   }
@@ -516,6 +541,7 @@ __device__ __forceinline__ int wave_inclusive_sum(int v) {
 
 // Compact delivery from the LDS-resident table: entry -> local start and end (the next
 // unit's start) -> targets, all in LDS; the quad's lanes take every kQuad-th target.
+template <bool CR>
 __device__ __forceinline__ void deliver_quad_lds(const FusedArgs& a, const uint32_t* ents, int n, int q,
                                                  int slot0, int B, int nown, int64_t* sring,
                                                  unsigned long long& st_del, long long q_lo, long long q_hi,
@@ -524,13 +550,59 @@ __device__ __forceinline__ void deliver_quad_lds(const FusedArgs& a, const uint3
     const uint32_t e = ents[r];
     const uint32_t id = e & kIdMask;
     const uint32_t s0 = slseg[id], s1 = slseg[id + 1];
-    const unsigned long long qv = (unsigned long long)((int)id < a.src_split ? q_lo : q_hi);
     int sl = slot0 + (int)(e >> 24) + a.src_delay;
     while (sl >= B) sl -= B;
     if (q == 0) st_del += (unsigned long long)(s1 - s0);
-    unsigned long long* row = (unsigned long long*)(sring + (size_t)sl * nown);
-    for (uint32_t k = s0 + (uint32_t)q; k < s1; k += kQuad) atomicAdd(&row[slrec[k]], qv);
+    if (CR) {
+      const uint32_t inc = (int)id < a.src_split ? 1u : 0x10000u;
+      uint32_t* row = (uint32_t*)sring + (size_t)sl * nown;
+      for (uint32_t k = s0 + (uint32_t)q; k < s1; k += kQuad) atomicAdd(&row[slrec[k]], inc);
+    } else {
+      const unsigned long long qv = (unsigned long long)((int)id < a.src_split ? q_lo : q_hi);
+      unsigned long long* row = (unsigned long long*)(sring + (size_t)sl * nown);
+      for (uint32_t k = s0 + (uint32_t)q; k < s1; k += kQuad) atomicAdd(&row[slrec[k]], qv);
+    }
   }
+}
+
+// One lane's delivery of one entry from the LDS-resident table: the entry's segment
+// (local start and end) and its targets, read four at a time with the loads in flight.
+template <bool CR>
+__device__ __forceinline__ void deliver_lane_lds(const FusedArgs& a, uint32_t e, int slot0, int B, int nown,
+                                                 int64_t* sring, unsigned long long& st_del, long long q_lo,
+                                                 long long q_hi, const uint16_t* slseg, const uint16_t* slrec) {
+  const uint32_t id = e & kIdMask;
+  const uint32_t s0 = slseg[id], s1 = slseg[id + 1];
+  int sl = slot0 + (int)(e >> 24) + a.src_delay;
+  while (sl >= B) sl -= B;
+  st_del += (unsigned long long)(s1 - s0);
+  const bool lo = (int)id < a.src_split;
+  uint32_t* crow = (uint32_t*)sring + (size_t)sl * nown;
+  unsigned long long* row = (unsigned long long*)(sring + (size_t)sl * nown);
+  const uint32_t inc = lo ? 1u : 0x10000u;
+  const unsigned long long qv = (unsigned long long)(lo ? q_lo : q_hi);
+  uint32_t k = s0;
+  for (; k + 4 <= s1; k += 4) {
+    const uint16_t t0 = slrec[k], t1 = slrec[k + 1], t2 = slrec[k + 2], t3 = slrec[k + 3];
+    if (CR) {
+      atomicAdd(&crow[t0], inc); atomicAdd(&crow[t1], inc); atomicAdd(&crow[t2], inc); atomicAdd(&crow[t3], inc);
+    } else {
+      atomicAdd(&row[t0], qv); atomicAdd(&row[t1], qv); atomicAdd(&row[t2], qv); atomicAdd(&row[t3], qv);
+    }
+  }
+  for (; k < s1; ++k) {
+    if (CR) atomicAdd(&crow[slrec[k]], inc);
+    else atomicAdd(&row[slrec[k]], qv);
+  }
+}
+
+template <bool CR>
+__device__ __forceinline__ void deliver_lane_lds2(const FusedArgs& a, bool has0, uint32_t e0, bool has1, uint32_t e1,
+                                                  int slot0, int B, int nown, int64_t* sring,
+                                                  unsigned long long& st_del, long long q_lo, long long q_hi,
+                                                  const uint16_t* slseg, const uint16_t* slrec) {
+  if (has0) deliver_lane_lds<CR>(a, e0, slot0, B, nown, sring, st_del, q_lo, q_hi, slseg, slrec);
+  if (has1) deliver_lane_lds<CR>(a, e1, slot0, B, nown, sring, st_del, q_lo, q_hi, slseg, slrec);
 }
 
 constexpr int kQuadEnt = 8;   // entries of one source a quad stages in LDS at a time
@@ -543,10 +615,14 @@ __device__ __forceinline__ void deliver_staged(const FusedArgs& a, const uint32_
                                                const uint16_t* slseg, const uint16_t* slrec) {
   for (int j0 = 0; j0 < n; j0 += kSpikesPerQuad) {
     const int m = min(kSpikesPerQuad, n - j0);
-    if (a.lseg_U > 0)
-      deliver_quad_lds(a, ents + j0, m, q, slot0, BL, nownL, sring, st_del, q_lo, q_hi, slseg, slrec);
+    if (a.lseg_U > 0 && a.cring)
+      deliver_quad_lds<true>(a, ents + j0, m, q, slot0, BL, nownL, sring, st_del, q_lo, q_hi, slseg, slrec);
+    else if (a.lseg_U > 0)
+      deliver_quad_lds<false>(a, ents + j0, m, q, slot0, BL, nownL, sring, st_del, q_lo, q_hi, slseg, slrec);
+    else if (a.syn16 != nullptr && a.cring)
+      deliver_quad_compact<true>(a, ents + j0, m, q, p, slot0, BL, nownL, sring, st_del, q_lo, q_hi);
     else if (a.syn16 != nullptr)
-      deliver_quad_compact(a, ents + j0, m, q, p, slot0, BL, nownL, sring, st_del, q_lo, q_hi);
+      deliver_quad_compact<false>(a, ents + j0, m, q, p, slot0, BL, nownL, sring, st_del, q_lo, q_hi);
     else if (a.scale == nullptr)
       deliver_quad<false>(a, ents + j0, m, q, p, slot0, BL, nownL, L, sring, st_del);
     else
@@ -611,6 +687,42 @@ __device__ __forceinline__ bool tagged_gather_deliver(const FusedArgs& a, int p,
     }
     if (!ok) break;
     if (!active) continue;
+    if (a.lseg_U > 0) {
+      // LDS table: every lane delivers the entries it holds -- word q is entry q - 1, word
+      // q + 4 entry q + 3 -- and longer lists eight at a time, two entries per lane
+      const bool h0 = q >= 1 && q - 1 < c, h1 = q + 3 < c;
+      if (a.cring)
+        deliver_lane_lds2<true>(a, h0, (uint32_t)w0, h1, (uint32_t)w1, slot0, BL, nownL, sring, st_del, q_lo, q_hi,
+                                slseg, slrec);
+      else
+        deliver_lane_lds2<false>(a, h0, (uint32_t)w0, h1, (uint32_t)w1, slot0, BL, nownL, sring, st_del, q_lo,
+                                 q_hi, slseg, slrec);
+      for (int j0 = kQuadEnt - 1; j0 < c; j0 += kQuadEnt) {
+        const int m = min(kQuadEnt, c - j0);
+        const unsigned long long* eb = blk + 1 + j0;
+        unsigned long long e0 = q < m ? xload64(eb + q) : 0ull;
+        unsigned long long e1 = q + kQuad < m ? xload64(eb + q + kQuad) : 0ull;
+        long long sp = 0;
+        while ((q < m && xtag(e0) != tag) || (q + kQuad < m && xtag(e1) != tag)) {
+          if (++sp > a.spin_limit) {
+            atomicCAS(a.error, 0, 1);
+            ok = false;
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+          if (q < m && xtag(e0) != tag) e0 = xload64(eb + q);
+          if (q + kQuad < m && xtag(e1) != tag) e1 = xload64(eb + q + kQuad);
+        }
+        if (!ok) break;
+        if (a.cring)
+          deliver_lane_lds2<true>(a, q < m, (uint32_t)e0, q + kQuad < m, (uint32_t)e1, slot0, BL, nownL, sring,
+                                  st_del, q_lo, q_hi, slseg, slrec);
+        else
+          deliver_lane_lds2<false>(a, q < m, (uint32_t)e0, q + kQuad < m, (uint32_t)e1, slot0, BL, nownL, sring,
+                                   st_del, q_lo, q_hi, slseg, slrec);
+      }
+      continue;
+    }
     // word q holds entry q - 1, word q + 4 entry q + 3
     if (q >= 1 && q - 1 < c) qent[q - 1] = (uint32_t)w0;
     if (q + 3 < c) qent[q + 3] = (uint32_t)w1;
@@ -815,8 +927,10 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const int p, cons
   const int ilo = a.ilo[p], ihi = a.ilo[p + 1];
   const int nown = nhi - nlo, iown = ihi - ilo;
   const int L = L1 ? 1 : a.L, G = a.G, B = a.B, BL = a.BL, W = a.W;
-  const FusedLayout f = fused_layout(BL, L, W, G, a.nown_max, a.iown_max, a.xcap, a.P, a.lseg_U, a.lrec);
+  const FusedLayout f = fused_layout(BL, L, W, G, a.nown_max, a.iown_max, a.xcap, a.P, a.lseg_U, a.lrec, a.cring,
+                                     a.dahead);
   int64_t* sring = (int64_t*)(smem + f.ring);
+  uint32_t* scring = (uint32_t*)(smem + f.ring);   // count ring (a.cring)
   uint32_t* shist = (uint32_t*)(smem + f.hist);
   uint8_t* sext = (uint8_t*)(smem + f.ext);
   unsigned long long* simask = (unsigned long long*)(smem + f.imask);
@@ -846,15 +960,23 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const int p, cons
   // group starts at step t every pending input is for a step in [t, t + largest delay), so
   // step s lives in LDS slot s % BL, and in slot s % B of the global ring (B = the
   // reference's D + 2G + 1 slots, SpikingSynapses.cu:48), which holds zeros elsewhere.
+  // The count ring starts empty: the pending inputs stay in the global ring, read where
+  // their steps are (steps below carry_end).
+  int carry_end;
   {
     float tl;
     int t0;
     group_time(a, 0, tl, t0);
-    const int r0 = t0 % BL;
-    for (int j = 0; j < BL; ++j) {
-      const int s = t0 + (j - r0 + BL) % BL;
-      const int64_t* src = a.ring + (size_t)(s % B) * NL + (size_t)nlo * L;
-      for (int k = tid; k < nownL; k += BS) sring[(size_t)j * nownL + k] = src[k];
+    carry_end = t0 + BL;
+    if (a.cring) {
+      for (int k = tid; k < BL * nown; k += BS) scring[k] = 0u;
+    } else {
+      const int r0 = t0 % BL;
+      for (int j = 0; j < BL; ++j) {
+        const int s = t0 + (j - r0 + BL) % BL;
+        const int64_t* src = a.ring + (size_t)(s % B) * NL + (size_t)nlo * L;
+        for (int k = tid; k < nownL; k += BS) sring[(size_t)j * nownL + k] = src[k];
+      }
     }
   }
   for (int k = tid; k < W * nown; k += BS) {
@@ -954,7 +1076,9 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const int p, cons
     sguide[tid] = (uint8_t)c;
   }
   __syncthreads();
-  draw_group<BS>(a, 0, sext, simask, nlo, nown, ilo, iown, st_ext, (uint32_t*)sinp, scdf, sguide);
+  for (int k = tid; k < a.dahead * a.iown_max; k += BS) simask[k] = 0ull;
+  __syncthreads();
+  draw_groups<BS>(a, 0, min(a.dahead, a.ngroups), sext, simask, nlo, nown, ilo, iown, st_ext, scdf, sguide);
   __syncthreads();
   PhaseClock clk;
   clk.start(a, s_prof, vb);
@@ -967,13 +1091,15 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const int p, cons
     group_time(a, s, t, tidx);
     const int par = s & 1;
     int* scnt = scounter + par;
-    const uint8_t* ext = sext + (size_t)par * a.nown_max * G;
-    const unsigned long long* imask = simask + (size_t)par * a.iown_max;
+    const int dslot = s % a.dahead;
+    const uint8_t* ext = sext + (size_t)dslot * a.nown_max * G;
+    unsigned long long* imask = simask + (size_t)dslot * a.iown_max;
     const int b0 = tidx % bits;
     const int slot0 = tidx % BL;
     // -- A. Poisson inputs: append this group's spikes, update histories ------------
     for (int i = tid; i < iown; i += BS) {
       const unsigned long long m = imask[i];
+      imask[i] = 0ull;   // cleared for the draws of a later group
       const int c = __popcll(m);
       int pos = c ? atomicAdd(scnt, c) : 0;
       unsigned long long mm = m;
@@ -1005,7 +1131,17 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const int p, cons
       if (slot >= BL) slot -= BL;
       int64_t* rs = sring + (size_t)slot * nownL + (size_t)i * L;
       const int64_t ext_q = (a.ext_k > 0) ? (int64_t)ext[i * G + g] * a.q_ext : 0;
-      if (KIND == 0) {
+      if (a.cring) {
+        // one label: the counts of the two source types times their fixed-point weights,
+        // the external events and, early in a launch, the input pending at its start
+        uint32_t* cs = scring + (size_t)slot * nown + i;
+        const uint32_t cw = *cs;
+        *cs = 0u;
+        int64_t qs = (int64_t)(cw & 0xFFFFu) * q_lo + (int64_t)(cw >> 16) * q_hi + ext_q;
+        const int step = tidx + g;
+        if (step < carry_end) qs += a.ring[(size_t)(step % B) * NL + (size_t)(nlo + i)];
+        sinp[k] = fixed_to_float(qs) * a.inv_scale;
+      } else if (KIND == 0) {
         float total = 0.0f;
         for (int l = 0; l < L; ++l) {
           int64_t q = rs[l];
@@ -1198,10 +1334,9 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const int p, cons
     if (!PLASTIC) {
       clk.lap(a, 2);
       // -- D. the next group's random draws while the other workgroups finish ----------
-      if (s + 1 < a.ngroups)
-        draw_group<BS>(a, s + 1, sext + (size_t)(par ^ 1) * a.nown_max * G,
-                       simask + (size_t)(par ^ 1) * a.iown_max, nlo, nown, ilo, iown, st_ext,
-                       (uint32_t*)sinp, scdf, sguide);
+      if ((s + 1) % a.dahead == 0 && s + 1 < a.ngroups)
+        draw_groups<BS>(a, s + 1, min(a.dahead, a.ngroups - s - 1), sext, simask, nlo, nown, ilo, iown, st_ext,
+                        scdf, sguide);
       clk.lap(a, 3);
       // -- F. every source's list of this group, polled by its tags, delivered into the
       //    LDS ring (no grid barrier)
@@ -1223,10 +1358,9 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const int p, cons
     clk.lap(a, 2);
     // -- D. barrier: arrive, draw the next group while others finish, wait ----------
     if (tid == 0) barrier_arrive(a, s, vb);
-    if (s + 1 < a.ngroups)
-      draw_group<BS>(a, s + 1, sext + (size_t)(par ^ 1) * a.nown_max * G,
-                 simask + (size_t)(par ^ 1) * a.iown_max, nlo, nown, ilo, iown, st_ext,
-                 (uint32_t*)sinp, scdf, sguide);
+    if ((s + 1) % a.dahead == 0 && s + 1 < a.ngroups)
+      draw_groups<BS>(a, s + 1, min(a.dahead, a.ngroups - s - 1), sext, simask, nlo, nown, ilo, iown, st_ext,
+                      scdf, sguide);
     clk.lap(a, 3);
     if (tid == 0) s_ok = barrier_wait(a, s) ? 1 : 0;
     __syncthreads();
@@ -1305,10 +1439,8 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const int p, cons
         const int nquads = BS / kQuad;
         for (int j0 = quad * kSpikesPerQuad; j0 < nr; j0 += nquads * kSpikesPerQuad) {
           const int n = min(kSpikesPerQuad, nr - j0);
-          if (a.lseg_U > 0)
-            deliver_quad_lds(a, sent + j0, n, q, slot0, BL, nownL, sring, st_del, q_lo, q_hi, slseg, slrec);
-          else if (a.syn16 != nullptr)
-            deliver_quad_compact(a, sent + j0, n, q, p, slot0, BL, nownL, sring, st_del, q_lo, q_hi);
+          if (a.lseg_U > 0 || a.syn16 != nullptr)
+            deliver_staged(a, sent + j0, n, q, p, slot0, BL, nownL, L, sring, st_del, q_lo, q_hi, slseg, slrec);
           else if (a.scale == nullptr)
             deliver_quad<false>(a, sent + j0, n, q, p, slot0, BL, nownL, L, sring, st_del);
           else
@@ -1338,8 +1470,23 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const int p, cons
       const int s = t1 + (k - r1 + B) % B;
       int64_t* dst = a.ring + (size_t)k * NL + (size_t)nlo * L;
       const bool live = s - t1 < BL;
-      const int64_t* src = sring + (size_t)(s % BL) * nownL;
-      for (int j = tid; j < nownL; j += BS) dst[j] = live ? src[j] : 0;
+      if (a.cring) {
+        // counts back to fixed point, plus the pending input of the launch's start still
+        // unread (a launch shorter than BL steps)
+        const uint32_t* src = scring + (size_t)(s % BL) * nown;
+        for (int j = tid; j < nown; j += BS) {
+          int64_t v = 0;
+          if (live) {
+            const uint32_t cw = src[j];
+            v = (int64_t)(cw & 0xFFFFu) * q_lo + (int64_t)(cw >> 16) * q_hi;
+            if (s < carry_end) v += dst[j];
+          }
+          dst[j] = v;
+        }
+      } else {
+        const int64_t* src = sring + (size_t)(s % BL) * nownL;
+        for (int j = tid; j < nownL; j += BS) dst[j] = live ? src[j] : 0;
+      }
     }
   }
   for (int k = tid; k < W * nown; k += BS) {

@@ -256,6 +256,9 @@ struct spk_network {
   std::string fused_why;
   size_t fused_lds = 0;        // dynamic LDS of the persistent kernel
   size_t fused_lds_base = 0;   // the same without the LDS-resident delivery table
+  size_t fused_lds_i64 = 0;    // the same with the int64 ring (after caller weights end the compact form)
+  bool cring_ok = false;       // compact form: per-type in-degrees below 2^16 (count ring)
+  int dahead = 1;              // groups whose random draws one pass makes (single-network launches)
   int lseg_U = 0, lrec = 0;    // that table: source units, targets per partition (0: off)
   int cu_count = 0;
   std::vector<int> h_nlo, h_ilo;
@@ -269,7 +272,7 @@ struct spk_network {
   bool bar_clear = true;         // counters (and the error flag) must be cleared first
   bool rules_uploaded = false;   // device descriptors of the rules are current
   bool generated = false;        // connectivity generated on the device (persistent kernel only)
-  bool lds_attr[2] = {false, false};  // dynamic LDS limit raised for the non-plastic/plastic kernel
+  size_t lds_attr[2] = {0, 0};  // dynamic LDS limit set for the non-plastic/plastic kernel
   // window statistics (spk_stats_begin)
   unsigned int *d_st_cnt = nullptr, *d_st_nisi = nullptr, *d_st_hist = nullptr;
   float* d_st_prev = nullptr;
@@ -817,7 +820,7 @@ int plan_lds_table(spk_network* n, const uint32_t* seg, int U) {
   const int64_t mx = *std::max_element(cnt.begin(), cnt.end());
   if (mx > 65535) return SPK_OK;
   const size_t full = spk::fused_lds_bytes(n->BL, n->L, n->W, n->G, n->nown_max, n->iown_max, n->xcap, P,
-                                           U, (int)mx);
+                                           U, (int)mx, n->cring_ok ? 1 : 0);
   if (full > fused_lds_limit(n)) return SPK_OK;
   // and the grid stays co-resident with the larger LDS (both forms of the kernel)
   for (bool plastic : {false, true}) {
@@ -829,6 +832,27 @@ int plan_lds_table(spk_network* n, const uint32_t* seg, int U) {
   n->lrec = (int)mx;
   n->fused_lds = full;
   return SPK_OK;
+}
+
+// Groups whose random draws the persistent kernel makes in one pass: up to 8 while their
+// LDS slots fit beside the rest (the draws then leave all but one group in eight).  Called
+// once the LDS layout is otherwise final; adds the slots to fused_lds.
+void choose_dahead(spk_network* n) {
+  n->dahead = 1;
+  if (!n->fused_ok) return;
+  const int cr = (n->compact && n->rules.empty() && n->cring_ok) ? 1 : 0;
+  const size_t base = spk::fused_lds_bytes(n->BL, n->L, n->W, n->G, n->nown_max, n->iown_max, n->xcap, n->P,
+                                           n->lseg_U, n->lrec, cr, 1);
+  const size_t lim = fused_lds_limit(n);
+  for (int d : {8, 4, 2}) {
+    const size_t full = spk::fused_lds_bytes(n->BL, n->L, n->W, n->G, n->nown_max, n->iown_max, n->xcap, n->P,
+                                             n->lseg_U, n->lrec, cr, d);
+    if (n->fused_lds - base + full <= lim) {
+      n->fused_lds += full - base;
+      n->dahead = d;
+      return;
+    }
+  }
 }
 
 // Kernel arguments of the persistent kernel (all but the run position).
@@ -852,6 +876,7 @@ int fill_fused_args(spk_network* n, bool plastic, spk::FusedArgs& a) {
     a.w_lo = n->cw_lo; a.w_hi = n->cw_hi; a.src_split = n->csplit; a.src_delay = n->cdelay;
     a.lseg_U = n->lseg_U;
     a.lrec = n->lrec;
+    a.cring = n->cring_ok ? 1 : 0;
   }
   a.n_rules = plastic ? (int)n->rules.size() : 0;
   for (int r = 0; r < a.n_rules; ++r) {
@@ -883,6 +908,7 @@ int fill_fused_args(spk_network* n, bool plastic, spk::FusedArgs& a) {
   a.rules = n->d_rules;
   a.xlist = n->d_xlist; a.bar = n->d_bar; a.error = n->d_err;
   a.xlist64 = (unsigned long long*)n->d_xlist;
+  a.dahead = std::max(1, n->dahead);
   // short lists (partitions of at most 64 neurons): each quad polls and delivers one
   // source's list; longer ones are staged over the whole workgroup
   a.xquad = n->nown_max + n->iown_max <= 64 ? 1 : 0;
@@ -951,9 +977,9 @@ int launch_fused(spk_network* n, spk::FusedArgs& a, float start, int64_t group0,
                  hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr) {
   if (int rc = stage_fused(n, a, start, group0, ng, n->stream)) return rc;
   const void* fn = fused_kernel(n, a.plastic != 0);
-  if (n->fused_lds > 65536 && !n->lds_attr[a.plastic != 0]) {
+  if (n->fused_lds > 65536 && n->lds_attr[a.plastic != 0] < n->fused_lds) {
     SPK_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)n->fused_lds));
-    n->lds_attr[a.plastic != 0] = true;
+    n->lds_attr[a.plastic != 0] = n->fused_lds;
   }
   void* kargs[] = {&a};
   if (ev0) SPK_HIP(hipEventRecord(ev0, n->stream));
@@ -1600,7 +1626,24 @@ int spk_prepare(spk_network* n, int G) {
       }
     }
   }
+  // count ring of the compact form: a neuron's deliveries of one step from each source type
+  // are at most its in-degree from that type, which must stay below 2^16
+  n->fused_lds_i64 = n->fused_lds;
+  n->cring_ok = false;
+  if (n->compact) {
+    std::vector<uint32_t> c_lo(n->N, 0u), c_hi(n->N, 0u);
+    for (int64_t s = 0; s < S; ++s) {
+      const int u = uid(n->h_pre[s]);
+      ++(u < n->csplit ? c_lo : c_hi)[n->h_post[s]];
+    }
+    const uint32_t m_lo = c_lo.empty() ? 0u : *std::max_element(c_lo.begin(), c_lo.end());
+    const uint32_t m_hi = c_hi.empty() ? 0u : *std::max_element(c_hi.begin(), c_hi.end());
+    n->cring_ok = m_lo < 65536u && m_hi < 65536u;
+    if (n->cring_ok)
+      n->fused_lds = spk::fused_lds_bytes(n->BL, n->L, n->W, G, n->nown_max, n->iown_max, n->xcap, n->P, 0, 0, 1);
+  }
   if (int rc = plan_lds_table(n, seg.data(), U)) return rc;
+  choose_dahead(n);
   // list capacity: every neuron of both populations may spike once per sub-step
   n->cap = std::max(1, U);
   n->slots = 1;
@@ -2232,6 +2275,10 @@ int spk_set_weights(spk_network* n, const float* in) {
   }
   SPK_HIP(hipStreamSynchronize(n->stream));
   n->compact = false;   // caller weights: deliveries read the full records from now on
+  n->cring_ok = false;  // and the LDS ring holds fixed-point sums again
+  n->lseg_U = n->lrec = 0;
+  n->dahead = 1;
+  if (n->fused_lds_i64) n->fused_lds = n->fused_lds_base = n->fused_lds_i64;
   for (auto& r : n->rules) r.dense_next = true;   // caller weights may lie below 0
   return SPK_OK;
 }
@@ -2370,7 +2417,9 @@ int spk_create_brunel(const spk_brunel_params* p, int partitions, spk_network** 
   for (int q = 0; q < P; ++q) n->nown_max = std::max(n->nown_max, n->h_nlo[q + 1] - n->h_nlo[q]);
   n->xcap = std::max(1, n->nown_max * max_spikes_per_group(n->tref, n->dt, G));
   n->BL = std::max(n->dmax, G);
-  n->fused_lds = spk::fused_lds_bytes(n->BL, 1, n->W, G, n->nown_max, 0, n->xcap, P);
+  n->cring_ok = p->ce < 65536 && p->ci < 65536;   // counts per step and source type
+  n->fused_lds_i64 = spk::fused_lds_bytes(n->BL, 1, n->W, G, n->nown_max, 0, n->xcap, P);
+  n->fused_lds = spk::fused_lds_bytes(n->BL, 1, n->W, G, n->nown_max, 0, n->xcap, P, 0, 0, n->cring_ok ? 1 : 0);
   if (n->nown_max > spk::kFusedBlock) return bail(fail(SPK_ERR_ARG, "more neurons per partition than threads per workgroup"));
   if (P > spk::kMaxPartitions) return bail(fail(SPK_ERR_ARG, "too many partitions"));
   if (n->fused_lds > fused_lds_limit(n))
@@ -2454,6 +2503,7 @@ int spk_create_brunel(const spk_brunel_params* p, int partitions, spk_network** 
     cleanup();
     return bail(rc);
   }
+  choose_dahead(n);
   if (S > 0) {
     hipLaunchKernelGGL(spk::k_gen_scatter, dim3(grid), dim3(256), 0, st, ga, S);
     SPK_HIP(hipGetLastError());
@@ -2657,6 +2707,7 @@ int spk_batch_run(spk_batch* b, const float* starts, const int64_t* ngroups, int
     spk_network* n = b->nets[k];
     if (int rc = fill_fused_args(n, false, args[k])) return rc;
     if (!b->lds_tables) args[k].lseg_U = args[k].lrec = 0;
+    args[k].dahead = 1;   // the batch's LDS holds one group's draws
     if (int rc = stage_fused(n, args[k], starts[k], 0, (int)ngroups[k], b->stream)) return rc;
     if (n->profiling) {
       if (!n->d_prof) {

@@ -54,6 +54,7 @@ class Mlp:
         h = C.c_void_p()
         abi.check(self.lib.spk_mlp_create(int(models), int(max_batch), C.byref(h)))
         self.h, self.models, self.max_batch = h, int(models), int(max_batch)
+        abi._track(self, 1)
 
     def close(self):
         if getattr(self, "h", None):

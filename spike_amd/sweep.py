@@ -100,9 +100,19 @@ def brunel_params(p: SweepPoint) -> abi.BrunelParams:
 LDS_HALF_CU = 80 * 1024 - 1024
 
 
-def build(p: SweepPoint, partitions: int = 0, lds_max: int = 0) -> abi.Network:
+DRIVE_SEED_STEP = 7919
+
+
+def drive_seed(p: SweepPoint, realisation: int) -> int:
+    """Seed of the external drive's realisation r of a point (r = 0: the point's own seed;
+    the connectivity always comes from the point's seed)."""
+    return int(p.seed) + DRIVE_SEED_STEP * int(realisation)
+
+
+def build(p: SweepPoint, partitions: int = 0, lds_max: int = 0, realisation: int = 0) -> abi.Network:
     """The point's network on the device; with partitions > 0 the count is raised until the
-    per-workgroup input ring fits in LDS (and in lds_max bytes when given)."""
+    per-workgroup input ring fits in LDS (and in lds_max bytes when given).  `realisation`
+    selects the external drive's seed (drive_seed)."""
     P = partitions
     while True:
         try:
@@ -113,6 +123,8 @@ def build(p: SweepPoint, partitions: int = 0, lds_max: int = 0) -> abi.Network:
             P = min(p.n, int(P * 1.25) + 1)
             continue
         if lds_max <= 0 or P <= 0 or P >= p.n or net.info().lds_bytes <= lds_max:
+            if realisation:
+                abi.check(net.lib.spk_set_seed(net.h, drive_seed(p, realisation)))
             return net
         net.close()
         P = min(p.n, int(P * 1.25) + 1)
@@ -158,10 +170,10 @@ def run_plain(points: Sequence[SweepPoint], seconds: float, bin_steps: int = 10)
                        dict(mode="plain"))
 
 
-def _build_all(points, partitions, seconds, bin_steps, profiling=False, lds_max=0):
+def _build_all(points, partitions, seconds, bin_steps, profiling=False, lds_max=0, realisation=0):
     nets = []
     for p, P in zip(points, partitions):
-        n = build(p, int(P), lds_max)
+        n = build(p, int(P), lds_max, realisation)
         n.stats_begin(bin_steps, p.groups(seconds) * p.delay_steps)
         if profiling:
             n.set_profiling(True)
@@ -173,18 +185,22 @@ def default_plan(points: Sequence[SweepPoint], neurons_per_wg: int = 384) -> Dic
     """Tiling from sizes alone: ceil(N / neurons_per_wg) workgroups, largest first."""
     parts = [max(1, -(-p.n // neurons_per_wg)) for p in points]
     order = sorted(range(len(points)), key=lambda k: -points[k].n)
-    return dict(partitions=parts, order=order, cost=None, name=f"size/{neurons_per_wg}")
+    return dict(partitions=parts, order=order, cost=None, name=f"size/{neurons_per_wg}",
+                policy=dict(target=None, npw=int(neurons_per_wg)))
 
 
 def run_batched(points: Sequence[SweepPoint], seconds: float, plan: Optional[Dict] = None,
-                workgroups: int = 0, bin_steps: int = 10, profiling: bool = False) -> SweepResult:
+                workgroups: int = 0, bin_steps: int = 10, profiling: bool = False,
+                realisation: int = 0) -> SweepResult:
     """All networks in one persistent launch.  `plan` gives every network's workgroup count
     (partitions), the scheduling order and optionally the per-group cost the scheduler
-    uses (microseconds); None uses default_plan."""
+    uses (microseconds); None uses default_plan.  `realisation` selects the external
+    drive's realisation of every point (drive_seed)."""
     plan = plan or default_plan(points)
     t0 = time.perf_counter()
     order = list(plan["order"])
-    nets = _build_all(points, plan["partitions"], seconds, bin_steps, profiling, plan.get("lds_max", 0))
+    nets = _build_all(points, plan["partitions"], seconds, bin_steps, profiling, plan.get("lds_max", 0),
+                      realisation)
     batch = abi.Batch([nets[k] for k in order], workgroups)
     if plan.get("cost") is not None:
         batch.set_costs([plan["cost"][k] for k in order])
@@ -224,7 +240,8 @@ def plan_from_work(points, events_per_group, target_events, neurons_per_wg_max, 
     total = [cost[k] * points[k].groups(1.0) * parts[k] for k in range(K)]
     order = sorted(range(K), key=lambda k: -cost[k] * points[k].groups(1.0))
     return dict(partitions=parts, order=order, cost=cost, name=name or f"work/{target_events:.0f}",
-                lds_max=LDS_HALF_CU if neurons_per_wg_max <= 512 else 0)
+                lds_max=LDS_HALF_CU if neurons_per_wg_max <= 512 else 0,
+                policy=dict(target=float(target_events), npw=int(neurons_per_wg_max)))
 
 
 def autotune(points: Sequence[SweepPoint], probe_seconds: float = 0.02,
@@ -282,4 +299,26 @@ def autotune(points: Sequence[SweepPoint], probe_seconds: float = 0.02,
                 best = dict(plan=plan, ms=r.ms)
     best["trials"] = trials
     best["fit"] = dict(fixed_us=fixed, per_event_us=per_ev)
+    best["policy"] = best["plan"].get("policy")
     return best
+
+
+def plan_for(points: Sequence[SweepPoint], policy: Optional[Dict], probe_seconds: float = 0.02):
+    """The tiling of a tuned policy (autotune(...)["policy"], tuned on other points) for
+    `points`: a probe run with the size-only plan measures their deliveries per group and the
+    cost fit, then plan_from_work applies the policy.  Returns (plan, probe device ms)."""
+    npw = policy["npw"] if policy else 384
+    base = default_plan(points, npw)
+    if not policy or policy.get("target") is None:
+        return base, 0.0            # a size-only tiling needs no probe
+    r0 = run_batched(points, probe_seconds, base, profiling=True)
+    groups = np.array([p.groups(probe_seconds) for p in points], dtype=np.float64)
+    ev = r0.extra["events"] / np.maximum(groups, 1.0)
+    us = r0.extra["duration_us"] / np.maximum(groups, 1.0)
+    x = ev / np.array(r0.extra["partitions"], dtype=np.float64)
+    A = np.stack([np.ones_like(x), x], axis=1)
+    (fixed, per_ev), *_ = np.linalg.lstsq(A, us, rcond=None)
+    fixed, per_ev = float(max(fixed, 1.0)), float(max(per_ev, 1e-6))
+    plan = plan_from_work(points, ev, policy["target"], policy["npw"], r0.config["workgroups"], fixed, per_ev,
+                          name=f"work/{policy['target']:.0f}/{policy['npw']}")
+    return plan, r0.ms

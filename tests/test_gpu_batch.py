@@ -6,7 +6,14 @@
    batch only changes which workgroups run which partitions), also when a few workgroups
    must run many networks one after another.
 3. Batched FP32 rates are within 2 % of the FP64 oracle's.
+4. The full sweep at its configured ranges (256 networks, N 1250-12500, 1 s): batched
+   equals one-network-at-a-time bit for bit, and the FP32 rates averaged over 16
+   realisations of the external drive lie within 2 % of the FP64 oracle's averages
+   (tests/golden/sweep_fp64_rates.json).
 """
+import json
+import os
+
 import numpy as np
 import pytest
 
@@ -45,6 +52,30 @@ def test_generated_network_matches_oracle(partitions):
     s = gpu.stats()
     oi, ot = ora.take_spikes(0)
     assert s.spikes == len(oi)
+
+
+@pytest.mark.parametrize("realisation", [1, 5])
+def test_drive_realisation_matches_oracle(realisation):
+    """Another realisation of the external drive (spk_set_seed after generation) keeps the
+    connectivity and equals the oracle replaying the point with that drive seed."""
+    p = sweep.SweepPoint(n=900, g=4.0, nu_ratio=2.5, J=0.1e-3, delay_steps=15, seed=77)
+    ng = 120
+    print(f"[batch] drive realisation {realisation}", flush=True)
+    gpu = sweep.build(p, 0, realisation=realisation)
+    gpu.run(0.0, ng, plasticity=False)
+    d = host_replay(p)
+    d.seed = sweep.drive_seed(p, realisation)
+    ora = oracle_py.OracleNetwork(d, precision=32)
+    ora.run(0.0, ng)
+    assert np.array_equal(gpu.membrane().astype(np.float64), ora.membrane())
+    assert np.array_equal(gpu.ring(), ora.ring())
+    c, oc = gpu.counters(), ora.counters()
+    assert c.spikes_neurons == oc["spikes_neurons"] > 0 and c.external_events == oc["external_events"]
+    gpu.close()
+    base = sweep.build(p, 0)
+    base.run(0.0, ng, plasticity=False)
+    assert base.counters().external_events != c.external_events
+    base.close()
 
 
 def _individual(points, ng_of, partitions_of):
@@ -121,3 +152,32 @@ def test_every_partition_count_builds_co_resident_or_reports_lds():
         b.close()
         net.close()
     assert built > 0 and refused > 0
+
+
+def test_full_sweep_at_configured_size():
+    """The benchmark's sweep: 256 networks of N log-uniform in [1250, 12500], g U(3, 8),
+    nu_ext/nu_thr U(0.9, 4), J U(0.05, 0.2) mV, delays 10-20 steps, 1 s each."""
+    here = os.path.dirname(os.path.abspath(__file__))
+    fx = json.load(open(os.path.join(here, "golden", "sweep_fp64_rates.json")))
+    pts = sweep.sample_points(256)
+    assert min(p.n for p in pts) >= 1250 and max(p.n for p in pts) <= 12500
+    print("[batch] 256-network sweep: batched", flush=True)
+    rb = sweep.run_batched(pts, 1.0, sweep.default_plan(pts, 384))
+    print(f"[batch] batched {rb.ms:.1f} ms; plain", flush=True)
+    rp = sweep.run_plain(pts, 1.0)
+    print(f"[batch] plain {rp.ms:.1f} ms", flush=True)
+    assert np.array_equal(rb.rate, rp.rate) and np.array_equal(rb.cv, rp.cv)
+    assert np.array_equal(rb.sync, rp.sync)
+    assert rb.ms < rp.ms
+    idx = [e["index"] for e in fx["points"]]
+    R = fx["realisations"]
+    sub = [pts[k] for k in idx]
+    r32 = np.zeros((len(idx), R))
+    r32[:, 0] = rb.rate[idx]
+    for r in range(1, R):
+        r32[:, r] = sweep.run_batched(sub, 1.0, None, realisation=r).rate
+    m64 = np.array([np.mean(e["rate_fp64"]) for e in fx["points"]])
+    err = np.abs(r32.mean(1) - m64) / m64
+    for j, k in enumerate(idx):
+        print(f"point {k} N={pts[k].n}: FP32 {r32[j].mean():.3f} Hz FP64 {m64[j]:.3f} Hz err {100 * err[j]:.2f} %")
+    assert err.max() <= 0.02
