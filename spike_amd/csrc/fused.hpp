@@ -456,13 +456,18 @@ __device__ __forceinline__ void deliver_quad(const FusedArgs& a, const uint32_t*
 
 // Compact form of deliver_quad: 2-byte records, the fixed-point weight and the slot are
 // those of the spike's source (q_lo below src_split, q_hi from it on), computed once per
-// spike with the same expression as the full form.
+// spike with the same expression as the full form.  The targets are read eight at a time:
+// a spike's segment [s0, s1) of syn16 is covered by the 16-byte aligned chunks it touches
+// (the array is padded to whole chunks), lane q of the quad takes chunks q, q + 4, ... and
+// adds the chunk's targets that lie inside the segment; the first chunk of every spike is
+// loaded before any is used.  With the count ring the increment is 1 (sources below
+// src_split) or 1 << 16.
 template <bool CR>
 __device__ __forceinline__ void deliver_quad_compact(const FusedArgs& a, const uint32_t* ents, int n, int q,
                                                      int p, int slot0, int B, int nown, int64_t* sring,
                                                      unsigned long long& st_del, long long q_lo, long long q_hi) {
   uint32_t s0[kSpikesPerQuad], s1[kSpikesPerQuad];
-  long long qv[kSpikesPerQuad];   // count ring: the increment (1 or 1 << 16)
+  long long qv[kSpikesPerQuad];
   int slot[kSpikesPerQuad];
 #pragma unroll
   for (int r = 0; r < kSpikesPerQuad; ++r) {
@@ -485,42 +490,36 @@ __device__ __forceinline__ void deliver_quad_compact(const FusedArgs& a, const u
 #pragma unroll
   for (int r = 0; r < kSpikesPerQuad; ++r)
     if (q == 0) st_del += (unsigned long long)(s1[r] - s0[r]);
-  // first two targets per lane of every segment, loads in flight together
-  uint16_t t0[kSpikesPerQuad][2];
-#pragma unroll
-  for (int r = 0; r < kSpikesPerQuad; ++r)
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const uint32_t k = s0[r] + (uint32_t)(q + h * kQuad);
-      t0[r][h] = k < s1[r] ? a.syn16[k] : (uint16_t)0;
-    }
+  const uint4* chunks = (const uint4*)a.syn16;
+  uint4 v[kSpikesPerQuad];
 #pragma unroll
   for (int r = 0; r < kSpikesPerQuad; ++r) {
+    const uint32_t c = (s0[r] >> 3) + (uint32_t)q;
+    v[r] = c < ((s1[r] + 7u) >> 3) && s0[r] < s1[r] ? chunks[c] : make_uint4(0u, 0u, 0u, 0u);
+  }
+#pragma unroll
+  for (int r = 0; r < kSpikesPerQuad; ++r) {
+    if (s0[r] >= s1[r]) continue;
     int64_t* row = sring + (size_t)slot[r] * nown;
     uint32_t* crow = (uint32_t*)sring + (size_t)slot[r] * nown;
+    const uint32_t cend = (s1[r] + 7u) >> 3;
+    uint32_t c = (s0[r] >> 3) + (uint32_t)q;
+    uint4 w = v[r];
+    while (c < cend) {
+      const uint32_t cn = c + kQuad;
+      const uint4 wn = cn < cend ? chunks[cn] : make_uint4(0u, 0u, 0u, 0u);
+      const uint32_t words[4] = {w.x, w.y, w.z, w.w};
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const uint32_t k = s0[r] + (uint32_t)(q + h * kQuad);
-      if (k < s1[r]) {
-        if (CR) atomicAdd(&crow[t0[r][h]], (uint32_t)qv[r]);
-        else atomicAdd((unsigned long long*)&row[t0[r][h]], (unsigned long long)qv[r]);
-      }
-    }
-    for (uint32_t k0 = s0[r] + (uint32_t)(q + 2 * kQuad); k0 < s1[r]; k0 += 4 * kQuad) {
-      uint16_t tt[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const uint32_t k = k0 + (uint32_t)(u * kQuad);
-        tt[u] = k < s1[r] ? a.syn16[k] : (uint16_t)0;
-      }
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const uint32_t k = k0 + (uint32_t)(u * kQuad);
-        if (k < s1[r]) {
-          if (CR) atomicAdd(&crow[tt[u]], (uint32_t)qv[r]);
-          else atomicAdd((unsigned long long*)&row[tt[u]], (unsigned long long)qv[r]);
+      for (int h = 0; h < 8; ++h) {
+        const uint32_t k = c * 8u + (uint32_t)h;
+        if (k >= s0[r] && k < s1[r]) {
+          const uint32_t t = (words[h >> 1] >> ((h & 1) * 16)) & 0xFFFFu;
+          if (CR) atomicAdd(&crow[t], (uint32_t)qv[r]);
+          else atomicAdd((unsigned long long*)&row[t], (unsigned long long)qv[r]);
         }
       }
+      c = cn;
+      w = wn;
     }
   }
 }
@@ -607,26 +606,30 @@ __device__ __forceinline__ void deliver_lane_lds2(const FusedArgs& a, bool has0,
 
 constexpr int kQuadEnt = 8;   // entries of one source a quad stages in LDS at a time
 
+// Delivery forms a kernel is compiled for: every form chosen at run time, or only the
+// compact form with the LDS-resident table, or only the compact form read from global
+// memory (a kernel holding only the form it runs keeps its registers for it).
+enum { kDeliverAny = 0, kDeliverLdsTable = 1, kDeliverCompact = 2 };
+
 // One quad's delivery of n entries staged in LDS, kSpikesPerQuad at a time, by the form of
 // the delivery table the launch uses.
+template <int DM>
 __device__ __forceinline__ void deliver_staged(const FusedArgs& a, const uint32_t* ents, int n, int q, int p,
                                                int slot0, int BL, int nownL, int L, int64_t* sring,
                                                unsigned long long& st_del, long long q_lo, long long q_hi,
                                                const uint16_t* slseg, const uint16_t* slrec) {
   for (int j0 = 0; j0 < n; j0 += kSpikesPerQuad) {
     const int m = min(kSpikesPerQuad, n - j0);
-    if (a.lseg_U > 0 && a.cring)
-      deliver_quad_lds<true>(a, ents + j0, m, q, slot0, BL, nownL, sring, st_del, q_lo, q_hi, slseg, slrec);
-    else if (a.lseg_U > 0)
-      deliver_quad_lds<false>(a, ents + j0, m, q, slot0, BL, nownL, sring, st_del, q_lo, q_hi, slseg, slrec);
-    else if (a.syn16 != nullptr && a.cring)
-      deliver_quad_compact<true>(a, ents + j0, m, q, p, slot0, BL, nownL, sring, st_del, q_lo, q_hi);
-    else if (a.syn16 != nullptr)
-      deliver_quad_compact<false>(a, ents + j0, m, q, p, slot0, BL, nownL, sring, st_del, q_lo, q_hi);
-    else if (a.scale == nullptr)
-      deliver_quad<false>(a, ents + j0, m, q, p, slot0, BL, nownL, L, sring, st_del);
-    else
-      deliver_quad<true>(a, ents + j0, m, q, p, slot0, BL, nownL, L, sring, st_del);
+    if (DM != kDeliverCompact && a.lseg_U > 0) {
+      if (a.cring) deliver_quad_lds<true>(a, ents + j0, m, q, slot0, BL, nownL, sring, st_del, q_lo, q_hi, slseg, slrec);
+      else deliver_quad_lds<false>(a, ents + j0, m, q, slot0, BL, nownL, sring, st_del, q_lo, q_hi, slseg, slrec);
+    } else if (DM != kDeliverLdsTable && a.syn16 != nullptr) {
+      if (a.cring) deliver_quad_compact<true>(a, ents + j0, m, q, p, slot0, BL, nownL, sring, st_del, q_lo, q_hi);
+      else deliver_quad_compact<false>(a, ents + j0, m, q, p, slot0, BL, nownL, sring, st_del, q_lo, q_hi);
+    } else if (DM == kDeliverAny) {
+      if (a.scale == nullptr) deliver_quad<false>(a, ents + j0, m, q, p, slot0, BL, nownL, L, sring, st_del);
+      else deliver_quad<true>(a, ents + j0, m, q, p, slot0, BL, nownL, L, sring, st_del);
+    }
   }
 }
 
@@ -637,7 +640,7 @@ __device__ __forceinline__ void deliver_staged(const FusedArgs& a, const uint32_
 // longer lists follow eight entries at a time with their loads in flight together.
 // Returns false (in every lane of the wave) when a wait exceeded the spin limit or another
 // workgroup reported an error; the error flag is then set.
-template <int BS>
+template <int BS, int DM>
 __device__ __forceinline__ bool tagged_gather_deliver(const FusedArgs& a, int p, int par, uint32_t tag, int slot0,
                                                       int BL, int nownL, int L, int64_t* sring,
                                                       unsigned long long& st_del, long long q_lo, long long q_hi,
@@ -687,7 +690,7 @@ __device__ __forceinline__ bool tagged_gather_deliver(const FusedArgs& a, int p,
     }
     if (!ok) break;
     if (!active) continue;
-    if (a.lseg_U > 0) {
+    if (DM != kDeliverCompact && a.lseg_U > 0) {
       // LDS table: every lane delivers the entries it holds -- word q is entry q - 1, word
       // q + 4 entry q + 3 -- and longer lists eight at a time, two entries per lane
       const bool h0 = q >= 1 && q - 1 < c, h1 = q + 3 < c;
@@ -727,8 +730,8 @@ __device__ __forceinline__ bool tagged_gather_deliver(const FusedArgs& a, int p,
     if (q >= 1 && q - 1 < c) qent[q - 1] = (uint32_t)w0;
     if (q + 3 < c) qent[q + 3] = (uint32_t)w1;
     __builtin_amdgcn_wave_barrier();
-    deliver_staged(a, qent, min(c, kQuadEnt - 1), q, p, slot0, BL, nownL, L, sring, st_del, q_lo, q_hi, slseg,
-                   slrec);
+    deliver_staged<DM>(a, qent, min(c, kQuadEnt - 1), q, p, slot0, BL, nownL, L, sring, st_del, q_lo, q_hi, slseg,
+                       slrec);
     for (int j0 = kQuadEnt - 1; j0 < c; j0 += kQuadEnt) {
       const int m = min(kQuadEnt, c - j0);
       const unsigned long long* eb = blk + 1 + j0;
@@ -750,7 +753,7 @@ __device__ __forceinline__ bool tagged_gather_deliver(const FusedArgs& a, int p,
       if (q < m) qent[q] = (uint32_t)e0;
       if (q + kQuad < m) qent[q + kQuad] = (uint32_t)e1;
       __builtin_amdgcn_wave_barrier();
-      deliver_staged(a, qent, m, q, p, slot0, BL, nownL, L, sring, st_del, q_lo, q_hi, slseg, slrec);
+      deliver_staged<DM>(a, qent, m, q, p, slot0, BL, nownL, L, sring, st_del, q_lo, q_hi, slseg, slrec);
     }
     __builtin_amdgcn_wave_barrier();
   }
@@ -765,7 +768,7 @@ __device__ __forceinline__ bool tagged_gather_deliver(const FusedArgs& a, int p,
 // against the tag; (4) quads deliver kSpikesPerQuad staged entries at a time.  `scnt` is
 // LDS for P counts, `sscan` for P + 1 offsets.  Returns false when a wait failed (the error
 // flag is then set); uniform across the workgroup.
-template <int BS>
+template <int BS, int DM>
 __device__ __forceinline__ bool tagged_gather_stage(const FusedArgs& a, int p, int par, uint32_t tag, int slot0,
                                                     int BL, int nownL, int L, int64_t* sring,
                                                     unsigned long long& st_del, long long q_lo, long long q_hi,
@@ -854,8 +857,8 @@ __device__ __forceinline__ bool tagged_gather_stage(const FusedArgs& a, int p, i
     __syncthreads();
     if (!*s_ok) return false;
     for (int j0 = quad * kSpikesPerQuad; j0 < nst; j0 += nquads * kSpikesPerQuad)
-      deliver_staged(a, sent + j0, min(kSpikesPerQuad, nst - j0), q, p, slot0, BL, nownL, L, sring, st_del,
-                     q_lo, q_hi, slseg, slrec);
+      deliver_staged<DM>(a, sent + j0, min(kSpikesPerQuad, nst - j0), q, p, slot0, BL, nownL, L, sring, st_del,
+                         q_lo, q_hi, slseg, slrec);
     __syncthreads();
   }
   return true;
@@ -919,7 +922,7 @@ __device__ __forceinline__ int partition_of_block(int b, int P) {
 
 // The group loop of one network for partition p, run by the workgroup that is member vb of
 // the network's P workgroups.
-template <int KIND, bool L1, bool PLASTIC, int BS>
+template <int KIND, bool L1, bool PLASTIC, int BS, int DM>
 __device__ __forceinline__ void fused_body(const FusedArgs& a, const int p, const int vb, unsigned char* smem) {
   const int tid = threadIdx.x;
   const int lane = (int)__lane_id();
@@ -1052,6 +1055,8 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const int p, cons
   // fixed-point weights of the compact form: the full form's expression for one record
   const long long q_lo = __float2ll_rn((a.w_lo * a.scale0) * a.two_k);
   const long long q_hi = __float2ll_rn((a.w_hi * a.scale0) * a.two_k);
+  const double qd_lo = (double)q_lo, qd_hi = (double)q_hi, qd_ext = (double)a.q_ext;
+  const float inv_G = 1.0f / (float)G;
   const int bits = W * 32;
   const int nown_waves = ((nown + kWave - 1) / kWave) * kWave;
   // spike counters of the exchange list, one per group parity (the other parity's is
@@ -1126,22 +1131,28 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const int p, cons
     // clear them in parallel.  Voltage synapses keep the label sum, conductance synapses
     // the per-label values.
     for (int k = tid; k < nown * G; k += BS) {
-      const int i = k / G, g = k - i * G;
+      // i = k / G without an integer division: (k + 1/2) / G lies at least 1/(2G) from an
+      // integer and k < 2^16, far inside float precision
+      const int i = (int)(((float)k + 0.5f) * inv_G), g = k - i * G;
       int slot = slot0 + g;
       if (slot >= BL) slot -= BL;
       int64_t* rs = sring + (size_t)slot * nownL + (size_t)i * L;
-      const int64_t ext_q = (a.ext_k > 0) ? (int64_t)ext[i * G + g] * a.q_ext : 0;
       if (a.cring) {
         // one label: the counts of the two source types times their fixed-point weights,
-        // the external events and, early in a launch, the input pending at its start
+        // the external events and, early in a launch, the input pending at its start --
+        // integers below 2^53, so their double sum is the exact fixed-point sum
         uint32_t* cs = scring + (size_t)slot * nown + i;
         const uint32_t cw = *cs;
         *cs = 0u;
-        int64_t qs = (int64_t)(cw & 0xFFFFu) * q_lo + (int64_t)(cw >> 16) * q_hi + ext_q;
+        double qs = (double)(cw & 0xFFFFu) * qd_lo + (double)(cw >> 16) * qd_hi;
+        if (a.ext_k > 0) qs += (double)ext[i * G + g] * qd_ext;
         const int step = tidx + g;
-        if (step < carry_end) qs += a.ring[(size_t)(step % B) * NL + (size_t)(nlo + i)];
-        sinp[k] = fixed_to_float(qs) * a.inv_scale;
-      } else if (KIND == 0) {
+        if (step < carry_end) qs += (double)a.ring[(size_t)(step % B) * NL + (size_t)(nlo + i)];
+        sinp[k] = (float)qs * a.inv_scale;
+        continue;
+      }
+      const int64_t ext_q = (a.ext_k > 0) ? (int64_t)ext[i * G + g] * a.q_ext : 0;
+      if (KIND == 0) {
         float total = 0.0f;
         for (int l = 0; l < L; ++l) {
           int64_t q = rs[l];
@@ -1341,13 +1352,13 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const int p, cons
       // -- F. every source's list of this group, polled by its tags, delivered into the
       //    LDS ring (no grid barrier)
       if (a.xquad) {
-        if (!tagged_gather_deliver<BS>(a, p, par, tag, slot0, BL, nownL, L, sring, st_del, q_lo, q_hi, slseg,
-                                       slrec, sent))
+        if (!tagged_gather_deliver<BS, DM>(a, p, par, tag, slot0, BL, nownL, L, sring, st_del, q_lo, q_hi, slseg,
+                                           slrec, sent))
           s_ok = 0;
         __syncthreads();
       } else {
-        tagged_gather_stage<BS>(a, p, par, tag, slot0, BL, nownL, L, sring, st_del, q_lo, q_hi, slseg, slrec,
-                                sent, (int*)shead, sscan, &s_ok);
+        tagged_gather_stage<BS, DM>(a, p, par, tag, slot0, BL, nownL, L, sring, st_del, q_lo, q_hi, slseg, slrec,
+                                    sent, (int*)shead, sscan, &s_ok);
       }
       if (!s_ok) return;
       clk.lap(a, 8);
@@ -1440,7 +1451,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const int p, cons
         for (int j0 = quad * kSpikesPerQuad; j0 < nr; j0 += nquads * kSpikesPerQuad) {
           const int n = min(kSpikesPerQuad, nr - j0);
           if (a.lseg_U > 0 || a.syn16 != nullptr)
-            deliver_staged(a, sent + j0, n, q, p, slot0, BL, nownL, L, sring, st_del, q_lo, q_hi, slseg, slrec);
+            deliver_staged<DM>(a, sent + j0, n, q, p, slot0, BL, nownL, L, sring, st_del, q_lo, q_hi, slseg, slrec);
           else if (a.scale == nullptr)
             deliver_quad<false>(a, sent + j0, n, q, p, slot0, BL, nownL, L, sring, st_del);
           else
@@ -1512,10 +1523,10 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const int p, cons
   }
 }
 
-template <int KIND, bool L1, bool PLASTIC>
+template <int KIND, bool L1, bool PLASTIC, int DM = kDeliverAny>
 __global__ __launch_bounds__(kFusedBlock) void k_fused(FusedArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
-  fused_body<KIND, L1, PLASTIC, kFusedBlock>(a, partition_of_block(blockIdx.x, gridDim.x), blockIdx.x, smem);
+  fused_body<KIND, L1, PLASTIC, kFusedBlock, DM>(a, partition_of_block(blockIdx.x, gridDim.x), blockIdx.x, smem);
 }
 
 // Many independent networks in one cooperative launch.  Workgroup b runs the tasks
@@ -1535,7 +1546,7 @@ __global__ __launch_bounds__(BS, 4) void k_fused_batch(const FusedArgs* __restri
   for (int j = j0; j < j1; ++j) {
     const int k = __builtin_amdgcn_readfirstlane(tasks[j].x);
     const int p = __builtin_amdgcn_readfirstlane(tasks[j].y);
-    fused_body<KIND, L1, PLASTIC, BS>(nets[k], p, p, smem);
+    fused_body<KIND, L1, PLASTIC, BS, kDeliverAny>(nets[k], p, p, smem);
     __syncthreads();
   }
 }

@@ -273,6 +273,7 @@ struct spk_network {
   bool rules_uploaded = false;   // device descriptors of the rules are current
   bool generated = false;        // connectivity generated on the device (persistent kernel only)
   size_t lds_attr[2] = {0, 0};  // dynamic LDS limit set for the non-plastic/plastic kernel
+  const void* lds_fn[2] = {nullptr, nullptr};  // ... and the kernel it was set for
   // window statistics (spk_stats_begin)
   unsigned int *d_st_cnt = nullptr, *d_st_nisi = nullptr, *d_st_hist = nullptr;
   float* d_st_prev = nullptr;
@@ -777,6 +778,11 @@ int download(spk_network* n, T* out, const T* d, size_t cnt) {
 // plasticity applied or not).
 const void* fused_kernel(const spk_network* n, bool plastic) {
   const bool l1 = n->L == 1;
+  // Brunel-type networks (voltage synapses, compact records): kernels holding only their
+  // delivery form
+  if (!plastic && n->kind == SPK_SYN_VOLTAGE && l1 && n->compact && n->rules.empty())
+    return n->lseg_U > 0 ? (const void*)spk::k_fused<0, true, false, spk::kDeliverLdsTable>
+                         : (const void*)spk::k_fused<0, true, false, spk::kDeliverCompact>;
 #define SPK_FUSED_PICK(K)                                                                  \
   (plastic ? (l1 ? (const void*)spk::k_fused<K, true, true> : (const void*)spk::k_fused<K, false, true>) \
            : (l1 ? (const void*)spk::k_fused<K, true, false> : (const void*)spk::k_fused<K, false, false>))
@@ -977,9 +983,11 @@ int launch_fused(spk_network* n, spk::FusedArgs& a, float start, int64_t group0,
                  hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr) {
   if (int rc = stage_fused(n, a, start, group0, ng, n->stream)) return rc;
   const void* fn = fused_kernel(n, a.plastic != 0);
-  if (n->fused_lds > 65536 && n->lds_attr[a.plastic != 0] < n->fused_lds) {
+  const int fi = a.plastic != 0;
+  if (n->fused_lds > 65536 && (n->lds_attr[fi] < n->fused_lds || n->lds_fn[fi] != fn)) {
     SPK_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)n->fused_lds));
-    n->lds_attr[a.plastic != 0] = n->fused_lds;
+    n->lds_attr[fi] = n->fused_lds;
+    n->lds_fn[fi] = fn;
   }
   void* kargs[] = {&a};
   if (ev0) SPK_HIP(hipEventRecord(ev0, n->stream));
@@ -1731,7 +1739,8 @@ int spk_prepare(spk_network* n, int G) {
     host_chunks(S, T, [&](int, int64_t a, int64_t b) {
       for (int64_t k = a; k < b; ++k) t16[k] = (uint16_t)(packed[k] & 0xFFFFFu);
     });
-    if ((rc = dalloc(&n->d_syn16, S))) return rc;
+    // padded to whole 16-byte chunks plus one: deliveries read aligned 8-target chunks
+    if ((rc = dalloc(&n->d_syn16, (size_t)(S + 7) / 8 * 8 + 8))) return rc;
     if ((rc = upload(n->d_syn16, t16.data(), S, st))) return rc;
   }
   if ((rc = upload(n->d_delay, dl.data(), S, st))) return rc;
@@ -2438,7 +2447,7 @@ int spk_create_brunel(const spk_brunel_params* p, int partitions, spk_network** 
   ALLOC(n->d_decay, 1); ALLOC(n->d_erev, 1); ALLOC(n->d_in_thr, 1);
   ALLOC(n->d_ext_cdf, std::max<size_t>(1, n->ext_cdf.size()));
   ALLOC(n->d_syn, (size_t)std::max<int64_t>(1, S));
-  ALLOC(n->d_syn16, (size_t)std::max<int64_t>(1, S));
+  ALLOC(n->d_syn16, (size_t)(std::max<int64_t>(1, S) + 7) / 8 * 8 + 8);   // whole 16-byte chunks
   ALLOC(n->d_stats, 4);
   ALLOC(n->d_nlo, (size_t)P + 1); ALLOC(n->d_ilo, (size_t)P + 1);
   ALLOC(n->d_seg, (size_t)N * (P + 1));
@@ -2707,7 +2716,16 @@ int spk_batch_run(spk_batch* b, const float* starts, const int64_t* ngroups, int
     spk_network* n = b->nets[k];
     if (int rc = fill_fused_args(n, false, args[k])) return rc;
     if (!b->lds_tables) args[k].lseg_U = args[k].lrec = 0;
-    args[k].dahead = 1;   // the batch's LDS holds one group's draws
+    // as many groups of draws as the batch's LDS per workgroup holds for this network
+    args[k].dahead = 1;
+    for (int d : {8, 4, 2}) {
+      const spk::FusedArgs& x = args[k];
+      if (spk::fused_lds_bytes(x.BL, x.L, x.W, x.G, x.nown_max, x.iown_max, x.xcap, x.P, x.lseg_U, x.lrec, x.cring,
+                               d) <= b->lds) {
+        args[k].dahead = d;
+        break;
+      }
+    }
     if (int rc = stage_fused(n, args[k], starts[k], 0, (int)ngroups[k], b->stream)) return rc;
     if (n->profiling) {
       if (!n->d_prof) {
