@@ -302,10 +302,10 @@ __device__ __forceinline__ void group_time(const FusedArgs& a, int s, float& t, 
 // Chen & Asau 1974), so the scan takes about one step.  The input masks are ORed into slots
 // that phase A cleared when it read them.
 template <int BS>
-__device__ __forceinline__ void draw_groups(const FusedArgs& a, int s0, int cnt, uint8_t* sext,
-                                            unsigned long long* simask, int nlo, int nown, int ilo,
-                                            int iown, unsigned long long& st_ext, const uint32_t* scdf,
-                                            const uint8_t* sguide) {
+__device__ __forceinline__ void draw_groups_flat(const FusedArgs& a, int s0, int cnt, uint8_t* sext,
+                                                 unsigned long long* simask, int nlo, int nown, int ilo,
+                                                 int iown, unsigned long long& st_ext, const uint32_t* scdf,
+                                                 const uint8_t* sguide) {
   const int G = a.G;
   const int tid = threadIdx.x;
   const int nbm = (G + 3) / 4 + 1;   // 4-step blocks a group's G steps can touch
@@ -362,6 +362,80 @@ __device__ __forceinline__ void draw_groups(const FusedArgs& a, int s0, int cnt,
       if (bits) atomicOr(&simask[(size_t)((s0 + j) % a.dahead) * a.iown_max + i], bits);
     }
   }
+}
+
+template <int BS>
+__device__ __forceinline__ void draw_groups_seq(const FusedArgs& a, int s0, int cnt, uint8_t* sext,
+                                            unsigned long long* simask, int nlo, int nown, int ilo,
+                                            int iown, unsigned long long& st_ext, const uint32_t* scdf,
+                                            const uint8_t* sguide) {
+  const int G = a.G;
+  const int tid = threadIdx.x;
+  for (int j = 0; j < cnt; ++j) {
+    // per group (uniform): its first step and the 4-step blocks its G steps touch
+    float t;
+    int tidx;
+    group_time(a, s0 + j, t, tidx);
+    const int64_t b0 = (int64_t)tidx >> 2;
+    const int nbk = (int)((((int64_t)tidx + G - 1) >> 2) - b0) + 1;
+    const float inv_nbk = 1.0f / (float)nbk;
+    const int slot = (s0 + j) % a.dahead;
+    if (a.ext_k > 0) {
+      uint8_t* ext = sext + (size_t)slot * a.nown_max * G;
+      for (int k = tid; k < nown * nbk; k += BS) {
+        // k / nbk by a float product: (k + 1/2) / nbk is at least 1/(2 nbk) from an
+        // integer and k < 2^16
+        const int i = (int)(((float)k + 0.5f) * inv_nbk);
+        const int64_t blk = b0 + (k - i * nbk);
+        const u32x4 r = draw_block(a.seed, blk, (uint32_t)(nlo + i), kStreamExternal);
+        const int gb = (int)(blk * 4 - (int64_t)tidx);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const int g = gb + c;
+          if (g >= 0 && g < G) {
+            const uint32_t u = pick(r, c);
+            int n = sguide[u >> 24];
+            while (n < a.ext_k && u >= scdf[n]) ++n;
+            ext[i * G + g] = (uint8_t)n;
+            st_ext += (unsigned long long)n;
+          }
+        }
+      }
+    }
+    if (iown > 0) {
+      unsigned long long* im = simask + (size_t)slot * a.iown_max;
+      for (int k = tid; k < iown * nbk; k += BS) {
+        const int i = (int)(((float)k + 0.5f) * inv_nbk);
+        const int64_t blk = b0 + (k - i * nbk);
+        const u32x4 r = draw_block(a.seed, blk, (uint32_t)(ilo + i), kStreamInput);
+        const uint32_t thr = a.in_thr[ilo + i];
+        unsigned long long bits = 0ull;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const int64_t step = blk * 4 + c;
+          const int g = (int)(step - (int64_t)tidx);
+          if (g >= 0 && g < G && pick(r, c) < thr && step != a.arm_step) bits |= 1ull << g;
+        }
+        if (bits) atomicOr(&im[i], bits);
+      }
+    }
+  }
+}
+
+// The draws of small partitions (at most four items per thread) take one pass over
+// (group, unit, block) items, so that the groups' Philox calls run side by side; larger
+// ones go one group at a time, with each group's constants computed once and no integer
+// divisions per item.
+template <int BS>
+__device__ __forceinline__ void draw_groups(const FusedArgs& a, int s0, int cnt, uint8_t* sext,
+                                            unsigned long long* simask, int nlo, int nown, int ilo,
+                                            int iown, unsigned long long& st_ext, const uint32_t* scdf,
+                                            const uint8_t* sguide) {
+  const int nbm = (a.G + 3) / 4 + 1;
+  if (cnt * ((a.ext_k > 0 ? nown : 0) + iown) * nbm <= 4 * BS)
+    draw_groups_flat<BS>(a, s0, cnt, sext, simask, nlo, nown, ilo, iown, st_ext, scdf, sguide);
+  else
+    draw_groups_seq<BS>(a, s0, cnt, sext, simask, nlo, nown, ilo, iown, st_ext, scdf, sguide);
 }
 
 // One quad's share of kSpikesPerQuad spikes: every lane loads its synapses of each spike's
@@ -609,7 +683,7 @@ constexpr int kQuadEnt = 8;   // entries of one source a quad stages in LDS at a
 // Delivery forms a kernel is compiled for: every form chosen at run time, or only the
 // compact form with the LDS-resident table, or only the compact form read from global
 // memory (a kernel holding only the form it runs keeps its registers for it).
-enum { kDeliverAny = 0, kDeliverLdsTable = 1, kDeliverCompact = 2 };
+enum { kDeliverAny = 0, kDeliverLdsTable = 1, kDeliverCompact = 2, kDeliverCompactAny = 3 };
 
 // One quad's delivery of n entries staged in LDS, kSpikesPerQuad at a time, by the form of
 // the delivery table the launch uses.
@@ -1536,7 +1610,7 @@ __global__ __launch_bounds__(kFusedBlock) void k_fused(FusedArgs a) {
 // that are finishing earlier networks: every wait ends.  BS threads per workgroup: with
 // kBatchSmallBlock, two workgroups share a compute unit, so one network's partition runs
 // while the other's waits at its exchange (partitions of at most BS neurons).
-template <int KIND, bool L1, bool PLASTIC, int BS>
+template <int KIND, bool L1, bool PLASTIC, int BS, int DM = kDeliverAny>
 __global__ __launch_bounds__(BS, 4) void k_fused_batch(const FusedArgs* __restrict__ nets,
                                                     const int2* __restrict__ tasks,
                                                     const int* __restrict__ task_off) {
@@ -1546,7 +1620,7 @@ __global__ __launch_bounds__(BS, 4) void k_fused_batch(const FusedArgs* __restri
   for (int j = j0; j < j1; ++j) {
     const int k = __builtin_amdgcn_readfirstlane(tasks[j].x);
     const int p = __builtin_amdgcn_readfirstlane(tasks[j].y);
-    fused_body<KIND, L1, PLASTIC, BS, kDeliverAny>(nets[k], p, p, smem);
+    fused_body<KIND, L1, PLASTIC, BS, DM>(nets[k], p, p, smem);
     __syncthreads();
   }
 }

@@ -1528,6 +1528,8 @@ int spk_prepare(spk_network* n, int G) {
   auto no_fused = [&](const char* why) { if (n->fused_ok) { n->fused_ok = false; n->fused_why = why; } };
   if (n->N < 1) no_fused("no LIF neurons");
   if (n->nown_max > spk::kFusedBlock) no_fused("more neurons per partition than threads per workgroup");
+  // the draws split work items (input, block) by a float product, exact below 2^21 items
+  if (n->iown_max > 65536) no_fused("more than 65536 inputs per partition");
   if (G > 64) no_fused("timestep grouping above 64");
   if (n->dmax > 255) no_fused("delays above 255 steps");
   if (n->L > 15) no_fused("more than 15 labels");
@@ -2565,7 +2567,25 @@ const void* batch_kernel_bs(const spk_batch* b) {
   return b->l1 ? (const void*)spk::k_fused_batch<2, true, false, BS> : (const void*)spk::k_fused_batch<2, false, false, BS>;
 }
 
+// Batches of Brunel-type networks (voltage synapses, one label, compact records) run
+// kernels without the full-record delivery: compact only when no network keeps its table in
+// LDS, else both compact forms.
+template <int BS>
+const void* batch_kernel_compact(const spk_batch* b, bool tables) {
+  return tables ? (const void*)spk::k_fused_batch<0, true, false, BS, spk::kDeliverCompactAny>
+                : (const void*)spk::k_fused_batch<0, true, false, BS, spk::kDeliverCompact>;
+}
+
 const void* batch_kernel(const spk_batch* b, int bs) {
+  bool compact = b->kind == SPK_SYN_VOLTAGE && b->l1, tables = false;
+  for (auto* n : b->nets) {
+    compact = compact && n->compact && n->rules.empty();
+    tables = tables || n->lseg_U > 0;
+  }
+  tables = tables && bs != spk::kBatchSmallBlock && b->lds_tables;
+  if (compact)
+    return bs == spk::kBatchSmallBlock ? batch_kernel_compact<spk::kBatchSmallBlock>(b, false)
+                                       : batch_kernel_compact<spk::kFusedBlock>(b, tables);
   return bs == spk::kBatchSmallBlock ? batch_kernel_bs<spk::kBatchSmallBlock>(b) : batch_kernel_bs<spk::kFusedBlock>(b);
 }
 
