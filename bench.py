@@ -291,6 +291,9 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--form", default="config", choices=["config", "reference"])
+    ap.add_argument("--nu-ext-ratio", type=float, default=2.0,
+                    help="external drive nu_ext / nu_thr of the Brunel network (0: no drive, no spikes: the "
+                         "persistent kernel's per-group cost with nothing to deliver)")
     ap.add_argument("--path", default="fused", choices=["fused", "launches"])
     ap.add_argument("--cpu-groups", type=int, default=100,
                     help="timestep groups of the CPU-baseline sample (15 steps each), after the warm-up")
@@ -330,7 +333,7 @@ def main():
         torch.cuda.synchronize()
         del x, y
 
-    desc = builders.brunel(seed=42, form=args.form)
+    desc = builders.brunel(seed=42, form=args.form, nu_ext_ratio=args.nu_ext_ratio)
     G = desc.timestep_grouping()
     groups = desc.groups_for(1.0)
     net = abi.Network(desc)
@@ -402,7 +405,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic (Brunel network, seed 42)",
         "config": {"workload": f"Brunel10K {args.form} form: 8000E+2000I LIF, C_E=800/C_I=200, "
-                               f"J=0.1 mV g=5, delay 1.5 ms, nu_ext=2nu_thr, dt 0.1 ms, 1 s per step",
+                               f"J=0.1 mV g=5, delay 1.5 ms, nu_ext={args.nu_ext_ratio:g}nu_thr, dt 0.1 ms, 1 s per step",
                    "n_neurons": desc.n_neurons, "n_synapses": desc.n_synapses,
                    "timestep_grouping": G, "groups_per_step": groups, "path": args.path,
                    "record_bytes": info.record_bytes if args.path == "fused" else 8},
