@@ -242,36 +242,54 @@ def mlp_leg(log, members=8, rows=1 << 20, batch=4096, steps=200):
     }
 
 
+# Algorithmic bytes of the large plastic network's step (per-phase kernels), per item:
+#   neuron   56 B: V read+write 8, v0/vreset/threshold/decay/resistance 20, last spike 4,
+#                  its ring slot read+clear 16, history word read+write 8
+#   input    12 B: last spike 4, history word read+write 8
+#   delivery 24 B: target 4, FP16 weight 2, delay 2, scale 4, 64-bit ring add (read+write) 12
+#   plastic update 24 B: per synapse of a spiking excitatory neuron's rule lists (800 out, 800 in
+#                  on average): its index or position 4, the other side's last spike 4,
+#                  weight + Kahan word read+write 8, the other side's trace 4, source unit 4
+LARGE_BYTES = dict(neuron=56, input=12, delivery=24, plastic=24)
+
+
 def large_leg(seconds, log):
     """Large plastic network at full size: 5e5 LIF neurons (80 % excitatory), 1000 random
     outgoing synapses each (5e8 synapses, delays 1..20 steps so G = 1, FP16 weights),
     2.5e4 Poisson inputs at 10 Hz with 1000 targets each, nearest-spike weight-dependent
-    STDP on the 3.2e8 E->E synapses; dt = 0.1 ms.  Per-phase kernels (the ring slice of a
-    partition does not fit in LDS), weight-dependent rule driven by spikes.  Host-built
-    connectivity (seed 42).  Times are HIP events on the network's stream, after 20
-    warm-up steps (the first runs the rule's per-synapse form once)."""
+    STDP on the 3.2e8 E->E synapses; dt = 0.1 ms.  Synapses generated on the device by the
+    Philox fixed out-degree rule (seed 42); per-phase kernels (the ring slice of a
+    partition does not fit in LDS), weight-dependent rule driven by spikes.  Times are HIP
+    events on the network's stream, after 20 warm-up steps (the first runs the rule's
+    per-synapse form once)."""
     from spike_amd import abi, builders
     from spike_amd.netdesc import WEIGHTS_FP16
     t0 = time.perf_counter()
     d = builders.large_plastic(monitors=())
     d.weight_format = WEIGHTS_FP16
-    build_s = time.perf_counter() - t0
-    t0 = time.perf_counter()
     net = abi.Network(d)
-    prepare_s = time.perf_counter() - t0
-    n, n_in, S, P = d.n_neurons, d.n_inputs, d.n_synapses, len(d.rules[0].synapses)
+    build_s = time.perf_counter() - t0
+    n, n_in, S = d.n_neurons, d.n_inputs, d.n_synapses
+    P = net.rule_size(0)
+    o = d.outdegree
     warm = 20
     t = float(d.group_time(0.0, warm))
     groups = d.groups_for(seconds)
-    del d
-    log(f"large: built in {build_s:.1f} s, prepared in {prepare_s:.1f} s")
+    log(f"large: synapses generated and prepared on the device in {build_s:.2f} s")
     net.run(0.0, warm)
     tm = net.run(t, groups)
     net.close()
     steps = groups   # G = 1
     ms = tm.total_ms
+    # plastic updates: each excitatory spike visits its rule synapses as source and as target
+    plastic_per_spike = 2.0 * P / o.n_exc
+    exc_spikes = tm.spikes_neurons * o.n_exc / n
+    B = LARGE_BYTES
+    algo = (steps * (n * B["neuron"] + n_in * B["input"]) + tm.synaptic_events * B["delivery"]
+            + exc_spikes * plastic_per_spike * B["plastic"])
+    achieved = algo / (ms / 1e3) / 1e9
     log(f"large: {seconds} s simulated in {ms:.1f} ms ({ms / steps * 1e3:.1f} us per step), "
-        f"{tm.spikes_neurons} spikes, {tm.synaptic_events} deliveries")
+        f"{tm.spikes_neurons} spikes, {tm.synaptic_events} deliveries, {achieved:.0f} GB/s algorithmic")
     return {
         "workload": "large plastic net: 5e5 LIF (80/20), 1000 random out-synapses each, delays 1-20 steps, "
                     "FP16 weights, 2.5e4 Poisson inputs at 10 Hz, nearest-spike WD STDP on E->E, dt 0.1 ms",
@@ -281,7 +299,10 @@ def large_leg(seconds, log):
         "syn_events_per_s": round(tm.synaptic_events / (ms / 1e3), 1),
         "rate_hz": round(tm.spikes_neurons / n / (steps * 1e-4), 3),
         "spikes": tm.spikes_neurons, "input_spikes": tm.spikes_inputs, "deliveries": tm.synaptic_events,
-        "build_s": round(build_s, 1), "prepare_s": round(prepare_s, 1), "path": "per-phase kernels",
+        "build_s": round(build_s, 2), "builder": "device (Philox fixed out-degree)", "path": "per-phase kernels",
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "bytes_model": B, "scope": "whole step (all six launches), algorithmic bytes"},
     }
 
 

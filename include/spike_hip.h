@@ -365,6 +365,8 @@ int spk_get_history(spk_network* net, int population, uint32_t* words);
 /* Plasticity traces of a rule in its synapse order (two arrays of n); for a weight
  * normalization rule, its initial weights and zeros. */
 int spk_get_traces(spk_network* net, int rule_id, float* trace_a, float* trace_b);
+/* Synapse count of a rule (after spk_prepare; a generated rule's size is known only then). */
+int spk_get_rule_size(spk_network* net, int rule_id, int64_t* count);
 
 /* ---------------------------------------------------------------------------- */
 /* Brunel networks generated on the device (no reference counterpart: the reference builds
@@ -385,6 +387,33 @@ typedef struct {
   uint64_t seed;
 } spk_brunel_params;
 int spk_create_brunel(const spk_brunel_params* params, int partitions, spk_network** out);
+
+/* Fixed out-degree connectivity generated on the device (the fast builder for large
+ * networks; no reference counterpart: the reference's CONNECTIVITY_TYPE_RANDOM draws one
+ * rand() per (pre, post) pair and keeps 36 B of host arrays per synapse,
+ * Synapses.cpp:208-228 and :391-421).  Called on a created network with LIF parameters
+ * (and Poisson rates) set, in place of spk_set_synapses and the rule calls; spk_prepare
+ * then generates the synapses and the rule tables on the device, with no host copy.
+ * Unit u in [0, n_neurons + n_inputs) (neurons, then inputs) has out_degree
+ * (input_out_degree) synapses, targets drawn with replacement; synapse j of unit u takes
+ * r = Philox4x32-10(ctr = {j, 0, u, 5}, key = seed):
+ *   post = floor(r.x * n_neurons / 2^32), delay = 1 + floor(r.y * delay_max / 2^32),
+ *   weight = (r.z >> 8) * 2^-24 for excitatory sources (u < n_exc), else weight_inh /
+ *   weight_input; scale by the same source classes.
+ * Caller synapse s is CSR position s: unit u's synapses are s = row(u) + j with
+ * row(u) = u*out_degree for u <= n_neurons, then input_out_degree per input.  With
+ * ee_stdp, every synapse from and to the first n_exc neurons belongs to one
+ * weight-dependent STDP rule (rule id 0), listed in synapse order.  Such a network runs
+ * on the per-phase kernels; weights and traces are readable as for a listed network. */
+typedef struct {
+  uint64_t seed;
+  int n_exc, out_degree, input_out_degree, delay_max;
+  float scale_exc, scale_inh, scale_input;
+  float weight_inh, weight_input;
+  int ee_stdp;
+  spk_wd_stdp_params stdp;
+} spk_outdegree_params;
+int spk_set_outdegree_synapses(spk_network* net, const spk_outdegree_params* params);
 
 /* Batches: many prepared networks (same synapse kind; each able to use the persistent
  * kernel) run in one cooperative launch of `workgroups` workgroups (0 = as many as are

@@ -76,6 +76,10 @@ SIG = {
     "or_philox4x32_10": (None, [up, up, up]),
     "or_brunel_connectivity": (C.c_int, [C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                          C.POINTER(C.c_int32)]),
+    "or_outdegree_connectivity": (C.c_int, [C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                            C.c_float, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int,
+                                            C.POINTER(C.c_int32), C.POINTER(C.c_int32), fp, fp,
+                                            C.POINTER(C.c_int32)]),
     "or_bernoulli_threshold": (C.c_uint32, [C.c_float, C.c_float]),
     "or_poisson_count_table": (C.c_int, [C.c_double, up, C.c_int]),
     "or_choose_frac_bits": (C.c_int, [C.c_double]),
@@ -117,6 +121,41 @@ def brunel_connectivity(seed, ne, ni, ce, ci):
     pre = np.empty((n, ce + ci), np.int32)
     check(lib.or_brunel_connectivity(int(seed), ne, ni, ce, ci, n, _p(pre, C.c_int32)))
     return pre
+
+
+def outdegree_connectivity(seed, n, n_in, o, threads=8):
+    """Synapse arrays (pre, post, weight, scale, delay) of the Philox fixed out-degree
+    rule with the parameters of a NetDesc `outdegree` spec."""
+    lib = load()
+    S = o.n_synapses(n, n_in)
+    pre = np.empty(S, np.int32)
+    post = np.empty(S, np.int32)
+    w = np.empty(S, np.float32)
+    sc = np.empty(S, np.float32)
+    dl = np.empty(S, np.int32)
+    check(lib.or_outdegree_connectivity(int(seed), n, n_in, o.n_exc, o.out_degree, o.input_out_degree,
+                                        o.delay_max, o.scale_exc, o.scale_inh, o.scale_input, o.weight_inh,
+                                        o.weight_input, int(threads), _p(pre, C.c_int32), _p(post, C.c_int32),
+                                        _p(w, C.c_float), _p(sc, C.c_float), _p(dl, C.c_int32)))
+    return pre, post, w, sc, dl
+
+
+def expand_outdegree(d, threads=8):
+    """The same network as a NetDesc with host synapse arrays (and its E->E rule listed in
+    synapse order): what the library generates on the device for `d.outdegree`."""
+    import copy
+    from spike_amd.netdesc import Rule, RULE_WD
+    o = d.outdegree
+    e = copy.copy(d)
+    e.outdegree = None
+    e.pre, e.post, e.weight, e.scale, e.delay = outdegree_connectivity(d.seed, d.n_neurons, d.n_inputs, o, threads)
+    e.label = None
+    e.grouping = d.timestep_grouping()
+    e.rules = []
+    if o.ee_stdp:
+        ee = np.flatnonzero((e.pre >= 0) & (e.pre < o.n_exc) & (e.post < o.n_exc)).astype(np.int64)
+        e.rules = [Rule(RULE_WD, ee, dict(o.ee_stdp))]
+    return e
 
 
 def philox(ctr, key):

@@ -284,6 +284,15 @@ struct Net {
     for (auto& x : th) x.join();
   }
 
+  template <typename F>
+  void parallel64(int64_t n, F f) {
+    const int T = (int)std::max<int64_t>(1, std::min<int64_t>(nthreads, n / 4096));
+    if (T == 1) { f((int64_t)0, n); return; }
+    std::vector<std::thread> th;
+    for (int t = 0; t < T; ++t) th.emplace_back([=, &f] { f(n * t / T, n * (t + 1) / T); });
+    for (auto& x : th) x.join();
+  }
+
   // conversion of one delivery to the ring's accumulator
   static Acc to_acc(R v, int k) {
     if constexpr (kFixed) return (Acc)std::llrint(std::ldexp((double)v, k));
@@ -593,7 +602,9 @@ struct Net {
         iEm[k] = 1.0f / Em[k];
         iEp[k] = 1.0f / Ep[k];
       }
-      for (int64_t j = 0; j < P; ++j) {
+      // every rule synapse is updated from its own state only: split over threads
+      parallel64(P, [&](int64_t jlo, int64_t jhi) {
+      for (int64_t j = jlo; j < jhi; ++j) {
         int64_t s = r.syn[j];
         R xpre = (R)r.ta[j], xpost = (R)r.tb[j];
         int pg = (int)std::roundf((pre_last(spre[s]) - t) / dt);
@@ -628,6 +639,7 @@ struct Net {
         wstore(s, w, nw, 0.0f, INFINITY);
         r.ta[j] = tstore(xpre); r.tb[j] = tstore(xpost);
       }
+      });
     } else if (r.kind == 1) {
       // InhibitorySTDPPlasticity.cu:72-155 (the target term is evaluated in double)
       const float decay = std::exp(-dt / r.vg[0]);
@@ -1148,6 +1160,51 @@ int or_brunel_connectivity(uint64_t seed, int ne, int ni, int ce, int ci, int n,
       pre[(size_t)i * C + j] = j < ce ? (int32_t)(((uint64_t)u * (uint64_t)ne) >> 32)
                                       : (int32_t)(ne + (((uint64_t)u * (uint64_t)ni) >> 32));
     }
+  return 0;
+}
+
+// Fixed out-degree connectivity drawn from Philox (the rule the library generates on the
+// device for large networks): unit u (neurons [0, n), then inputs) has k (k_in) synapses;
+// synapse j of unit u takes r = philox(ctr = {j, 0, u, 5}, key = seed) and gets
+//   post = floor(r0*n/2^32), delay = 1 + floor(r1*dmax/2^32),
+//   weight = (r2 >> 8) * 2^-24 for u < ne, else w_inh (neurons) / w_in (inputs),
+//   scale = s_e / s_i / s_in by the same classes,
+// at index row(u) + j, row(u) = u*k for u <= n, then k_in per input; pre is u for a
+// neuron and -(u-n)-1 for input u-n.  Rows are split over `threads` threads.
+int or_outdegree_connectivity(uint64_t seed, int n, int n_in, int ne, int k, int k_in, int dmax,
+                              float s_e, float s_i, float s_in, float w_inh, float w_in, int threads,
+                              int32_t* pre, int32_t* post, float* weight, float* scale, int32_t* delay) {
+  const int U = n + n_in;
+  auto row = [&](int u) { return u <= n ? (int64_t)u * k : (int64_t)n * k + (int64_t)(u - n) * k_in; };
+  auto rows = [&](int u0, int u1) {
+    for (int u = u0; u < u1; ++u) {
+      const int len = u < n ? k : k_in;
+      const int64_t base = row(u);
+      for (int j = 0; j < len; ++j) {
+        uint32_t r[4];
+        Philox::draws4(seed, (int64_t)j, (uint32_t)u, 5u, r);
+        const int64_t s = base + j;
+        pre[s] = u < n ? u : -(u - n) - 1;
+        post[s] = (int32_t)(((uint64_t)r[0] * (uint64_t)n) >> 32);
+        delay[s] = 1 + (int32_t)(((uint64_t)r[1] * (uint64_t)dmax) >> 32);
+        if (u < ne) {
+          weight[s] = (float)(r[2] >> 8) * 0x1p-24f;
+          scale[s] = s_e;
+        } else if (u < n) {
+          weight[s] = w_inh;
+          scale[s] = s_i;
+        } else {
+          weight[s] = w_in;
+          scale[s] = s_in;
+        }
+      }
+    }
+  };
+  const int T = std::max(1, std::min(threads, std::max(1, U)));
+  std::vector<std::thread> th;
+  for (int t = 0; t < T; ++t)
+    th.emplace_back(rows, (int)((int64_t)U * t / T), (int)((int64_t)U * (t + 1) / T));
+  for (auto& x : th) x.join();
   return 0;
 }
 

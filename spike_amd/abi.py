@@ -107,6 +107,14 @@ class BrunelParams(C.Structure):
                 ("seed", C.c_uint64)]
 
 
+class OutDegreeParams(C.Structure):
+    """spk_outdegree_params of include/spike_hip.h."""
+    _fields_ = [("seed", C.c_uint64), ("n_exc", C.c_int), ("out_degree", C.c_int),
+                ("input_out_degree", C.c_int), ("delay_max", C.c_int), ("scale_exc", C.c_float),
+                ("scale_inh", C.c_float), ("scale_input", C.c_float), ("weight_inh", C.c_float),
+                ("weight_input", C.c_float), ("ee_stdp", C.c_int), ("stdp", WDParams)]
+
+
 class Stats(C.Structure):
     """spk_stats of include/spike_hip.h."""
     _fields_ = [("rate_hz", C.c_double), ("isi_cv", C.c_double), ("synchrony", C.c_double),
@@ -161,6 +169,8 @@ SIGNATURES = {
     "spk_monitor_reset": (C.c_int, [P, C.c_int]),
     "spk_stats_begin": (C.c_int, [P, C.c_int, C.c_int64]),
     "spk_create_brunel": (C.c_int, [C.POINTER(BrunelParams), C.c_int, C.POINTER(P)]),
+    "spk_set_outdegree_synapses": (C.c_int, [P, C.POINTER(OutDegreeParams)]),
+    "spk_get_rule_size": (C.c_int, [P, C.c_int, C.POINTER(C.c_int64)]),
     "spk_batch_create": (C.c_int, [C.POINTER(P), C.c_int, C.c_int, C.POINTER(P)]),
     "spk_batch_destroy": (C.c_int, [P]),
     "spk_batch_set_cost_model": (C.c_int, [P, C.c_double, C.c_double, C.c_double]),
@@ -282,7 +292,17 @@ class Network:
         if d.decay_g is not None:
             dg, rv = f32(d.decay_g), f32(d.reversal)
             check(lib.spk_set_labels(h, _ptr(dg, C.c_float), _ptr(rv, C.c_float)))
-        if d.n_synapses:
+        if d.outdegree is not None:
+            o = d.outdegree
+            p = OutDegreeParams(seed=int(d.seed), n_exc=o.n_exc, out_degree=o.out_degree,
+                                input_out_degree=o.input_out_degree, delay_max=o.delay_max,
+                                scale_exc=o.scale_exc, scale_inh=o.scale_inh, scale_input=o.scale_input,
+                                weight_inh=o.weight_inh, weight_input=o.weight_input,
+                                ee_stdp=1 if o.ee_stdp else 0)
+            if o.ee_stdp:
+                p.stdp = WDParams(**{("lambda_" if k == "lambda" else k): v for k, v in o.ee_stdp.items()})
+            check(lib.spk_set_outdegree_synapses(h, C.byref(p)))
+        elif d.n_synapses:
             pre, post, dl = i32(d.pre), i32(d.post), i32(d.delay)
             w = f32(d.weight)
             sc = None if d.scale is None else f32(d.scale)
@@ -290,7 +310,7 @@ class Network:
             check(lib.spk_set_synapses(h, d.n_synapses, _ptr(pre, C.c_int32), _ptr(post, C.c_int32),
                                        _ptr(w, C.c_float), _ptr(sc, C.c_float), _ptr(dl, C.c_int32),
                                        _ptr(lb, C.c_int32)))
-        self.rule_ids = []
+        self.rule_ids = [0] if d.outdegree is not None and d.outdegree.ee_stdp else []
         for r in d.rules:
             syn = np.ascontiguousarray(r.synapses, dtype=np.int64)
             rid = C.c_int()
@@ -455,8 +475,13 @@ class Network:
         check(self.lib.spk_get_history(self.h, pop, _ptr(r, C.c_uint32)))
         return r
 
+    def rule_size(self, rule: int) -> int:
+        c = C.c_int64()
+        check(self.lib.spk_get_rule_size(self.h, int(rule), C.byref(c)))
+        return c.value
+
     def traces(self, rule: int):
-        n = len(self.desc.rules[rule].synapses)
+        n = self.rule_size(rule)
         a = np.empty(n, np.float32)
         b = np.empty(n, np.float32)
         check(self.lib.spk_get_traces(self.h, rule, _ptr(a, C.c_float), _ptr(b, C.c_float)))

@@ -1,14 +1,15 @@
 """Synthetic networks of the benchmark configurations, as NetDesc objects.
 
-Connectivity here is drawn with numpy's PCG64 from a fixed seed (the C++ front-end keeps
-the reference's glibc rand() construction order; these builders are for tests and the
-benchmark, where only the network's statistics matter).
+Connectivity here is drawn with numpy's PCG64 from a fixed seed, or, for the large plastic
+network, by the library's Philox fixed out-degree generator on the device (the C++
+front-end keeps the reference's glibc rand() construction order; these builders are for
+tests and the benchmark, where only the network's statistics matter).
 """
 from __future__ import annotations
 
 import numpy as np
 
-from .netdesc import (F32, NetDesc, Rule, RULE_EVANS, RULE_VOGELS, RULE_WD, SYN_CONDUCTANCE, SYN_CURRENT,
+from .netdesc import (F32, NetDesc, OutDegree, Rule, RULE_EVANS, RULE_VOGELS, RULE_WD, SYN_CONDUCTANCE, SYN_CURRENT,
                       SYN_VOLTAGE, conductance_decay, lif_constants)
 
 
@@ -156,19 +157,33 @@ def coba_vogels(n_exc=3200, n_inh=800, p=0.02, seed=42, dt=1e-4, delay_steps=8,
 
 def large_plastic(n=500_000, out_degree=1000, n_inputs=None, input_out_degree=1000, input_rate=10.0,
                   delay_max=20, J=0.1e-3, J_in=2.0e-3, g=5.0, theta=20e-3, vreset=10e-3, tau=20e-3,
-                  tref=2e-3, seed=42, dt=1e-4, plastic=True, monitors=()) -> NetDesc:
+                  tref=2e-3, seed=42, dt=1e-4, plastic=True, monitors=(), generated=True) -> NetDesc:
     """Large plastic network (shrinks with n): 80 % excitatory LIF neurons, each with
     `out_degree` random targets (drawn with replacement), delays U{1..delay_max} steps
     (so G = 1); excitatory weights w ~ U(0, 1) scaled by J, inhibitory ones fixed at 1
     scaled by -g J; n/20 Poisson inputs at `input_rate`, each with `input_out_degree`
     targets, weight 1 scaled by J_in.  Weight-dependent nearest-spike STDP on E->E with
     tau+ 16.8 ms, tau- 33.7 ms, learning rate 0.005, LTD/LTP ratio 1.05 (A- 0.00525 vs
-    A+ 0.005), w in [0, 1]."""
-    rng = np.random.default_rng(seed)
+    A+ 0.005), w in [0, 1].
+
+    generated=True: the synapses are drawn on the device by the Philox fixed out-degree
+    rule (NetDesc.outdegree; no host arrays).  generated=False: host arrays drawn with
+    numpy (another realisation of the same network statistics)."""
     ne = int(round(0.8 * n))
     n_in = n // 20 if n_inputs is None else n_inputs
     d = NetDesc(n_neurons=n, n_inputs=n_in, kind=SYN_VOLTAGE, dt=dt, seed=seed, name="large_plastic")
     _lif(d, n, 0.0, vreset, theta, cm=200e-12, g0=200e-12 / tau, tref=tref)
+    d.rates = np.full((1, n_in), F32(input_rate), F32)
+    d.monitors = list(monitors)
+    stdp = wd_params(a_minus=1.0, a_plus=1.0, tau_minus=33.7e-3, tau_plus=16.8e-3, lam=0.005, alpha=1.05,
+                     w_max=1.0, nearest=1)
+    if generated:
+        d.outdegree = OutDegree(n_exc=ne, out_degree=out_degree, input_out_degree=input_out_degree,
+                                delay_max=delay_max, scale_exc=float(F32(J)), scale_inh=float(F32(-g * J)),
+                                scale_input=float(F32(J_in)), weight_inh=1.0, weight_input=1.0,
+                                ee_stdp=stdp if plastic else None)
+        return d
+    rng = np.random.default_rng(seed)
     pre = np.repeat(np.arange(n, dtype=np.int32), out_degree)
     post = rng.integers(0, n, size=n * out_degree, dtype=np.int64).astype(np.int32)
     ipre = np.repeat(np.arange(n_in, dtype=np.int32), input_out_degree)
@@ -181,12 +196,9 @@ def large_plastic(n=500_000, out_degree=1000, n_inputs=None, input_out_degree=10
     d.weight = np.concatenate([w, np.ones(len(ipre), F32)])
     d.scale = np.concatenate([sc, np.full(len(ipre), F32(J_in), F32)])
     d.delay = rng.integers(1, delay_max + 1, size=len(d.pre), dtype=np.int64).astype(np.int32)
-    d.rates = np.full((1, n_in), F32(input_rate), F32)
     if plastic:
         ee = np.nonzero(exc & (post < ne))[0].astype(np.int64)
-        d.rules.append(Rule(RULE_WD, ee, wd_params(a_minus=1.0, a_plus=1.0, tau_minus=33.7e-3, tau_plus=16.8e-3,
-                                                   lam=0.005, alpha=1.05, w_max=1.0, nearest=1)))
-    d.monitors = list(monitors)
+        d.rules.append(Rule(RULE_WD, ee, stdp))
     return d
 
 

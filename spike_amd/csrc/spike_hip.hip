@@ -111,7 +111,8 @@ using hvec = std::vector<T, NoInit<T>>;
 
 struct RuleHost {
   int kind = 0;  // 0 WD, 1 Vogels, 2 Evans, 3 Custom
-  std::vector<int64_t> syn;
+  std::vector<int64_t> syn;   // caller synapse indices (empty for a rule generated on the device)
+  int64_t cnt = 0;            // synapses of the rule
   spk_wd_stdp_params wd{};
   spk_vogels_stdp_params vg{};
   spk_evans_stdp_params ev{};
@@ -202,6 +203,10 @@ struct spk_network {
   hvec<int32_t> h_pre, h_post, h_delay, h_label;
   hvec<float> h_w, h_scale;
   bool syn_set = false;
+  // fixed out-degree synapses generated on the device by spk_prepare (no host arrays;
+  // caller synapse index = CSR position, rule order = CSR order)
+  bool od = false;
+  spk_outdegree_params odp{};
   std::vector<RuleHost> rules;
   std::vector<MonitorHost> mons;
   // derived sizes
@@ -472,6 +477,82 @@ int refresh_rule_tables(spk_network* n, RuleHost& r) {
   return SPK_OK;
 }
 
+// Device state of the neurons, inputs, spike lists and counters (every prepared network
+// that runs the per-phase kernels).
+int alloc_state(spk_network* n) {
+  int rc;
+  const size_t N = n->N, Nin = n->Nin, L = n->L;
+#define ALLOC(p, cnt) if ((rc = dalloc(&(p), (cnt)))) return rc
+  ALLOC(n->d_V, N); ALLOC(n->d_v0, N); ALLOC(n->d_vreset, N); ALLOC(n->d_thr, N);
+  ALLOC(n->d_c, N); ALLOC(n->d_rp, N); ALLOC(n->d_last_n, N); ALLOC(n->d_last_in, Nin);
+  ALLOC(n->d_hist_n, (size_t)n->W * N); ALLOC(n->d_hist_in, (size_t)n->W * Nin);
+  ALLOC(n->d_ring, (size_t)n->B * N * L); ALLOC(n->d_g, N * L);
+  ALLOC(n->d_decay, L); ALLOC(n->d_erev, L);
+  ALLOC(n->d_in_thr, (size_t)std::max(1, n->n_stim) * Nin);
+  ALLOC(n->d_ext_cdf, std::max<size_t>(1, n->ext_cdf.size()));
+  ALLOC(n->d_lists, (size_t)n->slots * n->G * n->cap);
+  ALLOC(n->d_counts, (size_t)n->slots * n->G);
+  ALLOC(n->d_slot_time, n->slots); ALLOC(n->d_slot_tidx, n->slots);
+  ALLOC(n->d_stats, 4);
+  for (size_t r = 0; r < n->rate_pop.size(); ++r) {
+    const int pop = n->rate_pop[r];
+    const size_t cnt = std::max<size_t>(1, pop == SPK_POP_NEURONS ? N : Nin);
+    if (!n->d_rate[pop]) ALLOC(n->d_rate[pop], cnt);
+    n->rate_base[r].assign(cnt, 0u);
+  }
+  if (n->gen) {
+    ALLOC(n->d_gen_ids, std::max<size_t>(1, n->gen_ids.size()));
+    ALLOC(n->d_gen_times, std::max<size_t>(1, n->gen_times.size()));
+  }
+#undef ALLOC
+  return SPK_OK;
+}
+
+int upload_state(spk_network* n) {
+  int rc;
+  const size_t N = n->N, Nin = n->Nin, L = n->L;
+  hipStream_t st = n->stream;
+  if ((rc = upload(n->d_v0, n->v0.data(), N, st))) return rc;
+  if ((rc = upload(n->d_vreset, n->vreset.data(), N, st))) return rc;
+  if ((rc = upload(n->d_thr, n->thr.data(), N, st))) return rc;
+  if ((rc = upload(n->d_c, n->c.data(), N, st))) return rc;
+  if ((rc = upload(n->d_rp, n->rp.data(), N, st))) return rc;
+  if ((rc = upload(n->d_decay, n->decay_g.data(), L, st))) return rc;
+  if ((rc = upload(n->d_erev, n->erev.data(), L, st))) return rc;
+  for (int pop = 0; pop < 2; ++pop)
+    if (n->d_rate[pop])
+      SPK_HIP(hipMemsetAsync(n->d_rate[pop], 0, sizeof(uint32_t) * std::max<size_t>(1, pop == SPK_POP_NEURONS ? N : Nin), st));
+  if (n->gen) {
+    if ((rc = upload(n->d_gen_ids, n->gen_ids.data(), n->gen_ids.size(), st))) return rc;
+    if ((rc = upload(n->d_gen_times, n->gen_times.data(), n->gen_times.size(), st))) return rc;
+  }
+  {
+    std::vector<uint32_t> thr(n->rates.size());
+    for (size_t i = 0; i < thr.size(); ++i) thr[i] = bernoulli_threshold(n->rates[i], n->dt);
+    if ((rc = upload(n->d_in_thr, thr.data(), thr.size(), st))) return rc;
+  }
+  if ((rc = upload(n->d_ext_cdf, n->ext_cdf.data(), n->ext_cdf.size(), st))) return rc;
+  SPK_HIP(hipMemsetAsync(n->d_counts, 0, sizeof(int) * n->slots * n->G, st));
+  SPK_HIP(hipMemsetAsync(n->d_stats, 0, sizeof(unsigned long long) * 4, st));
+  return SPK_OK;
+}
+
+// Spike monitors: device stores of the requested capacity; waits for the stream.
+int alloc_monitors(spk_network* n) {
+  int rc;
+  for (auto& m : n->mons) {
+    if ((rc = dalloc(&m.d_ids, (size_t)m.capacity))) return rc;
+    if ((rc = dalloc(&m.d_times, (size_t)m.capacity))) return rc;
+    if ((rc = dalloc(&m.d_count, 1))) return rc;
+    SPK_HIP(hipMemsetAsync(m.d_count, 0, sizeof(unsigned long long), n->stream));
+  }
+  SPK_HIP(hipStreamSynchronize(n->stream));
+  return SPK_OK;
+}
+
+// CSR position of caller synapse s (the identity for generated fixed out-degree synapses).
+inline int64_t csr_pos(const spk_network* n, int64_t s) { return n->od ? s : n->perm[(size_t)s]; }
+
 int check_time(spk_network* n, float t) {
   if (!(t >= n->t_hi - 0.5f * n->dt))
     return fail(SPK_ERR_ARG, "group start time precedes the latest group simulated "
@@ -604,7 +685,7 @@ int copy_traces(spk_network* n, RuleHost& r, bool to_neurons) {
   a.syn = traces_of(n, r);
   a.neu = neuron_traces_of(n, r);
   a.pos = r.d_pos; a.pre = r.d_pre; a.tgt = n->d_tgt;
-  a.P = (int64_t)r.syn.size();
+  a.P = r.cnt;
   a.to_neurons = to_neurons ? 1 : 0;
   hipLaunchKernelGGL(spk::k_trace_copy, dim3(grid_for(a.P, 256)), dim3(256), 0, n->stream, a);
   SPK_HIP(hipGetLastError());
@@ -648,7 +729,7 @@ int launch_wd_events(spk_network* n, RuleHost& r, float t) {
 
 int launch_rule(spk_network* n, int rid, float t) {
   RuleHost& r = n->rules[rid];
-  if (r.syn.empty()) return SPK_OK;
+  if (r.cnt == 0) return SPK_OK;
   int rc = refresh_rule_tables(n, r);
   if (rc) return rc;
   if (r.kind == 0 && r.ev_ok && !r.dense_next) return launch_wd_events(n, r, t);
@@ -667,7 +748,7 @@ int launch_rule(spk_network* n, int rid, float t) {
   a.pos = r.d_pos; a.pre = r.d_pre; a.tgt = n->d_tgt; a.delay = n->d_delay; a.wts = weights_of(n);
   a.tr = traces_of(n, r); a.last_n = n->d_last_n; a.last_in = n->d_last_in;
   a.hist_n = n->d_hist_n; a.hist_in = n->d_hist_in; a.tab = r.d_tab; a.dterm = r.dterm;
-  a.P = (int64_t)r.syn.size(); a.N = n->N; a.Nin = n->Nin; a.G = n->G; a.W = n->W;
+  a.P = r.cnt; a.N = n->N; a.Nin = n->Nin; a.G = n->G; a.W = n->W;
   a.tidx = (int)roundf(t / n->dt); a.nearest = r.nearest; a.t = t; a.dt = n->dt;
   const int grid = grid_for(a.P, 256);
   if (r.kind == 0) hipLaunchKernelGGL(spk::k_stdp_wd, dim3(grid), dim3(256), 0, n->stream, a);
@@ -900,7 +981,7 @@ int fill_fused_args(spk_network* n, bool plastic, spk::FusedArgs& a) {
       ra.pos = rh.d_pos; ra.pre = rh.d_pre; ra.tgt = n->d_tgt; ra.delay = n->d_delay; ra.wts = weights_of(n);
       ra.tr = traces_of(n, rh); ra.last_n = n->d_last_n; ra.last_in = n->d_last_in;
       ra.hist_n = n->d_hist_n; ra.hist_in = n->d_hist_in; ra.tab = rh.d_tab; ra.dterm = rh.dterm;
-      ra.P = (int64_t)rh.syn.size(); ra.N = n->N; ra.Nin = n->Nin; ra.G = G; ra.W = n->W;
+      ra.P = rh.cnt; ra.N = n->N; ra.Nin = n->Nin; ra.G = G; ra.W = n->W;
       ra.nearest = rh.nearest; ra.dt = n->dt;
       fr[r].kind = rh.kind;
       fr[r].part = rh.d_part;
@@ -1318,10 +1399,38 @@ int spk_set_synapses(spk_network* n, int64_t S, const int32_t* pre, const int32_
   return SPK_OK;
 }
 
+int spk_set_outdegree_synapses(spk_network* n, const spk_outdegree_params* p) {
+  if (int rc = check_net(n)) return rc;
+  if (!p) return fail(SPK_ERR_ARG, "null parameters");
+  if (n->prepared) return fail(SPK_ERR_STATE, "network already prepared");
+  if (n->syn_set || !n->rules.empty()) return fail(SPK_ERR_STATE, "synapses or rules already set");
+  if (n->L != 1) return fail(SPK_ERR_ARG, "generated synapses carry label 0 of a one-label network");
+  if (p->n_exc < 0 || p->n_exc > n->N) return fail(SPK_ERR_ARG, "n_exc out of range");
+  if (p->out_degree < 0 || p->input_out_degree < 0) return fail(SPK_ERR_ARG, "negative out-degree");
+  if (p->delay_max < 1 || p->delay_max > 65535) return fail(SPK_ERR_ARG, "delay_max must be in [1, 65535] steps");
+  if (n->N < 1 && (p->out_degree > 0 || p->input_out_degree > 0)) return fail(SPK_ERR_ARG, "synapses without target neurons");
+  const int64_t S = (int64_t)n->N * p->out_degree + (int64_t)n->Nin * p->input_out_degree;
+  // the rule's by-target table keeps 32-bit CSR positions
+  if (S >= (int64_t)UINT32_MAX) return fail(SPK_ERR_ARG, "2^32 - 1 or more synapses");
+  n->odp = *p;
+  n->od = true;
+  n->S = S;
+  n->syn_set = true;
+  if (p->ee_stdp) {
+    RuleHost r;
+    r.kind = 0;
+    r.wd = p->stdp;
+    r.nearest = p->stdp.nearest_spike_only;
+    n->rules.push_back(std::move(r));
+  }
+  return SPK_OK;
+}
+
 static int add_rule(spk_network* n, int kind, const int64_t* syn, int64_t cnt, RuleHost&& r,
                     int* rule_id) {
   if (int rc = check_net(n)) return rc;
   if (n->prepared) return fail(SPK_ERR_STATE, "network already prepared");
+  if (n->od) return fail(SPK_ERR_STATE, "synapses generated on the device: their rule comes with spk_set_outdegree_synapses");
   if (cnt < 0 || (cnt > 0 && !syn)) return fail(SPK_ERR_ARG, "bad plastic synapse list");
   // a synapse listed twice would be updated by two lanes at once
   std::vector<bool> seen((size_t)n->S, false);
@@ -1332,6 +1441,7 @@ static int add_rule(spk_network* n, int kind, const int64_t* syn, int64_t cnt, R
   }
   r.kind = kind;
   r.syn.assign(syn, syn + cnt);
+  r.cnt = cnt;
   n->rules.push_back(std::move(r));
   if (rule_id) *rule_id = (int)n->rules.size() - 1;
   return SPK_OK;
@@ -1421,6 +1531,170 @@ int spk_add_rate_monitor(spk_network* n, int population, int* monitor_id) {
   return SPK_OK;
 }
 
+// spk_prepare of a network whose fixed out-degree synapses are generated on the device
+// (spk_set_outdegree_synapses): the delay range, fraction bits and rule tables come from
+// two passes over the same Philox draws; it runs on the per-phase kernels.
+static int prepare_outdegree(spk_network* n, int G) {
+  const spk_outdegree_params& p = n->odp;
+  const int N = n->N, Nin = n->Nin, U = n->nu();
+  const int64_t S = n->S;
+  const bool plastic = !n->rules.empty();
+  hipStream_t st = n->stream;
+  n->G = n->Gprep = G;
+  n->uniform_scale = false;
+  n->scale0 = p.scale_exc;
+  {
+    hipDeviceProp_t prop;
+    SPK_HIP(hipGetDeviceProperties(&prop, n->device));
+    n->cu_count = prop.multiProcessorCount;
+  }
+  n->P = 1;
+  n->h_nlo = {0, N};
+  n->h_ilo = {0, Nin};
+  n->nown_max = N;
+  n->iown_max = Nin;
+  n->fused_ok = false;
+  n->fused_why = "fixed out-degree synapses generated on the device";
+  n->cap = std::max(1, U);
+  n->slots = 1;
+  spk::OutDegreeArgs a{};
+  a.seed = p.seed; a.N = N; a.Nin = Nin; a.ne = p.n_exc; a.K = p.out_degree; a.Kin = p.input_out_degree;
+  a.dmax = p.delay_max; a.scale_e = p.scale_exc; a.scale_i = p.scale_inh; a.scale_in = p.scale_input;
+  a.w_inh = p.weight_inh; a.w_in = p.weight_input;
+  a.plastic = plastic ? 1 : 0;
+  a.cap_plastic = plastic ? std::fabs(p.stdp.w_max) : 0.0f;
+  a.wfmt = n->wfmt;
+  int rc;
+  // pass 1: delay range, per-neuron input bound, rule counts
+  double* d_bound = nullptr;
+  unsigned int *d_post_cnt = nullptr, *d_row_cnt = nullptr, *d_cursor = nullptr;
+  int* d_range = nullptr;
+  auto cleanup = [&]() { dfree(d_bound); dfree(d_post_cnt); dfree(d_row_cnt); dfree(d_cursor); dfree(d_range); };
+  if ((rc = dalloc(&d_bound, (size_t)std::max(1, N))) || (rc = dalloc(&d_post_cnt, (size_t)std::max(1, N))) ||
+      (rc = dalloc(&d_cursor, (size_t)std::max(1, N))) || (rc = dalloc(&d_row_cnt, (size_t)std::max(1, U))) ||
+      (rc = dalloc(&d_range, 2))) {
+    cleanup();
+    return rc;
+  }
+  const int range0[2] = {1 << 30, 0};
+  SPK_HIP(hipMemsetAsync(d_bound, 0, sizeof(double) * std::max(1, N), st));
+  SPK_HIP(hipMemsetAsync(d_post_cnt, 0, sizeof(unsigned int) * std::max(1, N), st));
+  SPK_HIP(hipMemsetAsync(d_cursor, 0, sizeof(unsigned int) * std::max(1, N), st));
+  if ((rc = upload(d_range, range0, 2, st))) {
+    cleanup();
+    return rc;
+  }
+  a.bound = d_bound; a.post_cnt = d_post_cnt; a.row_cnt = d_row_cnt; a.drange = d_range; a.cursor = d_cursor;
+  const int grid = std::max(1, std::min(U, 16384));
+  if (S > 0) {
+    hipLaunchKernelGGL(spk::k_od_count, dim3(grid), dim3(256), 0, st, a);
+    SPK_HIP(hipGetLastError());
+  }
+  std::vector<double> bound(std::max(1, N));
+  std::vector<unsigned int> post_cnt(std::max(1, N)), row_cnt(std::max(1, U));
+  int range[2] = {0, 0};
+  SPK_HIP(hipMemcpyAsync(bound.data(), d_bound, sizeof(double) * bound.size(), hipMemcpyDeviceToHost, st));
+  SPK_HIP(hipMemcpyAsync(post_cnt.data(), d_post_cnt, sizeof(unsigned int) * post_cnt.size(), hipMemcpyDeviceToHost, st));
+  SPK_HIP(hipMemcpyAsync(row_cnt.data(), d_row_cnt, sizeof(unsigned int) * row_cnt.size(), hipMemcpyDeviceToHost, st));
+  SPK_HIP(hipMemcpyAsync(range, d_range, sizeof(range), hipMemcpyDeviceToHost, st));
+  SPK_HIP(hipStreamSynchronize(st));
+  n->dmin = S ? range[0] : 0;
+  n->dmax = S ? range[1] : 0;
+  if (S && G > n->dmin) {
+    cleanup();
+    return fail(SPK_ERR_ARG, "timestep grouping exceeds the minimum delay");
+  }
+  n->B = n->dmax + 2 * G + 1;           // SpikingSynapses.cu:48
+  n->W = (n->dmax + 2 * G) / 32 + 1;
+  double mx = 0.0;
+  for (int i = 0; i < N; ++i) mx = std::max(mx, bound[i]);
+  n->ext_cdf = poisson_count_table((double)n->ext_rate * (double)n->dt);
+  if ((int)n->ext_cdf.size() > spk::kMaxExtTable) {
+    cleanup();
+    return fail(SPK_ERR_ARG, "external drive rate*dt too large for the count table");
+  }
+  if (!n->ext_cdf.empty()) mx += (double)n->ext_cdf.size() * std::fabs((double)n->ext_w);
+  n->frac = choose_frac_bits(mx);
+  n->q_ext = n->ext_cdf.empty() ? 0 : (int64_t)std::llrint(std::ldexp((double)n->ext_w, n->frac));
+  // rule tables: exclusive scans of the counts
+  std::vector<int64_t> pre_ptr((size_t)U + 1, 0), post_ptr((size_t)N + 1, 0);
+  for (int u = 0; u < U; ++u) pre_ptr[u + 1] = pre_ptr[u] + (plastic ? row_cnt[u] : 0u);
+  for (int i = 0; i < N; ++i) post_ptr[i + 1] = post_ptr[i] + (plastic ? post_cnt[i] : 0u);
+  const int64_t PR = pre_ptr[U];
+  if (PR >= (int64_t)INT32_MAX) {
+    cleanup();
+    return fail(SPK_ERR_ARG, "2^31 or more plastic synapses");
+  }
+  // device arrays
+  if ((rc = alloc_state(n))) {
+    cleanup();
+    return rc;
+  }
+  const size_t SS = (size_t)std::max<int64_t>(1, S);
+#define ALLOC(ptr, cnt) if ((rc = dalloc(&(ptr), (cnt)))) { cleanup(); return rc; }
+  ALLOC(n->d_rowptr, (size_t)U + 1); ALLOC(n->d_tgt, SS); ALLOC(n->d_syn, SS);
+  ALLOC(n->d_delay, SS); ALLOC(n->d_scale, SS);
+  if (n->wfmt != SPK_WEIGHTS_FP32) {
+    ALLOC(n->d_w16, SS); ALLOC(n->d_c16, SS);
+  }
+  if (plastic) {
+    RuleHost& r = n->rules[0];
+    const size_t P = (size_t)std::max<int64_t>(1, PR);
+    r.cnt = PR;
+    r.order.clear();
+    r.part = {0, PR};
+    r.ev_ok = PR > 0;
+    ALLOC(r.d_pos, P); ALLOC(r.d_pre, P); ALLOC(r.d_ta, P); ALLOC(r.d_tb, P);
+    if (n->wfmt != SPK_WEIGHTS_FP32) {
+      ALLOC(r.d_ta16, P); ALLOC(r.d_tb16, P);
+    }
+    ALLOC(r.d_part, 2);
+    ALLOC(r.d_ev_pre_ptr, (size_t)U + 1); ALLOC(r.d_ev_post_ptr, (size_t)N + 1);
+    ALLOC(r.d_ev_post_pos, P); ALLOC(r.d_ev_post_pre, P);
+    if (n->wfmt != SPK_WEIGHTS_FP32) {
+      ALLOC(r.d_xn_a16, (size_t)U); ALLOC(r.d_xn_b16, (size_t)N);
+    } else {
+      ALLOC(r.d_xn_a, (size_t)U); ALLOC(r.d_xn_b, (size_t)N);
+    }
+    ALLOC(r.d_xcur_a, (size_t)U); ALLOC(r.d_xcur_b, (size_t)N);
+    ALLOC(r.d_act, (size_t)U + N); ALLOC(r.d_act_cnt, 4);
+  }
+#undef ALLOC
+  std::vector<int64_t> rowptr((size_t)U + 1);
+  for (int u = 0; u <= U; ++u) rowptr[u] = spk::od_row(a, u);
+  if ((rc = upload(n->d_rowptr, rowptr.data(), rowptr.size(), st))) {
+    cleanup();
+    return rc;
+  }
+  if (plastic) {
+    RuleHost& r = n->rules[0];
+    if ((rc = upload(r.d_part, r.part.data(), 2, st)) ||
+        (rc = upload(r.d_ev_pre_ptr, pre_ptr.data(), pre_ptr.size(), st)) ||
+        (rc = upload(r.d_ev_post_ptr, post_ptr.data(), post_ptr.size(), st))) {
+      cleanup();
+      return rc;
+    }
+    a.pre_ptr = r.d_ev_pre_ptr; a.post_ptr = r.d_ev_post_ptr;
+    a.rpos = r.d_pos; a.rpre = r.d_pre; a.epos = r.d_ev_post_pos; a.epre = r.d_ev_post_pre;
+  }
+  // pass 2: the synapse arrays and the rule tables
+  a.tgt = n->d_tgt; a.syn = n->d_syn; a.delay = n->d_delay; a.scale = n->d_scale;
+  a.w16 = n->d_w16; a.c16 = n->d_c16;
+  if (S > 0) {
+    hipLaunchKernelGGL(spk::k_od_fill, dim3(grid), dim3(256), 0, st, a);
+    SPK_HIP(hipGetLastError());
+  }
+  if ((rc = upload_state(n))) {
+    cleanup();
+    return rc;
+  }
+  SPK_HIP(hipStreamSynchronize(st));
+  cleanup();
+  if ((rc = alloc_monitors(n))) return rc;
+  n->prepared = true;
+  return spk_reset_state(n);
+}
+
 int spk_prepare(spk_network* n, int G) {
   if (int rc = check_net(n)) return rc;
   if (n->prepared) return fail(SPK_ERR_STATE, "network already prepared");
@@ -1430,6 +1704,7 @@ int spk_prepare(spk_network* n, int G) {
     n->rates.assign(n->Nin, 0.0f);
     n->n_stim = 1;
   }
+  if (n->od) return prepare_outdegree(n, G);
   const int64_t S = n->S;
   n->G = n->Gprep = G;
   n->dmin = S ? 1 << 30 : 0;
@@ -1659,33 +1934,13 @@ int spk_prepare(spk_network* n, int G) {
   n->slots = 1;
   // allocations
   int rc;
-  const size_t N = n->N, Nin = n->Nin, L = n->L;
+  if ((rc = alloc_state(n))) return rc;
+  const size_t N = n->N;
 #define ALLOC(p, cnt) if ((rc = dalloc(&(p), (cnt)))) return rc
-  ALLOC(n->d_V, N); ALLOC(n->d_v0, N); ALLOC(n->d_vreset, N); ALLOC(n->d_thr, N);
-  ALLOC(n->d_c, N); ALLOC(n->d_rp, N); ALLOC(n->d_last_n, N); ALLOC(n->d_last_in, Nin);
-  ALLOC(n->d_hist_n, (size_t)n->W * N); ALLOC(n->d_hist_in, (size_t)n->W * Nin);
-  ALLOC(n->d_ring, (size_t)n->B * N * L); ALLOC(n->d_g, N * L);
-  ALLOC(n->d_decay, L); ALLOC(n->d_erev, L);
-  ALLOC(n->d_in_thr, (size_t)std::max(1, n->n_stim) * Nin);
-  ALLOC(n->d_ext_cdf, std::max<size_t>(1, n->ext_cdf.size()));
   ALLOC(n->d_rowptr, (size_t)U + 1); ALLOC(n->d_tgt, S); ALLOC(n->d_syn, S);
   ALLOC(n->d_delay, S);
   if (!n->uniform_scale) ALLOC(n->d_scale, S);
   if (n->L > 1) ALLOC(n->d_label, S);
-  ALLOC(n->d_lists, (size_t)n->slots * G * n->cap);
-  ALLOC(n->d_counts, (size_t)n->slots * G);
-  ALLOC(n->d_slot_time, n->slots); ALLOC(n->d_slot_tidx, n->slots);
-  ALLOC(n->d_stats, 4);
-  for (size_t r = 0; r < n->rate_pop.size(); ++r) {
-    const int pop = n->rate_pop[r];
-    const size_t cnt = std::max<size_t>(1, pop == SPK_POP_NEURONS ? N : Nin);
-    if (!n->d_rate[pop]) ALLOC(n->d_rate[pop], cnt);
-    n->rate_base[r].assign(cnt, 0u);
-  }
-  if (n->gen) {
-    ALLOC(n->d_gen_ids, std::max<size_t>(1, n->gen_ids.size()));
-    ALLOC(n->d_gen_times, std::max<size_t>(1, n->gen_times.size()));
-  }
   if (n->fused_ok) {
     ALLOC(n->d_nlo, (size_t)n->P + 1); ALLOC(n->d_ilo, (size_t)n->P + 1);
     ALLOC(n->d_seg, seg.size());
@@ -1694,26 +1949,7 @@ int spk_prepare(spk_network* n, int G) {
   }
 #undef ALLOC
   hipStream_t st = n->stream;
-  if ((rc = upload(n->d_v0, n->v0.data(), N, st))) return rc;
-  if ((rc = upload(n->d_vreset, n->vreset.data(), N, st))) return rc;
-  if ((rc = upload(n->d_thr, n->thr.data(), N, st))) return rc;
-  if ((rc = upload(n->d_c, n->c.data(), N, st))) return rc;
-  if ((rc = upload(n->d_rp, n->rp.data(), N, st))) return rc;
-  if ((rc = upload(n->d_decay, n->decay_g.data(), L, st))) return rc;
-  if ((rc = upload(n->d_erev, n->erev.data(), L, st))) return rc;
-  for (int pop = 0; pop < 2; ++pop)
-    if (n->d_rate[pop])
-      SPK_HIP(hipMemsetAsync(n->d_rate[pop], 0, sizeof(uint32_t) * std::max<size_t>(1, pop == SPK_POP_NEURONS ? N : Nin), st));
-  if (n->gen) {
-    if ((rc = upload(n->d_gen_ids, n->gen_ids.data(), n->gen_ids.size(), st))) return rc;
-    if ((rc = upload(n->d_gen_times, n->gen_times.data(), n->gen_times.size(), st))) return rc;
-  }
-  {
-    std::vector<uint32_t> thr(n->rates.size());
-    for (size_t i = 0; i < thr.size(); ++i) thr[i] = bernoulli_threshold(n->rates[i], n->dt);
-    if ((rc = upload(n->d_in_thr, thr.data(), thr.size(), st))) return rc;
-  }
-  if ((rc = upload(n->d_ext_cdf, n->ext_cdf.data(), n->ext_cdf.size(), st))) return rc;
+  if ((rc = upload_state(n))) return rc;
   if ((rc = upload(n->d_rowptr, rowptr.data(), (size_t)U + 1, st))) return rc;
   if ((rc = upload(n->d_tgt, tgt.data(), S, st))) return rc;
   {
@@ -1754,8 +1990,6 @@ int spk_prepare(spk_network* n, int G) {
     if ((rc = upload(n->d_seg, seg.data(), seg.size(), st))) return rc;
     SPK_HIP(hipMemsetAsync(n->d_xlist, 0, sizeof(uint64_t) * 2 * n->P * spk::fused_xstride(n->xcap), st));
   }
-  SPK_HIP(hipMemsetAsync(n->d_counts, 0, sizeof(int) * n->slots * G, st));
-  SPK_HIP(hipMemsetAsync(n->d_stats, 0, sizeof(unsigned long long) * 4, st));
   // rules: synapses grouped by the partition of their postsynaptic neuron
   for (auto& r : n->rules) {
     const size_t P = r.syn.size();
@@ -1868,14 +2102,7 @@ int spk_prepare(spk_network* n, int G) {
       SPK_HIP(hipStreamSynchronize(st));
     }
   }
-  // monitors: device store of the requested capacity
-  for (auto& m : n->mons) {
-    if ((rc = dalloc(&m.d_ids, (size_t)m.capacity))) return rc;
-    if ((rc = dalloc(&m.d_times, (size_t)m.capacity))) return rc;
-    if ((rc = dalloc(&m.d_count, 1))) return rc;
-    SPK_HIP(hipMemsetAsync(m.d_count, 0, sizeof(unsigned long long), st));
-  }
-  SPK_HIP(hipStreamSynchronize(st));
+  if ((rc = alloc_monitors(n))) return rc;
   // the persistent grid must be co-resident: one workgroup per partition
   if (n->fused_ok) {
     int per_cu = 0;
@@ -1955,7 +2182,7 @@ int spk_reset_state(spk_network* n) {
     SPK_HIP(hipStreamSynchronize(st));
   }
   for (auto& r : n->rules) {
-    const size_t P = r.syn.size();
+    const size_t P = (size_t)r.cnt;
     if (P && r.kind != 4) {   // (weight normalization keeps its initial weights there)
       SPK_HIP(hipMemsetAsync(r.d_ta, 0, sizeof(float) * P, st));
       SPK_HIP(hipMemsetAsync(r.d_tb, 0, sizeof(float) * P, st));
@@ -2259,13 +2486,13 @@ int spk_get_weights(spk_network* n, float* out) {
     std::vector<uint16_t> w16(n->S);
     if (n->S) SPK_HIP(hipMemcpyAsync(w16.data(), n->d_w16, n->S * sizeof(uint16_t), hipMemcpyDeviceToHost, n->stream));
     SPK_HIP(hipStreamSynchronize(n->stream));
-    for (int64_t s = 0; s < n->S; ++s) out[s] = spk::dec16(w16[n->perm[s]], n->wfmt);
+    for (int64_t s = 0; s < n->S; ++s) out[s] = spk::dec16(w16[csr_pos(n, s)], n->wfmt);
     return SPK_OK;
   }
   std::vector<uint2> rec(n->S);
   if (n->S) SPK_HIP(hipMemcpyAsync(rec.data(), n->d_syn, n->S * sizeof(uint2), hipMemcpyDeviceToHost, n->stream));
   SPK_HIP(hipStreamSynchronize(n->stream));
-  for (int64_t s = 0; s < n->S; ++s) std::memcpy(&out[s], &rec[n->perm[s]].y, sizeof(float));
+  for (int64_t s = 0; s < n->S; ++s) std::memcpy(&out[s], &rec[csr_pos(n, s)].y, sizeof(float));
   return SPK_OK;
 }
 
@@ -2275,12 +2502,12 @@ int spk_set_weights(spk_network* n, const float* in) {
   std::vector<uint2> rec(n->S);
   if (n->S) SPK_HIP(hipMemcpyAsync(rec.data(), n->d_syn, n->S * sizeof(uint2), hipMemcpyDeviceToHost, n->stream));
   SPK_HIP(hipStreamSynchronize(n->stream));
-  for (int64_t s = 0; s < n->S; ++s) std::memcpy(&rec[n->perm[s]].y, &in[s], sizeof(float));
+  for (int64_t s = 0; s < n->S; ++s) std::memcpy(&rec[csr_pos(n, s)].y, &in[s], sizeof(float));
   if (n->S) SPK_HIP(hipMemcpyAsync(n->d_syn, rec.data(), n->S * sizeof(uint2), hipMemcpyHostToDevice, n->stream));
   if (n->wfmt != SPK_WEIGHTS_FP32 && n->S) {
     // the 16-bit words of the caller's weights; compensations start again from zero
     std::vector<uint16_t> w16(n->S), c16(n->S, 0);
-    for (int64_t s = 0; s < n->S; ++s) w16[n->perm[s]] = spk::enc16(in[s], n->wfmt);
+    for (int64_t s = 0; s < n->S; ++s) w16[csr_pos(n, s)] = spk::enc16(in[s], n->wfmt);
     SPK_HIP(hipMemcpyAsync(n->d_w16, w16.data(), n->S * sizeof(uint16_t), hipMemcpyHostToDevice, n->stream));
     SPK_HIP(hipMemcpyAsync(n->d_c16, c16.data(), n->S * sizeof(uint16_t), hipMemcpyHostToDevice, n->stream));
   }
@@ -2325,7 +2552,7 @@ int spk_get_traces(spk_network* n, int rid, float* a, float* b) {
   if (int rc = check_listed(n)) return rc;
   if (rid < 0 || rid >= (int)n->rules.size()) return fail(SPK_ERR_ARG, "rule id out of range");
   RuleHost& r = n->rules[rid];
-  const size_t P = r.syn.size();
+  const size_t P = (size_t)r.cnt;
   if (int rc = traces_per_synapse(n, r)) return rc;
   std::vector<float> ta(P), tb(P);
   if (r.kind == 4) {   // weight normalization: its initial weights, and zeros
@@ -2342,10 +2569,19 @@ int spk_get_traces(spk_network* n, int rid, float* a, float* b) {
     if (int rc = download(n, ta.data(), r.d_ta, P)) return rc;
     if (int rc = download(n, tb.data(), r.d_tb, P)) return rc;
   }
-  for (size_t j = 0; j < P; ++j) {
-    a[r.order[j]] = ta[j];
-    b[r.order[j]] = tb[j];
+  for (size_t j = 0; j < P; ++j) {   // device order -> rule order (identity when generated)
+    const size_t o = r.order.empty() ? j : (size_t)r.order[j];
+    a[o] = ta[j];
+    b[o] = tb[j];
   }
+  return SPK_OK;
+}
+
+int spk_get_rule_size(spk_network* n, int rid, int64_t* count) {
+  if (int rc = check_prepared(n)) return rc;
+  if (rid < 0 || rid >= (int)n->rules.size()) return fail(SPK_ERR_ARG, "rule id out of range");
+  if (!count) return fail(SPK_ERR_ARG, "null output");
+  *count = n->rules[rid].cnt;
   return SPK_OK;
 }
 

@@ -45,6 +45,28 @@ class Rule:
 
 
 @dataclasses.dataclass
+class OutDegree:
+    """Fixed out-degree synapses generated on the device (spk_set_outdegree_synapses):
+    every neuron has `out_degree` targets and every input `input_out_degree`, drawn with
+    replacement from Philox (stream 5), delays U{1..delay_max} steps, excitatory weights
+    U[0, 1) in float32 (sources below n_exc), the others weight_inh / weight_input; with
+    `ee_stdp` (weight-dependent STDP parameters) the E->E synapses form rule 0."""
+    n_exc: int
+    out_degree: int
+    input_out_degree: int
+    delay_max: int
+    scale_exc: float
+    scale_inh: float
+    scale_input: float
+    weight_inh: float = 1.0
+    weight_input: float = 1.0
+    ee_stdp: Optional[dict] = None
+
+    def n_synapses(self, n_neurons: int, n_inputs: int) -> int:
+        return n_neurons * self.out_degree + n_inputs * self.input_out_degree
+
+
+@dataclasses.dataclass
 class NetDesc:
     n_neurons: int
     n_inputs: int
@@ -83,14 +105,19 @@ class NetDesc:
     grouping: Optional[int] = None   # default: minimum delay (SpikingModel.cpp:98)
     name: str = "net"
     weight_format: int = 0        # 0 FP32, 1 BF16, 2 FP16 storage of weights and traces
+    outdegree: Optional[OutDegree] = None   # synapses generated on the device instead of pre..label
 
     @property
     def n_synapses(self) -> int:
+        if self.outdegree is not None:
+            return self.outdegree.n_synapses(self.n_neurons, self.n_inputs)
         return 0 if self.pre is None else int(len(self.pre))
 
     def timestep_grouping(self) -> int:
         if self.grouping is not None:
             return int(self.grouping)
+        if self.outdegree is not None:
+            return 1   # generated delays start at one step
         if self.n_synapses == 0:
             return 1
         return int(np.min(self.delay))

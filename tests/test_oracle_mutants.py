@@ -21,6 +21,7 @@ CXXFLAGS = ["-O1", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math",
 
 P = "tests/test_oracle_pinned.py::"
 S = "tests/test_oracle_stdp_pinned.py::"
+D = "tests/test_outdegree_oracle.py::"
 
 # (name, file, original text, planted text, pins that must fail)
 MUTANTS = [
@@ -108,6 +109,26 @@ MUTANTS = [
     ("generator_type_boundary", "spike_oracle.cpp",
      "pre[(size_t)i * C + j] = j < ce ?", "pre[(size_t)i * C + j] = j <= ce ?",
      [P + "test_brunel_connectivity_counts_and_rows"]),
+    ("outdegree_delay_from_target_draw", "spike_oracle.cpp",
+     "delay[s] = 1 + (int32_t)(((uint64_t)r[1] * (uint64_t)dmax) >> 32);",
+     "delay[s] = 1 + (int32_t)(((uint64_t)r[0] * (uint64_t)dmax) >> 32);",
+     [D + "test_rows_recomputed_from_philox", D + "test_target_delay_weight_laws"]),
+    ("outdegree_delay_from_zero", "spike_oracle.cpp",
+     "delay[s] = 1 + (int32_t)(((uint64_t)r[1] * (uint64_t)dmax) >> 32);",
+     "delay[s] = (int32_t)(((uint64_t)r[1] * (uint64_t)dmax) >> 32);",
+     [D + "test_rows_recomputed_from_philox", D + "test_degrees_layout_and_ranges"]),
+    ("outdegree_input_rows_at_neuron_degree", "spike_oracle.cpp",
+     "auto row = [&](int u) { return u <= n ? (int64_t)u * k : (int64_t)n * k + (int64_t)(u - n) * k_in; };",
+     "auto row = [&](int u) { return u <= n ? (int64_t)u * k : (int64_t)n * k_in + (int64_t)(u - n) * k_in; };",
+     [D + "test_rows_recomputed_from_philox"]),
+    ("outdegree_class_boundary", "spike_oracle.cpp",
+     "        if (u < ne) {\n          weight[s] = (float)(r[2] >> 8) * 0x1p-24f;",
+     "        if (u <= ne) {\n          weight[s] = (float)(r[2] >> 8) * 0x1p-24f;",
+     [D + "test_rows_recomputed_from_philox"]),
+    ("outdegree_wrong_stream", "spike_oracle.cpp",
+     "Philox::draws4(seed, (int64_t)j, (uint32_t)u, 5u, r);",
+     "Philox::draws4(seed, (int64_t)j, (uint32_t)u, 3u, r);",
+     [D + "test_rows_recomputed_from_philox"]),
     ("wd_taus_swapped", "spike_oracle.cpp",
      "Em[k] = std::exp(-((float)k * dt) / tm);", "Em[k] = std::exp(-((float)k * dt) / tp);",
      [S + "test_wd_pair_protocol"]),
