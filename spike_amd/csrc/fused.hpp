@@ -83,7 +83,7 @@ struct FusedArgs {
   // synapses
   const uint32_t* seg;      // [U][P+1] absolute segment offsets
   const uint2* syn;         // per synapse {local target | delay << 20 | label << 28, weight bits}
-  const uint16_t* w16;      // 16-bit weights (wfmt 1 bfloat16, 2 binary16) instead of the records'
+  const uint32_t* wc;       // 16-bit weights (wfmt 1 bfloat16, 2 binary16) in the low halves, instead of the records'
   int wfmt;
   const float* scale;       // nullptr: uniform scale0
   // compact form (no plasticity, one label, uniform scale): every synapse of a source
@@ -440,9 +440,9 @@ __device__ __forceinline__ void draw_groups(const FusedArgs& a, int s0, int cnt,
 
 // One quad's share of kSpikesPerQuad spikes: every lane loads its synapses of each spike's
 // segment (lane q takes s0+q, s0+q+4, ...) with all loads issued before the LDS atomics.
-// With 16-bit weights only the packed word of a record is read, plus the 2-byte weight.
+// With 16-bit weights only the packed word of a record is read, plus the weight's 4-byte word.
 __device__ __forceinline__ float syn_weight(const FusedArgs& a, const uint32_t* synw, uint32_t k) {
-  return a.wfmt ? dec16(a.w16[k], a.wfmt) : __uint_as_float(synw[(size_t)k * 2 + 1]);
+  return a.wfmt ? dec16((uint16_t)(a.wc[k] & 0xFFFFu), a.wfmt) : __uint_as_float(synw[(size_t)k * 2 + 1]);
 }
 
 template <bool PER_SYN_SCALE>

@@ -111,8 +111,7 @@ struct OutDegreeArgs {
   uint2* syn;               // {0, weight bits}
   uint16_t* delay;
   float* scale;
-  uint16_t* w16;
-  uint16_t* c16;
+  uint32_t* wc;             // 16-bit weight words (compensation zero)
   int64_t* rpos;            // rule: CSR position, in CSR order
   int32_t* rpre;            //       its source unit
   uint32_t* epos;           // rule by target: CSR position
@@ -201,10 +200,7 @@ __global__ __launch_bounds__(256) void k_od_fill(OutDegreeArgs a) {
         a.syn[k] = make_uint2(0u, __float_as_uint(s.w));
         a.delay[k] = (uint16_t)s.delay;
         a.scale[k] = s.scale;
-        if (a.wfmt) {
-          a.w16[k] = enc16(s.w, a.wfmt);
-          a.c16[k] = 0;
-        }
+        if (a.wfmt) a.wc[k] = enc16(s.w, a.wfmt);
       }
       // rule index: the row's rule synapses before this one, in j order
       const uint64_t m = __ballot(s.plastic);

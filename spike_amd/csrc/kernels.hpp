@@ -86,17 +86,28 @@ __host__ __device__ __forceinline__ float dec16(uint16_t h, int fmt) {
   return fmt == 1 ? bits_f32((uint32_t)h << 16) : f16_to_f32(h);
 }
 
-// Synaptic weights: FP32 in the synapse records (w[k * kWStride]), or, with fmt != 0,
-// 16-bit words w16[k] plus a 16-bit Kahan compensation c16[k] for plastic updates.
+// Synaptic weights: FP32 in the synapse records (w[k * kWStride]), or, with fmt != 0, one
+// 32-bit word per synapse: the 16-bit weight in the low half and its 16-bit Kahan
+// compensation for plastic updates in the high half (one memory line per random update).
 struct Weights {
   float* w;
-  uint16_t* w16;
-  uint16_t* c16;
+  uint32_t* wc;
   int fmt;
 };
 
 __device__ __forceinline__ float wload(const Weights& W, int64_t k) {
-  return W.fmt ? dec16(W.w16[k], W.fmt) : W.w[(size_t)k * kWStride];
+  return W.fmt ? dec16((uint16_t)(W.wc[k] & 0xFFFFu), W.fmt) : W.w[(size_t)k * kWStride];
+}
+
+// The weight and (16-bit formats) its compensation word, in one load.
+__device__ __forceinline__ float wload_c(const Weights& W, int64_t k, uint16_t& c16) {
+  if (!W.fmt) {
+    c16 = 0;
+    return W.w[(size_t)k * kWStride];
+  }
+  const uint32_t word = W.wc[k];
+  c16 = (uint16_t)(word >> 16);
+  return dec16((uint16_t)(word & 0xFFFFu), W.fmt);
 }
 
 // Store a plastic weight that a rule moved from w0 (the stored value) to nw (already
@@ -105,13 +116,16 @@ __device__ __forceinline__ float wload(const Weights& W, int64_t k) {
 // that the compensation would push outside [lo, hi] is clamped and the compensation
 // cleared.  A zero increment is no step at all (the weights a spike-driven rule skips
 // must be those the per-synapse form leaves).
-__device__ __forceinline__ void wupdate(const Weights& W, int64_t k, float w0, float nw, float lo, float hi) {
+// Returns the weight as stored afterwards.  c16 is the synapse's compensation word, when
+// the caller has loaded it already (wupdate loads it otherwise).
+__device__ __forceinline__ float wupdate_c(const Weights& W, int64_t k, float w0, uint16_t c16, float nw,
+                                           float lo, float hi) {
   if (!W.fmt) {
     W.w[(size_t)k * kWStride] = nw;
-    return;
+    return nw;
   }
-  if (nw == w0) return;   // a zero increment leaves the word and its compensation as they are
-  const float c = dec16(W.c16[k], W.fmt);
+  if (nw == w0) return w0;   // a zero increment leaves the word and its compensation as they are
+  const float c = dec16(c16, W.fmt);
   const float d = nw - w0;
   const float y = d - c;
   const float t = w0 + y;
@@ -127,8 +141,14 @@ __device__ __forceinline__ void wupdate(const Weights& W, int64_t k, float w0, f
   } else {
     c1 = enc16((v1 - w0) - y, W.fmt);
   }
-  W.w16[k] = s1;
-  W.c16[k] = c1;
+  W.wc[k] = (uint32_t)s1 | ((uint32_t)c1 << 16);
+  return dec16(s1, W.fmt);
+}
+
+__device__ __forceinline__ float wupdate(const Weights& W, int64_t k, float w0, float nw, float lo, float hi) {
+  if (!W.fmt) return wupdate_c(W, k, w0, 0, nw, lo, hi);
+  if (nw == w0) return w0;
+  return wupdate_c(W, k, w0, (uint16_t)(W.wc[k] >> 16), nw, lo, hi);
 }
 
 // Plasticity traces per plastic synapse: FP32, or 16-bit words of the weight format.
@@ -258,22 +278,25 @@ struct LifArgs {
   uint64_t seed;
   StatsArgs st;
   uint32_t* rate;             // [N] spike counts of a rate monitor, or nullptr
+  int uni;                    // 1: every neuron has the parameters u_* (the arrays are not read)
+  float u_v0, u_vreset, u_thr, u_c, u_rp;
 };
 
+// One lane of the LIF update for neuron i (lanes with i >= N take part in the wavefront's
+// list appends only).  Returns the neuron's last spike time after the group.
 template <int KIND>
-__global__ __launch_bounds__(256) void k_lif(LifArgs a) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ float lif_unit(const LifArgs& a, int i) {
   const bool active = i < a.N;
   const int ii = active ? i : 0;
   float V = a.V[ii];
   float last = a.last[ii];
-  const float cc = a.c[ii];
-  const float cv0 = cc * a.v0[ii];
+  const float cc = a.uni ? a.u_c : a.c[ii];
+  const float cv0 = cc * (a.uni ? a.u_v0 : a.v0[ii]);
   const float omc = 1.0f - cc;
   const float cib = cc * a.ibg;
-  const float thr = a.thr[ii];
-  const float vr = a.vreset[ii];
-  const float rp = a.rp[ii];
+  const float thr = a.uni ? a.u_thr : a.thr[ii];
+  const float vr = a.uni ? a.u_vreset : a.vreset[ii];
+  const float rp = a.uni ? a.u_rp : a.rp[ii];
   const size_t NL = (size_t)a.N * a.L;
   float gl[kMaxLabels];
   if (KIND != 0) {
@@ -384,6 +407,12 @@ __global__ __launch_bounds__(256) void k_lif(LifArgs a) {
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
     if (__lane_id() == 0 && s) atomicAdd(a.ext_events, s);
   }
+  return last;
+}
+
+template <int KIND>
+__global__ __launch_bounds__(256) void k_lif(LifArgs a) {
+  lif_unit<KIND>(a, (int)(blockIdx.x * blockDim.x + threadIdx.x));
 }
 
 // ---------------------------------------------------------------------------------
@@ -403,8 +432,8 @@ struct PoissonArgs {
   uint32_t* rate;          // [Nin] spike counts of a rate monitor, or nullptr
 };
 
-__global__ __launch_bounds__(256) void k_poisson(PoissonArgs a) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+// One lane of the Poisson inputs for input i; returns its last spike time after the group.
+__device__ __forceinline__ float poisson_unit(const PoissonArgs& a, int i) {
   const bool active = i < a.Nin;
   const int ii = active ? i : 0;
   const uint32_t T = a.thr[ii];
@@ -443,6 +472,11 @@ __global__ __launch_bounds__(256) void k_poisson(PoissonArgs a) {
     a.last[i] = last;
     if (a.rate != nullptr && nspk) a.rate[i] += nspk;
   }
+  return last;
+}
+
+__global__ __launch_bounds__(256) void k_poisson(PoissonArgs a) {
+  poisson_unit(a, (int)(blockIdx.x * blockDim.x + threadIdx.x));
 }
 
 // ---------------------------------------------------------------------------------
@@ -711,32 +745,45 @@ __device__ __forceinline__ void wave_push(bool pred, uint32_t id, uint32_t* __re
   if (pred && p < cap) list[p] = id;
 }
 
-__global__ __launch_bounds__(256) void k_wd_neurons(WdEventArgs a) {
-  const int U = a.N + a.Nin;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+// Per-neuron traces of unit u as a presynaptic source (l: its last spike time); lists it
+// (when `push`) if it spiked in the group and has rule synapses.  Wavefront-uniform call.
+__device__ __forceinline__ void wd_pre_unit(const WdEventArgs& a, int u, bool active, float l, bool push) {
+  const int G = a.G;
+  const float* Ep = a.tab + 5 + (G + 1);
+  bool listed = false;
+  if (active) {
+    const int pg = wd_pre_step(l, a.t, a.dt, G);
+    const float x = wd_trace(tload(a.trn, 0, u), pg, a.tab[1], Ep, G, a.nearest);
+    a.xa[u] = x;
+    tstore(a.trn, 0, u, x);
+    listed = pg >= 0 && a.pre_ptr[u + 1] > a.pre_ptr[u];
+  }
+  if (push) wave_push(listed, (uint32_t)u, a.act, a.cnt + 2 * a.par, a.N + a.Nin);
+}
+
+// The same for neuron v as a postsynaptic target (l: its last spike time).
+__device__ __forceinline__ void wd_post_unit(const WdEventArgs& a, int v, bool active, float l) {
   const int G = a.G;
   const float* Em = a.tab + 5;
-  const float* Ep = Em + (G + 1);
   bool listed = false;
-  if (i < U) {
-    const float l = i < a.N ? a.last_n[i] : a.last_in[i - a.N];
-    const int pg = wd_pre_step(l, a.t, a.dt, G);
-    const float x = wd_trace(tload(a.trn, 0, i), pg, a.tab[1], Ep, G, a.nearest);
-    a.xa[i] = x;
-    tstore(a.trn, 0, i, x);
-    listed = pg >= 0 && a.pre_ptr[i + 1] > a.pre_ptr[i];
-  }
-  wave_push(listed, (uint32_t)i, a.act, a.cnt + 2 * a.par, U);
-  listed = false;
-  const int v = i - U;
-  if (v >= 0 && v < a.N) {
-    const int qg = wd_post_step(a.last_n[v], a.t, a.dt, G);
+  if (active) {
+    const int qg = wd_post_step(l, a.t, a.dt, G);
     const float x = wd_trace(tload(a.trn, 1, v), qg, a.tab[0], Em, G, a.nearest);
     a.xb[v] = x;
     tstore(a.trn, 1, v, x);
     listed = qg >= 0 && a.post_ptr[v + 1] > a.post_ptr[v];
   }
-  wave_push(listed, (uint32_t)v, a.act + U, a.cnt + 2 * a.par + 1, a.N);
+  wave_push(listed, (uint32_t)v, a.act + a.N + a.Nin, a.cnt + 2 * a.par + 1, a.N);
+}
+
+__global__ __launch_bounds__(256) void k_wd_neurons(WdEventArgs a) {
+  const int U = a.N + a.Nin;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool pre = i < U;
+  wd_pre_unit(a, i, pre, pre ? (i < a.N ? a.last_n[i] : a.last_in[i - a.N]) : 0.0f, true);
+  const int v = i - U;
+  const bool post = v >= 0 && v < a.N;
+  wd_post_unit(a, v, post, post ? a.last_n[v] : 0.0f);
 }
 
 __global__ __launch_bounds__(256) void k_wd_events(WdEventArgs a) {
@@ -779,6 +826,147 @@ __global__ __launch_bounds__(256) void k_wd_events(WdEventArgs a) {
         wupdate(a.wts, s, w, wd_weight(w, a.xa[u], xpost, pg, qg, a.tab, G), 0.0f, __builtin_inff());
       }
     }
+  }
+}
+
+// ---------------------------------------------------------------------------------
+// One group (G = 1) of the per-phase path in two launches, for networks with Poisson (or
+// no) inputs and at most one rule, a weight-dependent one in its spike-driven form:
+//   k_units   LIF neurons and Poisson inputs (one lane each), the rule's per-neuron traces
+//             and its list of spiking targets; clears the spike counts of the next group's
+//             list slot (so no separate group-start launch is needed);
+//   k_deliver propagation of this group's spikes and both sides of the rule:
+//             - source side: each spiking unit's row is walked once; a synapse the rule
+//               lists (bit of `mask`) is updated first and delivered with its new weight
+//               (the reference applies plasticity before propagation, SpikingModel.cpp:228-233);
+//             - target side: the rule synapses onto each listed spiking neuron whose source
+//               did not spike in the group (those the source side does not reach).
+//             The two sides touch disjoint synapses, and propagation reads only the rows of
+//             this group's spikers, so all of it runs in one launch.
+// Results equal those of the separate kernels (k_begin_group, k_lif, k_poisson,
+// k_wd_neurons, k_wd_events, k_propagate) bit for bit.
+
+struct UnitsArgs {
+  LifArgs lif;
+  PoissonArgs in;
+  WdEventArgs wd;
+  int* next_counts;     // [G] counts of the next group's list slot
+  float* slot_time;     // this group's slot: its start time and step (monitors read them)
+  int* slot_tidx;
+  int slot, nb_neurons; // blocks [0, nb_neurons) run neurons, the rest inputs
+};
+
+template <int KIND, bool WD>
+__global__ __launch_bounds__(256) void k_units(UnitsArgs a) {
+  if (blockIdx.x == 0) {
+    if ((int)threadIdx.x < a.lif.G) a.next_counts[threadIdx.x] = 0;
+    if (threadIdx.x == 0) {
+      a.slot_time[a.slot] = a.lif.t;
+      a.slot_tidx[a.slot] = a.lif.tidx;
+    }
+  }
+  if ((int)blockIdx.x < a.nb_neurons) {
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const float last = lif_unit<KIND>(a.lif, i);
+    if (WD) {
+      const bool on = i < a.lif.N;
+      wd_pre_unit(a.wd, i, on, last, false);
+      wd_post_unit(a.wd, i, on, last);
+    }
+  } else {
+    const int i = (int)((blockIdx.x - a.nb_neurons) * blockDim.x + threadIdx.x);
+    const float last = poisson_unit(a.in, i);
+    if (WD) wd_pre_unit(a.wd, a.in.N + i, i < a.in.Nin, last, false);
+  }
+}
+
+struct DeliverArgs {
+  PropArgs p;
+  WdEventArgs wd;
+  const uint32_t* mask;   // bit s: CSR position s belongs to the rule
+  int row_waves;          // wavefronts per source row (ceil(longest row / 64))
+  int post_waves;         // wavefronts per target list
+};
+
+template <bool WD>
+__global__ __launch_bounds__(256) void k_deliver(DeliverArgs a) {
+  const int lane = (int)__lane_id();
+  const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) / kWave);
+  const int nwaves = (int)((gridDim.x * blockDim.x) / kWave);
+  const PropArgs& p = a.p;
+  const WdEventArgs& r = a.wd;
+  const size_t NL = (size_t)p.N * p.L;
+  const int slot0 = p.tidx % p.B;
+  const int nspk = min(p.counts[0], p.cap);
+  int npost = 0;
+  if (WD) {
+    npost = min(r.cnt[2 * r.par + 1], r.N);
+    // the other buffer is next group's: clear it (nothing reads it in this group)
+    if (blockIdx.x == 0 && threadIdx.x < 2) r.cnt[2 * (r.par ^ 1) + threadIdx.x] = 0;
+  }
+  const int tasks_src = nspk * a.row_waves;
+  const int tasks = tasks_src + npost * a.post_waves;
+  unsigned long long ev = 0;
+  for (int w = wave; w < tasks; w += nwaves) {
+    if (w < tasks_src) {
+      const int j = w / a.row_waves, c = w - j * a.row_waves;
+      const uint32_t u = p.lists[j];
+      const int64_t s0 = p.rowptr[u], s1 = p.rowptr[u + 1];
+      if (lane == 0 && c == 0) ev += (unsigned long long)(s1 - s0);
+      float xpre = 0.0f;
+      int pg = 0;
+      if (WD) {
+        xpre = r.xa[u];
+        pg = wd_pre_step((int)u < r.N ? r.last_n[u] : r.last_in[u - r.N], r.t, r.dt, r.G);
+      }
+      for (int64_t s = s0 + c * kWave + lane; s < s1; s += (int64_t)a.row_waves * kWave) {
+        const int post = p.tgt[s];
+        const int d = p.delay[s];
+        const float sc = p.scale ? p.scale[s] : p.scale0;
+        uint16_t c16;
+        float wt = wload_c(p.wts, s, c16);
+        if (WD && ((a.mask[s >> 5] >> (s & 31)) & 1u)) {
+          const float lpost = r.last_n[post], xpost = r.xb[post];
+          const int qg = wd_post_step(lpost, r.t, r.dt, r.G);
+          wt = wupdate_c(p.wts, s, wt, c16, wd_weight(wt, xpre, xpost, pg, qg, r.tab, r.G), 0.0f,
+                         __builtin_inff());
+        }
+        const long long q = __float2ll_rn(wt * sc * p.two_k);
+        const int slot = (slot0 + d) % p.B;
+        const int l = p.label ? (int)p.label[s] : 0;
+        atomicAdd((unsigned long long*)(p.ring + (size_t)slot * NL + (size_t)post * p.L + l),
+                  (unsigned long long)q);
+      }
+    } else if (WD) {
+      const int k = (w - tasks_src) / a.post_waves, c = (w - tasks_src) - k * a.post_waves;
+      const int v = (int)r.act[r.N + r.Nin + k];
+      const int qg = wd_post_step(r.last_n[v], r.t, r.dt, r.G);
+      const float xpost = r.xb[v];
+      const int64_t e1 = r.post_ptr[v + 1];
+      for (int64_t e = r.post_ptr[v] + c * kWave + lane; e < e1; e += (int64_t)a.post_waves * kWave) {
+        // every load of the update issued before the test (two levels of dependent loads)
+        const int u = r.post_pre[e];
+        const int64_t s = (int64_t)r.post_pos[e];
+        const float l = u < r.N ? r.last_n[u] : r.last_in[u - r.N];
+        const float xu = r.xa[u];
+        uint16_t c16;
+        const float w0 = wload_c(r.wts, s, c16);
+        const int pgu = wd_pre_step(l, r.t, r.dt, r.G);
+        if (pgu >= 0) continue;              // updated from the source side
+        wupdate_c(r.wts, s, w0, c16, wd_weight(w0, xu, xpost, pgu, qg, r.tab, r.G), 0.0f, __builtin_inff());
+      }
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) ev += __shfl_xor(ev, off);
+  if (lane == 0 && ev) atomicAdd(p.events, ev);
+}
+
+// Marks the rule's synapses in a bitmask over CSR positions (k_deliver's source side).
+__global__ __launch_bounds__(256) void k_mark_rule(const int64_t* __restrict__ pos, int64_t P,
+                                                   uint32_t* __restrict__ mask) {
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < P; j += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t s = pos[j];
+    atomicOr(mask + (s >> 5), 1u << (s & 31));
   }
 }
 

@@ -147,6 +147,7 @@ struct RuleHost {
   int64_t* d_ev_post_ptr = nullptr;
   uint32_t* d_ev_post_pos = nullptr;  // CSR position of each synapse, grouped by postsynaptic neuron
   int32_t* d_ev_post_pre = nullptr;   // its presynaptic unit
+  int64_t max_post = 0;               // longest list of one postsynaptic neuron
   float* d_xn_a = nullptr;           // stored per-neuron traces (FP32 or binary16)
   float* d_xn_b = nullptr;
   uint16_t* d_xn_a16 = nullptr;
@@ -186,6 +187,7 @@ struct spk_network {
   std::vector<float> v0, vreset, thr, c, rp;
   float tref = 0.002f, ibg = 0.0f;
   bool lif_set = false;
+  bool lif_uniform = false;   // every neuron has the same parameters
   std::vector<float> rates;
   int n_stim = 0, stim = 0;
   // generator inputs (scheduled spikes) instead of Poisson inputs: per stimulus the
@@ -238,10 +240,13 @@ struct spk_network {
   // targets; sources below csplit weigh cw_lo, the others cw_hi; one delay cdelay
   uint16_t* d_syn16 = nullptr;
   bool compact = false;
+  // per-phase two-launch groups (k_units, k_deliver): bitmask of the CSR positions of a
+  // spike-driven weight-dependent rule, and the longest source row
+  uint32_t* d_wd_mask = nullptr;
+  int64_t max_row = 0;
   // 16-bit weights (spk_set_weight_format): words and Kahan compensations in device order
   int wfmt = SPK_WEIGHTS_FP32;
-  uint16_t* d_w16 = nullptr;
-  uint16_t* d_c16 = nullptr;
+  uint32_t* d_wc = nullptr;   // per synapse: 16-bit weight | compensation << 16
   float cw_lo = 0.0f, cw_hi = 0.0f;
   int csplit = 0, cdelay = 0;
   float* d_scale = nullptr;
@@ -319,8 +324,7 @@ float* weights_view(spk_network* n) { return reinterpret_cast<float*>(n->d_syn) 
 spk::Weights weights_of(spk_network* n) {
   spk::Weights w{};
   w.w = weights_view(n);
-  w.w16 = n->d_w16;
-  w.c16 = n->d_c16;
+  w.wc = n->d_wc;
   w.fmt = n->wfmt;
   return w;
 }
@@ -360,7 +364,7 @@ void free_device(spk_network* n) {
   dfree(n->d_hist_in); dfree(n->d_ring); dfree(n->d_g); dfree(n->d_decay); dfree(n->d_erev);
   dfree(n->d_in_thr); dfree(n->d_ext_cdf); dfree(n->d_gen_ids); dfree(n->d_gen_times);
   dfree(n->d_rate[0]); dfree(n->d_rate[1]); dfree(n->d_rowptr); dfree(n->d_tgt); dfree(n->d_syn); dfree(n->d_syn16);
-  dfree(n->d_w16); dfree(n->d_c16);
+  dfree(n->d_wc);
   dfree(n->d_scale); dfree(n->d_delay); dfree(n->d_label); dfree(n->d_lists);
   dfree(n->d_counts); dfree(n->d_slot_time); dfree(n->d_slot_tidx); dfree(n->d_stats);
   for (auto& r : n->rules) {
@@ -372,7 +376,7 @@ void free_device(spk_network* n) {
     dfree(r.d_xcur_a); dfree(r.d_xcur_b); dfree(r.d_act); dfree(r.d_act_cnt);
     dfree(r.d_wn_sumsq); dfree(r.d_wn_inset); dfree(r.d_wn_ptr); dfree(r.d_wn_idx);
   }
-  dfree(n->d_nlo); dfree(n->d_ilo); dfree(n->d_seg); dfree(n->d_xlist);
+  dfree(n->d_nlo); dfree(n->d_ilo); dfree(n->d_seg); dfree(n->d_xlist); dfree(n->d_wd_mask);
   dfree(n->d_bar); dfree(n->d_err); dfree(n->d_prof); dfree(n->d_rules);
   dfree(n->d_st_cnt); dfree(n->d_st_nisi); dfree(n->d_st_hist); dfree(n->d_st_prev);
   dfree(n->d_st_s1); dfree(n->d_st_s2); dfree(n->d_st_out);
@@ -550,6 +554,21 @@ int alloc_monitors(spk_network* n) {
   return SPK_OK;
 }
 
+// Bitmask of the CSR positions of a network's single spike-driven weight-dependent rule
+// (the source side of k_deliver); other networks keep none.
+int build_wd_mask(spk_network* n) {
+  dfree(n->d_wd_mask);
+  if (n->rules.size() != 1 || n->rules[0].kind != 0 || !n->rules[0].ev_ok || n->rules[0].cnt == 0) return SPK_OK;
+  const size_t words = (size_t)(n->S + 31) / 32;
+  if (int rc = dalloc(&n->d_wd_mask, words)) return rc;
+  SPK_HIP(hipMemsetAsync(n->d_wd_mask, 0, sizeof(uint32_t) * words, n->stream));
+  const int64_t P = n->rules[0].cnt;
+  const int grid = (int)std::min<int64_t>(16384, (P + 255) / 256);
+  hipLaunchKernelGGL(spk::k_mark_rule, dim3(grid), dim3(256), 0, n->stream, n->rules[0].d_pos, P, n->d_wd_mask);
+  SPK_HIP(hipGetLastError());
+  return SPK_OK;
+}
+
 // CSR position of caller synapse s (the identity for generated fixed out-degree synapses).
 inline int64_t csr_pos(const spk_network* n, int64_t s) { return n->od ? s : n->perm[(size_t)s]; }
 
@@ -602,9 +621,12 @@ void stats_advance(spk_network* n, int64_t first, int64_t ng) {
   n->st_steps = std::max(n->st_steps, std::min(end, n->st_max));
 }
 
-int launch_neurons(spk_network* n, float t, int slot) {
-  if (n->N == 0) return SPK_OK;
-  spk::LifArgs a{};
+void fill_lif_args(spk_network* n, float t, int slot, spk::LifArgs& a) {
+  a = spk::LifArgs{};
+  if (n->lif_uniform) {
+    a.uni = 1;
+    a.u_v0 = n->v0[0]; a.u_vreset = n->vreset[0]; a.u_thr = n->thr[0]; a.u_c = n->c[0]; a.u_rp = n->rp[0];
+  }
   a.V = n->d_V; a.last = n->d_last_n; a.hist = n->d_hist_n; a.ring = n->d_ring; a.gcond = n->d_g;
   a.v0 = n->d_v0; a.vreset = n->d_vreset; a.thr = n->d_thr; a.c = n->d_c; a.rp = n->d_rp;
   a.decay_g = n->d_decay; a.erev = n->d_erev; a.ext_cdf = n->d_ext_cdf;
@@ -619,6 +641,12 @@ int launch_neurons(spk_network* n, float t, int slot) {
   a.st = stats_args(n, t);
   a.rate = n->d_rate[SPK_POP_NEURONS];
   stats_advance(n, a.tidx, 1);
+}
+
+int launch_neurons(spk_network* n, float t, int slot) {
+  if (n->N == 0) return SPK_OK;
+  spk::LifArgs a;
+  fill_lif_args(n, t, slot, a);
   const int grid = grid_for(n->N, 256);
   if (n->kind == SPK_SYN_VOLTAGE)
     hipLaunchKernelGGL(spk::k_lif<0>, dim3(grid), dim3(256), 0, n->stream, a);
@@ -652,15 +680,13 @@ int launch_generator(spk_network* n, float t, int slot) {
   return SPK_OK;
 }
 
-int launch_inputs(spk_network* n, float t, int slot) {
-  if (n->Nin == 0) return SPK_OK;
-  if (n->gen) return launch_generator(n, t, slot);
+void fill_poisson_args(spk_network* n, float t, int slot, spk::PoissonArgs& a) {
   const int tidx = (int)roundf(t / n->dt);
   if (n->pending_arm) {
     n->arm_step = tidx;
     n->pending_arm = false;
   }
-  spk::PoissonArgs a{};
+  a = spk::PoissonArgs{};
   a.last = n->d_last_in; a.hist = n->d_hist_in;
   a.thr = n->d_in_thr + (size_t)n->stim * n->Nin;
   a.lists = slot_lists(n, slot); a.counts = slot_counts(n, slot);
@@ -668,6 +694,13 @@ int launch_inputs(spk_network* n, float t, int slot) {
   a.N = n->N; a.Nin = n->Nin; a.G = n->G; a.W = n->W; a.cap = n->cap; a.tidx = tidx;
   a.arm_step = n->arm_step; a.t = t; a.dt = n->dt; a.seed = n->seed;
   a.rate = n->d_rate[SPK_POP_INPUTS];
+}
+
+int launch_inputs(spk_network* n, float t, int slot) {
+  if (n->Nin == 0) return SPK_OK;
+  if (n->gen) return launch_generator(n, t, slot);
+  spk::PoissonArgs a;
+  fill_poisson_args(n, t, slot, a);
   hipLaunchKernelGGL(spk::k_poisson, dim3(grid_for(n->Nin, 256)), dim3(256), 0, n->stream, a);
   SPK_HIP(hipGetLastError());
   return SPK_OK;
@@ -701,12 +734,8 @@ int traces_per_synapse(spk_network* n, RuleHost& r) {
 }
 
 // One group of the weight-dependent rule in its spike-driven form (k_wd_neurons, k_wd_events).
-int launch_wd_events(spk_network* n, RuleHost& r, float t) {
-  if (!r.neu_tr) {
-    if (int rc = copy_traces(n, r, true)) return rc;
-    r.neu_tr = true;
-  }
-  spk::WdEventArgs a{};
+void fill_wd_args(spk_network* n, RuleHost& r, float t, spk::WdEventArgs& a) {
+  a = spk::WdEventArgs{};
   a.trn = neuron_traces_of(n, r);
   a.xa = r.d_xcur_a; a.xb = r.d_xcur_b;
   a.wts = weights_of(n);
@@ -717,6 +746,15 @@ int launch_wd_events(spk_network* n, RuleHost& r, float t) {
   a.act = r.d_act; a.cnt = r.d_act_cnt; a.par = r.par;
   a.N = n->N; a.Nin = n->Nin; a.G = n->G; a.nearest = r.nearest;
   a.t = t; a.dt = n->dt;
+}
+
+int launch_wd_events(spk_network* n, RuleHost& r, float t) {
+  if (!r.neu_tr) {
+    if (int rc = copy_traces(n, r, true)) return rc;
+    r.neu_tr = true;
+  }
+  spk::WdEventArgs a;
+  fill_wd_args(n, r, t, a);
   hipLaunchKernelGGL(spk::k_wd_neurons, dim3(grid_for((int64_t)n->nu() + n->N, 256)), dim3(256), 0,
                      n->stream, a);
   SPK_HIP(hipGetLastError());
@@ -759,9 +797,8 @@ int launch_rule(spk_network* n, int rid, float t) {
   return SPK_OK;
 }
 
-int launch_propagate(spk_network* n, float t, int slot) {
-  if (n->S == 0) return SPK_OK;
-  spk::PropArgs a{};
+void fill_prop_args(spk_network* n, float t, int slot, spk::PropArgs& a) {
+  a = spk::PropArgs{};
   a.lists = slot_lists(n, slot); a.counts = slot_counts(n, slot);
   a.rowptr = n->d_rowptr; a.tgt = n->d_tgt; a.wts = weights_of(n);
   a.scale = n->uniform_scale ? nullptr : n->d_scale;
@@ -770,7 +807,81 @@ int launch_propagate(spk_network* n, float t, int slot) {
   a.scale0 = n->scale0; a.two_k = ldexpf(1.0f, n->frac);
   a.N = n->N; a.L = n->L; a.G = n->G; a.B = n->B; a.cap = n->cap;
   a.tidx = (int)roundf(t / n->dt);
+}
+
+int launch_propagate(spk_network* n, float t, int slot) {
+  if (n->S == 0) return SPK_OK;
+  spk::PropArgs a;
+  fill_prop_args(n, t, slot, a);
   hipLaunchKernelGGL(spk::k_propagate, dim3(1024), dim3(256), 0, n->stream, a);
+  SPK_HIP(hipGetLastError());
+  return SPK_OK;
+}
+
+// Groups of the per-phase path that run as two launches (k_units, k_deliver): G = 1,
+// LIF neurons, Poisson or no inputs, and no rule applied or one weight-dependent rule in
+// its spike-driven form.
+bool fast_group_ok(const spk_network* n, bool plastic) {
+  if (n->G != 1 || n->Gprep != 1 || n->gen || n->N == 0 || n->S == 0 || n->slots < 2) return false;
+  if (n->rules.empty() || !plastic) return true;
+  if (n->rules.size() != 1) return false;
+  const RuleHost& r = n->rules[0];
+  return r.kind == 0 && r.ev_ok && r.cnt > 0 && !r.dense_next && n->d_wd_mask != nullptr;
+}
+
+constexpr int kDeliverBlocks = 4096;
+
+// One group in two launches.  The current slot's spike counts were cleared by the previous
+// group's k_units, or, when `clear_now`, are cleared here by a group-start launch.
+int launch_group_fast(spk_network* n, float t, bool plastic, bool clear_now) {
+  if (clear_now) {
+    if (int rc = begin_group(n, t, true)) return rc;
+  } else {
+    if (int rc = check_time(n, t)) return rc;
+    n->cur_tidx = (int)roundf(t / n->dt);
+    n->cur_slot = (n->cur_tidx / n->G) % n->slots;
+  }
+  const int slot = n->cur_slot;
+  const int next = (n->cur_tidx / n->G + 1) % n->slots;
+  const bool wd = plastic && !n->rules.empty();
+  RuleHost* r = wd ? &n->rules[0] : nullptr;
+  spk::UnitsArgs u{};
+  if (wd) {
+    if (int rc = refresh_rule_tables(n, *r)) return rc;
+    if (!r->neu_tr) {
+      if (int rc = copy_traces(n, *r, true)) return rc;
+      r->neu_tr = true;
+    }
+    fill_wd_args(n, *r, t, u.wd);
+  }
+  fill_lif_args(n, t, slot, u.lif);
+  if (n->Nin > 0) fill_poisson_args(n, t, slot, u.in);
+  u.next_counts = n->d_counts + (size_t)next * n->Gprep;
+  u.slot_time = n->d_slot_time;
+  u.slot_tidx = n->d_slot_tidx;
+  u.slot = slot;
+  u.nb_neurons = grid_for(n->N, 256);
+  const int blocks = u.nb_neurons + (n->Nin > 0 ? grid_for(n->Nin, 256) : 0);
+  const void* fn;
+  if (n->kind == SPK_SYN_VOLTAGE) fn = wd ? (const void*)spk::k_units<0, true> : (const void*)spk::k_units<0, false>;
+  else if (n->kind == SPK_SYN_CONDUCTANCE) fn = wd ? (const void*)spk::k_units<1, true> : (const void*)spk::k_units<1, false>;
+  else fn = wd ? (const void*)spk::k_units<2, true> : (const void*)spk::k_units<2, false>;
+  void* uargs[] = {&u};
+  SPK_HIP(hipLaunchKernel(fn, dim3(blocks), dim3(256), uargs, 0, n->stream));
+  spk::DeliverArgs d{};
+  fill_prop_args(n, t, slot, d.p);
+  d.row_waves = (int)std::min<int64_t>(64, std::max<int64_t>(1, (n->max_row + spk::kWave - 1) / spk::kWave));
+  if (wd) {
+    d.wd = u.wd;
+    d.mask = n->d_wd_mask;
+    d.post_waves = (int)std::min<int64_t>(64, std::max<int64_t>(1, (r->max_post + spk::kWave - 1) / spk::kWave));
+    hipLaunchKernelGGL(spk::k_deliver<true>, dim3(kDeliverBlocks), dim3(256), 0, n->stream, d);
+    r->par ^= 1;
+    r->syn_tr = false;
+  } else {
+    d.post_waves = 1;
+    hipLaunchKernelGGL(spk::k_deliver<false>, dim3(kDeliverBlocks), dim3(256), 0, n->stream, d);
+  }
   SPK_HIP(hipGetLastError());
   return SPK_OK;
 }
@@ -956,7 +1067,7 @@ int fill_fused_args(spk_network* n, bool plastic, spk::FusedArgs& a) {
   a.last_in = n->d_last_in; a.hist_in = n->d_hist_in;
   a.in_thr = n->d_in_thr + (size_t)n->stim * n->Nin;
   a.seg = n->d_seg; a.syn = n->d_syn;
-  a.w16 = n->d_w16; a.wfmt = n->wfmt;
+  a.wc = n->d_wc; a.wfmt = n->wfmt;
   a.scale = n->uniform_scale ? nullptr : n->d_scale;
   if (n->compact && n->rules.empty()) {
     a.syn16 = n->d_syn16;
@@ -1247,6 +1358,13 @@ int spk_set_lif(spk_network* n, const float* v0, const float* vreset, const floa
   n->tref = refractory_s;
   n->ibg = background_current;
   n->lif_set = true;
+  // one parameter set for every neuron (bit for bit): the kernels take it as scalars
+  auto same = [](const std::vector<float>& v) {
+    for (size_t i = 1; i < v.size(); ++i)
+      if (std::memcmp(&v[i], &v[0], sizeof(float)) != 0) return false;
+    return true;
+  };
+  n->lif_uniform = n->N > 0 && same(n->v0) && same(n->vreset) && same(n->thr) && same(n->c) && same(n->rp);
   return SPK_OK;
 }
 
@@ -1556,7 +1674,7 @@ static int prepare_outdegree(spk_network* n, int G) {
   n->fused_ok = false;
   n->fused_why = "fixed out-degree synapses generated on the device";
   n->cap = std::max(1, U);
-  n->slots = 1;
+  n->slots = 2;
   spk::OutDegreeArgs a{};
   a.seed = p.seed; a.N = N; a.Nin = Nin; a.ne = p.n_exc; a.K = p.out_degree; a.Kin = p.input_out_degree;
   a.dmax = p.delay_max; a.scale_e = p.scale_exc; a.scale_i = p.scale_inh; a.scale_in = p.scale_input;
@@ -1635,7 +1753,7 @@ static int prepare_outdegree(spk_network* n, int G) {
   ALLOC(n->d_rowptr, (size_t)U + 1); ALLOC(n->d_tgt, SS); ALLOC(n->d_syn, SS);
   ALLOC(n->d_delay, SS); ALLOC(n->d_scale, SS);
   if (n->wfmt != SPK_WEIGHTS_FP32) {
-    ALLOC(n->d_w16, SS); ALLOC(n->d_c16, SS);
+    ALLOC(n->d_wc, SS);
   }
   if (plastic) {
     RuleHost& r = n->rules[0];
@@ -1644,6 +1762,8 @@ static int prepare_outdegree(spk_network* n, int G) {
     r.order.clear();
     r.part = {0, PR};
     r.ev_ok = PR > 0;
+    r.max_post = 0;
+    for (int i = 0; i < N; ++i) r.max_post = std::max<int64_t>(r.max_post, post_cnt[i]);
     ALLOC(r.d_pos, P); ALLOC(r.d_pre, P); ALLOC(r.d_ta, P); ALLOC(r.d_tb, P);
     if (n->wfmt != SPK_WEIGHTS_FP32) {
       ALLOC(r.d_ta16, P); ALLOC(r.d_tb16, P);
@@ -1679,10 +1799,15 @@ static int prepare_outdegree(spk_network* n, int G) {
   }
   // pass 2: the synapse arrays and the rule tables
   a.tgt = n->d_tgt; a.syn = n->d_syn; a.delay = n->d_delay; a.scale = n->d_scale;
-  a.w16 = n->d_w16; a.c16 = n->d_c16;
+  a.wc = n->d_wc;
   if (S > 0) {
     hipLaunchKernelGGL(spk::k_od_fill, dim3(grid), dim3(256), 0, st, a);
     SPK_HIP(hipGetLastError());
+  }
+  n->max_row = S > 0 ? std::max(N > 0 ? p.out_degree : 0, Nin > 0 ? p.input_out_degree : 0) : 0;
+  if ((rc = build_wd_mask(n))) {
+    cleanup();
+    return rc;
   }
   if ((rc = upload_state(n))) {
     cleanup();
@@ -1797,6 +1922,8 @@ int spk_prepare(spk_network* n, int G) {
   hvec<int64_t> order(S);
   counting_sort(S, U, T, [&](int64_t s) { return uid(n->h_pre[s]); },
                 [&](int64_t s, int64_t k) { order[k] = s; }, rowptr);
+  n->max_row = 0;
+  for (int u = 0; u < U; ++u) n->max_row = std::max(n->max_row, rowptr[u + 1] - rowptr[u]);
   // ---- eligibility of the persistent kernel ----
   n->fused_ok = true;
   n->fused_why.clear();
@@ -1929,9 +2056,10 @@ int spk_prepare(spk_network* n, int G) {
   }
   if (int rc = plan_lds_table(n, seg.data(), U)) return rc;
   choose_dahead(n);
-  // list capacity: every neuron of both populations may spike once per sub-step
+  // list capacity: every neuron of both populations may spike once per sub-step; two
+  // slots, so that a group can clear the next group's counts (k_units)
   n->cap = std::max(1, U);
-  n->slots = 1;
+  n->slots = 2;
   // allocations
   int rc;
   if ((rc = alloc_state(n))) return rc;
@@ -1964,13 +2092,12 @@ int spk_prepare(spk_network* n, int G) {
   }
   if (n->wfmt != SPK_WEIGHTS_FP32) {
     // 16-bit weights of the records' weights, compensations zero
-    hvec<uint16_t> w16(S);
-    std::vector<uint16_t> c16(S, 0);
+    hvec<uint32_t> wc(S);
     host_chunks(S, T, [&](int, int64_t a, int64_t b) {
-      for (int64_t k = a; k < b; ++k) w16[k] = spk::enc16(w[k], n->wfmt);
+      for (int64_t k = a; k < b; ++k) wc[k] = spk::enc16(w[k], n->wfmt);
     });
-    if ((rc = dalloc(&n->d_w16, S)) || (rc = dalloc(&n->d_c16, S))) return rc;
-    if ((rc = upload(n->d_w16, w16.data(), S, st)) || (rc = upload(n->d_c16, c16.data(), S, st))) return rc;
+    if ((rc = dalloc(&n->d_wc, S))) return rc;
+    if ((rc = upload(n->d_wc, wc.data(), S, st))) return rc;
   }
   if (n->compact) {
     hvec<uint16_t> t16(S);
@@ -2049,6 +2176,8 @@ int spk_prepare(spk_network* n, int G) {
                       qs[k] = (uint32_t)pos[j];
                       qu[k] = pre[j];
                     }, qp);
+      r.max_post = 0;
+      for (size_t v = 0; v + 1 < qp.size(); ++v) r.max_post = std::max(r.max_post, qp[v + 1] - qp[v]);
       if ((rc = dalloc(&r.d_ev_pre_ptr, (size_t)U + 1))) return rc;
       if ((rc = dalloc(&r.d_ev_post_ptr, (size_t)n->N + 1))) return rc;
       if ((rc = dalloc(&r.d_ev_post_pos, P))) return rc;
@@ -2102,6 +2231,7 @@ int spk_prepare(spk_network* n, int G) {
       SPK_HIP(hipStreamSynchronize(st));
     }
   }
+  if ((rc = build_wd_mask(n))) return rc;
   if ((rc = alloc_monitors(n))) return rc;
   // the persistent grid must be co-resident: one workgroup per partition
   if (n->fused_ok) {
@@ -2259,16 +2389,23 @@ int spk_run(spk_network* n, float start, int64_t ngroups, int flags) {
     }
     const bool plastic = (flags & SPK_RUN_PLASTICITY) != 0;
     int since_drain = 0;
+    bool cleared = false;   // the previous group cleared this group's spike counts
     for (int64_t s = 0; s < ngroups; ++s) {
       const float t = start + (float)(s * n->G) * n->dt;
       int rc;
-      if ((rc = begin_group(n, t, true))) return rc;
-      if ((rc = launch_neurons(n, t, n->cur_slot))) return rc;
-      if ((rc = launch_inputs(n, t, n->cur_slot))) return rc;
-      if (plastic)
-        for (int r = 0; r < (int)n->rules.size(); ++r)
-          if ((rc = launch_rule(n, r, t))) return rc;
-      if ((rc = launch_propagate(n, t, n->cur_slot))) return rc;
+      if (fast_group_ok(n, plastic)) {
+        if ((rc = launch_group_fast(n, t, plastic, !cleared))) return rc;
+        cleared = true;
+      } else {
+        cleared = false;
+        if ((rc = begin_group(n, t, true))) return rc;
+        if ((rc = launch_neurons(n, t, n->cur_slot))) return rc;
+        if ((rc = launch_inputs(n, t, n->cur_slot))) return rc;
+        if (plastic)
+          for (int r = 0; r < (int)n->rules.size(); ++r)
+            if ((rc = launch_rule(n, r, t))) return rc;
+        if ((rc = launch_propagate(n, t, n->cur_slot))) return rc;
+      }
       for (int m = 0; m < (int)n->mons.size(); ++m)
         if ((rc = launch_collect(n, m, n->cur_slot, 1))) return rc;
       if (!n->mons.empty() && ++since_drain == 64) {
@@ -2483,10 +2620,10 @@ int spk_get_weights(spk_network* n, float* out) {
   if (int rc = check_listed(n)) return rc;
   if (!out && n->S) return fail(SPK_ERR_ARG, "null output");
   if (n->wfmt != SPK_WEIGHTS_FP32) {
-    std::vector<uint16_t> w16(n->S);
-    if (n->S) SPK_HIP(hipMemcpyAsync(w16.data(), n->d_w16, n->S * sizeof(uint16_t), hipMemcpyDeviceToHost, n->stream));
+    std::vector<uint32_t> wc(n->S);
+    if (n->S) SPK_HIP(hipMemcpyAsync(wc.data(), n->d_wc, n->S * sizeof(uint32_t), hipMemcpyDeviceToHost, n->stream));
     SPK_HIP(hipStreamSynchronize(n->stream));
-    for (int64_t s = 0; s < n->S; ++s) out[s] = spk::dec16(w16[csr_pos(n, s)], n->wfmt);
+    for (int64_t s = 0; s < n->S; ++s) out[s] = spk::dec16((uint16_t)(wc[csr_pos(n, s)] & 0xFFFFu), n->wfmt);
     return SPK_OK;
   }
   std::vector<uint2> rec(n->S);
@@ -2506,10 +2643,9 @@ int spk_set_weights(spk_network* n, const float* in) {
   if (n->S) SPK_HIP(hipMemcpyAsync(n->d_syn, rec.data(), n->S * sizeof(uint2), hipMemcpyHostToDevice, n->stream));
   if (n->wfmt != SPK_WEIGHTS_FP32 && n->S) {
     // the 16-bit words of the caller's weights; compensations start again from zero
-    std::vector<uint16_t> w16(n->S), c16(n->S, 0);
-    for (int64_t s = 0; s < n->S; ++s) w16[csr_pos(n, s)] = spk::enc16(in[s], n->wfmt);
-    SPK_HIP(hipMemcpyAsync(n->d_w16, w16.data(), n->S * sizeof(uint16_t), hipMemcpyHostToDevice, n->stream));
-    SPK_HIP(hipMemcpyAsync(n->d_c16, c16.data(), n->S * sizeof(uint16_t), hipMemcpyHostToDevice, n->stream));
+    std::vector<uint32_t> wc(n->S);
+    for (int64_t s = 0; s < n->S; ++s) wc[csr_pos(n, s)] = spk::enc16(in[s], n->wfmt);
+    SPK_HIP(hipMemcpyAsync(n->d_wc, wc.data(), n->S * sizeof(uint32_t), hipMemcpyHostToDevice, n->stream));
   }
   SPK_HIP(hipStreamSynchronize(n->stream));
   n->compact = false;   // caller weights: deliveries read the full records from now on

@@ -68,7 +68,7 @@ def test_coba_vogels_first_1000_steps():
 
 
 def test_large_plastic_scaled_first_1000_steps():
-    d = builders.large_plastic(n=5000, out_degree=100, monitors=(0, 1))
+    d = builders.large_plastic(n=5000, out_degree=100, monitors=(0, 1), generated=False)
     _bit_exact_first_steps(d, tag="large plastic / 100")
 
 
@@ -78,7 +78,7 @@ def test_large_plastic_threaded_preparation(launches):
     gathers, the rule's index sorts); the device layout and so every result must be the
     sequential one: bit-exact against the oracle with FP16 weights, on the persistent
     kernel and on the per-phase kernels (whose rule is driven by the spike-indexed arrays)."""
-    d = builders.large_plastic(n=10000, out_degree=170, monitors=(0, 1))
+    d = builders.large_plastic(n=10000, out_degree=170, monitors=(0, 1), generated=False)
     d.weight_format = 2
     assert len(d.rules[0].synapses) > (1 << 20)
     _bit_exact_first_steps(d, steps=300, tag=f"large plastic / 170, threaded preparation, launches={launches}",
