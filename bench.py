@@ -316,9 +316,9 @@ def main():
                     help="external drive nu_ext / nu_thr of the Brunel network (0: no drive, no spikes: the "
                          "persistent kernel's per-group cost with nothing to deliver)")
     ap.add_argument("--path", default="fused", choices=["fused", "launches"])
-    ap.add_argument("--cpu-groups", type=int, default=100,
+    ap.add_argument("--cpu-groups", type=int, default=2000,
                     help="timestep groups of the CPU-baseline sample (15 steps each), after the warm-up")
-    ap.add_argument("--cpu-one-groups", type=int, default=20,
+    ap.add_argument("--cpu-one-groups", type=int, default=200,
                     help="timestep groups of the one-thread CPU-baseline sample")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
