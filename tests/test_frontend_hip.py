@@ -140,7 +140,11 @@ def test_frontend_run_matches_oracle(tmp_path, case, simtime, plastic):
 @pytest.mark.gpu
 def test_brunel_example_unchanged_runs():
     """Examples/Brunel10K.cpp, compiled unchanged against the HIP backend, runs 0.2 s of
-    the 10,000-neuron network with its spike monitor."""
+    the 10,000-neuron network with its spike monitor and prints the number of spikes
+    recorded: exactly the FP32 oracle's count for the same network, rebuilt by the oracle's
+    restatement of the front-end's construction (tests/golden/make_brunel_example_fixture.py,
+    which wrote tests/golden/brunel10k_example_0p2s.json)."""
+    gold = json.load(open(os.path.join(ROOT, "tests", "golden", "brunel10k_example_0p2s.json")))
     prog = os.path.join(ROOT, "oracle", "_ref", "Brunel10K_hip")
     print("Brunel10K_hip --simtime 0.2", flush=True)
     r = subprocess.run([prog, "--simtime", "0.2"], capture_output=True, text=True, timeout=300)
@@ -148,8 +152,7 @@ def test_brunel_example_unchanged_runs():
     lines = [l for l in r.stdout.splitlines() if "Number of Spikes Recorded" in l]
     assert lines, r.stdout[-2000:]
     n = int(lines[-1].split(":")[1])
-    # 10,000 neurons over 0.2 s: a silent or runaway network is a failure
-    assert 0 < n < 10000 * 0.2 * 500
+    assert n == gold["spikes"] > 0
 
 
 @pytest.mark.gpu
