@@ -302,7 +302,7 @@ def large_leg(seconds, log):
         "build_s": round(build_s, 2), "builder": "device (Philox fixed out-degree)", "path": "per-phase kernels",
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "bytes_model": B, "scope": "whole step (all six launches), algorithmic bytes"},
+                     "bytes_model": B, "scope": "whole step (both launches), algorithmic bytes"},
     }
 
 

@@ -132,3 +132,45 @@ def test_coba_vogels_2s_ensemble_within_2pc_of_fp64(fmt):
     assert abs(cv) <= t_cv
     assert abs(wm) <= t_mean
     assert l1 <= t_hist
+
+
+GOLD_LARGE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "large_plastic_fp64_2s.json")
+
+
+@pytest.mark.parametrize("fmt", [0, 2])
+def test_large_plastic_2s_ensemble_within_2pc_of_fp64(fmt):
+    """The large plastic configuration at 1/25 of its size (2e4 neurons with the full-size
+    in-degrees, drive and rule; tests/golden/make_large_fp64.py), 2 s on the GPU with FP32
+    or FP16 (Kahan) weight storage against the FP64 oracle over 4 seeds: population rate,
+    ISI CV, mean E->E weight and the E->E weight histogram over [0, 1], as means over the
+    seeds (single runs of a chaotic network are single realisations)."""
+    with open(GOLD_LARGE) as f:
+        gold = json.load(f)
+    rows = []
+    for seed, g in gold["seeds"].items():
+        d = builders.large_plastic(n=gold["n_neurons"], seed=int(seed), monitors=(0,))
+        d.weight_format = fmt
+        print(f"[configs] large plastic / 25, 2 s, seed {seed}, weights format {fmt}", flush=True)
+        gpu = abi.Network(d)
+        gpu.run(0.0, d.groups_for(gold["seconds"]))
+        ids, times = gpu.take_spikes(0)
+        w = gpu.weights()
+        gpu.close()
+        pre, post, _, _, _ = oracle_py.outdegree_connectivity(d.seed, d.n_neurons, d.n_inputs, d.outdegree, threads=8)
+        ee = (pre >= 0) & (pre < d.outdegree.n_exc) & (post < d.outdegree.n_exc)
+        w = w[ee].astype(np.float64)
+        l1 = hist_l1(histogram(w, np.asarray(g["edges"])), g["hist"])
+        rows.append((len(ids), isi_cv(ids, times, d.n_neurons), w.mean(), l1, g["spikes"], g["cv"], g["w_mean"]))
+        r = rows[-1]
+        print(f"  spikes {r[0]} vs {r[4]} ({(r[0] - r[4]) / r[4]:+.2%}), CV {r[1]:.4f} vs {r[5]:.4f} "
+              f"({(r[1] - r[5]) / r[5]:+.2%}), mean weight {(r[2] - r[6]) / r[6]:+.3%}, histogram L1/2 {l1:.4f}",
+              flush=True)
+    a = np.array(rows)
+    rate, cv, wm = [(a[:, k].mean() - a[:, k + 4].mean()) / a[:, k + 4].mean() for k in range(3)]
+    l1 = a[:, 3].mean()
+    print(f"means over {len(a)} seeds: rate {rate:+.2%}, CV {cv:+.2%}, mean weight {wm:+.3%}, "
+          f"histogram L1/2 {l1:.4f}", flush=True)
+    assert abs(rate) <= 0.02
+    assert abs(cv) <= 0.02
+    assert abs(wm) <= 0.02
+    assert l1 <= 0.02
