@@ -425,6 +425,13 @@ int spk_set_outdegree_synapses(spk_network* net, const spk_outdegree_params* par
  * No reference counterpart (parameter sweeps run one model at a time in the reference). */
 typedef struct spk_batch spk_batch;
 int spk_batch_create(spk_network* const* nets, int n_nets, int workgroups, spk_batch** out);
+/* A batch run is a hipGraph: the first run captures the argument copies (staged in pinned
+ * host memory) and the cooperative launch, and every later run of the same batch replays
+ * it with that run's arguments (start times, group counts, seeds, statistics windows).
+ * on = 0 runs every launch directly instead.  Default on.  No reference counterpart. */
+int spk_batch_set_graph(spk_batch* batch, int on);
+/* Graph captures and replays since the batch was created. */
+int spk_batch_graph_info(const spk_batch* batch, int64_t* captures, int64_t* replays);
 int spk_batch_destroy(spk_batch* batch);
 int spk_batch_set_cost_model(spk_batch* batch, double fixed_us, double per_neuron_us,
                              double per_synapse_us);

@@ -179,6 +179,8 @@ SIGNATURES = {
     "spk_batch_set_costs": (C.c_int, [P, C.POINTER(C.c_double)]),
     "spk_batch_run": (C.c_int, [P, C.POINTER(C.c_float), C.POINTER(C.c_int64), C.c_int]),
     "spk_batch_last_ms": (C.c_int, [P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "spk_batch_set_graph": (C.c_int, [P, C.c_int]),
+    "spk_batch_graph_info": (C.c_int, [P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "spk_stats_get": (C.c_int, [P, C.POINTER(Stats)]),
     "spk_stats_end": (C.c_int, [P]),
     "spk_synchronize": (C.c_int, [P]),
@@ -565,6 +567,16 @@ class Batch:
         c = np.ascontiguousarray(cost_per_group_us, dtype=np.float64)
         self._costs = c
         check(self.lib.spk_batch_set_costs(self.h, _ptr(c, C.c_double)))
+
+    def set_graph(self, on: bool = True):
+        """Run through a captured hipGraph (default) or launch directly."""
+        check(self.lib.spk_batch_set_graph(self.h, 1 if on else 0))
+
+    def graph_info(self):
+        """(captures, replays) of the batch's hipGraph."""
+        c, r = C.c_int64(), C.c_int64()
+        check(self.lib.spk_batch_graph_info(self.h, C.byref(c), C.byref(r)))
+        return c.value, r.value
 
     def run(self, starts, ngroups):
         st = np.ascontiguousarray(starts, dtype=np.float32)

@@ -2926,9 +2926,31 @@ struct spk_batch {
   std::vector<double> cost_per_group_us;   // measured per-network costs (empty: the model)
   double last_ms = 0.0;
   double est_ms = 0.0;
+  // hipGraph of a run: the argument copies (from pinned staging, read when they execute)
+  // and the cooperative launch, captured at the first run and replayed by every later run
+  // of the same shape (kernel, task count, device buffers)
+  bool use_graph = true;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  const void* g_fn = nullptr;
+  size_t g_tasks = 0;
+  const int2* g_dtasks = nullptr;
+  spk::FusedArgs* h_args = nullptr;
+  int2* h_tasks = nullptr;
+  size_t h_tasks_cap = 0;
+  int* h_off = nullptr;
+  int64_t captures = 0, replays = 0;
 };
 
 namespace {
+
+void batch_drop_graph(spk_batch* b) {
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->gexec = nullptr;
+  b->graph = nullptr;
+  b->g_fn = nullptr;
+}
 
 template <int BS>
 const void* batch_kernel_bs(const spk_batch* b) {
@@ -3022,6 +3044,10 @@ int spk_batch_create(spk_network* const* nets, int n_nets, int workgroups, spk_b
 int spk_batch_destroy(spk_batch* b) {
   if (!b) return SPK_OK;
   if (b->stream) (void)hipStreamSynchronize(b->stream);
+  batch_drop_graph(b);
+  if (b->h_args) (void)hipHostFree(b->h_args);
+  if (b->h_tasks) (void)hipHostFree(b->h_tasks);
+  if (b->h_off) (void)hipHostFree(b->h_off);
   dfree(b->d_args);
   dfree(b->d_tasks);
   dfree(b->d_off);
@@ -3127,16 +3153,59 @@ int spk_batch_run(spk_batch* b, const float* starts, const int64_t* ngroups, int
       args[k].prof = n->d_prof;
     }
   }
-  SPK_HIP(hipMemcpyAsync(b->d_tasks, tasks.data(), tasks.size() * sizeof(int2), hipMemcpyHostToDevice, b->stream));
-  SPK_HIP(hipMemcpyAsync(b->d_off, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice, b->stream));
-  SPK_HIP(hipMemcpyAsync(b->d_args, args.data(), args.size() * sizeof(spk::FusedArgs), hipMemcpyHostToDevice, b->stream));
+  // this run's arguments in pinned staging buffers: the copies read them when they execute
+  if (!b->h_args) SPK_HIP(hipHostMalloc((void**)&b->h_args, sizeof(spk::FusedArgs) * (size_t)K));
+  if (!b->h_off) SPK_HIP(hipHostMalloc((void**)&b->h_off, sizeof(int) * ((size_t)b->nb + 1)));
+  if (tasks.size() > b->h_tasks_cap) {
+    if (b->h_tasks) SPK_HIP(hipHostFree(b->h_tasks));
+    b->h_tasks = nullptr;
+    SPK_HIP(hipHostMalloc((void**)&b->h_tasks, sizeof(int2) * tasks.size()));
+    b->h_tasks_cap = tasks.size();
+  }
+  std::memcpy(b->h_args, args.data(), sizeof(spk::FusedArgs) * (size_t)K);
+  std::memcpy(b->h_off, off.data(), sizeof(int) * off.size());
+  std::memcpy(b->h_tasks, tasks.data(), sizeof(int2) * tasks.size());
   const void* fn = batch_kernel(b, b->bs);
   const spk::FusedArgs* pa = b->d_args;
   const int2* pt = b->d_tasks;
   const int* po = b->d_off;
   void* kargs[] = {(void*)&pa, (void*)&pt, (void*)&po};
-  SPK_HIP(hipEventRecord(b->e0, b->stream));
-  SPK_HIP(hipLaunchCooperativeKernel(fn, dim3(b->nb), dim3(b->bs), kargs, (unsigned int)b->lds, b->stream));
+  // the run: three copies and the launch (no memset: a captured one is not allowed here)
+  auto enqueue = [&]() -> hipError_t {
+    hipError_t e = hipMemcpyAsync(b->d_tasks, b->h_tasks, tasks.size() * sizeof(int2), hipMemcpyHostToDevice, b->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(b->d_off, b->h_off, off.size() * sizeof(int), hipMemcpyHostToDevice, b->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(b->d_args, b->h_args, (size_t)K * sizeof(spk::FusedArgs), hipMemcpyHostToDevice, b->stream);
+    if (e == hipSuccess) e = hipLaunchCooperativeKernel(fn, dim3(b->nb), dim3(b->bs), kargs, (unsigned int)b->lds, b->stream);
+    return e;
+  };
+  if (b->use_graph) {
+    if (!b->gexec || b->g_fn != fn || b->g_tasks != tasks.size() || b->g_dtasks != b->d_tasks) {
+      batch_drop_graph(b);
+      SPK_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+      const hipError_t e = enqueue();
+      hipGraph_t g = nullptr;
+      const hipError_t ec = hipStreamEndCapture(b->stream, &g);
+      if (e != hipSuccess || ec != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return fail(SPK_ERR_HIP, std::string("capturing the batch launch: ") +
+                                     hipGetErrorString(e != hipSuccess ? e : ec));
+      }
+      b->graph = g;
+      SPK_HIP(hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0));
+      b->g_fn = fn;
+      b->g_tasks = tasks.size();
+      b->g_dtasks = b->d_tasks;
+      ++b->captures;
+    } else {
+      ++b->replays;
+    }
+    SPK_HIP(hipEventRecord(b->e0, b->stream));
+    SPK_HIP(hipGraphLaunch(b->gexec, b->stream));
+  } else {
+    SPK_HIP(hipEventRecord(b->e0, b->stream));
+    const hipError_t e = enqueue();
+    if (e != hipSuccess) return fail(SPK_ERR_HIP, std::string("batch launch: ") + hipGetErrorString(e));
+  }
   SPK_HIP(hipEventRecord(b->e1, b->stream));
   SPK_HIP(hipEventSynchronize(b->e1));
   float ms = 0.0f;
@@ -3160,6 +3229,23 @@ int spk_batch_run(spk_batch* b, const float* starts, const int64_t* ngroups, int
       return fail(SPK_ERR_HIP, "batched persistent kernel: grid barrier wait exceeded its spin limit");
     }
   }
+  return SPK_OK;
+}
+
+int spk_batch_set_graph(spk_batch* b, int on) {
+  if (!b) return fail(SPK_ERR_ARG, "null batch");
+  b->use_graph = on != 0;
+  if (!b->use_graph) {
+    SPK_HIP(hipStreamSynchronize(b->stream));
+    batch_drop_graph(b);
+  }
+  return SPK_OK;
+}
+
+int spk_batch_graph_info(const spk_batch* b, int64_t* captures, int64_t* replays) {
+  if (!b) return fail(SPK_ERR_ARG, "null batch");
+  if (captures) *captures = b->captures;
+  if (replays) *replays = b->replays;
   return SPK_OK;
 }
 

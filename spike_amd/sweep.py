@@ -224,6 +224,39 @@ def run_batched(points: Sequence[SweepPoint], seconds: float, plan: Optional[Dic
     return r
 
 
+def run_realisations(points: Sequence[SweepPoint], seconds: float, realisations: Sequence[int],
+                     plan: Optional[Dict] = None, workgroups: int = 0, bin_steps: int = 10):
+    """The same networks under several realisations of the external drive (drive_seed):
+    generated once, run as one batch, and for each realisation the drive seed set, the
+    state reset and the batch run again -- every run after the first replays the batch's
+    captured hipGraph.  Returns (rates [points][realisations], device ms per run,
+    (graph captures, replays))."""
+    plan = plan or default_plan(points)
+    order = list(plan["order"])
+    nets = _build_all(points, plan["partitions"], seconds, bin_steps, lds_max=plan.get("lds_max", 0))
+    batch = abi.Batch([nets[k] for k in order], workgroups)
+    if plan.get("cost") is not None:
+        batch.set_costs([plan["cost"][k] for k in order])
+    ng = [points[k].groups(seconds) for k in order]
+    rates = np.zeros((len(points), len(realisations)))
+    ms = []
+    try:
+        for j, r in enumerate(realisations):
+            for p, n in zip(points, nets):
+                abi.check(n.lib.spk_set_seed(n.h, drive_seed(p, r)))
+                n.reset_state()
+                n.stats_begin(bin_steps, p.groups(seconds) * p.delay_steps)
+            m, _ = batch.run(np.zeros(len(order)), ng)
+            ms.append(m)
+            rates[:, j] = _stats(nets, points, seconds)[0]
+        info = batch.graph_info()
+    finally:
+        batch.close()
+        for n in nets:
+            n.close()
+    return rates, ms, info
+
+
 def plan_from_work(points, events_per_group, target_events, neurons_per_wg_max, workgroups,
                    fixed_us, per_event_us, name=None) -> Dict:
     """Tiling from measured work: network k gets enough workgroups that each handles about

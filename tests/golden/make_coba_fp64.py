@@ -1,9 +1,10 @@
 """FP64 oracle statistics of the COBA + Vogels iSTDP network (3200 E + 800 I, p = 0.02,
-2 s) for connectivity seeds 42..49: spike count, mean ISI CV, the final I->E weights'
-mean and 20-bin histogram (edges from their range), and the histogram of the same weights
-rounded to bfloat16.  Writes coba_vogels_fp64_2s.json beside this script.
+2 s, or the configuration's 10 s) for connectivity seeds 42..49: spike count, mean ISI CV,
+the final I->E weights' mean and 20-bin histogram (edges from their range), and the
+histogram of the same weights rounded to bfloat16.  Writes coba_vogels_fp64_<seconds>s.json
+beside this script.
 
-    python tests/golden/make_coba_fp64.py [threads]
+    python tests/golden/make_coba_fp64.py [threads] [seconds]
 """
 import json
 import os
@@ -20,14 +21,13 @@ from chaos_stats import histogram, isi_cv, round_bf16  # noqa: E402
 from spike_amd import builders  # noqa: E402
 
 SEEDS = list(range(42, 50))
-SECONDS = 2.0
 BINS = 20
 
 
-def fp64_entry(seed, threads):
+def fp64_entry(seed, threads, seconds):
     d = builders.coba_vogels(seed=seed, monitors=(0,))
     o = oracle_py.OracleNetwork(d, precision=64, threads=threads)
-    o.run(0.0, d.groups_for(SECONDS))
+    o.run(0.0, d.groups_for(seconds))
     ids, times = o.take_spikes(0)
     w = o.weights()[d.rules[0].synapses]
     o.close()
@@ -39,11 +39,12 @@ def fp64_entry(seed, threads):
 
 def main():
     threads = int(sys.argv[1]) if len(sys.argv) > 1 else 8
-    out = dict(seconds=SECONDS, network="builders.coba_vogels(seed)", seeds={})
+    seconds = float(sys.argv[2]) if len(sys.argv) > 2 else 2.0
+    out = dict(seconds=seconds, network="builders.coba_vogels(seed)", seeds={})
     for seed in SEEDS:
-        out["seeds"][str(seed)] = e = fp64_entry(seed, threads)
+        out["seeds"][str(seed)] = e = fp64_entry(seed, threads, seconds)
         print(seed, e["spikes"], round(e["cv"], 4), flush=True)
-    with open(os.path.join(HERE, "coba_vogels_fp64_2s.json"), "w") as f:
+    with open(os.path.join(HERE, "coba_vogels_fp64_%gs.json" % seconds), "w") as f:
         json.dump(out, f, indent=1)
 
 

@@ -14,5 +14,5 @@ def test_fixture_entry_recomputed():
     with open(os.path.join(HERE, "golden", "coba_vogels_fp64_2s.json")) as f:
         gold = json.load(f)
     assert sorted(int(s) for s in gold["seeds"]) == make_coba_fp64.SEEDS
-    e = json.loads(json.dumps(make_coba_fp64.fp64_entry(45, threads=8)))
+    e = json.loads(json.dumps(make_coba_fp64.fp64_entry(45, threads=8, seconds=2.0)))
     assert e == gold["seeds"]["45"]

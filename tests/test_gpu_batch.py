@@ -118,6 +118,50 @@ def test_batch_equals_individual_runs(workgroups):
         n.close()
 
 
+def test_batch_graph_replays_equal_direct_launches():
+    """A batch run is a captured hipGraph replayed by later runs.  Two copies of the same
+    networks, one batch launching directly and one through its graph, run three times in a
+    row (continuing, uneven group counts) with the host's own work between the runs; every
+    run of the graph batch equals the direct one bit for bit."""
+    pts = small_points(5, seed=3, n_range=(300, 700))
+    part_of = lambda p: max(1, -(-p.n // 160))
+    print("[batch] graph replays vs direct launches, 5 networks", flush=True)
+    nets_d = [sweep.build(p, part_of(p)) for p in pts]
+    nets_g = [sweep.build(p, part_of(p)) for p in pts]
+    bd, bg = abi.Batch(nets_d, 0), abi.Batch(nets_g, 0)
+    bd.set_graph(False)
+    starts = np.zeros(len(pts))
+    for k, ng in enumerate(([40] * 5, [25, 30, 35, 40, 45], [60] * 5)):
+        for nd, ngr, p, g in zip(nets_d, nets_g, pts, ng):
+            nd.stats_begin(10, g * p.delay_steps)
+            ngr.stats_begin(10, g * p.delay_steps)
+        bd.run(starts, ng)
+        bg.run(starts, ng)
+        for j, (nd, ngr) in enumerate(zip(nets_d, nets_g)):
+            sd, sg = nd.stats(), ngr.stats()
+            assert (sd.spikes, sd.isi_cv, sd.synchrony) == (sg.spikes, sg.isi_cv, sg.synchrony), f"run {k} net {j}"
+            assert np.array_equal(nd.membrane(), ngr.membrane()), f"run {k} net {j} membrane"
+            assert np.array_equal(nd.ring(), ngr.ring()), f"run {k} net {j} ring"
+        starts = np.array([float(F32(F32(s) + F32(F32(g * p.delay_steps) * F32(p.dt))))
+                           for s, g, p in zip(starts, ng, pts)])
+    assert bg.graph_info() == (1, 2) and bd.graph_info() == (0, 0)
+    for x in [bd, bg] + nets_d + nets_g:
+        x.close()
+
+
+def test_realisations_reuse_equals_fresh_builds():
+    """sweep.run_realisations (one batch, drive seed set and state reset between runs, the
+    graph replayed) gives exactly the rates of networks built afresh per realisation."""
+    pts = small_points(4, seed=21, n_range=(400, 900))
+    print("[batch] realisations through one batch vs fresh builds", flush=True)
+    rates, ms, info = sweep.run_realisations(pts, 0.1, [0, 2, 5])
+    assert info == (1, 2)
+    for j, r in enumerate([0, 2, 5]):
+        fresh = sweep.run_batched(pts, 0.1, realisation=r)
+        assert np.array_equal(rates[:, j], fresh.rate), f"realisation {r}"
+    assert not np.array_equal(rates[:, 0], rates[:, 1])
+
+
 def test_batched_rates_within_2pc_of_fp64():
     pts = small_points(4, seed=11, n_range=(800, 1600))
     seconds = 1.0
@@ -174,8 +218,8 @@ def test_full_sweep_at_configured_size():
     sub = [pts[k] for k in idx]
     r32 = np.zeros((len(idx), R))
     r32[:, 0] = rb.rate[idx]
-    for r in range(1, R):
-        r32[:, r] = sweep.run_batched(sub, 1.0, None, realisation=r).rate
+    r32[:, 1:], _, info = sweep.run_realisations(sub, 1.0, range(1, R))
+    assert info == (1, R - 2)
     m64 = np.array([np.mean(e["rate_fp64"]) for e in fx["points"]])
     err = np.abs(r32.mean(1) - m64) / m64
     for j, k in enumerate(idx):

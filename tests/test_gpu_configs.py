@@ -99,20 +99,14 @@ GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "coba_
 COBA_TOL = {0: (0.02, 0.02, 0.02, 0.02), 1: (0.03, 0.03, 0.005, 0.10), 2: (0.02, 0.02, 0.02, 0.02)}
 
 
-@pytest.mark.parametrize("fmt", [0, 1, 2])
-def test_coba_vogels_2s_ensemble_within_2pc_of_fp64(fmt):
-    """COBA + iSTDP, 2 s on the GPU with FP32, BF16 (Kahan) or FP16 (Kahan) weight storage
-    against FP64 oracle statistics of the same networks (tests/golden/make_coba_fp64.py),
-    over 8 connectivity seeds: a single 2 s run is one realisation of a chaotic network
-    whose rate and ISI CV move by 1-3 % under a one-ulp perturbation (chaos_stats), so the
-    criterion is on the means over the seeds; per-seed differences are printed."""
-    with open(GOLD) as f:
+def _coba_ensemble(gold_path, fmt):
+    with open(gold_path) as f:
         gold = json.load(f)
     rows = []
     for seed, g in gold["seeds"].items():
         d = builders.coba_vogels(seed=int(seed), monitors=(0,))
         d.weight_format = fmt
-        print(f"[configs] coba + vogels 2 s seed {seed} weights format {fmt}", flush=True)
+        print(f"[configs] coba + vogels {gold['seconds']:g} s seed {seed} weights format {fmt}", flush=True)
         gpu = abi.Network(d)
         gpu.run(0.0, d.groups_for(gold["seconds"]))
         ids, times = gpu.take_spikes(0)
@@ -130,6 +124,32 @@ def test_coba_vogels_2s_ensemble_within_2pc_of_fp64(fmt):
     l1 = a[:, 3].mean()
     print(f"means over {len(a)} seeds: rate {rate:+.2%}, CV {cv:+.2%}, mean weight {wm:+.3%}, "
           f"histogram L1/2 {l1:.4f}", flush=True)
+    return rate, cv, wm, l1
+
+
+@pytest.mark.parametrize("fmt", [0, 1, 2])
+def test_coba_vogels_2s_ensemble_within_2pc_of_fp64(fmt):
+    """COBA + iSTDP, 2 s on the GPU with FP32, BF16 (Kahan) or FP16 (Kahan) weight storage
+    against FP64 oracle statistics of the same networks (tests/golden/make_coba_fp64.py),
+    over 8 connectivity seeds: a single 2 s run is one realisation of a chaotic network
+    whose rate and ISI CV move by 1-3 % under a one-ulp perturbation (chaos_stats), so the
+    criterion is on the means over the seeds; per-seed differences are printed."""
+    rate, cv, wm, l1 = _coba_ensemble(GOLD, fmt)
+    t_rate, t_cv, t_mean, t_hist = COBA_TOL[fmt]
+    assert abs(rate) <= t_rate
+    assert abs(cv) <= t_cv
+    assert abs(wm) <= t_mean
+    assert l1 <= t_hist
+
+
+GOLD10 = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "coba_vogels_fp64_10s.json")
+
+
+@pytest.mark.parametrize("fmt", [0, 1, 2])
+def test_coba_vogels_10s_ensemble_within_2pc_of_fp64(fmt):
+    """The COBA configuration's own duration, 10 s (tests/golden/make_coba_fp64.py 8 10):
+    the same comparison and tolerances as the 2 s runs, over the same 8 seeds."""
+    rate, cv, wm, l1 = _coba_ensemble(GOLD10, fmt)
     t_rate, t_cv, t_mean, t_hist = COBA_TOL[fmt]
     assert abs(rate) <= t_rate
     assert abs(cv) <= t_cv
