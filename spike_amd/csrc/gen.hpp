@@ -1,4 +1,5 @@
-// gen.hpp -- Brunel-type connectivity generated on the device.
+// gen.hpp -- connectivity generated on the device: the Brunel fixed in-degree rule (below)
+// and the fixed out-degree rule of large networks (further down).
 //
 // Fixed in-degree (Brunel 2000; the rule of Examples/UtilityFunctions.hpp:51-82, sources
 // drawn with replacement): every neuron i receives ce synapses from the excitatory

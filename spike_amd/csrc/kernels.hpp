@@ -1,9 +1,10 @@
 // kernels.hpp -- per-phase gfx950 kernels of the Spike backend.
 //
-// One lane per neuron for the neuron phases, one 64-lane wavefront per spiking
-// presynaptic neuron for delivery.  State is structure-of-arrays FP32; the input ring is
-// int64 fixed point (exact, order-independent sums); spike lists are compacted with a
-// wavefront ballot + popcount prefix and one atomic per wavefront.
+// One lane per neuron for the neuron phases; for delivery, each spiking unit's row is
+// shared by several 64-lane wavefronts.  State is structure-of-arrays FP32; the input ring
+// is int64 fixed point (exact, order-independent sums); spike lists are compacted with a
+// wavefront ballot + popcount prefix and one atomic per wavefront.  Networks with G = 1 run
+// a group as two launches (k_units, k_deliver) built from the same per-unit functions.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
