@@ -113,7 +113,9 @@ def sweep_leg(n_points, seconds, cpu_points, threads, log, tune_seed=7):
     nu_ext/nu_thr U(0.9,4), J U(0.05,0.2) mV, delay U{10..20} steps, eps 0.1, seeds
     2024 + index) simulated for `seconds` each, (1) in one persistent batched launch and
     (2) one network per launch.  The batched tiling policy is tuned on a disjoint sample
-    of the same distribution (seed `tune_seed`); the benchmark points then get their plan
+    of the same distribution (seed `tune_seed`; its finalists timed three times each over the
+    full `seconds`);
+    the benchmark points then get their plan
     from a short probe run, whose device time counts in the batched time.  Points/s =
     points / device seconds (HIP events; network generation is reported separately).
 
@@ -128,7 +130,8 @@ def sweep_leg(n_points, seconds, cpu_points, threads, log, tune_seed=7):
     from spike_amd import sweep
     pts = sweep.sample_points(n_points)
     t0 = time.perf_counter()
-    best = sweep.autotune(sweep.sample_points(n_points, seed=tune_seed), log=log)
+    best = sweep.autotune(sweep.sample_points(n_points, seed=tune_seed), log=log, refine_seconds=seconds,
+                          refine_repeats=3)
     tune_s = time.perf_counter() - t0
     plan, probe_ms = sweep.plan_for(pts, best["policy"])
     rb = sweep.run_batched(pts, seconds, plan)

@@ -112,7 +112,9 @@ struct FusedArgs {
   unsigned long long* xlist64;
   unsigned int xtag_base;     // tag of group s of this launch: xtag_base + s + 1
   int xquad;                  // tagged delivery by quads per source (small partitions) or staged
-  int dahead;                 // groups whose random draws are made in one pass (LDS slots)
+  int early;                  // with xquad and no plasticity: wavefront 0 publishes its list right
+                              // after its LIF, the other wavefronts make the next group's draws
+  int dahead;                 // groups whose random draws are made in one pass (LDS slots; 1)
   // two-level grid barrier: counters bar[g * kBarStride] (g < 8), bar[8 * kBarStride]
   // (level 2) and bar[9 * kBarStride] (release generation)
   unsigned int* bar;
@@ -295,7 +297,8 @@ __device__ __forceinline__ void group_time(const FusedArgs& a, int s, float& t, 
 // Random draws of groups s0 .. s0 + cnt - 1: external Poisson counts per (neuron, sub-step)
 // and the spike masks of the Poisson inputs (Bernoulli per sub-step), into LDS slot
 // s % dahead of each group.  Independent of network state, so they are made ahead, once
-// every dahead groups, while the workgroup waits for the other workgroups' spikes.  One work
+// every dahead groups, while the workgroup waits for the other workgroups' spikes (the host
+// sets dahead = 1: one group per pass hides in that wait, eight did not).  One work
 // item per (group, unit, 4-step Philox block) uses all four outputs of its call; a count is
 // the number of thresholds C[k] <= u, scanned upwards from the guide entry of u (`sguide[j]`
 // is the count of u = j * 2^24, a lower bound for every u in [j * 2^24, (j + 1) * 2^24),
@@ -305,13 +308,13 @@ template <int BS>
 __device__ __forceinline__ void draw_groups_flat(const FusedArgs& a, int s0, int cnt, uint8_t* sext,
                                                  unsigned long long* simask, int nlo, int nown, int ilo,
                                                  int iown, unsigned long long& st_ext, const uint32_t* scdf,
-                                                 const uint8_t* sguide) {
+                                                 const uint8_t* sguide, int t0 = 0) {
   const int G = a.G;
-  const int tid = threadIdx.x;
+  const int tid = (int)threadIdx.x - t0;   // threads t0 .. BS-1 draw
   const int nbm = (G + 3) / 4 + 1;   // 4-step blocks a group's G steps can touch
   const int per = nown * nbm;
   if (a.ext_k > 0) {
-    for (int k = tid; k < cnt * per; k += BS) {
+    for (int k = tid; k < cnt * per; k += BS - t0) {
       const int j = k / per;
       const int r0 = k - j * per;
       const int i = r0 / nbm, bb = r0 - i * nbm;
@@ -339,7 +342,7 @@ __device__ __forceinline__ void draw_groups_flat(const FusedArgs& a, int s0, int
   }
   if (iown > 0) {
     const int peri = iown * nbm;
-    for (int k = tid; k < cnt * peri; k += BS) {
+    for (int k = tid; k < cnt * peri; k += BS - t0) {
       const int j = k / peri;
       const int r0 = k - j * peri;
       const int i = r0 / nbm, bb = r0 - i * nbm;
@@ -368,9 +371,9 @@ template <int BS>
 __device__ __forceinline__ void draw_groups_seq(const FusedArgs& a, int s0, int cnt, uint8_t* sext,
                                             unsigned long long* simask, int nlo, int nown, int ilo,
                                             int iown, unsigned long long& st_ext, const uint32_t* scdf,
-                                            const uint8_t* sguide) {
+                                            const uint8_t* sguide, int t0 = 0) {
   const int G = a.G;
-  const int tid = threadIdx.x;
+  const int tid = (int)threadIdx.x - t0;   // threads t0 .. BS-1 draw
   for (int j = 0; j < cnt; ++j) {
     // per group (uniform): its first step and the 4-step blocks its G steps touch
     float t;
@@ -382,29 +385,45 @@ __device__ __forceinline__ void draw_groups_seq(const FusedArgs& a, int s0, int 
     const int slot = (s0 + j) % a.dahead;
     if (a.ext_k > 0) {
       uint8_t* ext = sext + (size_t)slot * a.nown_max * G;
-      for (int k = tid; k < nown * nbk; k += BS) {
+      for (int k = tid; k < nown * nbk; k += BS - t0) {
         // k / nbk by a float product: (k + 1/2) / nbk is at least 1/(2 nbk) from an
         // integer and k < 2^16
         const int i = (int)(((float)k + 0.5f) * inv_nbk);
         const int64_t blk = b0 + (k - i * nbk);
         const u32x4 r = draw_block(a.seed, blk, (uint32_t)(nlo + i), kStreamExternal);
         const int gb = (int)(blk * 4 - (int64_t)tidx);
+        // the four counts' table scans advance together, so that their LDS reads overlap
+        uint32_t u4[4];
+        int n4[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          u4[c] = pick(r, c);
+          n4[c] = sguide[u4[c] >> 24];
+        }
+        bool more = true;
+        while (more) {
+          more = false;
+#pragma unroll
+          for (int c = 0; c < 4; ++c) {
+            if (n4[c] < a.ext_k && u4[c] >= scdf[n4[c]]) {
+              ++n4[c];
+              more = true;
+            }
+          }
+        }
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
           const int g = gb + c;
           if (g >= 0 && g < G) {
-            const uint32_t u = pick(r, c);
-            int n = sguide[u >> 24];
-            while (n < a.ext_k && u >= scdf[n]) ++n;
-            ext[i * G + g] = (uint8_t)n;
-            st_ext += (unsigned long long)n;
+            ext[i * G + g] = (uint8_t)n4[c];
+            st_ext += (unsigned long long)n4[c];
           }
         }
       }
     }
     if (iown > 0) {
       unsigned long long* im = simask + (size_t)slot * a.iown_max;
-      for (int k = tid; k < iown * nbk; k += BS) {
+      for (int k = tid; k < iown * nbk; k += BS - t0) {
         const int i = (int)(((float)k + 0.5f) * inv_nbk);
         const int64_t blk = b0 + (k - i * nbk);
         const u32x4 r = draw_block(a.seed, blk, (uint32_t)(ilo + i), kStreamInput);
@@ -422,20 +441,22 @@ __device__ __forceinline__ void draw_groups_seq(const FusedArgs& a, int s0, int 
   }
 }
 
-// The draws of small partitions (at most four items per thread) take one pass over
-// (group, unit, block) items, so that the groups' Philox calls run side by side; larger
-// ones go one group at a time, with each group's constants computed once and no integer
-// divisions per item.
+// The draws of several groups for small partitions (at most four items per thread) take one
+// pass over (group, unit, block) items, so that the groups' Philox calls run side by side;
+// otherwise one group at a time, with each group's constants computed once and no integer
+// divisions per item -- also for a single group with external drive (its count scans run
+// interleaved there; Poisson inputs alone keep the one-pass form, measured faster).
+// Threads t0 .. BS-1 share the items.
 template <int BS>
 __device__ __forceinline__ void draw_groups(const FusedArgs& a, int s0, int cnt, uint8_t* sext,
                                             unsigned long long* simask, int nlo, int nown, int ilo,
                                             int iown, unsigned long long& st_ext, const uint32_t* scdf,
-                                            const uint8_t* sguide) {
+                                            const uint8_t* sguide, int t0 = 0) {
   const int nbm = (a.G + 3) / 4 + 1;
-  if (cnt * ((a.ext_k > 0 ? nown : 0) + iown) * nbm <= 4 * BS)
-    draw_groups_flat<BS>(a, s0, cnt, sext, simask, nlo, nown, ilo, iown, st_ext, scdf, sguide);
+  if ((cnt > 1 || a.ext_k == 0) && cnt * ((a.ext_k > 0 ? nown : 0) + iown) * nbm <= 4 * (BS - t0))
+    draw_groups_flat<BS>(a, s0, cnt, sext, simask, nlo, nown, ilo, iown, st_ext, scdf, sguide, t0);
   else
-    draw_groups_seq<BS>(a, s0, cnt, sext, simask, nlo, nown, ilo, iown, st_ext, scdf, sguide);
+    draw_groups_seq<BS>(a, s0, cnt, sext, simask, nlo, nown, ilo, iown, st_ext, scdf, sguide, t0);
 }
 
 // One quad's share of kSpikesPerQuad spikes: every lane loads its synapses of each spike's
@@ -1133,6 +1154,8 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const int p, cons
   const float inv_G = 1.0f / (float)G;
   const int bits = W * 32;
   const int nown_waves = ((nown + kWave - 1) / kWave) * kWave;
+  // one wavefront holds every unit of the partition (nown_waves <= kWave)
+  const bool early = !PLASTIC && a.xquad && a.early && BS > kWave;
   // spike counters of the exchange list, one per group parity (the other parity's is
   // cleared while the group runs)
   if (tid < 2) scounter[tid] = 0;
@@ -1247,7 +1270,12 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const int p, cons
     __syncthreads();
     clk.lap(a, 0);
     // -- B2. LIF recurrence, one lane per neuron (LIFSpikingNeurons.cu:115-190) -------
-    if (tid < nown_waves) {
+    if (early && tid >= kWave) {
+      // the next group's random draws on the other wavefronts while wavefront 0 integrates
+      if ((s + 1) % a.dahead == 0 && s + 1 < a.ngroups)
+        draw_groups<BS>(a, s + 1, min(a.dahead, a.ngroups - s - 1), sext, simask, nlo, nown, ilo, iown, st_ext,
+                        scdf, sguide, kWave);
+    } else if (tid < nown_waves || (early && tid < kWave)) {
       unsigned long long smask = 0ull;
       if (own) {
         if (KIND == 0 && G <= kPrefetch) {
@@ -1313,6 +1341,8 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const int p, cons
           }
         }
         clk.lap(a, 9);
+      }
+      if (own) {
         // spike history: clear this group's G bits [b0, b0 + G) of the bit ring and set
         // those of the spikes, one 32-bit word (or the part up to the ring's end) at a time
         {
@@ -1346,7 +1376,37 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const int p, cons
           sst_s1[tid] = d1; sst_s2[tid] = d2;
         }
       }
-      if (smask) {
+      if (early) {
+        // wavefront 0 holds the partition: its spikes' list positions by a wavefront prefix
+        // sum after the inputs' entries of phase A, and the list goes to the exchange block
+        // at once (phase C publishes nothing); after the history update, whose loop holds a
+        // vector-memory counter wait that would otherwise wait for these stores to complete
+        const int c = __popcll(smask);
+        const int incl = wave_inclusive_sum(c);
+        const int base = *scnt;
+        int pos = base + incl - c;
+        const int nxr = base + __builtin_amdgcn_readlane(incl, kWave - 1);
+        unsigned long long* xl = a.xlist64 + ((size_t)par * a.P + p) * a.xstride;
+        const uint32_t tag = a.xtag_base + (uint32_t)s + 1u;
+        st_spk += (unsigned long long)c;
+        unsigned long long mm = smask;
+        while (mm) {
+          const int g = __ffsll((unsigned long long)mm) - 1;
+          mm &= mm - 1ull;
+          if (pos < a.xcap) {
+            const uint32_t e = ((uint32_t)g << 24) | (uint32_t)(nlo + tid);
+            sxbuf[pos] = e;
+            xstore64(xl + 1 + pos, e, tag);
+          }
+          ++pos;
+        }
+        for (int k = lane; k < min(base, a.xcap); k += kWave) xstore64(xl + 1 + k, sxbuf[k], tag);
+        if (lane == 0) {
+          xstore64(xl, (uint32_t)min(nxr, a.xcap), tag);
+          *scnt = nxr;
+        }
+      }
+      if (!early && smask) {
         const int c = __popcll(smask);
         int pos = atomicAdd(scnt, c);
         st_spk += (unsigned long long)c;
@@ -1377,9 +1437,11 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const int p, cons
     if (tid == 0 && nxr > a.xcap) atomicExch(a.error, 2);
     const uint32_t tag = a.xtag_base + (uint32_t)s + 1u;
     if (!PLASTIC) {
-      unsigned long long* xl = a.xlist64 + ((size_t)par * a.P + p) * a.xstride;
-      for (int k = tid; k < nx; k += BS) xstore64(xl + 1 + k, sxbuf[k], tag);
-      if (tid == 0) xstore64(xl, (uint32_t)nx, tag);
+      if (!early) {
+        unsigned long long* xl = a.xlist64 + ((size_t)par * a.P + p) * a.xstride;
+        for (int k = tid; k < nx; k += BS) xstore64(xl + 1 + k, sxbuf[k], tag);
+        if (tid == 0) xstore64(xl, (uint32_t)nx, tag);
+      }
     } else {
       uint32_t* xl = a.xlist + ((size_t)par * a.P + p) * a.xstride;
       for (int k = tid; k < nx; k += BS) store_coherent(xl + 1 + k, sxbuf[k]);
@@ -1419,7 +1481,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const int p, cons
     if (!PLASTIC) {
       clk.lap(a, 2);
       // -- D. the next group's random draws while the other workgroups finish ----------
-      if ((s + 1) % a.dahead == 0 && s + 1 < a.ngroups)
+      if (!early && (s + 1) % a.dahead == 0 && s + 1 < a.ngroups)
         draw_groups<BS>(a, s + 1, min(a.dahead, a.ngroups - s - 1), sext, simask, nlo, nown, ilo, iown, st_ext,
                         scdf, sguide);
       clk.lap(a, 3);

@@ -268,7 +268,6 @@ struct spk_network {
   size_t fused_lds_base = 0;   // the same without the LDS-resident delivery table
   size_t fused_lds_i64 = 0;    // the same with the int64 ring (after caller weights end the compact form)
   bool cring_ok = false;       // compact form: per-type in-degrees below 2^16 (count ring)
-  int dahead = 1;              // groups whose random draws one pass makes (single-network launches)
   int lseg_U = 0, lrec = 0;    // that table: source units, targets per partition (0: off)
   int cu_count = 0;
   std::vector<int> h_nlo, h_ilo;
@@ -1032,27 +1031,6 @@ int plan_lds_table(spk_network* n, const uint32_t* seg, int U) {
   return SPK_OK;
 }
 
-// Groups whose random draws the persistent kernel makes in one pass: up to 8 while their
-// LDS slots fit beside the rest (the draws then leave all but one group in eight).  Called
-// once the LDS layout is otherwise final; adds the slots to fused_lds.
-void choose_dahead(spk_network* n) {
-  n->dahead = 1;
-  if (!n->fused_ok) return;
-  const int cr = (n->compact && n->rules.empty() && n->cring_ok) ? 1 : 0;
-  const size_t base = spk::fused_lds_bytes(n->BL, n->L, n->W, n->G, n->nown_max, n->iown_max, n->xcap, n->P,
-                                           n->lseg_U, n->lrec, cr, 1);
-  const size_t lim = fused_lds_limit(n);
-  for (int d : {8, 4, 2}) {
-    const size_t full = spk::fused_lds_bytes(n->BL, n->L, n->W, n->G, n->nown_max, n->iown_max, n->xcap, n->P,
-                                             n->lseg_U, n->lrec, cr, d);
-    if (n->fused_lds - base + full <= lim) {
-      n->fused_lds += full - base;
-      n->dahead = d;
-      return;
-    }
-  }
-}
-
 // Kernel arguments of the persistent kernel (all but the run position).
 int fill_fused_args(spk_network* n, bool plastic, spk::FusedArgs& a) {
   const int G = n->G;
@@ -1106,10 +1084,15 @@ int fill_fused_args(spk_network* n, bool plastic, spk::FusedArgs& a) {
   a.rules = n->d_rules;
   a.xlist = n->d_xlist; a.bar = n->d_bar; a.error = n->d_err;
   a.xlist64 = (unsigned long long*)n->d_xlist;
-  a.dahead = std::max(1, n->dahead);
+  // random draws one group ahead, made while the workgroup waits at the exchange (passes
+  // of 2, 4 or 8 groups are slower: their longer draws no longer hide in the wait)
+  a.dahead = 1;
   // short lists (partitions of at most 64 neurons): each quad polls and delivers one
   // source's list; longer ones are staged over the whole workgroup
   a.xquad = n->nown_max + n->iown_max <= 64 ? 1 : 0;
+  // one-wavefront partitions publish right after their LIF and draw beside it (Brunel10K
+  // 4.19 -> 4.02 ms per simulated second; the batch sets it off: 3 % slower there)
+  a.early = a.xquad;
   a.n_mons = (int)n->mons.size();
   for (int m = 0; m < a.n_mons; ++m) {
     const MonitorHost& mh = n->mons[m];
@@ -2055,7 +2038,6 @@ int spk_prepare(spk_network* n, int G) {
       n->fused_lds = spk::fused_lds_bytes(n->BL, n->L, n->W, G, n->nown_max, n->iown_max, n->xcap, n->P, 0, 0, 1);
   }
   if (int rc = plan_lds_table(n, seg.data(), U)) return rc;
-  choose_dahead(n);
   // list capacity: every neuron of both populations may spike once per sub-step; two
   // slots, so that a group can clear the next group's counts (k_units)
   n->cap = std::max(1, U);
@@ -2063,7 +2045,6 @@ int spk_prepare(spk_network* n, int G) {
   // allocations
   int rc;
   if ((rc = alloc_state(n))) return rc;
-  const size_t N = n->N;
 #define ALLOC(p, cnt) if ((rc = dalloc(&(p), (cnt)))) return rc
   ALLOC(n->d_rowptr, (size_t)U + 1); ALLOC(n->d_tgt, S); ALLOC(n->d_syn, S);
   ALLOC(n->d_delay, S);
@@ -2651,7 +2632,6 @@ int spk_set_weights(spk_network* n, const float* in) {
   n->compact = false;   // caller weights: deliveries read the full records from now on
   n->cring_ok = false;  // and the LDS ring holds fixed-point sums again
   n->lseg_U = n->lrec = 0;
-  n->dahead = 1;
   if (n->fused_lds_i64) n->fused_lds = n->fused_lds_base = n->fused_lds_i64;
   for (auto& r : n->rules) r.dense_next = true;   // caller weights may lie below 0
   return SPK_OK;
@@ -2886,7 +2866,6 @@ int spk_create_brunel(const spk_brunel_params* p, int partitions, spk_network** 
     cleanup();
     return bail(rc);
   }
-  choose_dahead(n);
   if (S > 0) {
     hipLaunchKernelGGL(spk::k_gen_scatter, dim3(grid), dim3(256), 0, st, ga, S);
     SPK_HIP(hipGetLastError());
@@ -3134,16 +3113,7 @@ int spk_batch_run(spk_batch* b, const float* starts, const int64_t* ngroups, int
     spk_network* n = b->nets[k];
     if (int rc = fill_fused_args(n, false, args[k])) return rc;
     if (!b->lds_tables) args[k].lseg_U = args[k].lrec = 0;
-    // as many groups of draws as the batch's LDS per workgroup holds for this network
-    args[k].dahead = 1;
-    for (int d : {8, 4, 2}) {
-      const spk::FusedArgs& x = args[k];
-      if (spk::fused_lds_bytes(x.BL, x.L, x.W, x.G, x.nown_max, x.iown_max, x.xcap, x.P, x.lseg_U, x.lrec, x.cring,
-                               d) <= b->lds) {
-        args[k].dahead = d;
-        break;
-      }
-    }
+    args[k].early = 0;
     if (int rc = stage_fused(n, args[k], starts[k], 0, (int)ngroups[k], b->stream)) return rc;
     if (n->profiling) {
       if (!n->d_prof) {

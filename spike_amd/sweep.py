@@ -279,15 +279,16 @@ def plan_from_work(points, events_per_group, target_events, neurons_per_wg_max, 
 
 def autotune(points: Sequence[SweepPoint], probe_seconds: float = 0.02,
              targets=(2e4, 5e4, 1e5, 2e5, 5e5), neurons_per_wg_max=(384, 1024, 256), log=None,
-             refine_seconds: float = 0.1, refine_top: int = 3) -> Dict:
+             refine_seconds: float = 0.1, refine_top: int = 3, refine_repeats: int = 1) -> Dict:
     """The project's tiling autotuner.  (1) A probe run of the whole sweep with the
     size-only plan measures every network's deliveries per group and its time per group;
     (2) a least-squares fit of time = fixed + per_event * events/workgroup gives the cost
     model; (3) every candidate (events-per-workgroup target, most neurons per workgroup)
     is turned into a plan (plan_from_work) and timed on a probe run; (4) the refine_top
-    fastest are timed again on refine_seconds (a short probe ranks plans within a few
-    per cent of each other unreliably: their schedules diverge later in the run) and the
-    fastest of those is returned.  Fewer, larger partitions pay the per-group fixed cost fewer times; a
+    fastest are timed again on refine_seconds, refine_repeats times each (a short probe
+    ranks plans within a few per cent of each other unreliably: their schedules diverge
+    later in the run, and one full-length run of a plan varies by about 5 %), and the
+    one with the lowest mean time is returned.  Fewer, larger partitions pay the per-group fixed cost fewer times; a
     partition whose ring slice would not fit in LDS is split further when built.  Plans of
     at most 512 neurons per workgroup also keep every partition's LDS within half a compute
     unit's, so that the batch runs two 512-thread workgroups per compute unit."""
@@ -324,12 +325,14 @@ def autotune(points: Sequence[SweepPoint], probe_seconds: float = 0.02,
         finalists = [pl for _, pl in sorted(timed, key=lambda x: x[0])[:refine_top]]
         best = None
         for plan in finalists:
-            r = run_batched(points, refine_seconds, plan)
-            trials.append(dict(plan=plan["name"], refine_ms=r.ms))
+            runs = [run_batched(points, refine_seconds, plan).ms for _ in range(max(1, refine_repeats))]
+            ms = float(np.mean(runs))
+            trials.append(dict(plan=plan["name"], refine_ms=ms, refine_runs=runs))
             if log:
-                log(f"autotune refine {plan['name']}: {r.ms:.2f} ms over {refine_seconds} s")
-            if best is None or r.ms < best["ms"]:
-                best = dict(plan=plan, ms=r.ms)
+                log(f"autotune refine {plan['name']}: {ms:.2f} ms over {refine_seconds} s "
+                    f"(mean of {len(runs)}: {', '.join(f'{x:.1f}' for x in runs)})")
+            if best is None or ms < best["ms"]:
+                best = dict(plan=plan, ms=ms)
     best["trials"] = trials
     best["fit"] = dict(fixed_us=fixed, per_event_us=per_ev)
     best["policy"] = best["plan"].get("policy")
