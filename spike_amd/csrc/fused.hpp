@@ -5,15 +5,19 @@
 // workgroup (the Poisson inputs likewise).  A workgroup keeps its neurons' slice of the
 // input ring in LDS for the whole launch and is the only writer of it, so delivery uses
 // LDS atomics and no global atomics.  Per timestep group:
-//   1. neuron phase: external Poisson counts and input spikes drawn in parallel over
-//      (neuron, sub-step); then one lane per neuron runs the G-step LIF recurrence
-//      (LIFSpikingNeurons.cu:91-195) on the LDS ring; spikes are compacted with a
-//      wavefront ballot into the workgroup's exchange list (id | g << 24);
-//   2. one grid barrier publishes every workgroup's list;
+//   1. neuron phase: the group's ring slots are read in parallel over (neuron, sub-step),
+//      with the external Poisson counts and input spikes drawn one group earlier; then one
+//      lane per neuron runs the G-step LIF recurrence (LIFSpikingNeurons.cu:91-195);
+//      spikes are compacted into the workgroup's exchange list (id | g << 24);
+//   2. the list is published as tagged 64-bit words in the workgroup's exchange block --
+//      by the LIF wavefront itself when one wavefront holds the partition -- and every
+//      workgroup polls the blocks it needs (no grid barrier); with plasticity a grid
+//      barrier orders the phases instead;
 //   3. plasticity of the plastic synapses whose postsynaptic neuron the workgroup owns
 //      (they are the only weights its own delivery reads);
 //   4. delivery: every spike's row segment that targets this partition
 //      (CSR rows sorted by (partition, delay)) is added into the LDS ring.
+// The next group's random draws run while the workgroup waits for the others' lists.
 // The arithmetic of every phase is identical to the per-phase kernels in kernels.hpp.
 #pragma once
 #include <hip/hip_runtime.h>
