@@ -195,7 +195,9 @@ __host__ __device__ inline FusedLayout fused_layout(int BL, int L, int W, int G,
   f.hist = o;    o = align16(o + (size_t)W * nown * 4);
   f.ext = o;     o = align16(o + (size_t)dahead * nown * G);   // slots of the drawn groups
   f.imask = o;   o = align16(o + (size_t)dahead * iown * 8);
-  f.inp = o;     o = align16(o + (size_t)nown * G * L * 4);   // this group's inputs as float
+  // this group's inputs as float; count-ring partitions of more than one wavefront without
+  // an LDS delivery table keep them in the ring's own slots instead (in_ring in fused_body)
+  f.inp = o;     o = align16(o + (cring && nown > kWave && segU == 0 ? 0 : (size_t)nown * G * L * 4));
   f.xbuf = o;    o = align16(o + (size_t)xcap * 4);
   f.head = o;    o = align16(o + (size_t)P * kHeadWords * 4);
   f.scan = o;    o = align16(o + (size_t)(P + 1) * 4);
@@ -1158,6 +1160,11 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const int p, cons
   const float inv_G = 1.0f / (float)G;
   const int bits = W * 32;
   const int nown_waves = ((nown + kWave - 1) / kWave) * kWave;
+  // count-ring partitions of more than one wavefront without an LDS delivery table: B1's
+  // inputs stay in the ring's slots (no separate LDS array, so larger partitions fit; with
+  // one LIF wavefront the extra LDS traffic would lengthen the critical path, so small
+  // partitions keep the array; kernels of the LDS-table form never take this path)
+  const bool in_ring = DM != kDeliverLdsTable && a.cring && a.nown_max > kWave && a.lseg_U == 0;
   // one wavefront holds every unit of the partition (nown_waves <= kWave)
   const bool early = !PLASTIC && a.xquad && a.early && BS > kWave;
   // spike counters of the exchange list, one per group parity (the other parity's is
@@ -1244,12 +1251,16 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const int p, cons
         // integers below 2^53, so their double sum is the exact fixed-point sum
         uint32_t* cs = scring + (size_t)slot * nown + i;
         const uint32_t cw = *cs;
-        *cs = 0u;
+        if (!in_ring) *cs = 0u;
         double qs = (double)(cw & 0xFFFFu) * qd_lo + (double)(cw >> 16) * qd_hi;
         if (a.ext_k > 0) qs += (double)ext[i * G + g] * qd_ext;
         const int step = tidx + g;
         if (step < carry_end) qs += (double)a.ring[(size_t)(step % B) * NL + (size_t)(nlo + i)];
-        sinp[k] = (float)qs * a.inv_scale;
+        const float v = (float)qs * a.inv_scale;
+        // in the ring: the input stays in its slot as a float, and the neuron's LIF lane
+        // takes it and clears the slot
+        if (in_ring) *cs = __float_as_uint(v);
+        else sinp[k] = v;
         continue;
       }
       const int64_t ext_q = (a.ext_k > 0) ? (int64_t)ext[i * G + g] * a.q_ext : 0;
@@ -1282,19 +1293,41 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const int p, cons
     } else if (tid < nown_waves || (early && tid < kWave)) {
       unsigned long long smask = 0ull;
       if (own) {
+        // with in_ring, this group's inputs are floats in the count ring's slots (B1): take
+        // them and clear the slots for later deliveries
+        auto take = [&](int g) -> float {
+          int sl = slot0 + g;
+          if (sl >= BL) sl -= BL;
+          uint32_t* w = scring + (size_t)sl * nown + tid;
+          const float v = __uint_as_float(*w);
+          *w = 0u;
+          return v;
+        };
         if (KIND == 0 && G <= kPrefetch) {
-          // G known at compile time: straight-line code, LDS reads at immediate offsets
-          const float* in = sinp + tid * G;
+          // G known at compile time: straight-line code, reading the LDS array at immediate
+          // offsets or the values taken from the ring into registers
           uint32_t mk = 0u;
-          switch (G) {
-#define SPK_LIF_CASE(GG) \
-            case GG: mk = lif_block<GG>(in, t, a.dt, a.tref, half_dt, cv0, omc, cib, thr, vr, V, last); break;
-            SPK_LIF_CASE(1) SPK_LIF_CASE(2) SPK_LIF_CASE(3) SPK_LIF_CASE(4) SPK_LIF_CASE(5) SPK_LIF_CASE(6)
-            SPK_LIF_CASE(7) SPK_LIF_CASE(8) SPK_LIF_CASE(9) SPK_LIF_CASE(10) SPK_LIF_CASE(11) SPK_LIF_CASE(12)
-            SPK_LIF_CASE(13) SPK_LIF_CASE(14) SPK_LIF_CASE(15) SPK_LIF_CASE(16)
-#undef SPK_LIF_CASE
-            default: break;
+#define SPK_LIF_CASE(GG, IN) \
+            case GG: mk = lif_block<GG>(IN, t, a.dt, a.tref, half_dt, cv0, omc, cib, thr, vr, V, last); break;
+#define SPK_LIF_SWITCH(IN) \
+          switch (G) { \
+            SPK_LIF_CASE(1, IN) SPK_LIF_CASE(2, IN) SPK_LIF_CASE(3, IN) SPK_LIF_CASE(4, IN) SPK_LIF_CASE(5, IN) \
+            SPK_LIF_CASE(6, IN) SPK_LIF_CASE(7, IN) SPK_LIF_CASE(8, IN) SPK_LIF_CASE(9, IN) SPK_LIF_CASE(10, IN) \
+            SPK_LIF_CASE(11, IN) SPK_LIF_CASE(12, IN) SPK_LIF_CASE(13, IN) SPK_LIF_CASE(14, IN) \
+            SPK_LIF_CASE(15, IN) SPK_LIF_CASE(16, IN) \
+            default: break; \
           }
+          if (in_ring) {
+            float cin[kPrefetch];
+#pragma unroll
+            for (int x = 0; x < kPrefetch; ++x) cin[x] = (x < G) ? take(x) : 0.0f;
+            SPK_LIF_SWITCH(cin)
+          } else {
+            const float* in = sinp + tid * G;
+            SPK_LIF_SWITCH(in)
+          }
+#undef SPK_LIF_SWITCH
+#undef SPK_LIF_CASE
           smask = mk;
         }
         for (int g0 = 0; g0 < G && !(KIND == 0 && G <= kPrefetch); g0 += kPrefetch) {
@@ -1302,7 +1335,8 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const int p, cons
           if (KIND == 0) {
             float dvs[kPrefetch];
 #pragma unroll
-            for (int x = 0; x < kPrefetch; ++x) dvs[x] = (x < gn) ? sinp[tid * G + g0 + x] : 0.0f;
+            for (int x = 0; x < kPrefetch; ++x)
+              dvs[x] = (x < gn) ? (in_ring ? take(g0 + x) : sinp[tid * G + g0 + x]) : 0.0f;
             const float tref = a.tref, dt = a.dt, tg0 = t;
             unsigned long long mk = 0ull;
 #pragma unroll

@@ -177,11 +177,13 @@ def test_every_partition_count_builds_co_resident_or_reports_lds():
     """Partition counts around the LDS limit: generation either fails with the LDS message
     (so the sweep builder splits further) or gives a network whose persistent grid is
     co-resident -- it runs alone and in a batch.  A partition sized just under the
-    per-workgroup limit, with the kernel's static LDS on top, must not get through."""
-    p = sweep.SweepPoint(n=12000, g=5.0, nu_ratio=2.0, J=0.1e-3, delay_steps=20, seed=41)
+    per-workgroup limit, with the kernel's static LDS on top, must not get through.  Delay 40
+    steps (a 40-slot ring, G = 40: the LIF's chunked path with the inputs kept in the ring)
+    puts the limit inside the range of partition counts probed."""
+    p = sweep.SweepPoint(n=12000, g=5.0, nu_ratio=2.0, J=0.1e-3, delay_steps=40, seed=41)
     built, refused = 0, 0
     for P in range(12, 40):   # 12: at most 1000 neurons per partition
-        print(f"[batch] N=12000 delay 20 P={P}", flush=True)
+        print(f"[batch] N=12000 delay 40 P={P}", flush=True)
         try:
             net = abi.generated_brunel(sweep.brunel_params(p), P)
         except abi.SpikeError as e:
