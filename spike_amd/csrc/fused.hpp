@@ -1494,6 +1494,8 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const int p, cons
     }
     // spike monitors (SpikingActivityMonitor.cu:69-98): one store slot per wavefront
     // ballot; time = group time + g * dt as the reference's collection kernel computes it
+    // (the guard keeps the monitors' argument loads out of launches without monitors)
+    if (a.n_mons > 0)
 #pragma unroll
     for (int mi = 0; mi < kMaxFusedMons; ++mi) {
       if (mi >= a.n_mons) continue;
