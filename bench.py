@@ -113,8 +113,8 @@ def sweep_leg(n_points, seconds, cpu_points, threads, log, tune_seed=7):
     nu_ext/nu_thr U(0.9,4), J U(0.05,0.2) mV, delay U{10..20} steps, eps 0.1, seeds
     2024 + index) simulated for `seconds` each, (1) in one persistent batched launch and
     (2) one network per launch.  The batched tiling policy is tuned on a disjoint sample
-    of the same distribution (seed `tune_seed`; its finalists timed three times each over the
-    full `seconds`);
+    of the same distribution (seed `tune_seed`; its finalists timed over the full `seconds`
+    on three such samples, seeds `tune_seed` .. `tune_seed` + 2, each with its own plan);
     the benchmark points then get their plan
     from a short probe run, whose device time counts in the batched time.  Points/s =
     points / device seconds (HIP events; network generation is reported separately).
@@ -131,7 +131,7 @@ def sweep_leg(n_points, seconds, cpu_points, threads, log, tune_seed=7):
     pts = sweep.sample_points(n_points)
     t0 = time.perf_counter()
     best = sweep.autotune(sweep.sample_points(n_points, seed=tune_seed), log=log, refine_seconds=seconds,
-                          refine_repeats=3)
+                          refine_samples=[sweep.sample_points(n_points, seed=tune_seed + j) for j in range(3)])
     tune_s = time.perf_counter() - t0
     plan, probe_ms = sweep.plan_for(pts, best["policy"])
     rb = sweep.run_batched(pts, seconds, plan)
@@ -148,7 +148,7 @@ def sweep_leg(n_points, seconds, cpu_points, threads, log, tune_seed=7):
         "gate_speedup": ">= 10", "gate_speedup_ok": bool(speedup >= 10.0),
         "batched_ms": round(batched_ms, 3), "batched_run_ms": round(rb.ms, 3), "probe_ms": round(probe_ms, 3),
         "plain_ms": round(rp.ms, 3),
-        "plan": plan["name"], "tuned_on": f"sample_points({n_points}, seed={tune_seed})",
+        "plan": plan["name"], "tuned_on": f"sample_points({n_points}, seed={tune_seed}); finalists on seeds {tune_seed}-{tune_seed + 2}",
         "workgroups": rb.config["workgroups"], "threads_per_workgroup": rb.config["threads"],
         "autotune_s": round(tune_s, 2), "cost_fit_us": best["fit"],
         "batched_vs_plain_rate_max_rel_diff": float(np.max(np.abs(rb.rate - rp.rate) / np.maximum(rp.rate, 1e-12))),

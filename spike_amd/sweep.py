@@ -279,16 +279,19 @@ def plan_from_work(points, events_per_group, target_events, neurons_per_wg_max, 
 
 def autotune(points: Sequence[SweepPoint], probe_seconds: float = 0.02,
              targets=(2e4, 5e4, 1e5, 2e5, 5e5), neurons_per_wg_max=(384, 1024, 256), log=None,
-             refine_seconds: float = 0.1, refine_top: int = 3, refine_repeats: int = 1) -> Dict:
+             refine_seconds: float = 0.1, refine_top: int = 3, refine_repeats: int = 1,
+             refine_samples: Optional[Sequence[Sequence[SweepPoint]]] = None) -> Dict:
     """The project's tiling autotuner.  (1) A probe run of the whole sweep with the
     size-only plan measures every network's deliveries per group and its time per group;
     (2) a least-squares fit of time = fixed + per_event * events/workgroup gives the cost
     model; (3) every candidate (events-per-workgroup target, most neurons per workgroup)
     is turned into a plan (plan_from_work) and timed on a probe run; (4) the refine_top
-    fastest are timed again on refine_seconds, refine_repeats times each (a short probe
-    ranks plans within a few per cent of each other unreliably: their schedules diverge
-    later in the run, and one full-length run of a plan varies by about 5 %), and the
-    one with the lowest mean time is returned.  Fewer, larger partitions pay the per-group fixed cost fewer times; a
+    fastest are timed again on refine_seconds, refine_repeats times each -- or, with
+    refine_samples, once on each of those further samples of the distribution (each
+    finalist's policy applied to the sample by plan_for) -- and the one with the lowest mean
+    time is returned.  (A short probe ranks plans within a few per cent of each other
+    unreliably: their schedules diverge later in the run; and one sample can favour a policy
+    that another does not, by as much as the policies differ.)  Fewer, larger partitions pay the per-group fixed cost fewer times; a
     partition whose ring slice would not fit in LDS is split further when built.  Plans of
     at most 512 neurons per workgroup also keep every partition's LDS within half a compute
     unit's, so that the batch runs two 512-thread workgroups per compute unit."""
@@ -325,7 +328,13 @@ def autotune(points: Sequence[SweepPoint], probe_seconds: float = 0.02,
         finalists = [pl for _, pl in sorted(timed, key=lambda x: x[0])[:refine_top]]
         best = None
         for plan in finalists:
-            runs = [run_batched(points, refine_seconds, plan).ms for _ in range(max(1, refine_repeats))]
+            if refine_samples:
+                runs = []
+                for smp in refine_samples:
+                    sp, pms = plan_for(smp, plan.get("policy"))
+                    runs.append(run_batched(smp, refine_seconds, sp).ms + pms)
+            else:
+                runs = [run_batched(points, refine_seconds, plan).ms for _ in range(max(1, refine_repeats))]
             ms = float(np.mean(runs))
             trials.append(dict(plan=plan["name"], refine_ms=ms, refine_runs=runs))
             if log:
