@@ -1291,6 +1291,9 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const int p, cons
         draw_groups<BS>(a, s + 1, min(a.dahead, a.ngroups - s - 1), sext, simask, nlo, nown, ilo, iown, st_ext,
                         scdf, sguide, kWave);
     } else if (tid < nown_waves || (early && tid < kWave)) {
+      // the LIF wavefront of the early form first at the SIMD's issue arbiter while the
+      // draws run beside it
+      if (early) __builtin_amdgcn_s_setprio(3);
       unsigned long long smask = 0ull;
       if (own) {
         // with in_ring, this group's inputs are floats in the count ring's slots (B1): take
@@ -1443,6 +1446,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const int p, cons
           xstore64(xl, (uint32_t)min(nxr, a.xcap), tag);
           *scnt = nxr;
         }
+        __builtin_amdgcn_s_setprio(0);
       }
       if (!early && smask) {
         const int c = __popcll(smask);
